@@ -1,0 +1,189 @@
+"""Benchmark: ghost-ray segments/s on the batched 24-surface workload.
+
+Workload (one step = one ghost render, everything resident on the device):
+seeded 24-surface wide-angle lens with 2 cemented doublets and 3-layer TMM AR
+coatings, 276 ghost pairs pre-filtered then probed / culled / concentrated,
+256x256 pupil grid, 31 wavelengths, 1920x1080 FP32 RGB output, one point
+source at (12 deg, 4 deg).  A step runs the probe passes, the trace + splat
+kernel and the spectral correction; the image stays in HBM (no D2H).
+
+Prints ONE JSON line.  `value` = executed ghost-ray segments per second
+(segment = one surface interaction reaching intersect_surface, counted exactly
+by the device for the same render).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import platform
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+FP32_VECTOR_PEAK_TFLOPS = 157.3   # MI355X published FP32 vector peak
+HBM_PEAK_GBS = 8000.0
+
+
+def workload():
+    from ghostlight_amd import FlareConfig, synthetic
+    lens = synthetic.config2_lens()
+    W, H = 1920, 1080
+    hw, hh = synthetic.sensor_for(lens, W, H, half_field_deg=20.0)
+    cfg = FlareConfig(ray_grid=256, spectral_samples=31)
+    src = np.array([[math.radians(12.0), math.radians(4.0), 1.0, 1.0, 1.0]], np.float32)
+    return lens, cfg, src, W, H, hw, hh
+
+
+def ops_per_segment(lens, pairs, cfg, src, n_pairs_sample=12, n_rays=256):
+    """Algorithmic FP32 ops per traced segment, counted by the oracle's
+    counting scalar on a sample of this workload's rays."""
+    import ctypes as C
+    from oracle import pyoracle
+    from ghostlight_amd import hostprep
+    surfs, arena = lens.pack()
+    n = lens.num_surfaces()
+    spec = hostprep.spectral_table(cfg.spectral_samples)
+    rng = np.random.default_rng(0)
+    fR, z0 = lens.surfaces[0].semi_aperture, lens.surfaces[0].z - 20.0
+    bx, by = math.tan(float(src[0, 0])), math.tan(float(src[0, 1]))
+    d = np.array([bx, by, 1.0]); d /= np.linalg.norm(d)
+    tot = np.zeros(3, np.uint64)
+    idx = np.linspace(0, len(pairs) - 1, min(n_pairs_sample, len(pairs))).astype(int)
+    for i in idx:
+        a, b = pairs[i]
+        uv = rng.uniform(-1, 1, (n_rays, 2))
+        uv = uv[(uv ** 2).sum(1) <= 1]
+        lam = spec[rng.integers(0, len(spec), len(uv)), 0]
+        rays = np.zeros((len(uv), 7), np.float32)
+        rays[:, 0] = uv[:, 0] * fR - 20.0 * bx
+        rays[:, 1] = uv[:, 1] * fR - 20.0 * by
+        rays[:, 2] = z0
+        rays[:, 3:6] = d
+        rays[:, 6] = lam
+        sp = np.zeros((len(rays), 4), np.float32); sp[:, 0] = lam
+        at, bf = pyoracle.ior_tables(surfs, n, sp)
+        out = np.zeros(3, np.uint64)
+        pyoracle.lib().orc_count_ops(surfs, n, pyoracle._f(arena), a, b, pyoracle._f(rays), len(rays),
+                                     pyoracle._f(at), pyoracle._f(bf), out.ctypes.data_as(C.POINTER(C.c_uint64)))
+        tot += out
+    return float(tot[0]) / max(1.0, float(tot[1]))
+
+
+def cpu_baseline(lens, cfg, src, pairs, boosts, W, H, hw, hh, target_s=15.0):
+    """The FP32 oracle (OpenMP over all host cores) on a bounded sample of
+    the same workload: the same lens, grid, spectra and source, on a subset
+    of the ghost pairs sized to ~target_s seconds."""
+    from oracle import pyoracle
+    from ghostlight_amd import hostprep
+    from ghostlight_amd.config import to_params
+    spec = hostprep.spectral_table(cfg.spectral_samples)
+    C = hostprep.spectral_correction(spec)
+    p, _keep = to_params(cfg)
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    k = 2
+    t0 = time.perf_counter()
+    _, info = pyoracle.render_ghosts(lens, pairs[:k], boosts[:k], src, spec, p, hw, hh, W, H, correction=C,
+                                     threads=threads)
+    dt = time.perf_counter() - t0
+    k2 = int(min(len(pairs), max(k, k * target_s / max(dt, 1e-3))))
+    t0 = time.perf_counter()
+    _, info = pyoracle.render_ghosts(lens, pairs[:k2], boosts[:k2], src, spec, p, hw, hh, W, H, correction=C,
+                                     threads=threads)
+    dt = time.perf_counter() - t0
+    return {"value": info["segments"] / dt, "unit": "segments/s", "cores": threads, "kind": "port",
+            "sample": f"FP32 oracle render of the first {k2} of {len(pairs)} ghost pairs of the same workload "
+                      f"({info['segments']} segments in {dt:.2f} s, {platform.processor() or 'x86_64'})"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print(f"bench.py: this build measures one GPU; --gpus {a.gpus} is not supported "
+              f"(multi-GPU sharding is a design note only)", file=sys.stderr)
+        sys.exit(2)
+
+    from ghostlight_amd import Context, filter_ghost_pairs, render_ghosts
+    lens, cfg, src, W, H, hw, hh = workload()
+    ctx = Context()
+    pairs, boosts = filter_ghost_pairs(lens, hw, hh, cfg, ctx=ctx)
+    print(f"bench: C2 lens {lens.num_surfaces()} surfaces, {len(pairs)} pre-filtered pairs, "
+          f"ray_grid {cfg.ray_grid}, {cfg.spectral_samples} lambda, {W}x{H}", file=sys.stderr, flush=True)
+
+    def step():
+        return render_ghosts(lens, src, W, H, cfg, hw, hh, pairs=pairs, boosts=boosts, ctx=ctx, device_out=True)
+
+    for _ in range(a.warmup):
+        step()
+    ctx.synchronize()
+    kms = []
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        step()
+        kms.append(ctx.last_kernel_ms())
+    ctx.synchronize()
+    t1 = time.perf_counter()
+    ms_per_step = (t1 - t0) * 1e3 / a.steps
+    # Exact segment count and survivor telemetry of the same render (untimed).
+    _, st = render_ghosts(lens, src, W, H, cfg, hw, hh, pairs=pairs, boosts=boosts, ctx=ctx, device_out=True,
+                          stats=True)
+    segs = st["segments"]
+    live_pairs = sum(1 for x in st["pair_on_sensor"] if x > 0)
+    value = segs / (ms_per_step * 1e-3)
+    kernel_ms = float(np.mean(kms))
+    print(f"bench: {segs} segments/step, {ms_per_step:.3f} ms/step, main kernel {kernel_ms:.3f} ms, "
+          f"{live_pairs} pairs reach the sensor", file=sys.stderr, flush=True)
+
+    roof = None
+    try:
+        ops = ops_per_segment(lens, pairs, cfg, src)
+        achieved = segs * ops / (kernel_ms * 1e-3) / 1e12
+        roof = {"bound": "valu", "achieved": round(achieved, 3), "peak": FP32_VECTOR_PEAK_TFLOPS,
+                "unit": "TFLOP/s", "frac": round(achieved / FP32_VECTOR_PEAK_TFLOPS, 4), "traffic": None,
+                "kernel": "ghost_kernel<false,true,false> (trace + splat)", "kernel_ms": round(kernel_ms, 4),
+                "ops_per_segment": round(ops, 2), "segments_per_launch": segs}
+    except Exception as e:  # the oracle library is test infrastructure; report its absence
+        print(f"bench: roofline op count unavailable ({e})", file=sys.stderr)
+
+    cpu = None
+    if not a.no_cpu:
+        try:
+            cpu = cpu_baseline(lens, cfg, src, pairs, boosts, W, H, hw, hh)
+        except Exception as e:
+            print(f"bench: cpu baseline unavailable ({e})", file=sys.stderr)
+
+    out = {
+        "metric": "ghost-ray segments/s",
+        "value": value,
+        "unit": "segments/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded lens, point source)",
+        "config": {"workload": "batched ghost trace: 24-surface lens, 2 cemented doublets, 3-layer TMM AR, "
+                               "276 pairs -> filter/probe/cull/concentrate, 256x256 pupil, 31 lambda, "
+                               "1920x1080 FP32 RGB", "pairs_prefiltered": len(pairs),
+                   "pairs_on_sensor": live_pairs, "segments_per_step": segs, "parallelism": "single GPU"},
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

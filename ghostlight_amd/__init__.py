@@ -1,0 +1,23 @@
+"""ghostlight_amd — MI355X-native flare hot path of Ghostlight.
+
+Ghost tracer, starburst and PSF/veil passes as hand-written gfx950 HIP kernels
+behind a C ABI (include/ghostlight_amd.h), with the reference's Python API
+(render_point_flare, render_source_flare, config classes, output dicts) on top.
+"""
+from .config import (BaffleShape, Baffle, DiffractionConfig, FlareConfig, GateConfig, GateLobe, GhostAovMode,
+                     GhostFilter, HurbKickDistribution, InputColorSpace, MatteBox, OutputColorSpace,
+                     PointFlareConfig, PSFCenterMode, PSFConfig, RenderConfig, SensorModel, StarburstEngine)
+from .lens import (ApertureShape, Coating, CoatingModel, CylinderAxis, DispersionModel, LensCalibration,
+                   OpticalSystem, Surface, SurfaceForm, TraceStatus)
+from .runtime import Context, device_available
+from .render import (enumerate_ghost_pairs, filter_ghost_pairs, render_ghosts, render_point_flare,
+                     render_source_flare, solve_source_angle, trace_ghost_rays)
+from .hostprep import build_spectral_lambdas
+
+
+def _cuda_available() -> bool:
+    """Reference name for the device check (bind_renderers.cpp:518)."""
+    return device_available()
+
+
+__all__ = [n for n in dir() if not n.startswith("_")] + ["_cuda_available"]

@@ -1,0 +1,876 @@
+// ghost.hip — the ghost path on gfx950: pair pre-filter probes, the
+// probe / cull / concentration passes, the ghost trace + splat, the spectral
+// colour correction and the host launcher that sequences them on a context
+// stream (launch_ghost_render / render_ghost_pipeline semantics,
+// ghostlight/src/ghost_render.cu:630-1581).
+//
+// Work mapping.  blockIdx.x = (pair, source) slot, blockIdx.y = 256-sample
+// chunk of the pupil grid; one lane per pupil sample loops over the spectral
+// samples (the reference's decomposition, ghost_render.cu:249-254,372), so a
+// wave shares its pair and walks the prescription with uniform indices.
+#include "gl_runtime.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+
+namespace gl {
+
+static constexpr int kBlock = 256;
+static constexpr int kProbeGrid = 40;
+static constexpr float kCullMargin = 1.25f;
+static constexpr int kProbePasses = 3;
+static constexpr float kConcDilateCells = 1.5f;
+static constexpr int kConcMinHits = 3;
+static constexpr int kRefineGrid = 128;
+static constexpr int kRefineHitsMax = 16;
+static constexpr int kRefineMaxEntries = 4096;
+
+struct SrcDev { float bx, by, dx, dy, dz, r, g, b; };
+
+struct ConcDev {
+    const float4* rects;      // nullptr = off
+    DevMask mask;
+    int jitter;
+    uint32_t seed_offset;
+    int nr_lattice, n_r;
+    float area_norm;
+    const int2* budgets;      // nullptr = uniform budget
+    float inv_aref;
+};
+
+struct GhostArgs {
+    const DevSurf* __restrict__ S;
+    const float* __restrict__ poly;
+    const float* __restrict__ arena;
+    int n;
+    const gl_pair* __restrict__ pairs;
+    int n_src;
+    const SrcDev* __restrict__ src;
+    const gl_sample* __restrict__ grid;
+    int n_grid;
+    float front_R, start_z;
+    float sw, sh;
+    int width, height;
+    float fmt_w, fmt_h, x0, y0;
+    float* __restrict__ out_r;
+    float* __restrict__ out_g;
+    float* __restrict__ out_b;
+    float gain, ray_weight;
+    const float4* __restrict__ spec;   // lambda, cmf_r, cmf_g, cmf_b
+    int n_spec;
+    const float* __restrict__ iat;
+    const float* __restrict__ ibf;
+    const unsigned char* __restrict__ pair_active;
+    const unsigned char* __restrict__ ps_refine;
+    ConcDev conc;
+    int* __restrict__ bbox;                     // STATS: mins | maxs | hits
+    unsigned long long* __restrict__ cnt;       // STATS: traces | survived | on-sensor (per pair)
+    unsigned long long* __restrict__ segs;      // optional segment counter
+};
+
+__device__ __forceinline__ int enc_pupil(float x) { return __float_as_int(x + 2.0f); }
+
+__device__ __forceinline__ unsigned long long wave_sum_u64(unsigned long long v) {
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+__device__ __forceinline__ float wave_min(float v) {
+    for (int off = 32; off > 0; off >>= 1) v = fminf(v, __shfl_xor(v, off, 64));
+    return v;
+}
+__device__ __forceinline__ float wave_max(float v) {
+    for (int off = 32; off > 0; off >>= 1) v = fmaxf(v, __shfl_xor(v, off, 64));
+    return v;
+}
+
+// Sample position and estimator weight (ghost_render.cu:280-335).
+__device__ __forceinline__ bool sample_at(const GhostArgs& A, int ps, int gi, float& u, float& v, float& w_ray) {
+    w_ray = A.ray_weight;
+    if (A.conc.rects != nullptr) {
+        const float4 rc = A.conc.rects[ps];
+        if (rc.x <= rc.z) {
+            int n_r_ps = A.conc.n_r, nr_lat = A.conc.nr_lattice;
+            if (A.conc.budgets) { const int2 b = A.conc.budgets[ps]; n_r_ps = b.x; nr_lat = b.y; }
+            if (gi >= n_r_ps) return false;
+            float fu, fv;
+            if (A.conc.jitter == 2) { fu = halton2((uint32_t)gi); fv = halton3((uint32_t)gi); }
+            else {
+                const int gx = gi % nr_lat, gy = gi / nr_lat, nn = nr_lat * nr_lat;
+                const float ju = A.conc.jitter == 1 ? (float)wang_hash((uint32_t)gi + A.conc.seed_offset) / 4294967296.0f : 0.5f;
+                const float jv = A.conc.jitter == 1 ? (float)wang_hash((uint32_t)gi + (uint32_t)nn + A.conc.seed_offset) / 4294967296.0f : 0.5f;
+                fu = ((float)gx + ju) / (float)nr_lat;
+                fv = ((float)gy + jv) / (float)nr_lat;
+            }
+            u = rc.x + fu * (rc.z - rc.x);
+            v = rc.y + fv * (rc.w - rc.y);
+            if (!mask_contains(A.conc.mask, u, v)) return false;
+            w_ray = (rc.z - rc.x) * (rc.w - rc.y) * (A.conc.budgets ? A.conc.inv_aref / (float)n_r_ps : A.conc.area_norm);
+            return true;
+        }
+    }
+    const gl_sample g = A.grid[gi];
+    u = g.u; v = g.v;
+    return true;
+}
+
+__device__ __forceinline__ void splat_px(float* __restrict__ o, int p, float c, float w) {
+    if (fabsf(c) > 1e-14f) atomicAdd(o + p, c * w);
+}
+
+// STATS = probe / diagnostic pass (counters, survivor bbox); otherwise the
+// render (SPLAT) and/or the segment count (COUNT).
+template <bool STATS, bool SPLAT, bool COUNT>
+__global__ __launch_bounds__(kBlock) void ghost_kernel(GhostArgs A) {
+    const int ps = blockIdx.x;
+    const int pair_idx = ps / A.n_src, src_idx = ps % A.n_src;
+    const int gi = (int)blockIdx.y * kBlock + (int)threadIdx.x;
+    // Whole-block exits are uniform; per-lane validity stays a flag so the
+    // wave reductions below see every lane.
+    if (A.pair_active != nullptr && A.pair_active[pair_idx] == 0) return;
+    if (A.ps_refine != nullptr && A.ps_refine[ps] == 0) return;
+    const gl_pair pr = A.pairs[pair_idx];
+    if (pr.surf_a < 0 || pr.surf_b <= pr.surf_a || pr.surf_b >= A.n) return;
+
+    float u = 0.f, v = 0.f, w_ray = 0.f;
+    bool live = gi < A.n_grid;
+    if (live) live = sample_at(A, ps, gi, u, v, w_ray);
+
+    unsigned long long n_tr = 0, n_sv = 0, n_on = 0, n_seg = 0;
+    if (live) {
+        const SrcDev sr = A.src[src_idx];
+        const F3 o0 = f3(u * A.front_R + (-20.0f * sr.bx), v * A.front_R + (-20.0f * sr.by), A.start_z);
+        const F3 d0 = f3(sr.dx, sr.dy, sr.dz);
+        for (int si = 0; si < A.n_spec; ++si) {
+            const float GLC* spf = as_const(reinterpret_cast<const float*>(A.spec)) + 4 * si;
+            const float4 sp = make_float4(spf[0], spf[1], spf[2], spf[3]);
+            if (STATS) ++n_tr;
+            TraceRes res = trace_ghost(as_const(A.S), A.n, as_const(A.poly), A.arena, pr.surf_a, pr.surf_b, o0, d0,
+                                       sp.x, as_const(A.iat) + (size_t)si * A.n, as_const(A.ibf) + (size_t)si * A.n,
+                                       false);
+            if (COUNT || STATS) n_seg += (unsigned long long)res.segs;
+            if (res.status != GL_TRACE_OK) continue;
+            if (!isfinite(res.px) || !isfinite(res.py)) continue;
+            if (STATS) {
+                ++n_sv;
+                float spx = (res.px / (2.0f * A.sw) + 0.5f) * A.fmt_w + A.x0;
+                float spy = (res.py / (2.0f * A.sh) + 0.5f) * A.fmt_h + A.y0;
+                if (isfinite(spx) && isfinite(spy) && spx >= 0.0f && spx < (float)A.width && spy >= 0.0f &&
+                    spy < (float)A.height) ++n_on;
+                continue;
+            }
+            if (!SPLAT) continue;
+            float base = res.w * w_ray * A.gain * pr.area_boost;
+            if (base < 1e-14f) continue;
+            float px = (res.px / (2.0f * A.sw) + 0.5f) * A.fmt_w + A.x0;
+            float py = (res.py / (2.0f * A.sh) + 0.5f) * A.fmt_h + A.y0;
+            if (!isfinite(px) || !isfinite(py)) continue;
+            px = fmaxf(-2.0e9f, fminf(2.0e9f, px));
+            py = fmaxf(-2.0e9f, fminf(2.0e9f, py));
+            const int x0 = (int)floorf(px - 0.5f), y0 = (int)floorf(py - 0.5f);
+            const float fx = (px - 0.5f) - (float)x0, fy = (py - 0.5f) - (float)y0;
+            const float w00 = (1.0f - fx) * (1.0f - fy), w10 = fx * (1.0f - fy);
+            const float w01 = (1.0f - fx) * fy, w11 = fx * fy;
+            const float P = sr.r * sp.y + sr.g * sp.z + sr.b * sp.w;
+            const float power = P * base;
+            const float cr = power * sp.y, cg = power * sp.z, cb = power * sp.w;
+            const int xs[4] = {x0, x0 + 1, x0, x0 + 1}, ys[4] = {y0, y0, y0 + 1, y0 + 1};
+            const float ws[4] = {w00, w10, w01, w11};
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                if (xs[k] < 0 || xs[k] >= A.width || ys[k] < 0 || ys[k] >= A.height) continue;
+                const int p = ys[k] * A.width + xs[k];
+                splat_px(A.out_r, p, cr, ws[k]);
+                splat_px(A.out_g, p, cg, ws[k]);
+                splat_px(A.out_b, p, cb, ws[k]);
+            }
+        }
+    }
+    if (STATS) {
+        const unsigned long long t = wave_sum_u64(n_tr), s = wave_sum_u64(n_sv), o = wave_sum_u64(n_on);
+        const unsigned long long sg = wave_sum_u64(n_seg);
+        const bool lane0 = (threadIdx.x & 63) == 0;
+        if (lane0) {
+            if (t) atomicAdd(A.cnt + pair_idx, t);
+            if (s) atomicAdd(A.cnt + gridDim.x / A.n_src + pair_idx, s);
+            if (o) atomicAdd(A.cnt + 2 * (gridDim.x / A.n_src) + pair_idx, o);
+            if (A.segs && sg) atomicAdd(A.segs, sg);
+        }
+        if (A.bbox != nullptr) {
+            const bool on = n_on > 0;
+            const unsigned long long bal = __ballot(on);
+            const float umn = wave_min(on ? u : 3.0f), vmn = wave_min(on ? v : 3.0f);
+            const float umx = wave_max(on ? u : -3.0f), vmx = wave_max(on ? v : -3.0f);
+            if (lane0 && bal) {
+                const int n_ps = (int)gridDim.x;
+                atomicMin(A.bbox + 2 * ps, enc_pupil(umn));
+                atomicMin(A.bbox + 2 * ps + 1, enc_pupil(vmn));
+                atomicMax(A.bbox + 2 * n_ps + 2 * ps, enc_pupil(umx));
+                atomicMax(A.bbox + 2 * n_ps + 2 * ps + 1, enc_pupil(vmx));
+                atomicAdd(A.bbox + 4 * n_ps + ps, (int)__popcll(bal));
+            }
+        }
+    } else if (COUNT) {
+        const unsigned long long sg = wave_sum_u64(n_seg);
+        if ((threadIdx.x & 63) == 0 && sg) atomicAdd(A.segs, sg);
+    }
+}
+
+// Output-space 3x3 colour correction (ghost_render.cu:610-622); contraction
+// is off for this file so each product and sum rounds separately.
+__global__ void spectral_correct_kernel(float* __restrict__ r, float* __restrict__ g, float* __restrict__ b, size_t n,
+                                        float c00, float c01, float c02, float c10, float c11, float c12,
+                                        float c20, float c21, float c22) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float rr = r[i], gg = g[i], bb = b[i];
+    r[i] = (c00 * rr + c01 * gg) + c02 * bb;
+    g[i] = (c10 * rr + c11 * gg) + c12 * bb;
+    b[i] = (c20 * rr + c21 * gg) + c22 * bb;
+}
+
+// Pre-filter probes (ghost.cpp:34-103) for every pair: lanes 0-2 the on-axis
+// intensity rays at 650/550/450 nm, lanes 3-66 the 8x8 green spread grid.
+// CPU-tracer semantics (sensor t >= 0).  out: float4 {x, y, w, status}.
+__global__ void filter_probe_kernel(const DevSurf* __restrict__ S, const float* __restrict__ poly,
+                                    const float* __restrict__ arena, int n, const int2* __restrict__ pairs,
+                                    const float* __restrict__ iat3, const float* __restrict__ ibf3, float front_R,
+                                    float z0, float4* __restrict__ out) {
+    const int p = blockIdx.x, t = threadIdx.x;
+    if (t >= 67) return;
+    const int2 pr = pairs[p];
+    F3 o, d = f3(0.f, 0.f, 1.f);
+    float lam;
+    int slot;
+    if (t < 3) {
+        o = f3(0.f, 0.f, z0);
+        lam = t == 0 ? 650.0f : (t == 1 ? 550.0f : 450.0f);
+        slot = t;
+    } else {
+        const int k = t - 3, gx = k % 8, gy = k / 8;
+        const float u = (((float)gx + 0.5f) / 8.0f) * 2.0f - 1.0f;
+        const float v = (((float)gy + 0.5f) / 8.0f) * 2.0f - 1.0f;
+        if (u * u + v * v > 1.0f) { out[(size_t)p * 67 + t] = make_float4(0.f, 0.f, 0.f, -1.0f); return; }
+        o = f3(u * front_R, v * front_R, z0);
+        lam = 550.0f;
+        slot = 1;
+    }
+    TraceRes r = trace_ghost(as_const(S), n, as_const(poly), arena, pr.x, pr.y, o, d, lam,
+                             as_const(iat3) + (size_t)slot * n, as_const(ibf3) + (size_t)slot * n, true);
+    out[(size_t)p * 67 + t] = make_float4(r.px, r.py, r.w, (float)r.status);
+}
+
+// Single-ray traces for tests and probes.
+__global__ void ray_kernel(const DevSurf* __restrict__ S, const float* __restrict__ poly, const float* __restrict__ arena,
+                           int n, int a, int b, const float* __restrict__ rays, int n_rays, const float* __restrict__ iat,
+                           const float* __restrict__ ibf, int t_ge0, float* __restrict__ pos, float* __restrict__ w,
+                           int* __restrict__ st, int* __restrict__ segs) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n_rays) return;
+    const float* q = rays + 7 * (size_t)i;
+    // Per-ray IOR slices: lane-varying, so plain global pointers.
+    TraceRes r = (b >= 0) ? trace_ghost(as_const(S), n, as_const(poly), arena, a, b, f3(q[0], q[1], q[2]),
+                                        f3(q[3], q[4], q[5]), q[6], as_const(iat + (size_t)i * n),
+                                        as_const(ibf + (size_t)i * n), t_ge0 != 0)
+                          : trace_primary(as_const(S), n, as_const(poly), arena, f3(q[0], q[1], q[2]),
+                                          f3(q[3], q[4], q[5]), q[6], as_const(iat + (size_t)i * n), t_ge0 != 0, nullptr);
+    pos[3 * i] = r.px; pos[3 * i + 1] = r.py; pos[3 * i + 2] = r.pz;
+    w[i] = r.w; st[i] = r.status;
+    if (segs) segs[i] = r.segs;
+}
+
+// ---------------------------------------------------------------------------
+// Host side
+// ---------------------------------------------------------------------------
+
+static int upload(gl_ctx* c, const char* name, const void* src, size_t bytes, void** dptr) {
+    DevBuf& b = c->buf(name);
+    int st = b.ensure(bytes);
+    if (st) return st;
+    if (bytes) GL_HIP(hipMemcpyAsync(b.p, src, bytes, hipMemcpyHostToDevice, c->stream));
+    *dptr = b.p;
+    return GL_OK;
+}
+
+template <class T> static void key_append(std::vector<uint8_t>& k, const T& v) {
+    const uint8_t* p = reinterpret_cast<const uint8_t*>(&v);
+    k.insert(k.end(), p, p + sizeof(T));
+}
+
+static SrcDev make_src(const gl_source& s) {
+    SrcDev d;
+    d.bx = (float)std::tan((double)s.angle_x);
+    d.by = (float)std::tan((double)s.angle_y);
+    // normalize(bx, by, 1) with the tracer's float operation order (vec3.h:67-71).
+    float l = std::sqrt(d.bx * d.bx + d.by * d.by + 1.0f * 1.0f);
+    float inv = (l > 1e-12f) ? 1.0f / l : 0.0f;
+    d.dx = d.bx * inv; d.dy = d.by * inv; d.dz = 1.0f * inv;
+    d.r = s.r; d.g = s.g; d.b = s.b;
+    return d;
+}
+
+template <bool STATS, bool SPLAT, bool COUNT>
+static int launch(const GhostArgs& A, int n_ps, int n_samples, hipStream_t s) {
+    if (n_ps <= 0 || n_samples <= 0) return GL_OK;
+    dim3 grid((unsigned)n_ps, (unsigned)((n_samples + kBlock - 1) / kBlock), 1);
+    hipLaunchKernelGGL((ghost_kernel<STATS, SPLAT, COUNT>), grid, dim3(kBlock), 0, s, A);
+    GL_HIP(hipGetLastError());
+    return GL_OK;
+}
+
+struct Launch {
+    gl_ctx* c;
+    gl_lens* L;
+    const gl_flare_params* p;
+    std::vector<gl_pair> pairs;
+    std::vector<SrcDev> src;
+    int n_src, width, height, fmt_w, fmt_h, fx0, fy0;
+    float sw, sh;
+};
+
+// Host caches: pupil grid (GridKey) and spectral table (SpecKey).
+static int prepare_tables(gl_ctx* c, const gl_lens* L, const gl_flare_params* p) {
+    std::vector<uint8_t> gk;
+    key_append(gk, p->ray_grid); key_append(gk, p->pupil_jitter); key_append(gk, p->jitter_seed);
+    key_append(gk, p->aperture_blades); key_append(gk, p->aperture_rotation);
+    for (const auto& s : L->host)
+        if (s.is_stop) {
+            key_append(gk, s.aperture_shape); key_append(gk, s.aperture_blades);
+            key_append(gk, s.aperture_aspect); key_append(gk, s.aperture_rotation_rad);
+            break;
+        }
+    if (gk != c->grid_key) {
+        c->grid = build_samples(L->host.data(), (int)L->host.size(), p->ray_grid, p->pupil_jitter, p->jitter_seed,
+                                p->aperture_blades, p->aperture_rotation);
+        c->grid_key = gk;
+        c->grid_uploaded = false;
+    }
+    const int ns = std::max(3, p->spectral_samples);
+    std::vector<uint8_t> sk;
+    key_append(sk, ns); key_append(sk, p->sensor_model); key_append(sk, p->output_cs);
+    key_append(sk, p->spectral_correction); key_append(sk, p->custom_xyz_to_output);
+    if (sk != c->spec_key) {
+        int st = spectral_table(ns, p->output_cs, p->custom_xyz_to_output, c->spec);
+        if (st) return st;
+        c->spec_do_correct = (ns != 3) && p->spectral_correction;
+        if (c->spec_do_correct) spectral_correction(c->spec, p->output_cs, p->custom_xyz_to_output, c->spec_C);
+        c->spec_key = sk;
+        c->spec_uploaded = false;
+    }
+    return GL_OK;
+}
+
+static int do_launch(Launch& Lc, float* out_r, float* out_g, float* out_b, float** dev_out, gl_ghost_stats* stats) {
+    gl_ctx* c = Lc.c;
+    gl_lens* L = Lc.L;
+    const gl_flare_params* p = Lc.p;
+    const double t_fn0 = now_ms();
+    const int n = (int)L->host.size();
+    const int n_pairs = (int)Lc.pairs.size(), n_src = Lc.n_src;
+    int st;
+    const double t_prep0 = now_ms();
+    if ((st = prepare_tables(c, L, p))) return st;
+    const int n_grid = (int)c->grid.size();
+    const int ns = (int)c->spec.size();
+    std::vector<float> iat((size_t)ns * n), ibf((size_t)ns * n);
+    ior_tables(L->host.data(), n, c->spec.data(), ns, iat.data(), ibf.data());
+    const double t_prep1 = now_ms();
+
+    const size_t npx = (size_t)Lc.width * Lc.height;
+    DevBuf &bo_r = c->buf("out_r"), &bo_g = c->buf("out_g"), &bo_b = c->buf("out_b");
+    if ((st = bo_r.ensure(npx * 4)) || (st = bo_g.ensure(npx * 4)) || (st = bo_b.ensure(npx * 4))) return st;
+    GL_HIP(hipMemsetAsync(bo_r.p, 0, npx * 4, c->stream));
+    GL_HIP(hipMemsetAsync(bo_g.p, 0, npx * 4, c->stream));
+    GL_HIP(hipMemsetAsync(bo_b.p, 0, npx * 4, c->stream));
+    if (dev_out) { dev_out[0] = bo_r.as<float>(); dev_out[1] = bo_g.as<float>(); dev_out[2] = bo_b.as<float>(); }
+    if (stats) {
+        stats->n_pairs = n_pairs; stats->n_sources = n_src; stats->n_grid = n_grid; stats->n_spec = ns;
+        stats->ms_grid_build = (float)(t_prep1 - t_prep0);
+    }
+    auto finish_empty = [&]() -> int {
+        if (out_r) {
+            GL_HIP(hipMemcpyAsync(out_r, bo_r.p, npx * 4, hipMemcpyDeviceToHost, c->stream));
+            GL_HIP(hipMemcpyAsync(out_g, bo_g.p, npx * 4, hipMemcpyDeviceToHost, c->stream));
+            GL_HIP(hipMemcpyAsync(out_b, bo_b.p, npx * 4, hipMemcpyDeviceToHost, c->stream));
+        }
+        GL_HIP(hipStreamSynchronize(c->stream));
+        return GL_OK;
+    };
+    if (n_pairs == 0 || n_src == 0 || n_grid == 0) return finish_empty();
+
+    const double t_up0 = now_ms();
+    if ((st = lens_sync(L, p->fp64_radius_min, c->stream))) return st;
+    void *d_pairs, *d_src, *d_iat, *d_ibf, *d_grid = c->buf("grid").p, *d_spec = c->buf("spec").p;
+    if ((st = upload(c, "pairs", Lc.pairs.data(), n_pairs * sizeof(gl_pair), &d_pairs))) return st;
+    if ((st = upload(c, "src", Lc.src.data(), n_src * sizeof(SrcDev), &d_src))) return st;
+    if ((st = upload(c, "iat", iat.data(), iat.size() * 4, &d_iat))) return st;
+    if ((st = upload(c, "ibf", ibf.data(), ibf.size() * 4, &d_ibf))) return st;
+    if (!c->grid_uploaded || !d_grid || c->buf("grid").cap < n_grid * sizeof(gl_sample)) {
+        if ((st = upload(c, "grid", c->grid.data(), n_grid * sizeof(gl_sample), &d_grid))) return st;
+        c->grid_uploaded = true;
+    }
+    if (!c->spec_uploaded || !d_spec || c->buf("spec").cap < ns * sizeof(gl_spec)) {
+        if ((st = upload(c, "spec", c->spec.data(), ns * sizeof(gl_spec), &d_spec))) return st;
+        c->spec_uploaded = true;
+    }
+    const double t_up1 = now_ms();
+
+    GhostArgs A{};
+    A.S = L->d_surf; A.poly = L->d_poly; A.arena = L->d_arena; A.n = n;
+    A.pairs = static_cast<const gl_pair*>(d_pairs);
+    A.n_src = n_src; A.src = static_cast<const SrcDev*>(d_src);
+    A.grid = static_cast<const gl_sample*>(d_grid); A.n_grid = n_grid;
+    A.front_R = L->host[0].semi_aperture; A.start_z = L->host[0].z - 20.0f;
+    A.sw = Lc.sw; A.sh = Lc.sh; A.width = Lc.width; A.height = Lc.height;
+    A.fmt_w = (float)Lc.fmt_w; A.fmt_h = (float)Lc.fmt_h; A.x0 = (float)Lc.fx0; A.y0 = (float)Lc.fy0;
+    A.out_r = bo_r.as<float>(); A.out_g = bo_g.as<float>(); A.out_b = bo_b.as<float>();
+    A.gain = p->flare_gain; A.ray_weight = 1.0f / (float)n_grid;
+    A.spec = static_cast<const float4*>(d_spec); A.n_spec = ns;
+    A.iat = static_cast<const float*>(d_iat); A.ibf = static_cast<const float*>(d_ibf);
+    A.conc.rects = nullptr;
+
+    // ---- probe / cull / concentrate (ghost_render.cu:969-1238)
+    const bool want_conc = p->concentrate_samples && n_pairs > 1 && ns != 3;
+    const int n_ps = n_pairs * n_src;
+    bool cull_ready = false, rect_ready = false;
+    std::vector<float4> rects;
+    std::vector<unsigned char> active(n_pairs, 1);
+    if ((p->cull_dead_pairs || want_conc) && n_pairs > 1 && ns != 3) {
+        DevBuf &bc = c->buf("probe_cnt"), &bb = c->buf("probe_bbox"), &bg = c->buf("probe_grid");
+        DevBuf &bm = c->buf("pair_mask"), &brf = c->buf("refine_mask");
+        if ((st = bc.ensure(3 * n_pairs * 8)) || (st = bb.ensure((size_t)n_ps * 5 * 4)) ||
+            (st = bg.ensure((size_t)kRefineGrid * kRefineGrid * sizeof(gl_sample))) || (st = bm.ensure(n_pairs)) ||
+            (st = brf.ensure(n_ps)))
+            return st;
+        GL_HIP(hipMemsetAsync(bc.p, 0, 3 * n_pairs * 8, c->stream));
+        if (want_conc) {
+            GL_HIP(hipMemsetAsync(bb.p, 0x7F, (size_t)n_ps * 2 * 4, c->stream));
+            GL_HIP(hipMemsetAsync(bb.as<int>() + 2 * n_ps, 0, (size_t)n_ps * 3 * 4, c->stream));
+        }
+        GhostArgs P = A;
+        P.sw = Lc.sw * kCullMargin; P.sh = Lc.sh * kCullMargin;
+        P.cnt = bc.as<unsigned long long>();
+        P.bbox = want_conc ? bb.as<int>() : nullptr;
+        P.segs = nullptr;
+        std::vector<unsigned char> cand(n_pairs, 0);
+        std::vector<unsigned long long> cnt(3 * n_pairs);
+        bool pass0_ok = false;
+        for (int pass = 0; pass < kProbePasses; ++pass) {
+            std::vector<gl_sample> probe = build_samples(L->host.data(), n, kProbeGrid, 1,
+                                                         (int)(0x9E3779B9u * (unsigned)(pass + 1)),
+                                                         p->aperture_blades, p->aperture_rotation);
+            if (probe.empty()) { if (pass == 0) break; else continue; }
+            GL_HIP(hipMemcpyAsync(bg.p, probe.data(), probe.size() * sizeof(gl_sample), hipMemcpyHostToDevice, c->stream));
+            P.grid = bg.as<gl_sample>(); P.n_grid = (int)probe.size();
+            P.pair_active = (pass == 0 || want_conc) ? nullptr : bm.as<unsigned char>();
+            P.ps_refine = nullptr;
+            if ((st = launch<true, false, false>(P, n_ps, P.n_grid, c->stream))) return st;
+            if (pass == 0) {
+                GL_HIP(hipStreamSynchronize(c->stream));
+                pass0_ok = true;
+                if (!want_conc) {
+                    GL_HIP(hipMemcpy(cnt.data(), bc.p, 3 * n_pairs * 8, hipMemcpyDeviceToHost));
+                    int nc = 0;
+                    for (int q = 0; q < n_pairs; ++q)
+                        if (cnt[q] > 0 && cnt[2 * n_pairs + q] == 0) { cand[q] = 1; ++nc; }
+                    if (nc == 0) break;
+                    GL_HIP(hipMemcpyAsync(bm.p, cand.data(), n_pairs, hipMemcpyHostToDevice, c->stream));
+                }
+            }
+        }
+        if (want_conc && pass0_ok) {
+            std::vector<int> hits(n_ps);
+            GL_HIP(hipMemcpyAsync(hits.data(), bb.as<int>() + 4 * n_ps, n_ps * 4, hipMemcpyDeviceToHost, c->stream));
+            GL_HIP(hipStreamSynchronize(c->stream));
+            std::vector<unsigned char> refine(n_ps, 0);
+            int n_ref = 0;
+            for (int i = 0; i < n_ps; ++i) if (hits[i] < kRefineHitsMax) { refine[i] = 1; ++n_ref; }
+            const int rns = std::min(3, ns);
+            const int li[3] = {0, ns / 2, ns - 1};
+            if (n_ref > 0 && n_ref <= kRefineMaxEntries) {
+                std::vector<gl_sample> rp = build_samples(L->host.data(), n, kRefineGrid, 1, (int)0xB5297A4Du,
+                                                          p->aperture_blades, p->aperture_rotation);
+                if (!rp.empty()) {
+                    std::vector<float4> s3(rns);
+                    std::vector<float> a3((size_t)rns * n), b3((size_t)rns * n);
+                    for (int k = 0; k < rns; ++k) {
+                        const gl_spec& q = c->spec[li[k]];
+                        s3[k] = make_float4(q.lambda, q.cmf_r, q.cmf_g, q.cmf_b);
+                        std::copy_n(&iat[(size_t)li[k] * n], n, &a3[(size_t)k * n]);
+                        std::copy_n(&ibf[(size_t)li[k] * n], n, &b3[(size_t)k * n]);
+                    }
+                    void *d_s3, *d_a3, *d_b3;
+                    if ((st = upload(c, "spec3", s3.data(), rns * sizeof(float4), &d_s3)) ||
+                        (st = upload(c, "iat3", a3.data(), a3.size() * 4, &d_a3)) ||
+                        (st = upload(c, "ibf3", b3.data(), b3.size() * 4, &d_b3)))
+                        return st;
+                    GL_HIP(hipMemcpyAsync(brf.p, refine.data(), n_ps, hipMemcpyHostToDevice, c->stream));
+                    GL_HIP(hipMemcpyAsync(bg.p, rp.data(), rp.size() * sizeof(gl_sample), hipMemcpyHostToDevice, c->stream));
+                    GhostArgs R = P;
+                    R.grid = bg.as<gl_sample>(); R.n_grid = (int)rp.size();
+                    R.spec = static_cast<const float4*>(d_s3); R.n_spec = rns;
+                    R.iat = static_cast<const float*>(d_a3); R.ibf = static_cast<const float*>(d_b3);
+                    R.pair_active = nullptr; R.ps_refine = brf.as<unsigned char>();
+                    if ((st = launch<true, false, false>(R, n_ps, R.n_grid, c->stream))) return st;
+                }
+            }
+        }
+        if (pass0_ok) {
+            GL_HIP(hipMemcpyAsync(cnt.data(), bc.p, 3 * n_pairs * 8, hipMemcpyDeviceToHost, c->stream));
+            GL_HIP(hipStreamSynchronize(c->stream));
+            for (int q = 0; q < n_pairs; ++q) active[q] = cnt[2 * n_pairs + q] > 0 ? 1 : 0;
+            if (want_conc) {
+                std::vector<int> bbh((size_t)n_ps * 5);
+                GL_HIP(hipMemcpy(bbh.data(), bb.p, (size_t)n_ps * 5 * 4, hipMemcpyDeviceToHost));
+                rects.assign(n_ps, make_float4(2.f, 2.f, -2.f, -2.f));
+                const float dil = kConcDilateCells * (2.0f / (float)kProbeGrid);
+                auto dec = [](int i) { float f; std::memcpy(&f, &i, 4); return f - 2.0f; };
+                for (int i = 0; i < n_ps; ++i) {
+                    if (bbh[(size_t)4 * n_ps + i] < kConcMinHits) continue;
+                    float u0 = dec(bbh[2 * (size_t)i]), v0 = dec(bbh[2 * (size_t)i + 1]);
+                    float u1 = dec(bbh[2 * (size_t)n_ps + 2 * i]), v1 = dec(bbh[2 * (size_t)n_ps + 2 * i + 1]);
+                    if (!(u0 <= u1 && v0 <= v1)) continue;
+                    u0 = std::max(-1.0f, u0 - dil); v0 = std::max(-1.0f, v0 - dil);
+                    u1 = std::min(1.0f, u1 + dil); v1 = std::min(1.0f, v1 + dil);
+                    rects[i] = make_float4(u0, v0, u1, v1);
+                }
+                if (stats && stats->ps_cap >= n_ps) {
+                    stats->n_ps = n_ps;
+                    for (int i = 0; i < n_ps; ++i) {
+                        if (stats->ps_hits) stats->ps_hits[i] = bbh[(size_t)4 * n_ps + i];
+                        if (stats->ps_rect) {
+                            stats->ps_rect[4 * i] = rects[i].x; stats->ps_rect[4 * i + 1] = rects[i].y;
+                            stats->ps_rect[4 * i + 2] = rects[i].z; stats->ps_rect[4 * i + 3] = rects[i].w;
+                        }
+                    }
+                }
+                rect_ready = true;
+            }
+        }
+        GL_HIP(hipMemcpyAsync(bm.p, active.data(), n_pairs, hipMemcpyHostToDevice, c->stream));
+        cull_ready = p->cull_dead_pairs != 0;
+    }
+
+    // ---- concentration + adaptive budgets (ghost_render.cu:1240-1304)
+    std::vector<int2> budgets;
+    if (rect_ready) {
+        void* d_rects;
+        if ((st = upload(c, "rects", rects.data(), n_ps * sizeof(float4), &d_rects))) return st;
+        A.conc.rects = static_cast<const float4*>(d_rects);
+        A.conc.mask = build_mask(L->host.data(), n, p->aperture_blades, p->aperture_rotation);
+        A.conc.jitter = p->pupil_jitter;
+        A.conc.seed_offset = (uint32_t)p->jitter_seed * 1000003u;
+        const int nr = std::max(1, (int)std::floor(std::sqrt((double)n_grid)));
+        A.conc.nr_lattice = nr;
+        A.conc.n_r = (p->pupil_jitter == 2) ? n_grid : nr * nr;
+        const float N = (float)p->ray_grid;
+        const float a_ref = 4.0f * (float)n_grid / (N * N);
+        A.conc.area_norm = 1.0f / (a_ref * (float)A.conc.n_r);
+        A.conc.budgets = nullptr;
+        if (p->adaptive_sample_budgets) {
+            budgets.assign(n_ps, make_int2(A.conc.n_r, A.conc.nr_lattice));
+            const float boost = std::max(1.0f, p->adaptive_density_boost);
+            int floor_s = std::max(256, p->adaptive_min_samples);
+            floor_s = ((floor_s + 255) / 256) * 256;
+            const int floor_nr = std::max(1, (int)std::lround(std::sqrt((double)floor_s)));
+            for (int i = 0; i < n_ps; ++i) {
+                const float4 r = rects[i];
+                if (!(r.x <= r.z)) continue;
+                const float area = (r.z - r.x) * (r.w - r.y);
+                const float raw = boost * (float)n_grid * area / a_ref;
+                if (p->pupil_jitter == 2) {
+                    int b = (int)std::ceil((double)raw);
+                    b = ((b + 255) / 256) * 256;
+                    b = std::max(floor_s, std::min(b, n_grid));
+                    budgets[i] = make_int2(b, A.conc.nr_lattice);
+                } else {
+                    int nrp = (int)std::lround(std::sqrt((double)std::max(0.0f, raw)));
+                    nrp = std::max(floor_nr, std::min(nrp, A.conc.nr_lattice));
+                    budgets[i] = make_int2(nrp * nrp, nrp);
+                }
+            }
+            void* d_b;
+            if ((st = upload(c, "budgets", budgets.data(), n_ps * sizeof(int2), &d_b))) return st;
+            A.conc.budgets = static_cast<const int2*>(d_b);
+            A.conc.inv_aref = 1.0f / a_ref;
+            if (stats && stats->ps_cap >= n_ps && stats->ps_budget)
+                for (int i = 0; i < n_ps; ++i) stats->ps_budget[i] = budgets[i].x;
+        }
+    }
+    A.pair_active = cull_ready ? c->buf("pair_mask").as<unsigned char>() : nullptr;
+
+    if (p->verbose) {
+        std::printf("ghostlight_amd: %d pairs x %d sources x %d samples -> grid (%d, %d)\n", n_pairs, n_src, n_grid,
+                    n_ps, (n_grid + kBlock - 1) / kBlock);
+        std::fflush(stdout);
+    }
+    GL_HIP(hipEventRecord(c->ev0, c->stream));
+    if ((st = launch<false, true, false>(A, n_ps, n_grid, c->stream))) return st;
+    GL_HIP(hipEventRecord(c->ev1, c->stream));
+    if (c->spec_do_correct) {
+        const float* C = c->spec_C;
+        hipLaunchKernelGGL(spectral_correct_kernel, dim3((unsigned)((npx + 255) / 256)), dim3(256), 0, c->stream,
+                           A.out_r, A.out_g, A.out_b, npx, C[0], C[1], C[2], C[3], C[4], C[5], C[6], C[7], C[8]);
+        GL_HIP(hipGetLastError());
+    }
+    const double t_k0 = now_ms();
+    if (out_r) {
+        GL_HIP(hipMemcpyAsync(out_r, A.out_r, npx * 4, hipMemcpyDeviceToHost, c->stream));
+        GL_HIP(hipMemcpyAsync(out_g, A.out_g, npx * 4, hipMemcpyDeviceToHost, c->stream));
+        GL_HIP(hipMemcpyAsync(out_b, A.out_b, npx * 4, hipMemcpyDeviceToHost, c->stream));
+    }
+    GL_HIP(hipStreamSynchronize(c->stream));
+    const double t_d1 = now_ms();
+    float kms = 0.f;
+    GL_HIP(hipEventElapsedTime(&kms, c->ev0, c->ev1));
+    c->last_kernel_ms = kms;
+
+    if (stats) {
+        stats->ms_upload = (float)(t_up1 - t_up0);
+        stats->ms_kernel = kms;
+        stats->ms_download = (float)(t_d1 - t_k0);
+        // Segment count of the main render (same inputs, no splat), then the
+        // pre-cull per-pair picture over the full grid (ghost_render.cu:1361-1438).
+        DevBuf& bsg = c->buf("segs");
+        DevBuf& bcn = c->buf("stat_cnt");
+        if ((st = bsg.ensure(8)) || (st = bcn.ensure(3 * n_pairs * 8))) return st;
+        GL_HIP(hipMemsetAsync(bsg.p, 0, 8, c->stream));
+        GL_HIP(hipMemsetAsync(bcn.p, 0, 3 * n_pairs * 8, c->stream));
+        GhostArgs Cg = A;
+        Cg.segs = bsg.as<unsigned long long>();
+        if ((st = launch<false, false, true>(Cg, n_ps, n_grid, c->stream))) return st;
+        GhostArgs Sg = A;
+        Sg.pair_active = nullptr; Sg.conc.rects = nullptr; Sg.bbox = nullptr; Sg.segs = nullptr;
+        Sg.cnt = bcn.as<unsigned long long>();
+        if ((st = launch<true, false, false>(Sg, n_ps, n_grid, c->stream))) return st;
+        std::vector<unsigned long long> h(3 * n_pairs);
+        unsigned long long segs = 0;
+        GL_HIP(hipMemcpyAsync(h.data(), bcn.p, 3 * n_pairs * 8, hipMemcpyDeviceToHost, c->stream));
+        GL_HIP(hipMemcpyAsync(&segs, bsg.p, 8, hipMemcpyDeviceToHost, c->stream));
+        GL_HIP(hipStreamSynchronize(c->stream));
+        stats->segments = segs;
+        stats->traces_total = stats->traces_survived = stats->traces_on_sensor = 0;
+        for (int q = 0; q < n_pairs; ++q) {
+            stats->traces_total += h[q]; stats->traces_survived += h[n_pairs + q];
+            stats->traces_on_sensor += h[2 * n_pairs + q];
+            if (q < stats->pair_cap) {
+                if (stats->pair_surf_a) stats->pair_surf_a[q] = Lc.pairs[q].surf_a;
+                if (stats->pair_surf_b) stats->pair_surf_b[q] = Lc.pairs[q].surf_b;
+                if (stats->pair_traces) stats->pair_traces[q] = h[q];
+                if (stats->pair_survived) stats->pair_survived[q] = h[n_pairs + q];
+                if (stats->pair_on_sensor) stats->pair_on_sensor[q] = h[2 * n_pairs + q];
+            }
+        }
+        stats->ms_total = (float)(now_ms() - t_fn0);
+    }
+    return GL_OK;
+}
+
+static int validate(const gl_lens* L, const gl_flare_params* p, int width, int height) {
+    if (!L || !p) return set_error(GL_ERR_INVALID, "null lens or params");
+    if (width <= 0 || height <= 0) return set_error(GL_ERR_INVALID, "width and height must be positive");
+    if ((size_t)width * height > (size_t)INT32_MAX) return set_error(GL_ERR_INVALID, "image has too many pixels");
+    if (p->ray_grid <= 0 || p->ray_grid > 4096) return set_error(GL_ERR_INVALID, "ray_grid %d outside [1, 4096]", p->ray_grid);
+    if (std::max(3, p->spectral_samples) > GL_MAX_SPECTRAL)
+        return set_error(GL_ERR_INVALID, "spectral_samples %d above %d", p->spectral_samples, GL_MAX_SPECTRAL);
+    return GL_OK;
+}
+
+}  // namespace gl
+
+using namespace gl;
+
+extern "C" {
+
+int gl_launch_ghost_render(gl_ctx* ctx, const gl_lens* lens, const gl_pair* pairs, int32_t n_pairs,
+                           const gl_source* sources, int32_t n_sources, float sw, float sh, int32_t width,
+                           int32_t height, int32_t fmt_w, int32_t fmt_h, int32_t fx0, int32_t fy0,
+                           const gl_flare_params* p, float* out_r, float* out_g, float* out_b, float** dev_out,
+                           gl_ghost_stats* stats) {
+    int st = validate(lens, p, width, height);
+    if (st) return st;
+    if (n_pairs < 0 || n_sources < 0 || (n_pairs && !pairs) || (n_sources && !sources))
+        return set_error(GL_ERR_INVALID, "bad pair/source arrays");
+    if (!(sw > 0.0f) || !(sh > 0.0f)) return set_error(GL_ERR_INVALID, "sensor half extents must be positive");
+    gl_ctx* c = resolve_ctx(ctx, &st);
+    if (!c) return st;
+    std::lock_guard<std::mutex> g(c->mu);
+    gl_lens* L = const_cast<gl_lens*>(lens);
+    std::lock_guard<std::mutex> gl_(L->mu);
+    Launch Lc{c, L, p, std::vector<gl_pair>(pairs, pairs + n_pairs), {}, n_sources, width, height,
+              fmt_w, fmt_h, fx0, fy0, sw, sh};
+    Lc.src.reserve(n_sources);
+    for (int i = 0; i < n_sources; ++i) Lc.src.push_back(make_src(sources[i]));
+    return do_launch(Lc, out_r, out_g, out_b, dev_out, stats);
+}
+
+int gl_filter_ghost_pairs(gl_ctx* ctx, const gl_lens* lens, float hw, float hh, const gl_flare_params* p,
+                          gl_pair* out, int32_t cap, int32_t* n_out) {
+    if (!lens || !p || !n_out) return set_error(GL_ERR_INVALID, "null argument");
+    const std::vector<gl_surface>& s = lens->host;
+    const int n = (int)s.size();
+    // IOR-contrast gate on n_d (ghost.cpp:112-125), then device probes.
+    auto ior_before = [&](int idx) {
+        for (int k = idx - 1; k >= 0; --k) if (s[k].is_active) return s[k].ior;
+        return 1.0f;
+    };
+    std::vector<int2> cand;
+    for (int a = 0; a < n; ++a) {
+        if (!s[a].is_active) continue;
+        for (int b = a + 1; b < n; ++b) {
+            if (!s[b].is_active) continue;
+            if (std::fabs(ior_before(a) - s[a].ior) < 0.001f || std::fabs(ior_before(b) - s[b].ior) < 0.001f) continue;
+            cand.push_back(make_int2(a, b));
+        }
+    }
+    *n_out = 0;
+    if (cand.empty()) return GL_OK;
+    int st;
+    gl_ctx* c = resolve_ctx(ctx, &st);
+    if (!c) return st;
+    std::lock_guard<std::mutex> g(c->mu);
+    gl_lens* L = const_cast<gl_lens*>(lens);
+    std::lock_guard<std::mutex> gl_(L->mu);
+    if ((st = lens_sync(L, 0.0f, c->stream))) return st;
+    const gl_spec s3[3] = {{650.f, 0, 0, 0}, {550.f, 0, 0, 0}, {450.f, 0, 0, 0}};
+    std::vector<float> a3(3 * (size_t)n), b3(3 * (size_t)n);
+    ior_tables(s.data(), n, s3, 3, a3.data(), b3.data());
+    void *d_c, *d_a, *d_b;
+    if ((st = upload(c, "f_pairs", cand.data(), cand.size() * sizeof(int2), &d_c)) ||
+        (st = upload(c, "f_iat", a3.data(), a3.size() * 4, &d_a)) ||
+        (st = upload(c, "f_ibf", b3.data(), b3.size() * 4, &d_b)))
+        return st;
+    DevBuf& bo = c->buf("f_out");
+    if ((st = bo.ensure(cand.size() * 67 * sizeof(float4)))) return st;
+    hipLaunchKernelGGL(filter_probe_kernel, dim3((unsigned)cand.size()), dim3(128), 0, c->stream, L->d_surf, L->d_poly,
+                       L->d_arena, n, static_cast<const int2*>(d_c), static_cast<const float*>(d_a),
+                       static_cast<const float*>(d_b), s[0].semi_aperture, s[0].z - 20.0f, bo.as<float4>());
+    GL_HIP(hipGetLastError());
+    std::vector<float4> res(cand.size() * 67);
+    GL_HIP(hipMemcpyAsync(res.data(), bo.p, res.size() * sizeof(float4), hipMemcpyDeviceToHost, c->stream));
+    GL_HIP(hipStreamSynchronize(c->stream));
+    int cnt = 0;
+    for (size_t i = 0; i < cand.size(); ++i) {
+        const float4* r = &res[i * 67];
+        float total = 0.0f;
+        for (int ch = 0; ch < 3; ++ch) if (r[ch].w == 0.0f) total += r[ch].z;
+        const float est = total / 3.0f;
+        if (est < p->min_ghost_intensity) continue;
+        float boost = 1.0f;
+        if (p->ghost_normalize) {
+            float mnx = 1e30f, mxx = -1e30f, mny = 1e30f, mxy = -1e30f;
+            int valid = 0;
+            for (int k = 0; k < 64; ++k) {
+                const float4 q = r[3 + k];
+                if (q.w != 0.0f) continue;
+                mnx = std::min(mnx, q.x); mxx = std::max(mxx, q.x);
+                mny = std::min(mny, q.y); mxy = std::max(mxy, q.y);
+                ++valid;
+            }
+            if (valid >= 2) {
+                const float gw = std::max(mxx - mnx, 0.01f), gh = std::max(mxy - mny, 0.01f);
+                const float ratio = (gw * gh) / ((2.0f * hw) * (2.0f * hh));
+                boost = std::min(std::max(ratio, 1.0f), p->max_area_boost);
+            }
+        }
+        if (cnt < cap && out) out[cnt] = gl_pair{cand[i].x, cand[i].y, boost};
+        ++cnt;
+    }
+    *n_out = cnt;
+    return cnt > cap ? set_error(GL_ERR_INVALID, "pair capacity %d < %d", cap, cnt) : GL_OK;
+}
+
+int gl_render_ghost_pipeline(gl_ctx* ctx, const gl_lens* lens, const gl_source* sources, int32_t n_sources,
+                             float calib_hw, float calib_hh, int32_t width, int32_t height, int32_t fmt_w,
+                             int32_t fmt_h, int32_t fx0, int32_t fy0, const gl_flare_params* p, float* out_r,
+                             float* out_g, float* out_b, float* aov_out, int32_t* aov_pairs, int32_t aov_cap,
+                             int32_t* aov_n, gl_ghost_stats* stats) {
+    int st = validate(lens, p, width, height);
+    if (st) return st;
+    if (aov_n) *aov_n = 0;
+    const size_t npx = (size_t)width * height;
+    if (out_r) { std::fill(out_r, out_r + npx, 0.0f); std::fill(out_g, out_g + npx, 0.0f); std::fill(out_b, out_b + npx, 0.0f); }
+    if (n_sources <= 0) return GL_OK;
+    const float hw = (p->sensor_half_w > 0.0f) ? p->sensor_half_w : calib_hw;
+    const float hh = (p->sensor_half_h > 0.0f) ? p->sensor_half_h : calib_hh;
+    std::vector<gl_pair> pairs(GL_MAX_SURFACES * GL_MAX_SURFACES / 2 + 1);
+    int32_t np = 0;
+    if ((st = gl_filter_ghost_pairs(ctx, lens, hw, hh, p, pairs.data(), (int32_t)pairs.size(), &np))) return st;
+    pairs.resize(np);
+    if (p->ghost_filter_mode != GL_FILTER_ALL && p->ghost_filter_count > 0 && p->ghost_filter_pairs) {
+        const bool include = p->ghost_filter_mode == GL_FILTER_INCLUDE;
+        std::vector<gl_pair> kept;
+        for (const auto& q : pairs) {
+            bool hit = false;
+            for (int k = 0; k < p->ghost_filter_count; ++k)
+                if (p->ghost_filter_pairs[2 * k] == q.surf_a && p->ghost_filter_pairs[2 * k + 1] == q.surf_b) { hit = true; break; }
+            if (include ? hit : !hit) kept.push_back(q);
+        }
+        pairs.swap(kept);
+    }
+    if (pairs.empty()) return GL_OK;
+    if (p->aov_mode == GL_AOV_PER_PAIR && aov_out) {
+        const int n_layers = p->aov_max_pairs < 0 ? (int)pairs.size() : std::min((int)pairs.size(), p->aov_max_pairs);
+        if (n_layers > aov_cap) return set_error(GL_ERR_INVALID, "AOV capacity %d < %d layers", aov_cap, n_layers);
+        std::vector<float> tr(npx), tg(npx), tb(npx);
+        for (int i = 0; i < n_layers; ++i) {
+            float* lr = aov_out + (size_t)i * 3 * npx;
+            if ((st = gl_launch_ghost_render(ctx, lens, &pairs[i], 1, sources, n_sources, hw, hh, width, height, fmt_w,
+                                             fmt_h, fx0, fy0, p, lr, lr + npx, lr + 2 * npx, nullptr, nullptr)))
+                return st;
+            if (aov_pairs) { aov_pairs[2 * i] = pairs[i].surf_a; aov_pairs[2 * i + 1] = pairs[i].surf_b; }
+            if (out_r)
+                for (size_t k = 0; k < npx; ++k) { out_r[k] += lr[k]; out_g[k] += lr[npx + k]; out_b[k] += lr[2 * npx + k]; }
+        }
+        if (aov_n) *aov_n = n_layers;
+        return GL_OK;
+    }
+    return gl_launch_ghost_render(ctx, lens, pairs.data(), (int32_t)pairs.size(), sources, n_sources, hw, hh, width,
+                                  height, fmt_w, fmt_h, fx0, fy0, p, out_r, out_g, out_b, nullptr, stats);
+}
+
+int gl_trace_ghost_rays(gl_ctx* ctx, const gl_lens* lens, int32_t a, int32_t b, const float* rays, int32_t n_rays,
+                        int32_t t_ge0, float fp64_rmin, float* out_pos, float* out_w, int32_t* out_status,
+                        int32_t* out_segments) {
+    if (!lens || !rays || n_rays < 0 || !out_pos || !out_w || !out_status) return set_error(GL_ERR_INVALID, "null argument");
+    if (n_rays == 0) return GL_OK;
+    const int n = (int)lens->host.size();
+    if (b >= 0 && (a < 0 || b <= a || b >= n)) {
+        for (int i = 0; i < n_rays; ++i) { out_status[i] = GL_TRACE_INVALID_INPUT; out_w[i] = 0.0f; }
+        return GL_OK;
+    }
+    int st;
+    gl_ctx* c = resolve_ctx(ctx, &st);
+    if (!c) return st;
+    std::lock_guard<std::mutex> g(c->mu);
+    gl_lens* L = const_cast<gl_lens*>(lens);
+    std::lock_guard<std::mutex> gl_(L->mu);
+    if ((st = lens_sync(L, fp64_rmin, c->stream))) return st;
+    // Per-ray IOR slices (each ray carries its own wavelength).
+    std::vector<float> iat((size_t)n_rays * n), ibf((size_t)n_rays * n);
+    for (int i = 0; i < n_rays; ++i) {
+        gl_spec sp{rays[7 * (size_t)i + 6], 0, 0, 0};
+        ior_tables(L->host.data(), n, &sp, 1, &iat[(size_t)i * n], &ibf[(size_t)i * n]);
+    }
+    void *d_r, *d_a, *d_b;
+    if ((st = upload(c, "r_rays", rays, (size_t)n_rays * 7 * 4, &d_r)) ||
+        (st = upload(c, "r_iat", iat.data(), iat.size() * 4, &d_a)) ||
+        (st = upload(c, "r_ibf", ibf.data(), ibf.size() * 4, &d_b)))
+        return st;
+    DevBuf &bp = c->buf("r_pos"), &bw = c->buf("r_w"), &bs = c->buf("r_st"), &bg = c->buf("r_seg");
+    if ((st = bp.ensure((size_t)n_rays * 12)) || (st = bw.ensure((size_t)n_rays * 4)) ||
+        (st = bs.ensure((size_t)n_rays * 4)) || (st = bg.ensure((size_t)n_rays * 4)))
+        return st;
+    hipLaunchKernelGGL(ray_kernel, dim3((unsigned)((n_rays + 255) / 256)), dim3(256), 0, c->stream, L->d_surf, L->d_poly,
+                       L->d_arena, n, a, b, static_cast<const float*>(d_r), n_rays, static_cast<const float*>(d_a),
+                       static_cast<const float*>(d_b), t_ge0, bp.as<float>(), bw.as<float>(), bs.as<int>(), bg.as<int>());
+    GL_HIP(hipGetLastError());
+    GL_HIP(hipMemcpyAsync(out_pos, bp.p, (size_t)n_rays * 12, hipMemcpyDeviceToHost, c->stream));
+    GL_HIP(hipMemcpyAsync(out_w, bw.p, (size_t)n_rays * 4, hipMemcpyDeviceToHost, c->stream));
+    GL_HIP(hipMemcpyAsync(out_status, bs.p, (size_t)n_rays * 4, hipMemcpyDeviceToHost, c->stream));
+    if (out_segments) GL_HIP(hipMemcpyAsync(out_segments, bg.p, (size_t)n_rays * 4, hipMemcpyDeviceToHost, c->stream));
+    GL_HIP(hipStreamSynchronize(c->stream));
+    return GL_OK;
+}
+
+}  // extern "C"

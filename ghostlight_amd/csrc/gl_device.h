@@ -1,0 +1,511 @@
+// gl_device.h — device-side prescription layout and the FP32 sequential
+// ghost tracer for gfx950.
+//
+// Layout.  The prescription is wave-uniform: every lane of a workgroup traces
+// the same ghost pair through the same surfaces, and the surface loop counter
+// is uniform.  DevSurf records therefore sit in HBM behind a __restrict__ const
+// pointer and are read with uniform indices, which the AMDGPU backend turns
+// into scalar (SMEM) loads into SGPRs: no LDS traffic and no VGPRs spent on
+// the prescription.  Per-lane state (origin, direction, weight, medium) is
+// the only vector data, kept in registers.
+//
+// Arithmetic.  Every expression follows the reference's operation order
+// (trace_core.h, trace_cuda.h, fresnel.h; see citations per function) and
+// the whole file is compiled with -ffp-contract=off and IEEE division and
+// square root, so statuses, hit/vignette masks and landing positions are
+// bit-identical to the FP32 CPU oracle.  The only non-IEEE functions are the
+// coating's cos/asin/exp, which scale weights, never masks.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/ghostlight_amd.h"
+
+// Constant address space: loads through it with a wave-uniform address are
+// selected as scalar (SMEM) loads.  Only read-only, wave-uniform data uses it.
+#define GLC __attribute__((address_space(4)))
+
+namespace gl {
+
+template <class T> __host__ __device__ inline const T GLC* as_const(const T* p) { return (const T GLC*)p; }
+
+enum : int32_t {
+    SF_ACTIVE = 1, SF_TRANSFORMED = 2, SF_FLAT = 4, SF_POLYGON = 8, SF_FP64 = 16,
+    SF_ASPECT = 32, SF_CYL_Y = 64,
+};
+
+// One surface as the tracer reads it (built on the host from gl_surface).
+struct DevSurf {
+    float z, R, sa, aspect;          // vertex z, radius, semi-aperture, X stretch
+    float dx, dy, inv_absR, conic;   // decenter, 1/|R| (float division), conic K
+    float rot[9];
+    int32_t flags, form, nblades, poly_off;   // poly_off: index into the normals array
+    float apothem;                   // polygon apothem (mm), rounded from FP64
+    int32_t coat_model, ar_layers, discard;
+    float angle_ref_ior;
+    int32_t tab_off, tab_n, sa_off, sa_nw;
+    int32_t sa_na, n_asph, pad0, pad1;
+    float tint_r, tint_g, tint_b, tint_strength;
+    float g_sigma, g_bg, g_peak, g_dx, g_dy;
+    float asph[GL_MAX_ASPHERE_TERMS];
+};
+
+struct TraceRes {
+    float px, py, pz, w;
+    int32_t status, segs;
+};
+
+#define GLD __device__ __forceinline__
+
+struct F3 { float x, y, z; };
+GLD F3 f3(float x, float y, float z) { F3 r; r.x = x; r.y = y; r.z = z; return r; }
+GLD F3 add(F3 a, F3 b) { return f3(a.x + b.x, a.y + b.y, a.z + b.z); }
+GLD F3 sub(F3 a, F3 b) { return f3(a.x - b.x, a.y - b.y, a.z - b.z); }
+GLD F3 mul(F3 a, float s) { return f3(a.x * s, a.y * s, a.z * s); }
+GLD float dot3(F3 a, F3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+GLD F3 neg(F3 a) { return f3(-a.x, -a.y, -a.z); }
+GLD F3 normalized(F3 v) {  // vec3.h:67-71
+    float l = sqrtf(v.x * v.x + v.y * v.y + v.z * v.z);
+    return (l > 1e-12f) ? mul(v, 1.0f / l) : f3(0.f, 0.f, 0.f);
+}
+template <class P> GLD F3 m_mul(P m, F3 v) {   // trace_core.h:33-39
+    return f3(m[0] * v.x + m[1] * v.y + m[2] * v.z, m[3] * v.x + m[4] * v.y + m[5] * v.z,
+              m[6] * v.x + m[7] * v.y + m[8] * v.z);
+}
+template <class P> GLD F3 m_tmul(P m, F3 v) {  // trace_core.h:41-47
+    return f3(m[0] * v.x + m[3] * v.y + m[6] * v.z, m[1] * v.x + m[4] * v.y + m[7] * v.z,
+              m[2] * v.x + m[5] * v.y + m[8] * v.z);
+}
+
+// ---- Fresnel and coatings (fresnel.h:69-178) -----------------------------
+GLD float fresnel_R(float cos_i, float n1, float n2) {
+    cos_i = fabsf(cos_i);
+    float eta = n1 / n2;
+    float sin2_t = eta * eta * (1.0f - cos_i * cos_i);
+    if (sin2_t >= 1.0f) return 1.0f;
+    float cos_t = sqrtf(1.0f - sin2_t);
+    float rs = (n1 * cos_i - n2 * cos_t) / (n1 * cos_i + n2 * cos_t);
+    float rp = (n2 * cos_i - n1 * cos_t) / (n2 * cos_i + n1 * cos_t);
+    return 0.5f * (rs * rs + rp * rp);
+}
+GLD float clamp01(float r) { return fminf(fmaxf(r, 0.0f), 1.0f); }
+GLD float coating_R(float cos_i, float n1, float n2, float cn, float d_nm, float lam) {
+    float sin2_c = (n1 / cn) * (n1 / cn) * (1.0f - cos_i * cos_i);
+    if (sin2_c >= 1.0f) return fresnel_R(cos_i, n1, n2);
+    float cos_c = sqrtf(1.0f - sin2_c);
+    float delta = 2.0f * 3.14159265358979323846f * cn * d_nm * cos_c / lam;
+    float r01 = (n1 * cos_i - cn * cos_c) / (n1 * cos_i + cn * cos_c);
+    float sin2_2 = (cn / n2) * (cn / n2) * (1.0f - cos_c * cos_c);
+    if (sin2_2 >= 1.0f) return fresnel_R(cos_i, n1, n2);
+    float cos_2 = sqrtf(1.0f - sin2_2);
+    float r12 = (cn * cos_c - n2 * cos_2) / (cn * cos_c + n2 * cos_2);
+    float c2d = cosf(2.0f * delta);
+    float num = r01 * r01 + r12 * r12 + 2.0f * r01 * r12 * c2d;
+    float den = 1.0f + r01 * r01 * r12 * r12 + 2.0f * r01 * r12 * c2d;
+    return clamp01(num / den);
+}
+GLD float ref_angle_deg(float cos_i, float n1, float aref) {   // trace_core.h:614-621
+    float sin_i = sqrtf(fmaxf(0.0f, 1.0f - cos_i * cos_i));
+    float s = (aref > 0.0f) ? (n1 * sin_i / aref) : sin_i;
+    if (s >= 1.0f) return 90.0f;
+    return asinf(s) * (180.0f / 3.14159265358979323846f);
+}
+GLD float lerp1d(const float* t, int n, float key, bool disc, bool* out_disc) {  // trace_core.h:575-602
+    if (n <= 0) return 0.0f;
+    if (key <= t[0]) { if (disc && key < t[0]) *out_disc = true; return t[1]; }
+    if (key >= t[2 * (n - 1)]) { if (disc && key > t[2 * (n - 1)]) *out_disc = true; return t[2 * (n - 1) + 1]; }
+    for (int i = 1; i < n; ++i)
+        if (key <= t[2 * i]) {
+            float span = t[2 * i] - t[2 * (i - 1)];
+            float f = (span > 0.0f) ? (key - t[2 * (i - 1)]) / span : 0.0f;
+            return t[2 * (i - 1) + 1] + f * (t[2 * i + 1] - t[2 * (i - 1) + 1]);
+        }
+    return t[2 * (n - 1) + 1];
+}
+
+// surface_weight (trace_core.h:635-783).
+GLD float surface_weight(F3 dir, F3 nrm, float n1, float n2, const DevSurf GLC& s, const float* __restrict__ arena,
+                         float lam, bool reflecting) {
+    float cos_i = fabsf(dot3(nrm, dir));
+    float R;
+    const int m = s.coat_model;
+    if (m == GL_COAT_SIMPLE) {
+        if (s.ar_layers <= 0) R = fresnel_R(cos_i, n1, n2);
+        else {
+            const float mg = 1.38f;
+            const float qw = 550.0f / (4.0f * mg);
+            R = coating_R(cos_i, n1, n2, mg, qw, lam);
+            for (int i = 1; i < s.ar_layers; ++i) R *= 0.25f;
+            R = clamp01(R);
+        }
+    } else if (m == GL_COAT_ARTIST) {
+        const float inv_2s2 = 1.0f / (2.0f * 65.0f * 65.0f);
+        float dr = lam - 650.0f, dg = lam - 550.0f, db = lam - 450.0f;
+        float br = expf(-dr * dr * inv_2s2), bg = expf(-dg * dg * inv_2s2), bb = expf(-db * db * inv_2s2);
+        float sum = br + bg + bb;
+        R = clamp01(s.tint_strength * (s.tint_r * br + s.tint_g * bg + s.tint_b * bb) / sum);
+    } else if (m == GL_COAT_SPECTRAL || m == GL_COAT_ANGULAR) {
+        if (s.tab_n <= 0) R = fresnel_R(cos_i, n1, n2);
+        else {
+            float key = (m == GL_COAT_SPECTRAL) ? lam : ref_angle_deg(cos_i, n1, s.angle_ref_ior);
+            bool d = false;
+            R = lerp1d(arena + s.tab_off, s.tab_n, key, s.discard != 0, &d);
+            if (d) return 0.0f;
+        }
+    } else if (m == GL_COAT_SPECTRAL_ANGULAR) {
+        const int nw = s.sa_nw, na = s.sa_na;
+        if (nw <= 0 || na <= 0) R = fresnel_R(cos_i, n1, n2);
+        else {
+            const float* wl = arena + s.sa_off;
+            const float* an = wl + nw;
+            const float* rr = an + na;
+            float ang = ref_angle_deg(cos_i, n1, s.angle_ref_ior);
+            bool disc = false;
+            const bool oob = s.discard != 0;
+            int iw = 0; float fw = 0.0f;
+            if (lam <= wl[0]) { if (oob && lam < wl[0]) disc = true; }
+            else if (lam >= wl[nw - 1]) { iw = nw > 1 ? nw - 2 : 0; fw = nw > 1 ? 1.0f : 0.0f; if (oob && lam > wl[nw - 1]) disc = true; }
+            else for (int i = 1; i < nw; ++i) if (lam <= wl[i]) {
+                iw = i - 1; float span = wl[i] - wl[i - 1];
+                fw = span > 0.0f ? (lam - wl[i - 1]) / span : 0.0f; break; }
+            int ia = 0; float fa = 0.0f;
+            if (ang <= an[0]) { if (oob && ang < an[0]) disc = true; }
+            else if (ang >= an[na - 1]) { ia = na > 1 ? na - 2 : 0; fa = na > 1 ? 1.0f : 0.0f; if (oob && ang > an[na - 1]) disc = true; }
+            else for (int i = 1; i < na; ++i) if (ang <= an[i]) {
+                ia = i - 1; float span = an[i] - an[i - 1];
+                fa = span > 0.0f ? (ang - an[i - 1]) / span : 0.0f; break; }
+            if (disc) return 0.0f;
+            const int iw1 = iw + 1 < nw ? iw + 1 : iw, ia1 = ia + 1 < na ? ia + 1 : ia;
+            float r00 = rr[iw * na + ia], r01 = rr[iw * na + ia1], r10 = rr[iw1 * na + ia], r11 = rr[iw1 * na + ia1];
+            R = (1.0f - fw) * ((1.0f - fa) * r00 + fa * r01) + fw * ((1.0f - fa) * r10 + fa * r11);
+        }
+    } else {
+        R = fresnel_R(cos_i, n1, n2);
+    }
+    R = clamp01(R);
+    return reflecting ? R : (1.0f - R);
+}
+
+GLD float attenuator(F3 hit, const DevSurf GLC& s) {   // trace_core.h:791-809
+    if (s.coat_model != GL_COAT_ATTENUATOR_GAUSS) return 1.0f;
+    if (!(s.g_sigma > 0.0f)) return s.g_bg;
+    F3 local = hit;
+    if (s.flags & SF_TRANSFORMED) local = add(m_tmul(s.rot, sub(hit, f3(s.dx, s.dy, s.z))), f3(0.f, 0.f, s.z));
+    float ddx = local.x - s.g_dx, ddy = local.y - s.g_dy;
+    float r2 = ddx * ddx + ddy * ddy;
+    float s2 = s.g_sigma * s.g_sigma;
+    return s.g_bg + s.g_peak * expf(-0.5f * r2 / s2);
+}
+
+// check_aperture (trace_core.h:133-170); the regular polygon is the
+// half-plane intersection max_k(h . n_k) <= apothem (same set as the
+// reference's r <= apothem / cos(offset)).
+GLD bool check_aperture(F3 hit, const DevSurf GLC& s, const float GLC* poly) {
+    float hx = (s.flags & SF_ASPECT) ? hit.x / s.aspect : hit.x;
+    float hy = hit.y;
+    float h2 = hx * hx + hy * hy;
+    if (!(h2 <= s.sa * s.sa)) return false;
+    if (s.flags & SF_POLYGON) {
+        const float GLC* pc = poly + s.poly_off;
+        const float GLC* ps = pc + GL_MAX_BLADES;
+        float m = hx * pc[0] + hy * ps[0];
+        for (int k = 1; k < s.nblades; ++k) {
+            float d = hx * pc[k] + hy * ps[k];
+            m = d > m ? d : m;
+        }
+        return m <= s.apothem;
+    }
+    return true;
+}
+
+GLD bool pick_root(float t1, float t2, float oz, float dz, float zv, float& t) {
+    if (t1 > 1e-6f && t2 > 1e-6f) {
+        float z1 = oz + t1 * dz, z2 = oz + t2 * dz;
+        t = (fabsf(z1 - zv) < fabsf(z2 - zv)) ? t1 : t2;
+    } else if (t1 > 1e-6f) t = t1;
+    else if (t2 > 1e-6f) t = t2;
+    else return false;
+    return true;
+}
+
+GLD float asph_sag(float r, float inv_R, float K, const DevSurf GLC& s) {     // trace_core.h:75-88
+    float r2 = r * r;
+    float D = 1.0f - (1.0f + K) * inv_R * inv_R * r2;
+    if (D <= 0.0f) return 1e30f;
+    float z = inv_R * r2 / (1.0f + sqrtf(D));
+    float rp = r2 * r2;
+    for (int i = 0; i < s.n_asph; ++i) { z += s.asph[i] * rp; rp *= r2; }
+    return z;
+}
+GLD float asph_dsag(float r, float inv_R, float K, const DevSurf GLC& s) {    // trace_core.h:96-112
+    float r2 = r * r;
+    float D = 1.0f - (1.0f + K) * inv_R * inv_R * r2;
+    if (D <= 0.0f) return 0.0f;
+    float dz = inv_R * r / sqrtf(D);
+    float rp = r2 * r;
+    for (int i = 0; i < s.n_asph; ++i) { dz += (4.0f + 2.0f * (float)i) * s.asph[i] * rp; rp *= r2; }
+    return dz;
+}
+
+// intersect_surface (trace_core.h:299-525).  Returns false on miss / clip.
+GLD bool intersect(F3 o, F3 d, const DevSurf GLC& s, const float GLC* poly, float& t_out, F3& n_out) {
+    const float zv = s.z;
+    const bool tr = (s.flags & SF_TRANSFORMED) != 0;
+    if (tr) {
+        o = add(m_tmul(s.rot, sub(o, f3(s.dx, s.dy, zv))), f3(0.f, 0.f, zv));
+        d = m_tmul(s.rot, d);
+    }
+    if (s.flags & SF_FLAT) {
+        if (fabsf(d.z) < 1e-12f) return false;
+        float t = (zv - o.z) / d.z;
+        if (!(t > 1e-6f)) return false;
+        F3 hit = add(o, mul(d, t));
+        if (!check_aperture(hit, s, poly)) return false;
+        t_out = t;
+        n_out = f3(0.f, 0.f, (d.z > 0.0f) ? -1.0f : 1.0f);
+        if (tr) n_out = m_mul(s.rot, n_out);
+        return true;
+    }
+    if (s.form == GL_FORM_SPHERE) {
+        const float R = s.R;
+        F3 ctr = f3(0.f, 0.f, zv + R);
+        float t;
+        if (s.flags & SF_FP64) {
+            // FP64 quadratic for large |R| (sag cancellation in |oc|^2 - R^2).
+            const double Rd = (double)R, zd = (double)zv;
+            const double ox = o.x, oy = o.y, oz = (double)o.z - (zd + Rd);
+            const double dx = d.x, dy = d.y, dz = d.z;
+            const double a = dx * dx + dy * dy + dz * dz;
+            const double b = 2.0 * (ox * dx + oy * dy + oz * dz);
+            const double c = ox * ox + oy * oy + oz * oz - Rd * Rd;
+            const double disc = b * b - 4.0 * a * c;
+            if (disc < 0.0) return false;
+            const double sd = sqrt(disc), inv_2a = 0.5 / a;
+            const double t1 = (-b - sd) * inv_2a, t2 = (-b + sd) * inv_2a;
+            double td;
+            if (t1 > 1e-6 && t2 > 1e-6) {
+                const double z1 = (double)o.z + t1 * dz, z2 = (double)o.z + t2 * dz;
+                td = (fabs(z1 - zd) < fabs(z2 - zd)) ? t1 : t2;
+            } else if (t1 > 1e-6) td = t1;
+            else if (t2 > 1e-6) td = t2;
+            else return false;
+            t = (float)td;
+        } else {
+            F3 oc = sub(o, ctr);
+            float a = dot3(d, d), b = 2.0f * dot3(oc, d), c = dot3(oc, oc) - R * R;
+            float disc = b * b - 4.0f * a * c;
+            if (disc < 0.0f) return false;
+            float sd = sqrtf(disc), inv_2a = 0.5f / a;
+            if (!pick_root((-b - sd) * inv_2a, (-b + sd) * inv_2a, o.z, d.z, zv, t)) return false;
+        }
+        F3 hit = add(o, mul(d, t));
+        if (!check_aperture(hit, s, poly)) return false;
+        F3 nrm = mul(sub(hit, ctr), s.inv_absR);
+        if (dot3(nrm, d) > 0.0f) nrm = neg(nrm);
+        if (tr) nrm = m_mul(s.rot, nrm);
+        t_out = t; n_out = nrm;
+        return true;
+    }
+    if (s.form == GL_FORM_CYLINDRICAL) {
+        const float vx = (s.flags & SF_CYL_Y) ? 1.0f : 0.0f, vy = (s.flags & SF_CYL_Y) ? 0.0f : 1.0f;
+        const float R = s.R;
+        float oc_v = o.x * vx + o.y * vy;
+        float oc_z = o.z - zv - R;
+        float dv = d.x * vx + d.y * vy;
+        float qa = dv * dv + d.z * d.z;
+        float qb = 2.0f * (oc_v * dv + oc_z * d.z);
+        float qc = oc_v * oc_v + oc_z * oc_z - R * R;
+        float disc = qb * qb - 4.0f * qa * qc;
+        if (disc < 0.0f) return false;
+        float sd = sqrtf(disc), inv_2a = 0.5f / qa, t;
+        if (!pick_root((-qb - sd) * inv_2a, (-qb + sd) * inv_2a, o.z, d.z, zv, t)) return false;
+        F3 hit = add(o, mul(d, t));
+        if (!check_aperture(hit, s, poly)) return false;
+        float hv = hit.x * vx + hit.y * vy;
+        F3 nrm = f3(hv * vx, hv * vy, hit.z - zv - R);
+        float nlen = sqrtf(nrm.x * nrm.x + nrm.y * nrm.y + nrm.z * nrm.z);
+        if (nlen < 1e-12f) return false;
+        nrm = mul(nrm, 1.0f / nlen);
+        if (dot3(nrm, d) > 0.0f) nrm = neg(nrm);
+        if (tr) nrm = m_mul(s.rot, nrm);
+        t_out = t; n_out = nrm;
+        return true;
+    }
+    // FORM_ASPHERE: Newton on the sag residual from the spherical guess.
+    {
+        const float R = s.R, inv_R = 1.0f / R, K = s.conic;
+        F3 ctr = f3(0.f, 0.f, zv + R);
+        F3 oc = sub(o, ctr);
+        float qa = dot3(d, d), qb = 2.0f * dot3(oc, d), qc = dot3(oc, oc) - R * R;
+        float disc = qb * qb - 4.0f * qa * qc;
+        float t;
+        if (disc >= 0.0f) {
+            float sd = sqrtf(disc), inv_2a = 0.5f / qa;
+            if (!pick_root((-qb - sd) * inv_2a, (-qb + sd) * inv_2a, o.z, d.z, zv, t)) return false;
+        } else {
+            if (fabsf(d.z) < 1e-12f) return false;
+            t = (zv - o.z) / d.z;
+            if (!(t > 1e-6f)) return false;
+        }
+        for (int it = 0; it < 15; ++it) {
+            F3 hi = add(o, mul(d, t));
+            float rx = hi.x, ry = hi.y, rad = sqrtf(rx * rx + ry * ry);
+            float sag = asph_sag(rad, inv_R, K, s);
+            if (sag > 1e29f) return false;
+            float f = hi.z - zv - sag;
+            if (fabsf(f) < 1e-5f) break;
+            float dsag = asph_dsag(rad, inv_R, K, s);
+            float r_safe = (rad > 1e-9f) ? rad : 1e-9f;
+            float dfdt = d.z - dsag * (rx * d.x + ry * d.y) / r_safe;
+            if (fabsf(dfdt) < 1e-12f) break;
+            t -= f / dfdt;
+        }
+        if (!(t > 1e-6f)) return false;
+        F3 hit = add(o, mul(d, t));
+        float rad = sqrtf(hit.x * hit.x + hit.y * hit.y);
+        if (!check_aperture(hit, s, poly)) return false;
+        float r_safe = (rad > 1e-9f) ? rad : 1e-9f;
+        float dsag = asph_dsag(rad, inv_R, K, s);
+        F3 nrm = f3(-dsag * hit.x / r_safe, -dsag * hit.y / r_safe, 1.0f);
+        float nlen = sqrtf(nrm.x * nrm.x + nrm.y * nrm.y + nrm.z * nrm.z);
+        nrm = mul(nrm, 1.0f / nlen);
+        if (dot3(nrm, d) > 0.0f) nrm = neg(nrm);
+        if (tr) nrm = m_mul(s.rot, nrm);
+        t_out = t; n_out = nrm;
+        return true;
+    }
+}
+
+GLD bool refract(F3 dir, F3 nrm, float n1, float n2, F3& out) {   // trace_core.h:533-552
+    float n_ratio = n1 / n2;
+    float cos_i = -dot3(nrm, dir);
+    float sin2_t = n_ratio * n_ratio * (1.0f - cos_i * cos_i);
+    if (sin2_t >= 1.0f) return false;
+    float cos_t = sqrtf(1.0f - sin2_t);
+    F3 tmp = add(mul(dir, n_ratio), mul(nrm, n_ratio * cos_i - cos_t));
+    float sq = tmp.x * tmp.x + tmp.y * tmp.y + tmp.z * tmp.z;
+    if (sq < 1e-18f) return false;
+    float inv = 1.0f / sqrtf(sq);
+    out = mul(tmp, inv);
+    return true;
+}
+GLD F3 reflect(F3 dir, F3 nrm) { return normalized(sub(dir, mul(nrm, 2.0f * dot3(dir, nrm)))); }
+
+// One surface interaction (trace_cuda.h:226-361 loop bodies).  Returns false
+// when the ray died; res.status is set to TIR for total internal reflection.
+GLD bool step(const DevSurf GLC& s, const float GLC* poly, const float* __restrict__ arena, F3& o, F3& d,
+              float lam, float n2, bool reflecting, float reflect_ior, float& cur, TraceRes& res) {
+    float t; F3 nrm;
+    ++res.segs;
+    if (!intersect(o, d, s, poly, t, nrm)) return false;
+    F3 hit = add(o, mul(d, t));
+    o = hit;
+    const float n1 = cur;
+    if (reflecting) {
+        res.w *= surface_weight(d, nrm, n1, n2, s, arena, lam, true);
+        res.w *= attenuator(hit, s);
+        d = reflect(d, nrm);
+        cur = reflect_ior;
+    } else {
+        F3 nd;
+        if (!refract(d, nrm, n1, n2, nd)) { res.status = GL_TRACE_TIR; return false; }
+        res.w *= surface_weight(d, nrm, n1, n2, s, arena, lam, false);
+        res.w *= attenuator(hit, s);
+        d = nd;
+        cur = n2;
+    }
+    return true;
+}
+
+GLD void to_sensor(F3 o, F3 d, bool t_ge0, TraceRes& res) {   // trace_cuda.h:366-380
+    if (fabsf(d.z) < 1e-12f) { res.status = GL_TRACE_MISSED_SURFACE; return; }
+    float ts = -o.z / d.z;
+    bool ok = t_ge0 ? (ts >= 0.0f) : (ts > 0.0f);
+    if (!ok) { res.status = GL_TRACE_MISSED_SURFACE; return; }
+    F3 p = add(o, mul(d, ts));
+    res.px = p.x; res.py = p.y; res.pz = p.z;
+    res.status = GL_TRACE_OK;
+}
+
+// d_trace_ghost_ray (trace_cuda.h:195-382) with per-lambda IOR slices.  The
+// surface record is copied from the constant address space per step; only
+// the fields a branch reads are loaded (scalar loads, uniform index).
+GLD TraceRes trace_ghost(const DevSurf GLC* S, int n, const float GLC* poly, const float* __restrict__ arena,
+                         int a, int b, F3 o, F3 d, float lam, const float GLC* iat, const float GLC* ibf,
+                         bool t_ge0) {
+    TraceRes res; res.status = GL_TRACE_VIGNETTED; res.w = 1.0f; res.segs = 0; res.px = res.py = res.pz = 0.f;
+    float cur = 1.0f;
+    for (int k = 0; k <= b; ++k) {
+        const DevSurf GLC& s = S[k];
+        if (!(s.flags & SF_ACTIVE)) continue;
+        if (!step(s, poly, arena, o, d, lam, iat[k], k == b, cur, cur, res)) return res;
+    }
+    const float refl_a = iat[a];
+    for (int k = b - 1; k >= a; --k) {
+        const DevSurf GLC& s = S[k];
+        if (!(s.flags & SF_ACTIVE)) continue;
+        if (!step(s, poly, arena, o, d, lam, ibf[k], k == a, refl_a, cur, res)) return res;
+    }
+    for (int k = a + 1; k < n; ++k) {
+        const DevSurf GLC& s = S[k];
+        if (!(s.flags & SF_ACTIVE)) continue;
+        if (!step(s, poly, arena, o, d, lam, iat[k], false, 0.0f, cur, res)) return res;
+    }
+    to_sensor(o, d, t_ge0, res);
+    return res;
+}
+
+// d_trace_primary_ray (trace_cuda.h:111-172).
+GLD TraceRes trace_primary(const DevSurf GLC* S, int n, const float GLC* poly, const float* __restrict__ arena,
+                           F3 o, F3 d, float lam, const float GLC* iat, bool t_ge0, F3* out_dir) {
+    TraceRes res; res.status = GL_TRACE_VIGNETTED; res.w = 1.0f; res.segs = 0; res.px = res.py = res.pz = 0.f;
+    float cur = 1.0f;
+    for (int k = 0; k < n; ++k) {
+        const DevSurf GLC& s = S[k];
+        if (!(s.flags & SF_ACTIVE)) continue;
+        if (!step(s, poly, arena, o, d, lam, iat[k], false, 0.0f, cur, res)) return res;
+    }
+    to_sensor(o, d, t_ge0, res);
+    if (out_dir) *out_dir = d;
+    return res;
+}
+
+// ---- pupil sampling on the device (aperture_sampler.h:136-170) ------------
+GLD uint32_t wang_hash(uint32_t s) {
+    s = (s ^ 61u) ^ (s >> 16u); s *= 9u; s ^= s >> 4u; s *= 0x27d4eb2du; s ^= s >> 15u; return s;
+}
+GLD float halton2(uint32_t n) {
+    n = (n << 16u) | (n >> 16u);
+    n = ((n & 0x00ff00ffu) << 8u) | ((n & 0xff00ff00u) >> 8u);
+    n = ((n & 0x0f0f0f0fu) << 4u) | ((n & 0xf0f0f0f0u) >> 4u);
+    n = ((n & 0x33333333u) << 2u) | ((n & 0xccccccccu) >> 2u);
+    n = ((n & 0x55555555u) << 1u) | ((n & 0xaaaaaaaau) >> 1u);
+    return (float)n * (1.0f / 4294967296.0f);
+}
+GLD float halton3(uint32_t n) {
+    float r = 0.0f, f = 1.0f / 3.0f;
+    while (n > 0) { r += (float)(n % 3u) * f; n /= 3u; f /= 3.0f; }
+    return r;
+}
+
+// Pupil silhouette (PupilMask, aperture_sampler.h:49-80) in half-plane form.
+struct DevMask {
+    float aspect;
+    int32_t nb;
+    float apothem;
+    float c[GL_MAX_BLADES], s[GL_MAX_BLADES];
+};
+GLD bool mask_contains(const DevMask& m, float u, float v) {
+    const float ux = u / m.aspect;
+    const float r2 = ux * ux + v * v;
+    if (r2 > 1.0f) return false;
+    if (m.nb >= 3) {
+        float mx = ux * m.c[0] + v * m.s[0];
+        for (int k = 1; k < m.nb; ++k) { float dd = ux * m.c[k] + v * m.s[k]; mx = dd > mx ? dd : mx; }
+        if (mx > m.apothem) return false;
+    }
+    return true;
+}
+
+}  // namespace gl

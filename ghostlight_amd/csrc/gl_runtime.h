@@ -1,0 +1,89 @@
+// gl_runtime.h — host-side runtime shared by the HIP translation units:
+// error state, contexts (stream + grow-only device buffers), device lenses.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+#include <cstdint>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/ghostlight_amd.h"
+#include "gl_device.h"
+
+namespace gl {
+
+int set_error(int code, const char* fmt, ...);
+#define GL_HIP(call)                                                                    \
+    do {                                                                                \
+        hipError_t e_ = (call);                                                         \
+        if (e_ != hipSuccess) return ::gl::set_error(GL_ERR_DEVICE, "%s -- %s", #call,  \
+                                                     hipGetErrorString(e_));            \
+    } while (0)
+
+// Grow-only device allocation owned by a context.
+struct DevBuf {
+    void* p = nullptr;
+    size_t cap = 0;
+    int ensure(size_t bytes);
+    template <class T> T* as() const { return static_cast<T*>(p); }
+    void release();
+};
+
+struct HostCacheKey;
+
+struct gl_ctx_impl;
+}  // namespace gl
+
+// One render context: a default-priority HIP stream, its buffers, caches.
+struct gl_ctx {
+    hipStream_t stream = nullptr;
+    int device = 0;
+    std::mutex mu;                       // one render at a time per context
+    std::map<std::string, gl::DevBuf> bufs;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    float last_kernel_ms = 0.0f;         // main trace+splat kernel, event-timed
+    // Host caches (pupil grid, spectral table) keyed like the reference's
+    // GridKey / SpecKey (ghost_render.cu:515-556).
+    std::vector<uint8_t> grid_key, spec_key;
+    std::vector<gl_sample> grid;
+    std::vector<gl_spec> spec;
+    float spec_C[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
+    bool spec_do_correct = false;
+    bool grid_uploaded = false, spec_uploaded = false;
+    gl::DevBuf& buf(const char* name) { return bufs[name]; }
+};
+
+// Device-resident prescription.
+struct gl_lens {
+    std::vector<gl_surface> host;
+    std::vector<float> arena;
+    std::vector<gl::DevSurf> dsurf;
+    std::vector<float> poly;             // per polygon surface: c[32] | s[32]
+    float fp64_rmin = -1.0f;             // value the device copy was built with
+    int device = -1;
+    gl::DevSurf* d_surf = nullptr;
+    float* d_arena = nullptr;
+    float* d_poly = nullptr;
+    std::mutex mu;
+};
+
+namespace gl {
+gl_ctx* resolve_ctx(gl_ctx* ctx, int* status);
+int lens_sync(gl_lens* lens, float fp64_rmin, hipStream_t stream);
+void build_devsurfs(const gl_surface* s, int n, float fp64_rmin, std::vector<DevSurf>& out, std::vector<float>& poly);
+DevMask build_mask(const gl_surface* s, int n, int blades_override, float rotation_deg);
+
+// host preparation (hostprep.cpp)
+std::vector<gl_sample> build_samples(const gl_surface* s, int n, int N, int jitter, int seed, int bo, float rdeg);
+void ior_tables(const gl_surface* s, int n, const gl_spec* spec, int ns, float* at, float* bf);
+float ior_at(const gl_surface& s, float lambda_nm);
+int spectral_table(int n, int output_cs, const float* custom9, std::vector<gl_spec>& out);
+int spectral_correction(const std::vector<gl_spec>& t, int output_cs, const float* custom9, float C[9]);
+void poly_normals(int n_blades, float rot_rad, float scale, float* c, float* s, float* apothem);
+double now_ms();
+}  // namespace gl

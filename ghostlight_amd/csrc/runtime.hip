@@ -1,0 +1,195 @@
+// runtime.hip — error state, device discovery, render contexts and
+// device-resident lenses.
+#include "gl_runtime.h"
+
+#include <cstdio>
+
+namespace gl {
+
+static thread_local std::string t_err = "no error";
+
+int set_error(int code, const char* fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    t_err = buf;
+    return code;
+}
+
+int DevBuf::ensure(size_t bytes) {
+    if (bytes <= cap && p) return GL_OK;
+    if (p) (void)hipFree(p);
+    p = nullptr; cap = 0;
+    size_t want = bytes < 256 ? 256 : bytes;
+    hipError_t e = hipMalloc(&p, want);
+    if (e != hipSuccess) {
+        p = nullptr;
+        return set_error(e == hipErrorOutOfMemory ? GL_ERR_NO_MEMORY : GL_ERR_DEVICE,
+                         "hipMalloc(%zu) -- %s", want, hipGetErrorString(e));
+    }
+    cap = want;
+    return GL_OK;
+}
+
+void DevBuf::release() {
+    if (p) (void)hipFree(p);
+    p = nullptr; cap = 0;
+}
+
+static int device_ok(int* dev) {
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    if (e != hipSuccess || n <= 0)
+        return set_error(GL_ERR_DEVICE, "no HIP device available (%s)",
+                         e == hipSuccess ? "device count 0" : hipGetErrorString(e));
+    GL_HIP(hipGetDevice(dev));
+    return GL_OK;
+}
+
+static int ctx_init(gl_ctx* c) {
+    int st = device_ok(&c->device);
+    if (st) return st;
+    GL_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    GL_HIP(hipEventCreate(&c->ev0));
+    GL_HIP(hipEventCreate(&c->ev1));
+    return GL_OK;
+}
+
+gl_ctx* resolve_ctx(gl_ctx* ctx, int* status) {
+    *status = GL_OK;
+    if (ctx) return ctx;
+    // Process default context: created once, never destroyed (the runtime may
+    // already be torn down at exit).
+    static std::mutex m;
+    static gl_ctx* def = nullptr;
+    std::lock_guard<std::mutex> g(m);
+    if (!def) {
+        gl_ctx* c = new gl_ctx();
+        int st = ctx_init(c);
+        if (st) { delete c; *status = st; return nullptr; }
+        def = c;
+    }
+    return def;
+}
+
+int lens_sync(gl_lens* L, float fp64_rmin, hipStream_t stream) {
+    int dev = 0;
+    GL_HIP(hipGetDevice(&dev));
+    if (L->d_surf && L->device == dev && L->fp64_rmin == fp64_rmin) return GL_OK;
+    build_devsurfs(L->host.data(), (int)L->host.size(), fp64_rmin, L->dsurf, L->poly);
+    if (L->device != dev) {
+        if (L->d_surf) { (void)hipFree(L->d_surf); (void)hipFree(L->d_arena); (void)hipFree(L->d_poly); }
+        L->d_surf = nullptr; L->d_arena = nullptr; L->d_poly = nullptr;
+        GL_HIP(hipMalloc(&L->d_surf, GL_MAX_SURFACES * sizeof(DevSurf)));
+        GL_HIP(hipMalloc(&L->d_arena, L->arena.size() * sizeof(float)));
+        GL_HIP(hipMalloc(&L->d_poly, GL_MAX_SURFACES * 2 * GL_MAX_BLADES * sizeof(float)));
+        GL_HIP(hipMemcpyAsync(L->d_arena, L->arena.data(), L->arena.size() * sizeof(float),
+                              hipMemcpyHostToDevice, stream));
+        L->device = dev;
+    }
+    GL_HIP(hipMemcpyAsync(L->d_surf, L->dsurf.data(), L->dsurf.size() * sizeof(DevSurf), hipMemcpyHostToDevice, stream));
+    GL_HIP(hipMemcpyAsync(L->d_poly, L->poly.data(), L->poly.size() * sizeof(float), hipMemcpyHostToDevice, stream));
+    GL_HIP(hipStreamSynchronize(stream));
+    L->fp64_rmin = fp64_rmin;
+    return GL_OK;
+}
+
+}  // namespace gl
+
+extern "C" {
+
+const char* gl_last_error(void) { return gl::t_err.c_str(); }
+
+int gl_device_available(void) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return 0;
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, 0) != hipSuccess) return 0;
+    return std::strncmp(prop.gcnArchName, "gfx950", 6) == 0 ? 1 : 0;
+}
+
+int gl_ctx_create(gl_ctx** out) {
+    if (!out) return gl::set_error(GL_ERR_INVALID, "null out");
+    gl_ctx* c = new gl_ctx();
+    int st = gl::ctx_init(c);
+    if (st) { delete c; *out = nullptr; return st; }
+    *out = c;
+    return GL_OK;
+}
+
+void gl_ctx_destroy(gl_ctx* c) {
+    if (!c) return;
+    {
+        std::lock_guard<std::mutex> g(c->mu);
+        if (c->stream) (void)hipStreamSynchronize(c->stream);
+        for (auto& kv : c->bufs) kv.second.release();
+        if (c->ev0) (void)hipEventDestroy(c->ev0);
+        if (c->ev1) (void)hipEventDestroy(c->ev1);
+        if (c->stream) (void)hipStreamDestroy(c->stream);
+    }
+    delete c;
+}
+
+int gl_ctx_synchronize(gl_ctx* ctx) {
+    int st;
+    gl_ctx* c = gl::resolve_ctx(ctx, &st);
+    if (!c) return st;
+    GL_HIP(hipStreamSynchronize(c->stream));
+    return GL_OK;
+}
+
+float gl_ctx_last_kernel_ms(gl_ctx* ctx) {
+    int st;
+    gl_ctx* c = gl::resolve_ctx(ctx, &st);
+    return c ? c->last_kernel_ms : -1.0f;
+}
+
+int gl_lens_create(const gl_surface* s, int32_t n, const float* arena, int32_t arena_len, gl_lens** out) {
+    if (!out) return gl::set_error(GL_ERR_INVALID, "null out");
+    *out = nullptr;
+    if (!s || n <= 0 || n > GL_MAX_SURFACES)
+        return gl::set_error(GL_ERR_INVALID, "surface count %d outside [1, %d]", n, GL_MAX_SURFACES);
+    for (int i = 0; i < n; ++i) {
+        const gl_surface& g = s[i];
+        if (g.table_count > 0 && (!arena || g.table_offset < 0 || g.table_offset + 2 * g.table_count > arena_len))
+            return gl::set_error(GL_ERR_INVALID, "surface %d: coating table outside the arena", i);
+        if (g.sa_n_wavelengths > 0 &&
+            (!arena || g.sa_offset < 0 ||
+             g.sa_offset + g.sa_n_wavelengths + g.sa_n_angles + g.sa_n_wavelengths * g.sa_n_angles > arena_len))
+            return gl::set_error(GL_ERR_INVALID, "surface %d: spectral-angular table outside the arena", i);
+        if (g.aperture_shape == GL_APERTURE_IMAGE)
+            return gl::set_error(GL_ERR_UNSUPPORTED, "surface %d: image apertures are not part of this build", i);
+    }
+    gl_lens* L = new gl_lens();
+    L->host.assign(s, s + n);
+    if (arena && arena_len > 0) L->arena.assign(arena, arena + arena_len);
+    else L->arena.assign(1, 0.0f);
+    *out = L;
+    return GL_OK;
+}
+
+void gl_lens_destroy(gl_lens* L) {
+    if (!L) return;
+    if (L->d_surf) { (void)hipFree(L->d_surf); (void)hipFree(L->d_arena); (void)hipFree(L->d_poly); }
+    delete L;
+}
+
+int gl_enumerate_ghost_pairs(const gl_lens* L, gl_pair* out, int32_t cap, int32_t* n_out) {
+    if (!L || !n_out) return gl::set_error(GL_ERR_INVALID, "null argument");
+    const int n = (int)L->host.size();
+    int cnt = 0;
+    for (int a = 0; a < n; ++a) {
+        if (!L->host[a].is_active) continue;
+        for (int b = a + 1; b < n; ++b) {
+            if (!L->host[b].is_active) continue;
+            if (cnt < cap && out) out[cnt] = gl_pair{a, b, 1.0f};
+            ++cnt;
+        }
+    }
+    *n_out = cnt;
+    return cnt > cap ? gl::set_error(GL_ERR_INVALID, "pair capacity %d < %d", cap, cnt) : GL_OK;
+}
+
+}  // extern "C"

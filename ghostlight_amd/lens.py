@@ -1,0 +1,324 @@
+"""Prescription types: Coating, Surface, OpticalSystem, LensCalibration.
+
+Attribute names, enum values and defaults follow the reference's bound types
+(ghostlight/src/optical_system.h:86-249, ghostlight/src/lens_calibration.h:11-57,
+ghostlight/bindings/python/src/bind_geometry.cpp:144-520).  The prescription is
+packed into the C ABI's ``gl_surface`` array plus one float arena for
+table-backed coatings (``OpticalSystem.pack``).
+"""
+from __future__ import annotations
+
+import enum
+import math
+from dataclasses import dataclass, field
+from typing import List, Optional
+
+import numpy as np
+
+from . import _abi
+
+
+class SurfaceForm(enum.IntEnum):
+    SPHERE = 0
+    ASPHERE = 1
+    CYLINDRICAL = 2
+
+
+class CylinderAxis(enum.IntEnum):
+    X = 0
+    Y = 1
+
+
+class ApertureShape(enum.IntEnum):
+    CIRCLE = 0
+    POLYGON = 1
+    IMAGE = 2
+
+
+class DispersionModel(enum.IntEnum):
+    AIR = 0
+    ABBE = 1
+    SELLMEIER = 2
+
+
+class CoatingModel(enum.IntEnum):
+    SIMPLE = 0
+    SPECTRAL = 1
+    ANGULAR = 2
+    SPECTRAL_ANGULAR = 3
+    ATTENUATOR_GAUSS = 4
+    ARTIST = 5
+
+
+class TraceStatus(enum.IntEnum):
+    OK = 0
+    VIGNETTED = 1
+    TIR = 2
+    MISSED_SURFACE = 3
+    INVALID_INPUT = 4
+
+
+@dataclass
+class Coating:
+    """Per-surface coating descriptor (optical_system.h:109-147)."""
+    model: CoatingModel = CoatingModel.SIMPLE
+    ar_layers: int = 0
+    out_of_range_discard: bool = False
+    gauss_sigma: float = 0.0
+    gauss_background: float = 0.0
+    gauss_peak: float = 0.0
+    gauss_decenter_x: float = 0.0
+    gauss_decenter_y: float = 0.0
+    angle_ref_ior: float = 1.0
+    tint_r: float = 1.0
+    tint_g: float = 1.0
+    tint_b: float = 1.0
+    tint_strength: float = 0.04
+    # table storage (CoatingTables, optical_system.h:289-301)
+    table: Optional[np.ndarray] = None        # (K, 2) key, r
+    sa_wavelengths: Optional[np.ndarray] = None
+    sa_angles: Optional[np.ndarray] = None
+    sa_r: Optional[np.ndarray] = None         # (nw, na)
+
+
+_IDENT = (1.0, 0.0, 0.0, 0.0, 1.0, 0.0, 0.0, 0.0, 1.0)
+
+
+@dataclass
+class Surface:
+    """One optical surface (optical_system.h:157-249)."""
+    radius: float = 0.0
+    thickness: float = 0.0
+    ior: float = 1.0
+    abbe_v: float = 0.0
+    semi_aperture: float = 10.0
+    coating: Coating = field(default_factory=Coating)
+    is_stop: bool = False
+    is_active: bool = True
+    z: float = 0.0
+    aperture_shape: ApertureShape = ApertureShape.CIRCLE
+    aperture_blades: int = 0
+    aperture_rotation_rad: float = 0.0
+    aperture_aspect: float = 1.0
+    aperture_semi_diameter: float = 0.0
+    decenter_x: float = 0.0
+    decenter_y: float = 0.0
+    rot: tuple = _IDENT
+    form: SurfaceForm = SurfaceForm.SPHERE
+    conic_k: float = 0.0
+    asphere_terms: tuple = ()
+    cyl_axis: CylinderAxis = CylinderAxis.X
+    disp_model: DispersionModel = DispersionModel.ABBE
+    sellmeier_B: tuple = (0.0, 0.0, 0.0)
+    sellmeier_C: tuple = (0.0, 0.0, 0.0)
+
+    @property
+    def n_asphere_terms(self) -> int:
+        return len(self.asphere_terms)
+
+    def ior_at(self, lambda_nm: float) -> float:
+        """Wavelength-dependent index (optical_system.h:230-235), float32 arithmetic."""
+        f = np.float32
+        lam = f(lambda_nm)
+        if self.disp_model == DispersionModel.SELLMEIER:
+            B = [f(b) for b in self.sellmeier_B]
+            Cc = [f(c) for c in self.sellmeier_C]
+            l = lam * f(1e-3)
+            l2 = l * l
+            n2 = f(1.0)
+            for i in range(3):
+                n2 = f(n2 + f(B[i] * l2) / f(l2 - Cc[i]))
+            return float(np.sqrt(n2)) if n2 > 0 else 1.0
+        n_d, V = f(self.ior), f(self.abbe_v)
+        if V < f(0.1) or n_d <= f(1.0001):
+            return float(n_d)
+        dn = (n_d - f(1)) / V
+        iF = f(1) / (f(486.13) * f(486.13))
+        iC = f(1) / (f(656.27) * f(656.27))
+        idl = f(1) / (f(587.56) * f(587.56))
+        B = dn / (iF - iC)
+        A = n_d - B * idl
+        return float(A + B / (lam * lam))
+
+
+@dataclass
+class LensCalibration:
+    """Calibration record (lens_calibration.h:11-57)."""
+    sensor_half_w: float = 18.0
+    sensor_half_h: float = 12.0
+    max_half_angle_h: float = 0.0
+    max_half_angle_v: float = 0.0
+    image_circle_semi_w: float = 0.0
+    image_circle_semi_h: float = 0.0
+    focal_length_x: float = 0.0
+    focal_length_y: float = 0.0
+    entrance_pupil_semi_x: float = 0.0
+    entrance_pupil_semi_y: float = 0.0
+    f_number_x: float = 0.0
+    f_number_y: float = 0.0
+    pupil_area_frac: float = 1.0
+
+    FIELDS = ("sensor_half_w", "sensor_half_h", "max_half_angle_h", "max_half_angle_v",
+              "image_circle_semi_w", "image_circle_semi_h", "focal_length_x", "focal_length_y",
+              "entrance_pupil_semi_x", "entrance_pupil_semi_y", "f_number_x", "f_number_y",
+              "pupil_area_frac")
+
+    @classmethod
+    def from_array(cls, a):
+        return cls(**{k: float(v) for k, v in zip(cls.FIELDS, a)})
+
+
+class OpticalSystem:
+    """Flat surface sequence with the sensor at z = 0 (optical_system.h:305-416)."""
+
+    def __init__(self, surfaces: Optional[List[Surface]] = None, name: str = ""):
+        self.name = name
+        self.focal_length = 0.0
+        self.surfaces: List[Surface] = list(surfaces or [])
+        self._dev = None        # cached device handle (gl_lens*) keyed by pack digest
+        self._dev_key = None
+
+    def num_surfaces(self) -> int:
+        return len(self.surfaces)
+
+    def finalize(self) -> bool:
+        """Relay z from the thickness chain so the chain ends at z = 0
+        (OpticalSystem::finalize, optical_system.cpp:1425)."""
+        if not self.surfaces:
+            return False
+        z = np.float32(0.0)
+        for s in reversed(self.surfaces):
+            z = np.float32(z - np.float32(s.thickness))
+            s.z = float(z)
+        return True
+
+    def ior_before(self, idx: int) -> float:
+        idx = min(idx, len(self.surfaces))
+        for k in range(idx - 1, -1, -1):
+            if self.surfaces[k].is_active:
+                return self.surfaces[k].ior
+        return 1.0
+
+    def ior_before_at(self, idx: int, lambda_nm: float) -> float:
+        idx = min(idx, len(self.surfaces))
+        for k in range(idx - 1, -1, -1):
+            if self.surfaces[k].is_active:
+                return self.surfaces[k].ior_at(lambda_nm)
+        return 1.0
+
+    # ---- coatings ---------------------------------------------------------
+    def set_coating_layers(self, i: int, layers) -> None:
+        """Bake a thin-film stack to a SPECTRAL_ANGULAR table (31 x 18) with the
+        transfer-matrix method (OpticalSystem::bake_coating_layers,
+        optical_system.cpp:1483-1570).  ``layers``: list of dicts with
+        ``thickness_nm`` and ``nk_table`` (K, 3) of (lambda_um, n, k)."""
+        from . import hostprep
+        wl = np.arange(31, dtype=np.float32) * np.float32(10.0) + np.float32(400.0)
+        n_amb = np.array([self.ior_before_at(i, float(l)) for l in wl], np.float32)
+        n_sub = np.array([self.surfaces[i].ior_at(float(l)) for l in wl], np.float32)
+        n_re = np.zeros((len(layers), 31), np.float32)
+        n_im = np.zeros((len(layers), 31), np.float32)
+        thick = np.zeros(len(layers), np.float32)
+        for li, d in enumerate(layers):
+            nk = np.asarray(d["nk_table"], np.float32)
+            thick[li] = d["thickness_nm"]
+            for w, lam in enumerate(wl):
+                n_re[li, w], n_im[li, w] = _interp_nk(nk, np.float32(lam * np.float32(1e-3)))
+        table = hostprep.bake_tmm_table(n_re, n_im, thick, n_amb, n_sub)
+        c = self.surfaces[i].coating
+        c.model = CoatingModel.SPECTRAL_ANGULAR
+        c.out_of_range_discard = False
+        c.angle_ref_ior = self.ior_before_at(i, 587.56)
+        c.sa_wavelengths = table[:31].copy()
+        c.sa_angles = table[31:49].copy()
+        c.sa_r = table[49:].reshape(31, 18).copy()
+
+    # ---- packing for the C ABI -------------------------------------------
+    def pack(self):
+        """Return (gl_surface array, float32 arena)."""
+        n = len(self.surfaces)
+        if n > _abi.GL_MAX_SURFACES:
+            raise ValueError(f"lens has {n} surfaces; at most {_abi.GL_MAX_SURFACES} supported")
+        arr = (_abi.gl_surface * max(n, 1))()
+        arena: List[float] = []
+        for i, s in enumerate(self.surfaces):
+            g = arr[i]
+            g.radius, g.thickness, g.ior, g.abbe_v = s.radius, s.thickness, s.ior, s.abbe_v
+            g.semi_aperture, g.z = s.semi_aperture, s.z
+            g.is_stop, g.is_active = int(bool(s.is_stop)), int(bool(s.is_active))
+            g.aperture_shape, g.aperture_blades = int(s.aperture_shape), int(s.aperture_blades)
+            g.aperture_rotation_rad, g.aperture_aspect = s.aperture_rotation_rad, s.aperture_aspect
+            g.decenter_x, g.decenter_y = s.decenter_x, s.decenter_y
+            for k in range(9):
+                g.rot[k] = s.rot[k]
+            g.form, g.conic_k = int(s.form), s.conic_k
+            at = list(s.asphere_terms)[: _abi.GL_MAX_ASPHERE_TERMS]
+            for k, t in enumerate(at):
+                g.asphere_terms[k] = t
+            g.n_asphere_terms = len(at)
+            g.cyl_axis, g.disp_model = int(s.cyl_axis), int(s.disp_model)
+            for k in range(3):
+                g.sellmeier_B[k], g.sellmeier_C[k] = s.sellmeier_B[k], s.sellmeier_C[k]
+            c = s.coating
+            g.coating_model, g.ar_layers = int(c.model), int(c.ar_layers)
+            g.out_of_range_discard, g.angle_ref_ior = int(bool(c.out_of_range_discard)), c.angle_ref_ior
+            g.tint_r, g.tint_g, g.tint_b, g.tint_strength = c.tint_r, c.tint_g, c.tint_b, c.tint_strength
+            g.gauss_sigma, g.gauss_background, g.gauss_peak = c.gauss_sigma, c.gauss_background, c.gauss_peak
+            g.gauss_decenter_x, g.gauss_decenter_y = c.gauss_decenter_x, c.gauss_decenter_y
+            g.table_offset = g.table_count = g.sa_offset = g.sa_n_wavelengths = g.sa_n_angles = 0
+            if c.table is not None and len(c.table):
+                t = np.asarray(c.table, np.float32).reshape(-1, 2)
+                g.table_offset, g.table_count = len(arena), len(t)
+                arena.extend(t.ravel().tolist())
+            if c.sa_r is not None:
+                wl = np.asarray(c.sa_wavelengths, np.float32)
+                an = np.asarray(c.sa_angles, np.float32)
+                r = np.asarray(c.sa_r, np.float32).reshape(len(wl), len(an))
+                g.sa_offset, g.sa_n_wavelengths, g.sa_n_angles = len(arena), len(wl), len(an)
+                arena.extend(wl.tolist()); arena.extend(an.tolist()); arena.extend(r.ravel().tolist())
+        return arr, np.asarray(arena if arena else [0.0], np.float32)
+
+    def pack_key(self) -> bytes:
+        arr, arena = self.pack()
+        import hashlib
+        h = hashlib.blake2b(bytes(arr), digest_size=16)
+        h.update(arena.tobytes())
+        return h.digest()
+
+    def device_lens(self):
+        """Device-resident handle for this prescription (re-created on change)."""
+        from . import runtime
+        key = self.pack_key()
+        if self._dev is None or self._dev_key != key:
+            self._dev = runtime.DeviceLens(self)
+            self._dev_key = key
+        return self._dev
+
+    def __repr__(self) -> str:
+        return f"OpticalSystem(name={self.name!r}, surfaces={len(self.surfaces)})"
+
+
+def _interp_nk(nk: np.ndarray, lu: np.float32):
+    """Complex index of a layer at lambda (um), linear in the table
+    (optical_system.cpp:1510-1529), float32 arithmetic."""
+    xs, ns, ks = nk[:, 0], nk[:, 1], nk[:, 2]
+    if lu <= xs[0]:
+        return ns[0], ks[0]
+    if lu >= xs[-1]:
+        return ns[-1], ks[-1]
+    for k in range(1, len(xs)):
+        if lu <= xs[k]:
+            span = np.float32(xs[k] - xs[k - 1])
+            f = np.float32((lu - xs[k - 1]) / span) if span > 0 else np.float32(0)
+            return (np.float32(ns[k - 1] + f * np.float32(ns[k] - ns[k - 1])),
+                    np.float32(ks[k - 1] + f * np.float32(ks[k] - ks[k - 1])))
+    return ns[-1], ks[-1]
+
+
+def rot_from_tilt(tilt_x_deg: float, tilt_y_deg: float) -> tuple:
+    """Row-major local->world rotation R = Ry(tilt_y) @ Rx(tilt_x)."""
+    ax, ay = math.radians(tilt_x_deg), math.radians(tilt_y_deg)
+    cx, sx, cy, sy = math.cos(ax), math.sin(ax), math.cos(ay), math.sin(ay)
+    Rx = np.array([[1, 0, 0], [0, cx, -sx], [0, sx, cx]])
+    Ry = np.array([[cy, 0, sy], [0, 1, 0], [-sy, 0, cy]])
+    return tuple(float(v) for v in (Ry @ Rx).ravel())

@@ -1,0 +1,256 @@
+"""Render entry points: the Python face of the drop-in boundary.
+
+`render_point_flare`, `render_source_flare` mirror the reference binding
+(ghostlight/bindings/python/src/bind_renderers.cpp:85-134): same argument
+names and order, the same output dict keys / shapes / dtypes
+(buffers.h:45-106) and the same exception types.  `render_ghosts` is the
+lower seam of `launch_ghost_render` (ghost_render.h:75-94) with explicit
+field-angle sources, used by the benchmark configurations.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+from typing import Optional, Sequence
+
+import numpy as np
+
+from . import _abi, hostprep
+from .config import FlareConfig, PointFlareConfig, to_params
+from .lens import LensCalibration, OpticalSystem
+from .runtime import ctx_handle, lib
+
+
+def _dims(fn: str, w: int, h: int):
+    if w <= 0 or h <= 0:
+        raise ValueError(f"{fn}: width and height must be positive")
+
+
+def _sources_array(sources) -> "C.Array":
+    s = np.ascontiguousarray(sources, np.float32).reshape(-1, 5)
+    arr = (_abi.gl_source * max(1, len(s)))()
+    if len(s):
+        C.memmove(arr, s.ctypes.data, s.nbytes)
+    return arr, len(s)
+
+
+class _Stats:
+    """Owner of the caller-side arrays of gl_ghost_stats."""
+
+    def __init__(self, n_pairs_cap: int, n_ps_cap: int):
+        self.pa = np.zeros(n_pairs_cap, np.int32)
+        self.pb = np.zeros(n_pairs_cap, np.int32)
+        self.pt = np.zeros(n_pairs_cap, np.uint64)
+        self.ps = np.zeros(n_pairs_cap, np.uint64)
+        self.po = np.zeros(n_pairs_cap, np.uint64)
+        self.hits = np.zeros(max(1, n_ps_cap), np.int32)
+        self.rect = np.zeros(max(1, n_ps_cap) * 4, np.float32)
+        self.budget = np.zeros(max(1, n_ps_cap), np.int32)
+        s = _abi.gl_ghost_stats()
+        s.pair_cap = n_pairs_cap
+        s.pair_surf_a = _abi.iptr(self.pa)
+        s.pair_surf_b = _abi.iptr(self.pb)
+        u64 = C.POINTER(C.c_uint64)
+        s.pair_traces = self.pt.ctypes.data_as(u64)
+        s.pair_survived = self.ps.ctypes.data_as(u64)
+        s.pair_on_sensor = self.po.ctypes.data_as(u64)
+        s.ps_cap = n_ps_cap
+        s.ps_hits = _abi.iptr(self.hits)
+        s.ps_rect = _abi.fptr(self.rect)
+        s.ps_budget = _abi.iptr(self.budget)
+        s.ms_grid_build = s.ms_upload = s.ms_kernel = s.ms_download = s.ms_total = -1.0
+        self.s = s
+
+    def to_dict(self) -> dict:
+        s = self.s
+        npair = s.n_pairs
+        nps = s.n_ps
+        return {
+            "ms_grid_build": s.ms_grid_build, "ms_upload": s.ms_upload, "ms_kernel": s.ms_kernel,
+            "ms_download": s.ms_download, "ms_total": s.ms_total,
+            "n_pairs": s.n_pairs, "n_sources": s.n_sources, "n_grid": s.n_grid, "n_spec": s.n_spec,
+            "traces_total": int(s.traces_total), "traces_survived": int(s.traces_survived),
+            "traces_on_sensor": int(s.traces_on_sensor), "segments": int(s.segments),
+            "pair_surf_a": self.pa[:npair].tolist(), "pair_surf_b": self.pb[:npair].tolist(),
+            "pair_traces": self.pt[:npair].tolist(), "pair_survived": self.ps[:npair].tolist(),
+            "pair_on_sensor": self.po[:npair].tolist(),
+            "ps_hits": self.hits[:nps].tolist(), "ps_rect": self.rect[: 4 * nps].tolist(),
+            "ps_budget": self.budget[:nps].tolist() if nps else [],
+        }
+
+
+def enumerate_ghost_pairs(lens: OpticalSystem):
+    """All active (a < b) surface pairs (ghost.cpp:17)."""
+    dl = lens.device_lens()
+    n = lens.num_surfaces()
+    cap = max(1, n * (n - 1) // 2)
+    out = (_abi.gl_pair * cap)()
+    k = C.c_int32()
+    _abi.check(lib().gl_enumerate_ghost_pairs(dl.handle, out, cap, C.byref(k)), "enumerate_ghost_pairs")
+    return [(out[i].surf_a, out[i].surf_b) for i in range(k.value)]
+
+
+def filter_ghost_pairs(lens: OpticalSystem, sensor_half_w: float, sensor_half_h: float,
+                       config: Optional[FlareConfig] = None, ctx=None):
+    """(pairs, area_boosts) after the IOR-contrast and intensity gates
+    (ghost.cpp:105)."""
+    config = config or FlareConfig()
+    dl = lens.device_lens()
+    n = lens.num_surfaces()
+    cap = max(1, n * (n - 1) // 2)
+    out = (_abi.gl_pair * cap)()
+    k = C.c_int32()
+    p, _keep = to_params(config)
+    _abi.check(lib().gl_filter_ghost_pairs(ctx_handle(ctx), dl.handle, float(sensor_half_w), float(sensor_half_h),
+                                           C.byref(p), out, cap, C.byref(k)), "filter_ghost_pairs")
+    return [(out[i].surf_a, out[i].surf_b) for i in range(k.value)], [out[i].area_boost for i in range(k.value)]
+
+
+def render_ghosts(lens: OpticalSystem, sources, width: int, height: int, config: FlareConfig,
+                  sensor_half_w: float, sensor_half_h: float, pairs=None, boosts=None,
+                  fmt=None, ctx=None, device_out: bool = False, stats: bool = False):
+    """launch_ghost_render (ghost_render.h:75-94) over field-angle sources.
+
+    sources: (N, 5) [angle_x, angle_y, r, g, b] (radians, output colour
+    space).  pairs/boosts: pre-filtered pairs; None runs filter_ghost_pairs.
+    fmt: (fmt_w, fmt_h, x0, y0) format window, default the full buffer.
+    Returns (r, g, b) host planes (H, W) float32, or three device pointers when
+    device_out, plus the stats dict when requested."""
+    _dims("render_ghosts", width, height)
+    if pairs is None:
+        pairs, boosts = filter_ghost_pairs(lens, sensor_half_w, sensor_half_h, config, ctx)
+    if boosts is None:
+        boosts = [1.0] * len(pairs)
+    parr = (_abi.gl_pair * max(1, len(pairs)))()
+    for i, (ab, bo) in enumerate(zip(pairs, boosts)):
+        parr[i].surf_a, parr[i].surf_b, parr[i].area_boost = int(ab[0]), int(ab[1]), float(bo)
+    sarr, n_src = _sources_array(sources)
+    fw, fh, x0, y0 = fmt if fmt is not None else (width, height, 0, 0)
+    p, _keep = to_params(config)
+    st = _Stats(max(1, len(pairs)), max(1, len(pairs) * n_src)) if stats else None
+    dl = lens.device_lens()
+    if device_out:
+        ptrs = (C.POINTER(C.c_float) * 3)()
+        _abi.check(lib().gl_launch_ghost_render(ctx_handle(ctx), dl.handle, parr, len(pairs), sarr, n_src,
+                                                float(sensor_half_w), float(sensor_half_h), width, height, fw, fh, x0,
+                                                y0, C.byref(p), None, None, None, ptrs,
+                                                C.byref(st.s) if st else None), "render_ghosts")
+        out = tuple(C.cast(ptrs[i], C.c_void_p).value for i in range(3))
+    else:
+        planes = [np.zeros((height, width), np.float32) for _ in range(3)]
+        _abi.check(lib().gl_launch_ghost_render(ctx_handle(ctx), dl.handle, parr, len(pairs), sarr, n_src,
+                                                float(sensor_half_w), float(sensor_half_h), width, height, fw, fh, x0,
+                                                y0, C.byref(p), _abi.fptr(planes[0]), _abi.fptr(planes[1]),
+                                                _abi.fptr(planes[2]), None, C.byref(st.s) if st else None),
+                   "render_ghosts")
+        out = tuple(planes)
+    return (out, st.to_dict()) if stats else out
+
+
+def trace_ghost_rays(lens: OpticalSystem, bounce_a: int, bounce_b: int, rays: np.ndarray, t_ge0: bool = True,
+                     fp64_radius_min: float = 0.0, ctx=None):
+    """Device traces of (N, 7) rays [ox, oy, oz, dx, dy, dz, lambda]; bounce_b < 0
+    traces the primary path.  Returns (pos (N,3), weight (N,), status (N,),
+    segments (N,))."""
+    r = np.ascontiguousarray(rays, np.float32).reshape(-1, 7)
+    n = len(r)
+    pos = np.zeros((n, 3), np.float32)
+    w = np.zeros(n, np.float32)
+    st = np.zeros(n, np.int32)
+    sg = np.zeros(n, np.int32)
+    dl = lens.device_lens()
+    _abi.check(lib().gl_trace_ghost_rays(ctx_handle(ctx), dl.handle, int(bounce_a), int(bounce_b), _abi.fptr(r), n,
+                                         int(bool(t_ge0)), float(fp64_radius_min), _abi.fptr(pos), _abi.fptr(w),
+                                         _abi.iptr(st), _abi.iptr(sg)), "trace_ghost_rays")
+    return pos, w, st, sg
+
+
+def solve_source_angle(lens: OpticalSystem, calib: LensCalibration, ndc_x: float, ndc_y: float,
+                       half_w: float, half_h: float):
+    """Screen NDC -> field angle.  First-order map tan(theta) = y' / f through
+    the calibrated focal lengths (the reference refines this with a traced
+    Newton solve, source_map.cpp:182; that solve is not part of this build)."""
+    fx = calib.focal_length_x or lens.focal_length
+    fy = calib.focal_length_y or lens.focal_length
+    if not (fx > 0 and fy > 0):
+        raise ValueError("calibration has no focal length; set LensCalibration.focal_length_x/y")
+    ax = math.atan(ndc_x * half_w / fx)
+    ay = math.atan(ndc_y * half_h / fy)
+    return np.float32(ax), np.float32(ay)
+
+
+def render_source_flare(offsets, width: int, height: int, lens: OpticalSystem, calibration: LensCalibration,
+                        config: PointFlareConfig, ctx=None) -> dict:
+    """Extended-source flare (SourceFlareRenderer.cpp:24-208): offsets is an
+    (N, 3) float32 array of [d_angle_x, d_angle_y, weight] rows."""
+    _dims("render_source_flare", width, height)
+    off = np.asarray(offsets, np.float32)
+    if off.ndim != 2 or off.shape[1] != 3 or off.shape[0] < 1:
+        raise ValueError("render_source_flare: offsets must be a (N, 3) float32 array of "
+                         "[d_angle_x, d_angle_y, weight] rows with N >= 1")
+    if not np.all(np.isfinite(off)):
+        raise RuntimeError("render_source_flare: HIP error: source offsets must be finite")
+    if config.gate.enabled:
+        raise NotImplementedError("render_source_flare: the film-gate pass is not part of this build")
+    if lens.num_surfaces() <= 0:
+        raise RuntimeError("render_source_flare: HIP error: lens has no surfaces")
+    hw = config.sensor_half_w if config.sensor_half_w > 0 else calibration.sensor_half_w
+    hh = config.sensor_half_h if config.sensor_half_h > 0 else calibration.sensor_half_h
+    ax0, ay0 = solve_source_angle(lens, calibration, (config.source_x - 0.5) * 2.0, (config.source_y - 0.5) * 2.0,
+                                  hw, hh)
+    Min, Mout = hostprep.color_matrices(config.input_cs, config.output_cs, config.custom_input_to_xyz,
+                                        config.custom_xyz_to_output)
+    f = np.float32
+    rgb = [f(config.source_r), f(config.source_g), f(config.source_b)]
+    xyz = [f(f(f(Min[i, 0] * rgb[0]) + f(Min[i, 1] * rgb[1])) + f(Min[i, 2] * rgb[2])) for i in range(3)]
+    src_rgb = [f(f(f(Mout[i, 0] * xyz[0]) + f(Mout[i, 1] * xyz[1])) + f(Mout[i, 2] * xyz[2])) for i in range(3)]
+    sources = np.zeros((len(off), 5), np.float32)
+    sources[:, 0] = ax0 + off[:, 0]
+    sources[:, 1] = ay0 + off[:, 1]
+    for c in range(3):
+        sources[:, 2 + c] = src_rgb[c] * off[:, 2]
+    out = {"width": int(width), "height": int(height)}
+    p, _keep = to_params(config)
+    sarr, n_src = _sources_array(sources)
+    dl = lens.device_lens()
+    planes = [np.zeros((height, width), np.float32) for _ in range(3)]
+    st = None
+    n = lens.num_surfaces()
+    max_pairs = max(1, n * (n - 1) // 2)
+    if config.collect_stats:
+        st = _Stats(max_pairs, max_pairs * n_src)
+    aov_buf = aov_pairs = None
+    aov_n = C.c_int32(0)
+    if int(config.aov_mode) == 1:
+        aov_buf = np.zeros((max_pairs, 3, height, width), np.float32)
+        aov_pairs = np.zeros(2 * max_pairs, np.int32)
+    _abi.check(lib().gl_render_ghost_pipeline(
+        ctx_handle(ctx), dl.handle, sarr, n_src, float(calibration.sensor_half_w), float(calibration.sensor_half_h),
+        width, height, width, height, 0, 0, C.byref(p), _abi.fptr(planes[0]), _abi.fptr(planes[1]),
+        _abi.fptr(planes[2]), _abi.fptr(aov_buf) if aov_buf is not None else None,
+        _abi.iptr(aov_pairs) if aov_pairs is not None else None, max_pairs if aov_buf is not None else 0,
+        C.byref(aov_n), C.byref(st.s) if st else None), "render_source_flare")
+    out["ghost_r"], out["ghost_g"], out["ghost_b"] = planes
+    if config.diffraction.starburst:
+        from . import starburst
+        out.update(starburst.render_starburst_layer(lens, calibration, sources, width, height, hw, hh, config,
+                                                    ctx=ctx))
+    if config.diffraction.veil:
+        from . import veil
+        out.update(veil.render_veil_layer(lens, calibration, sources, width, height, hw, hh, config, ctx=ctx))
+    if aov_buf is not None:
+        for i in range(aov_n.value):
+            a, b = int(aov_pairs[2 * i]), int(aov_pairs[2 * i + 1])
+            for c, ch in enumerate("rgb"):
+                out[f"ghost_s{a}_s{b}_{ch}"] = np.ascontiguousarray(aov_buf[i, c])
+    if st is not None:
+        out["stats"] = st.to_dict()
+    return out
+
+
+def render_point_flare(width: int, height: int, lens: OpticalSystem, calibration: LensCalibration,
+                       config: PointFlareConfig, ctx=None) -> dict:
+    """Single point source (PointFlareRenderer.cpp:11-20): one zero offset."""
+    _dims("render_point_flare", width, height)
+    return render_source_flare(np.array([[0.0, 0.0, 1.0]], np.float32), width, height, lens, calibration, config,
+                               ctx=ctx)

@@ -1,0 +1,306 @@
+/*
+ * ghostlight_amd.h — C ABI of the MI355X flare hot path.
+ *
+ * Every entry point is plain C: POD structs, raw pointers and sizes, an int
+ * status (GL_OK = 0) and a thread-local message from gl_last_error().  No
+ * torch, no C++ types.  The Python package `ghostlight_amd` binds this with
+ * ctypes; INTEGRATION.md shows the binding a maintainer of the reference
+ * would add.
+ *
+ * Each declaration names the reference interface it replaces
+ * (path:line relative to the reference repository root).
+ *
+ * Geometry convention (same as the reference): light travels in +z, the
+ * sensor is the plane z = 0, surfaces sit at z <= 0, surfaces[0] is the front
+ * element.  Lengths in mm, wavelengths in nm, angles in radians.
+ */
+#ifndef GHOSTLIGHT_AMD_H
+#define GHOSTLIGHT_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GL_OK                 0
+#define GL_ERR_INVALID       -1   /* bad argument (maps to Python ValueError)   */
+#define GL_ERR_DEVICE        -2   /* HIP error / no device (RuntimeError)       */
+#define GL_ERR_NO_MEMORY     -3   /* allocation or HBM-budget failure           */
+#define GL_ERR_UNSUPPORTED   -4   /* feature outside this build (§ out of scope) */
+
+#define GL_MAX_SURFACES       64
+#define GL_MAX_ASPHERE_TERMS   8
+#define GL_MAX_BLADES         32
+#define GL_MAX_SPECTRAL      128
+
+/* ---- enums (values equal the reference's, optical_system.h:54-97,
+ *      render_config.h, spectral.h:16-38) ---------------------------------- */
+enum { GL_FORM_SPHERE = 0, GL_FORM_ASPHERE = 1, GL_FORM_CYLINDRICAL = 2 };
+enum { GL_CYL_AXIS_X = 0, GL_CYL_AXIS_Y = 1 };
+enum { GL_APERTURE_CIRCLE = 0, GL_APERTURE_POLYGON = 1, GL_APERTURE_IMAGE = 2 };
+enum { GL_DISP_AIR = 0, GL_DISP_ABBE = 1, GL_DISP_SELLMEIER = 2 };
+enum { GL_COAT_SIMPLE = 0, GL_COAT_SPECTRAL = 1, GL_COAT_ANGULAR = 2,
+       GL_COAT_SPECTRAL_ANGULAR = 3, GL_COAT_ATTENUATOR_GAUSS = 4,
+       GL_COAT_ARTIST = 5 };
+enum { GL_TRACE_OK = 0, GL_TRACE_VIGNETTED = 1, GL_TRACE_TIR = 2,
+       GL_TRACE_MISSED_SURFACE = 3, GL_TRACE_INVALID_INPUT = 4 };
+enum { GL_CS_ACESCG = 0, GL_CS_SRGB_LINEAR = 1, GL_CS_P3_D65 = 2,
+       GL_CS_P3_D60 = 3, GL_CS_XYZ = 4, GL_CS_CUSTOM = 5 };
+enum { GL_CS_INPUT_ACESCG = 0, GL_CS_INPUT_SRGB_LINEAR = 1, GL_CS_INPUT_CUSTOM = 2 };
+enum { GL_AOV_NONE = 0, GL_AOV_PER_PAIR = 1 };
+enum { GL_FILTER_ALL = 0, GL_FILTER_INCLUDE = 1, GL_FILTER_EXCLUDE = 2 };
+
+/* One optical surface.  Field meaning follows `Surface` + `Coating`
+ * (optical_system.h:109-249); table-backed coatings index a float arena
+ * passed beside the surface array instead of holding raw pointers. */
+typedef struct gl_surface {
+    float   radius;           /* signed radius of curvature (0 = flat)            */
+    float   thickness;        /* axial distance to the next surface               */
+    float   ior;              /* n_d of the medium AFTER this surface             */
+    float   abbe_v;           /* V_d of the medium after (0 = air)                */
+    float   semi_aperture;    /* clear semi-diameter                              */
+    float   z;                /* world z of the vertex                            */
+    int32_t is_stop;
+    int32_t is_active;
+    int32_t aperture_shape;   /* GL_APERTURE_*                                    */
+    int32_t aperture_blades;
+    float   aperture_rotation_rad;
+    float   aperture_aspect;
+    float   decenter_x, decenter_y;
+    float   rot[9];           /* row-major local->world rotation                  */
+    int32_t form;             /* GL_FORM_*                                        */
+    float   conic_k;
+    float   asphere_terms[GL_MAX_ASPHERE_TERMS];
+    int32_t n_asphere_terms;
+    int32_t cyl_axis;
+    int32_t disp_model;       /* GL_DISP_*                                        */
+    float   sellmeier_B[3];
+    float   sellmeier_C[3];
+    /* coating */
+    int32_t coating_model;    /* GL_COAT_*                                        */
+    int32_t ar_layers;
+    int32_t out_of_range_discard;
+    float   angle_ref_ior;
+    float   tint_r, tint_g, tint_b, tint_strength;
+    float   gauss_sigma, gauss_background, gauss_peak;
+    float   gauss_decenter_x, gauss_decenter_y;
+    int32_t table_offset;     /* SPECTRAL/ANGULAR: arena[off + 2k] = key, [off+2k+1] = r */
+    int32_t table_count;
+    int32_t sa_offset;        /* SPECTRAL_ANGULAR: wl[nw] | ang[na] | r[nw*na]    */
+    int32_t sa_n_wavelengths;
+    int32_t sa_n_angles;
+} gl_surface;
+
+/* Pupil sample in normalised entrance-pupil coordinates
+ * (ApertureSample, aperture_sampler.h:21). */
+typedef struct gl_sample { float u, v; } gl_sample;
+
+/* Spectral table row (GPUSpectralSample, spectral.h:45). */
+typedef struct gl_spec { float lambda, cmf_r, cmf_g, cmf_b; } gl_spec;
+
+/* Ghost pair with its area boost (GPUPair, ghost_render.cu:122). */
+typedef struct gl_pair { int32_t surf_a, surf_b; float area_boost; } gl_pair;
+
+/* Flare source by field angle (FlareSource, ghost.h:21). */
+typedef struct gl_source { float angle_x, angle_y, r, g, b; } gl_source;
+
+/* The FlareConfig / RenderConfig fields the ghost path reads
+ * (render_config.h:14-46, 273-337).  Defaults are set by gl_flare_params_init. */
+typedef struct gl_flare_params {
+    /* RenderConfig */
+    int32_t ray_grid;
+    int32_t spectral_samples;
+    int32_t pupil_jitter;
+    int32_t jitter_seed;
+    int32_t sensor_model;
+    int32_t input_cs;
+    int32_t output_cs;
+    float   custom_input_to_xyz[9];
+    float   custom_xyz_to_output[9];
+    float   sensor_half_w;          /* 0 = use the calibrated value               */
+    float   sensor_half_h;
+    int32_t verbose;
+    int32_t collect_stats;
+    int32_t spectral_correction;
+    /* FlareConfig */
+    int32_t aperture_blades;
+    float   aperture_rotation;      /* degrees                                    */
+    float   flare_gain;
+    float   min_ghost_intensity;
+    int32_t ghost_normalize;
+    float   max_area_boost;
+    int32_t cull_dead_pairs;
+    int32_t concentrate_samples;
+    int32_t adaptive_sample_budgets;
+    float   adaptive_density_boost;
+    int32_t adaptive_min_samples;
+    int32_t aov_mode;
+    int32_t aov_max_pairs;
+    int32_t ghost_filter_mode;      /* GL_FILTER_*                                */
+    int32_t ghost_filter_count;
+    const int32_t* ghost_filter_pairs;  /* 2*count ints (surf_a, surf_b)          */
+    /* Build-specific precision knob: spherical surfaces with |R| at or above
+     * this use an FP64 ray-sphere solve (sag cancellation); <= 0 disables. */
+    float   fp64_radius_min;
+} gl_flare_params;
+
+/* Per-render diagnostics (GhostRenderStats, flare_buffers.h:21-74).  Arrays
+ * are caller-owned; *_cap gives their capacity, n_pairs / n_ps the fill. */
+typedef struct gl_ghost_stats {
+    float    ms_grid_build, ms_upload, ms_kernel, ms_download, ms_total;
+    int32_t  n_pairs, n_sources, n_grid, n_spec;
+    uint64_t traces_total, traces_survived, traces_on_sensor;
+    uint64_t segments;              /* executed surface segments, main render    */
+    int32_t  pair_cap;
+    int32_t* pair_surf_a;
+    int32_t* pair_surf_b;
+    uint64_t* pair_traces;
+    uint64_t* pair_survived;
+    uint64_t* pair_on_sensor;
+    int32_t  ps_cap;
+    int32_t  n_ps;
+    int32_t* ps_hits;
+    float*   ps_rect;               /* 4 floats per (pair, source)               */
+    int32_t* ps_budget;
+} gl_ghost_stats;
+
+/* Opaque handles. */
+typedef struct gl_ctx  gl_ctx;   /* one HIP stream + its grow-only buffers        */
+typedef struct gl_lens gl_lens;  /* device-resident prescription                  */
+
+/* ---- runtime ------------------------------------------------------------ */
+
+/* Replaces `_cuda_available` (bind_renderers.cpp:518-521): 1 when a gfx950
+ * device is usable, else 0. */
+int  gl_device_available(void);
+
+/* Message for the last non-zero status on this thread (never NULL). */
+const char* gl_last_error(void);
+
+/* A render context: one HIP stream (default priority) and its buffers.  The
+ * reference serialises every render behind process mutexes
+ * (ghost_render.cu:504,656); contexts let independent renders overlap.
+ * ctx = NULL in any call below means the process default context. */
+int  gl_ctx_create(gl_ctx** out);
+void gl_ctx_destroy(gl_ctx* ctx);
+int  gl_ctx_synchronize(gl_ctx* ctx);
+
+/* Device time (ms, HIP events on the context's stream) of the last main
+ * ghost trace + splat kernel run on ctx.  Build-specific instrumentation for
+ * the roofline report; no counterpart in the reference. */
+float gl_ctx_last_kernel_ms(gl_ctx* ctx);
+
+/* Device-resident lens (replaces the per-launch surface + coating-arena
+ * upload, ghost_render.cu:919-954, coating_upload.cu:46-148). */
+int  gl_lens_create(const gl_surface* surfaces, int32_t n_surfaces,
+                    const float* arena, int32_t arena_len, gl_lens** out);
+void gl_lens_destroy(gl_lens* lens);
+
+/* ---- host preparation (bit-exact restatements of the reference's CPU code) */
+
+/* gl_flare_params defaults = FlareConfig defaults (render_config.h:14-337). */
+void gl_flare_params_init(gl_flare_params* p);
+
+/* build_spectral_table (spectral.cpp:317) with resolve_output_matrix
+ * (spectral.cpp:272); n == 3 gives the fixed RGB table
+ * (ghost_render.cu:777-785).  Writes n rows. */
+int  gl_build_spectral_table(int32_t n, int32_t output_cs,
+                             const float custom_xyz_to_output[9], gl_spec* out);
+
+/* compute_spectral_correction (spectral.cpp:419). */
+int  gl_spectral_correction(const gl_spec* table, int32_t n, int32_t output_cs,
+                            const float custom_xyz_to_output[9], float C_out[9]);
+
+/* resolve_input_matrix / resolve_output_matrix (spectral.cpp:203, 272). */
+int  gl_color_matrices(int32_t input_cs, const float custom_in[9],
+                       int32_t output_cs, const float custom_out[9],
+                       float M_in[9], float M_out[9]);
+
+/* build_aperture_samples (aperture_sampler.h:181).  Writes at most cap
+ * samples, *n_out = count. */
+int  gl_build_aperture_samples(const gl_surface* surfaces, int32_t n_surfaces,
+                               int32_t ray_grid, int32_t pupil_jitter,
+                               int32_t jitter_seed, int32_t blades_override,
+                               float rotation_deg, gl_sample* out, int32_t cap,
+                               int32_t* n_out);
+
+/* ior_at / ior_before tables [lambda][surface]
+ * (ghost_render.cu:809-823; optical_system.h:230, 385-402). */
+int  gl_build_ior_tables(const gl_surface* surfaces, int32_t n_surfaces,
+                         const gl_spec* spec, int32_t n_spec,
+                         float* ior_at, float* ior_before);
+
+/* Bake a thin-film stack to a SPECTRAL_ANGULAR table, 31 lambda x 18 angles
+ * (OpticalSystem::bake_coating_layers, optical_system.cpp:1483-1570, with
+ * tmm::reflectance, tmm.h:89).  Layer i: complex index (n_re[i], n_im[i]) at
+ * every wavelength, thickness_nm[i]; n_ambient/n_substrate per wavelength
+ * (31 each).  out: wl[31] | ang[18] | r[31*18]. */
+int  gl_bake_tmm_table(int32_t n_layers, const float* layer_n_re,
+                       const float* layer_n_im, const float* thickness_nm,
+                       const float* n_ambient31, const float* n_substrate31,
+                       float* out);
+
+/* ---- ghost path -------------------------------------------------------- */
+
+/* enumerate_ghost_pairs (ghost.cpp:17): all active a < b.  Writes at most
+ * cap pairs (area_boost = 1). */
+int  gl_enumerate_ghost_pairs(const gl_lens* lens, gl_pair* out, int32_t cap,
+                              int32_t* n_out);
+
+/* filter_ghost_pairs (ghost.cpp:105): IOR-contrast gate, on-axis intensity
+ * gate and area boost.  The 3 + 64 probe traces per pair run on the device. */
+int  gl_filter_ghost_pairs(gl_ctx* ctx, const gl_lens* lens,
+                           float sensor_half_w, float sensor_half_h,
+                           const gl_flare_params* p,
+                           gl_pair* out, int32_t cap, int32_t* n_out);
+
+/* launch_ghost_render (ghost_render.h:75-94): probe / cull / concentrate /
+ * adaptive budgets, then the ghost trace + splat and the spectral correction.
+ * out_r/g/b: host planes (width*height floats), overwritten.  Pass NULL
+ * host planes and non-NULL dev_out to keep the result on the device
+ * (dev_out receives 3 device pointers owned by ctx, valid until the next
+ * call on ctx).  stats may be NULL. */
+int  gl_launch_ghost_render(gl_ctx* ctx, const gl_lens* lens,
+                            const gl_pair* pairs, int32_t n_pairs,
+                            const gl_source* sources, int32_t n_sources,
+                            float sensor_half_w, float sensor_half_h,
+                            int32_t width, int32_t height,
+                            int32_t fmt_w, int32_t fmt_h,
+                            int32_t fmt_x0_in_buf, int32_t fmt_y0_in_buf,
+                            const gl_flare_params* p,
+                            float* out_r, float* out_g, float* out_b,
+                            float** dev_out, gl_ghost_stats* stats);
+
+/* render_ghost_pipeline (ghost_render.cu:1448-1581): filter_ghost_pairs,
+ * GhostFilter include/exclude, then launch_ghost_render.  In PER_PAIR AOV
+ * mode aov_out receives n_layers x 3 planes (layer-major, r|g|b) and the
+ * pair of each layer in aov_pairs (2 ints per layer); both caller-owned with
+ * room for aov_cap layers, *aov_n = layers written. */
+int  gl_render_ghost_pipeline(gl_ctx* ctx, const gl_lens* lens,
+                              const gl_source* sources, int32_t n_sources,
+                              float calib_half_w, float calib_half_h,
+                              int32_t width, int32_t height,
+                              int32_t fmt_w, int32_t fmt_h,
+                              int32_t fmt_x0_in_buf, int32_t fmt_y0_in_buf,
+                              const gl_flare_params* p,
+                              float* out_r, float* out_g, float* out_b,
+                              float* aov_out, int32_t* aov_pairs,
+                              int32_t aov_cap, int32_t* aov_n,
+                              gl_ghost_stats* stats);
+
+/* Single-ray traces through the device tracer for tests and Designer-style
+ * probes (trace_ghost_ray / trace_primary_ray, trace.h; GPU semantics of
+ * trace_cuda.h:112,197).  rays: n x 7 floats (ox,oy,oz,dx,dy,dz,lambda).
+ * out_pos: n x 3, out_w: n, out_status: n, out_segments: n (may be NULL).
+ * t_ge0 = 1 selects the CPU tracer's t >= 0 sensor rule (trace.cpp:155). */
+int  gl_trace_ghost_rays(gl_ctx* ctx, const gl_lens* lens, int32_t bounce_a,
+                         int32_t bounce_b, const float* rays, int32_t n,
+                         int32_t t_ge0, float fp64_radius_min,
+                         float* out_pos, float* out_w, int32_t* out_status,
+                         int32_t* out_segments);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* GHOSTLIGHT_AMD_H */

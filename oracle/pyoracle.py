@@ -1,0 +1,190 @@
+"""ORACLE — test infrastructure only.  ctypes wrappers over the CPU checker
+(oracle/_build/liboracle.so, this project's restatement) and the reference
+build (oracle/_ref/libglref.so, present where the reference sources were
+available to compile).  Imported by tests/, __graft_entry__.smoke() and
+bench.py's cpu_baseline leg; never by the product package."""
+from __future__ import annotations
+
+import ctypes as C
+import math
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ORACLE_SO = os.path.join(HERE, "_build", "liboracle.so")
+REF_SO = os.path.join(HERE, "_ref", "libglref.so")
+
+_orc = None
+_ref = None
+
+
+def lib():
+    global _orc
+    if _orc is None:
+        if not os.path.exists(ORACLE_SO):
+            raise ImportError(f"oracle not built: {ORACLE_SO} (run make -C oracle)")
+        _orc = C.CDLL(ORACLE_SO)
+    return _orc
+
+
+def ref():
+    """The reference's CPU code, or None where oracle/_ref was not built."""
+    global _ref
+    if _ref is None and os.path.exists(REF_SO):
+        _ref = C.CDLL(REF_SO)
+        _ref.ref_tmm_reflectance.restype = C.c_float
+    return _ref
+
+
+def _f(a):
+    return a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+def _i(a):
+    return a.ctypes.data_as(C.POINTER(C.c_int32))
+
+
+def ior_tables(surfs, n, spec):
+    spec = np.ascontiguousarray(spec, np.float32).reshape(-1, 4)
+    at = np.zeros((len(spec), n), np.float32)
+    bf = np.zeros((len(spec), n), np.float32)
+    lib().orc_ior_tables(surfs, n, _f(spec), len(spec), _f(at), _f(bf))
+    return at, bf
+
+
+def trace_ghost(lens, a, b, rays, dbl=False, t_ge0=True, fp64_radius_min=0.0):
+    """Trace (N, 7) rays; returns pos (N,3) f64, w (N,), status (N,), segs (N,)."""
+    surfs, arena = lens.pack()
+    n = lens.num_surfaces()
+    rays = np.ascontiguousarray(rays, np.float32).reshape(-1, 7)
+    N = len(rays)
+    pos = np.zeros((N, 3)); w = np.zeros(N); st = np.zeros(N, np.int32); sg = np.zeros(N, np.int32)
+    L = lib()
+    for i in range(N):
+        spec = np.array([[rays[i, 6], 0, 0, 0]], np.float32)
+        at, bf = ior_tables(surfs, n, spec)
+        op = (C.c_double * 3)(); ow = C.c_double(); ost = C.c_int(); seg = C.c_int()
+        if b >= 0:
+            L.orc_trace_ghost(int(dbl), surfs, n, _f(arena), int(a), int(b), _f(rays[i]), _f(at), _f(bf), int(t_ge0),
+                              C.c_float(fp64_radius_min), op, C.byref(ow), C.byref(ost), C.byref(seg))
+        else:
+            L.orc_trace_primary(int(dbl), surfs, n, _f(arena), _f(rays[i]), _f(at), int(t_ge0),
+                                C.c_float(fp64_radius_min), op, C.byref(ow), C.byref(ost))
+        pos[i] = op[:]; w[i] = ow.value; st[i] = ost.value; sg[i] = seg.value
+    return pos, w, st, sg
+
+
+def build_samples(lens, N, jitter=0, seed=0, blades=0, rot_deg=0.0):
+    surfs, _ = lens.pack()
+    from ghostlight_amd import _abi
+    out = (_abi.gl_sample * max(1, N * N))()
+    k = C.c_int()
+    lib().orc_build_samples(surfs, lens.num_surfaces(), N, jitter, seed, blades, C.c_float(rot_deg), out, N * N,
+                            C.byref(k))
+    return np.frombuffer(bytes(out), np.float32).reshape(-1, 2)[: k.value].copy()
+
+
+def filter_ghost_pairs(lens, half_w, half_h, min_intensity=1e-7, normalize=True, max_boost=100.0):
+    from ghostlight_amd import _abi
+    surfs, arena = lens.pack()
+    n = lens.num_surfaces()
+    cap = max(1, n * n)
+    out = (_abi.gl_pair * cap)()
+    k = C.c_int()
+    lib().orc_filter_ghost_pairs(surfs, n, _f(arena), C.c_float(half_w), C.c_float(half_h), C.c_float(min_intensity),
+                                 int(normalize), C.c_float(max_boost), out, cap, C.byref(k))
+    return [(out[i].surf_a, out[i].surf_b) for i in range(k.value)], [out[i].area_boost for i in range(k.value)]
+
+
+def source_bxy(sources):
+    """tan(angle) rounded once to float, as the product computes it."""
+    s = np.asarray(sources, np.float32).reshape(-1, 5)
+    return np.array([[np.float32(math.tan(float(a))), np.float32(math.tan(float(b)))] for a, b in s[:, :2]],
+                    np.float32).reshape(-1, 2)
+
+
+def render_ghosts(lens, pairs, boosts, sources, spec, params, half_w, half_h, width, height, fmt=None,
+                  correction=None, dbl=False, threads=0, want_masks=False):
+    """launch_ghost_render semantics on the CPU.  params: gl_flare_params.
+    Returns (r, g, b) planes (H, W) float32 and a dict with the segment count
+    (+ cull mask, rects, budgets when want_masks)."""
+    from ghostlight_amd import _abi
+    surfs, arena = lens.pack()
+    n = lens.num_surfaces()
+    np_ = len(pairs)
+    parr = (_abi.gl_pair * max(1, np_))()
+    for i, ((a, b), bo) in enumerate(zip(pairs, boosts)):
+        parr[i].surf_a, parr[i].surf_b, parr[i].area_boost = a, b, bo
+    src = np.ascontiguousarray(sources, np.float32).reshape(-1, 5)
+    sarr = (_abi.gl_source * max(1, len(src)))()
+    C.memmove(sarr, src.ctypes.data, src.nbytes)
+    bxy = source_bxy(src)
+    spec = np.ascontiguousarray(spec, np.float32).reshape(-1, 4)
+    fw, fh, x0, y0 = fmt if fmt is not None else (width, height, 0, 0)
+    planes = [np.zeros((height, width), np.float32) for _ in range(3)]
+    counters = np.zeros(4, np.uint64)
+    n_ps = max(1, np_ * len(src))
+    active = np.zeros(max(1, np_), np.uint8)
+    rects = np.zeros(4 * n_ps, np.float32)
+    budgets = np.zeros(2 * n_ps, np.int32)
+    Cm = None if correction is None else np.ascontiguousarray(correction, np.float32).reshape(9)
+    lib().orc_render_ghosts(int(dbl), surfs, n, _f(arena), parr, np_, sarr, _f(bxy), len(src), _f(spec), len(spec),
+                            C.byref(params), C.c_float(half_w), C.c_float(half_h), width, height, fw, fh, x0, y0,
+                            _f(Cm) if Cm is not None else None, _f(planes[0]), _f(planes[1]), _f(planes[2]),
+                            counters.ctypes.data_as(C.POINTER(C.c_uint64)),
+                            active.ctypes.data_as(C.POINTER(C.c_uint8)), _f(rects), _i(budgets), int(threads))
+    info = {"segments": int(counters[3])}
+    if want_masks:
+        info.update(pair_active=active[:np_].copy(), rects=rects.reshape(-1, 4)[: np_ * len(src)].copy(),
+                    budgets=budgets.reshape(-1, 2)[: np_ * len(src)].copy())
+    return tuple(planes), info
+
+
+def ghost_stats(lens, pairs, boosts, sources, spec, params, half_w, half_h, width, height, threads=0):
+    """Per-pair traces / survived / on-sensor over the full grid + segments."""
+    from ghostlight_amd import _abi
+    surfs, arena = lens.pack()
+    n = lens.num_surfaces()
+    np_ = len(pairs)
+    parr = (_abi.gl_pair * max(1, np_))()
+    for i, ((a, b), bo) in enumerate(zip(pairs, boosts)):
+        parr[i].surf_a, parr[i].surf_b, parr[i].area_boost = a, b, bo
+    src = np.ascontiguousarray(sources, np.float32).reshape(-1, 5)
+    sarr = (_abi.gl_source * max(1, len(src)))()
+    C.memmove(sarr, src.ctypes.data, src.nbytes)
+    bxy = source_bxy(src)
+    spec = np.ascontiguousarray(spec, np.float32).reshape(-1, 4)
+    out = np.zeros(3 * max(1, np_), np.uint64)
+    segs = C.c_uint64()
+    lib().orc_ghost_stats(surfs, n, _f(arena), parr, np_, sarr, _f(bxy), len(src), _f(spec), len(spec),
+                          C.byref(params), C.c_float(half_w), C.c_float(half_h), width, height,
+                          out.ctypes.data_as(C.POINTER(C.c_uint64)), C.byref(segs), int(threads))
+    return out.reshape(3, -1)[:, :np_], int(segs.value)
+
+
+def rel_l2(a, b) -> float:
+    a = np.concatenate([np.asarray(x, np.float64).ravel() for x in a])
+    b = np.concatenate([np.asarray(x, np.float64).ravel() for x in b])
+    d = np.linalg.norm(b)
+    return float(np.linalg.norm(a - b) / d) if d > 0 else float(np.linalg.norm(a - b))
+
+
+class PackedLens:
+    """A prescription given as the packed C arrays (e.g. from a fixture)."""
+
+    def __init__(self, surf_bytes, arena):
+        from ghostlight_amd import _abi
+        raw = np.asarray(surf_bytes, np.uint8).tobytes()
+        n = len(raw) // C.sizeof(_abi.gl_surface)
+        self._surfs = (_abi.gl_surface * n).from_buffer_copy(raw)
+        self._arena = np.ascontiguousarray(arena, np.float32)
+        self._n = n
+        s0 = self._surfs[0]
+        self.front_R, self.z0 = s0.semi_aperture, s0.z
+
+    def pack(self):
+        return self._surfs, self._arena
+
+    def num_surfaces(self):
+        return self._n

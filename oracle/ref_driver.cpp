@@ -1,0 +1,184 @@
+// ORACLE — test infrastructure only.  A C-ABI shim, written for this project,
+// over the reference's own CPU functions.  oracle/Makefile compiles it together
+// with the reference sources where they lie (/root/reference/ghostlight/src)
+// into oracle/_ref/libglref.so; nothing here is shipped or called by the
+// product.  Tests use it to pin the oracle restatement and the product's host
+// preparation bit-for-bit against the reference.
+#include "optical_system.h"
+#include "trace.h"
+#include "ghost.h"
+#include "spectral.h"
+#include "aperture_sampler.h"
+#include "tmm.h"
+#include "lens_calibration.h"
+
+#include "../include/ghostlight_amd.h"
+
+#include <cstring>
+#include <vector>
+
+namespace {
+
+OpticalSystem make_lens(const gl_surface* s, int n, const float* arena) {
+    OpticalSystem L;
+    L.surfaces.resize(n);
+    L.coating_tables.resize(n);
+    L.aperture_images.resize(n);
+    L.surface_ids.resize(n);
+    for (int i = 0; i < n; ++i) {
+        Surface& d = L.surfaces[i];
+        const gl_surface& g = s[i];
+        d = Surface{};
+        d.radius = g.radius; d.thickness = g.thickness; d.ior = g.ior; d.abbe_v = g.abbe_v;
+        d.semi_aperture = g.semi_aperture; d.z = g.z;
+        d.is_stop = g.is_stop != 0; d.is_active = g.is_active != 0;
+        d.aperture_shape = g.aperture_shape; d.aperture_blades = g.aperture_blades;
+        d.aperture_rotation_rad = g.aperture_rotation_rad; d.aperture_aspect = g.aperture_aspect;
+        d.decenter_x = g.decenter_x; d.decenter_y = g.decenter_y;
+        std::memcpy(d.rot, g.rot, sizeof d.rot);
+        d.form = g.form; d.conic_k = g.conic_k;
+        std::memcpy(d.asphere_terms, g.asphere_terms, sizeof d.asphere_terms);
+        d.n_asphere_terms = g.n_asphere_terms; d.cyl_axis = g.cyl_axis;
+        d.disp_model = g.disp_model;
+        std::memcpy(d.sellmeier_B, g.sellmeier_B, sizeof d.sellmeier_B);
+        std::memcpy(d.sellmeier_C, g.sellmeier_C, sizeof d.sellmeier_C);
+        Coating& c = d.coating;
+        c.model = (CoatingModel)g.coating_model;
+        c.ar_layers = g.ar_layers;
+        c.out_of_range_discard = g.out_of_range_discard != 0;
+        c.angle_ref_ior = g.angle_ref_ior;
+        c.tint_r = g.tint_r; c.tint_g = g.tint_g; c.tint_b = g.tint_b; c.tint_strength = g.tint_strength;
+        c.gauss_sigma = g.gauss_sigma; c.gauss_background = g.gauss_background; c.gauss_peak = g.gauss_peak;
+        c.gauss_decenter_x = g.gauss_decenter_x; c.gauss_decenter_y = g.gauss_decenter_y;
+        CoatingTables& t = L.coating_tables[i];
+        if (arena && g.table_count > 0)
+            for (int k = 0; k < g.table_count; ++k)
+                t.table.push_back({arena[g.table_offset + 2 * k], arena[g.table_offset + 2 * k + 1]});
+        if (arena && g.sa_n_wavelengths > 0 && g.sa_n_angles > 0) {
+            const float* wl = arena + g.sa_offset;
+            t.sa_wavelengths.assign(wl, wl + g.sa_n_wavelengths);
+            t.sa_angles.assign(wl + g.sa_n_wavelengths, wl + g.sa_n_wavelengths + g.sa_n_angles);
+            const float* r = wl + g.sa_n_wavelengths + g.sa_n_angles;
+            t.sa_r.assign(r, r + (size_t)g.sa_n_wavelengths * g.sa_n_angles);
+        }
+        d.refresh_aperture_profile();
+    }
+    L.sync_coating_pointers();
+    return L;
+}
+
+void put(const TraceResult& r, float* pos, float* w, int* st) {
+    pos[0] = r.position.x; pos[1] = r.position.y; pos[2] = r.position.z;
+    *w = r.weight; *st = (int)r.status;
+}
+
+}  // namespace
+
+extern "C" {
+
+// trace_ghost_ray (trace.cpp:171) for n rays (7 floats each).
+int ref_trace_ghost_rays(const gl_surface* s, int n, const float* arena, int a, int b,
+                         const float* rays, int n_rays, float* pos, float* w, int* st) {
+    OpticalSystem L = make_lens(s, n, arena);
+    for (int i = 0; i < n_rays; ++i) {
+        Ray r; const float* q = rays + 7 * i;
+        r.origin = Vec3f(q[0], q[1], q[2]); r.dir = Vec3f(q[3], q[4], q[5]); r.lambda = q[6];
+        put(trace_ghost_ray(r, L, a, b), pos + 3 * i, w + i, st + i);
+    }
+    return 0;
+}
+
+// trace_primary_ray (trace.cpp:243).
+int ref_trace_primary_rays(const gl_surface* s, int n, const float* arena, const float* rays, int n_rays,
+                           float* pos, float* w, int* st) {
+    OpticalSystem L = make_lens(s, n, arena);
+    for (int i = 0; i < n_rays; ++i) {
+        Ray r; const float* q = rays + 7 * i;
+        r.origin = Vec3f(q[0], q[1], q[2]); r.dir = Vec3f(q[3], q[4], q[5]); r.lambda = q[6];
+        put(trace_primary_ray(r, L), pos + 3 * i, w + i, st + i);
+    }
+    return 0;
+}
+
+// filter_ghost_pairs (ghost.cpp:105).
+int ref_filter_ghost_pairs(const gl_surface* s, int n, const float* arena, float hw, float hh,
+                           float min_intensity, int normalize, float max_boost, gl_pair* out, int cap, int* n_out) {
+    OpticalSystem L = make_lens(s, n, arena);
+    FlareConfig cfg;
+    cfg.min_ghost_intensity = min_intensity; cfg.ghost_normalize = normalize != 0; cfg.max_area_boost = max_boost;
+    std::vector<GhostPair> pairs; std::vector<float> boosts;
+    filter_ghost_pairs(L, hw, hh, cfg, pairs, boosts);
+    *n_out = (int)pairs.size();
+    for (size_t i = 0; i < pairs.size() && (int)i < cap; ++i) out[i] = gl_pair{pairs[i].surf_a, pairs[i].surf_b, boosts[i]};
+    return 0;
+}
+
+// build_spectral_table (spectral.cpp:317) + resolve_output_matrix (:272).
+int ref_build_spectral_table(int n, int output_cs, const float* custom9, gl_spec* out) {
+    float custom[3][3] = {}; if (custom9) std::memcpy(custom, custom9, sizeof custom);
+    float M[3][3];
+    resolve_output_matrix((OutputColorSpace)output_cs, custom, M);
+    std::vector<GPUSpectralSample> t = build_spectral_table(n, 400.0f, 700.0f, get_sensor_profile(SENSOR_CIE_1931), M);
+    for (int i = 0; i < n; ++i) out[i] = gl_spec{t[i].lambda, t[i].cmf_r, t[i].cmf_g, t[i].cmf_b};
+    return 0;
+}
+
+// compute_spectral_correction (spectral.cpp:419).
+int ref_spectral_correction(const gl_spec* tab, int n, int output_cs, const float* custom9, float* C9) {
+    float custom[3][3] = {}; if (custom9) std::memcpy(custom, custom9, sizeof custom);
+    float M[3][3];
+    resolve_output_matrix((OutputColorSpace)output_cs, custom, M);
+    std::vector<GPUSpectralSample> t(n);
+    for (int i = 0; i < n; ++i) t[i] = GPUSpectralSample{tab[i].lambda, tab[i].cmf_r, tab[i].cmf_g, tab[i].cmf_b};
+    float C[3][3];
+    compute_spectral_correction(t, 400.0f, 700.0f, get_sensor_profile(SENSOR_CIE_1931), M, C);
+    std::memcpy(C9, C, sizeof C);
+    return 0;
+}
+
+// resolve_input_matrix / resolve_output_matrix (spectral.cpp:203,272).
+int ref_color_matrices(int in_cs, const float* cin9, int out_cs, const float* cout9, float* Min, float* Mout) {
+    float ci[3][3] = {}, co[3][3] = {};
+    if (cin9) std::memcpy(ci, cin9, sizeof ci);
+    if (cout9) std::memcpy(co, cout9, sizeof co);
+    float A[3][3], B[3][3];
+    resolve_input_matrix((InputColorSpace)in_cs, ci, A);
+    resolve_output_matrix((OutputColorSpace)out_cs, co, B);
+    std::memcpy(Min, A, sizeof A); std::memcpy(Mout, B, sizeof B);
+    return 0;
+}
+
+// build_aperture_samples (aperture_sampler.h:181).
+int ref_build_aperture_samples(const gl_surface* s, int n, int ray_grid, int jitter, int seed, int blades,
+                               float rot_deg, gl_sample* out, int cap, int* n_out) {
+    OpticalSystem L = make_lens(s, n, nullptr);
+    ApertureSamplerParams p;
+    p.ray_grid = ray_grid; p.pupil_jitter = jitter; p.jitter_seed = seed;
+    p.blades_override = blades; p.rotation_deg = rot_deg;
+    std::vector<ApertureSample> g = build_aperture_samples(L, p);
+    *n_out = (int)g.size();
+    for (size_t i = 0; i < g.size() && (int)i < cap; ++i) out[i] = gl_sample{g[i].u, g[i].v};
+    return 0;
+}
+
+// tmm::reflectance (tmm.h:89) for one (lambda, aoi).
+float ref_tmm_reflectance(int n_layers, const float* n_re, const float* n_im, const float* thick,
+                          float n_amb, float n_sub, float lambda_nm, float aoi_deg) {
+    std::vector<tmm::Layer> layers(n_layers);
+    for (int i = 0; i < n_layers; ++i) { layers[i].n = {n_re[i], n_im[i]}; layers[i].thickness_nm = thick[i]; }
+    return tmm::reflectance(layers, {n_amb, 0.0f}, {n_sub, 0.0f}, lambda_nm, aoi_deg);
+}
+
+// calibrate_lens (lens_calibration.cpp:344): 13 floats in LensCalibration order.
+int ref_calibrate_lens(const gl_surface* s, int n, const float* arena, float* out13) {
+    OpticalSystem L = make_lens(s, n, arena);
+    LensCalibration c = calibrate_lens(L);
+    const float v[13] = {c.sensor_half_w, c.sensor_half_h, c.max_half_angle_h, c.max_half_angle_v,
+                         c.image_circle_semi_w, c.image_circle_semi_h, c.focal_length_x, c.focal_length_y,
+                         c.entrance_pupil_semi_x, c.entrance_pupil_semi_y, c.f_number_x, c.f_number_y,
+                         c.pupil_area_frac};
+    std::memcpy(out13, v, sizeof v);
+    return 0;
+}
+
+}  // extern "C"

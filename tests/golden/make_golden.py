@@ -1,0 +1,126 @@
+"""Write the golden vectors in tests/golden/ from the reference's own CPU code.
+
+Runs only where oracle/_ref/libglref.so was built from the reference sources
+(oracle/Makefile `ref`).  The fixtures are data (inputs and the reference's
+outputs); they pin the oracle and the product's host preparation where the
+reference sources are not available (e.g. on the GPU box).
+
+    python tests/golden/make_golden.py
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+
+from ghostlight_amd import _abi, synthetic  # noqa: E402
+from oracle import pyoracle  # noqa: E402
+
+
+def rays_for(lens, n, seed, max_deg=8.0):
+    rng = np.random.default_rng(seed)
+    fR = lens.surfaces[0].semi_aperture
+    z0 = lens.surfaces[0].z - 20.0
+    ang = np.radians(rng.uniform(-max_deg, max_deg, (n, 2)))
+    d = np.stack([np.tan(ang[:, 0]), np.tan(ang[:, 1]), np.ones(n)], 1)
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    rays = np.zeros((n, 7), np.float32)
+    rays[:, :2] = rng.uniform(-1, 1, (n, 2)) * fR
+    rays[:, 2] = z0
+    rays[:, 3:6] = d
+    rays[:, 6] = rng.uniform(400, 700, n)
+    return rays
+
+
+def main():
+    ref = pyoracle.ref()
+    if ref is None:
+        raise SystemExit("oracle/_ref/libglref.so not built (needs the reference sources)")
+    out = {}
+    # Spectral tables + correction matrices (spectral.cpp:317, :419).
+    for n in (8, 16, 31, 64):
+        for cs in (0, 1, 2, 3, 4):
+            tab = (_abi.gl_spec * n)()
+            ref.ref_build_spectral_table(n, cs, None, tab)
+            t = np.frombuffer(bytes(tab), np.float32).reshape(n, 4).copy()
+            C9 = (C.c_float * 9)()
+            ref.ref_spectral_correction(tab, n, cs, None, C9)
+            out[f"spec_n{n}_cs{cs}"] = t
+            out[f"corr_n{n}_cs{cs}"] = np.array(C9[:], np.float32)
+    # Colour matrices.
+    for ics in (0, 1):
+        for ocs in (0, 1, 2, 3, 4):
+            Mi, Mo = (C.c_float * 9)(), (C.c_float * 9)()
+            ref.ref_color_matrices(ics, None, ocs, None, Mi, Mo)
+            out[f"cmat_in{ics}_out{ocs}"] = np.array(Mi[:] + Mo[:], np.float32)
+    lenses = {"c1": synthetic.config1_lens(), "c2": synthetic.config2_lens(), "c4": synthetic.config4_lens()}
+    for name, lens in lenses.items():
+        surfs, arena = lens.pack()
+        n = lens.num_surfaces()
+        out[f"lens_{name}_surfs"] = np.frombuffer(bytes(surfs), np.uint8).copy()
+        out[f"lens_{name}_arena"] = arena
+        # Pupil samples (aperture_sampler.h:181) for the render / probe / refine grids.
+        for (N, j, seed, bo, rd) in [(64, 0, 0, 0, 0.0), (64, 1, 3, 0, 0.0), (64, 2, 0, 0, 0.0),
+                                     (40, 1, int(np.uint32(0x9E3779B9).astype(np.int32)), 0, 0.0),
+                                     (128, 1, int(np.uint32(0xB5297A4D).astype(np.int32)), 0, 0.0),
+                                     (48, 0, 0, 5, 12.0)]:
+            cap = N * N
+            o = (_abi.gl_sample * cap)()
+            k = C.c_int()
+            ref.ref_build_aperture_samples(surfs, n, N, j, seed, bo, C.c_float(rd), o, cap, C.byref(k))
+            out[f"samples_{name}_N{N}_j{j}_s{seed}_b{bo}"] = np.frombuffer(bytes(o), np.float32).reshape(-1, 2)[: k.value].copy()
+        # Pair pre-filter (ghost.cpp:105).
+        hw, hh = synthetic.sensor_for(lens, 1920, 1080)
+        cap = n * n
+        o = (_abi.gl_pair * cap)()
+        k = C.c_int()
+        ref.ref_filter_ghost_pairs(surfs, n, pyoracle._f(arena), C.c_float(hw), C.c_float(hh), C.c_float(1e-7), 1,
+                                   C.c_float(100.0), o, cap, C.byref(k))
+        out[f"filter_{name}_pairs"] = np.array([[o[i].surf_a, o[i].surf_b] for i in range(k.value)], np.int32)
+        out[f"filter_{name}_boosts"] = np.array([o[i].area_boost for i in range(k.value)], np.float32)
+        out[f"filter_{name}_sensor"] = np.array([hw, hh], np.float32)
+        # Single-ray ghost traces (trace.cpp:171) through a spread of pairs.
+        rays = rays_for(lens, 1000, seed=11)
+        out[f"rays_{name}"] = rays
+        pair_list = [(0, 1), (1, n - 1), (2, n // 2), (n // 3, n - 2), (n - 3, n - 1)]
+        out[f"trace_{name}_pairs"] = np.array(pair_list, np.int32)
+        for (a, b) in pair_list:
+            pos = np.zeros((len(rays), 3), np.float32)
+            w = np.zeros(len(rays), np.float32)
+            st = np.zeros(len(rays), np.int32)
+            ref.ref_trace_ghost_rays(surfs, n, pyoracle._f(arena), a, b, pyoracle._f(rays), len(rays),
+                                     pyoracle._f(pos), pyoracle._f(w), pyoracle._i(st))
+            out[f"trace_{name}_{a}_{b}_pos"] = pos
+            out[f"trace_{name}_{a}_{b}_w"] = w
+            out[f"trace_{name}_{a}_{b}_st"] = st
+        pos = np.zeros((len(rays), 3), np.float32)
+        w = np.zeros(len(rays), np.float32)
+        st = np.zeros(len(rays), np.int32)
+        ref.ref_trace_primary_rays(surfs, n, pyoracle._f(arena), pyoracle._f(rays), len(rays), pyoracle._f(pos),
+                                   pyoracle._f(w), pyoracle._i(st))
+        out[f"primary_{name}_pos"], out[f"primary_{name}_w"], out[f"primary_{name}_st"] = pos, w, st
+    # TMM single-layer and 3-layer reflectances (tmm.h:89).
+    tm = []
+    for layers in ([(1.38, 0.0, 99.64)], [(1.38, 0.0, 99.6), (2.05, 0.0, 67.1), (1.63, 0.0, 84.4)]):
+        nre = np.array([l[0] for l in layers], np.float32)
+        nim = np.array([l[1] for l in layers], np.float32)
+        th = np.array([l[2] for l in layers], np.float32)
+        for lam in (400.0, 550.0, 700.0):
+            for aoi in (0.0, 30.0, 60.0, 85.0):
+                r = ref.ref_tmm_reflectance(len(layers), pyoracle._f(nre), pyoracle._f(nim), pyoracle._f(th),
+                                            C.c_float(1.0), C.c_float(1.5168), C.c_float(lam), C.c_float(aoi))
+                tm.append([len(layers), lam, aoi, r])
+    out["tmm"] = np.array(tm, np.float32)
+    path = os.path.join(HERE, "reference_vectors.npz")
+    np.savez_compressed(path, **out)
+    print(f"wrote {path} ({os.path.getsize(path) / 1024:.0f} KiB, {len(out)} arrays)")
+
+
+if __name__ == "__main__":
+    main()

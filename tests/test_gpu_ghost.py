@@ -1,0 +1,179 @@
+"""GPU parity of the ghost path against the CPU oracle (device calls go through
+the C ABI).  Bar: statuses / vignette masks / cull sets / concentration rects
+/ landing positions bit-exact vs the FP32 oracle; images rel-L2 <= 1e-4 vs the
+FP32 oracle and <= 1e-3 vs the FP64 oracle."""
+import math
+
+import numpy as np
+import pytest
+
+from ghostlight_amd import FlareConfig, hostprep, render_ghosts, synthetic, trace_ghost_rays
+from ghostlight_amd import filter_ghost_pairs
+from ghostlight_amd.config import to_params
+from oracle import pyoracle
+
+pytestmark = pytest.mark.gpu
+
+TOL_FP32 = 1e-4
+TOL_FP64 = 1e-3
+
+
+def _lenses():
+    return {"c1": synthetic.config1_lens(), "c2": synthetic.config2_lens(), "c4": synthetic.config4_lens()}
+
+
+LENSES = _lenses()
+
+
+def _random_rays(lens, n, seed, max_deg=14.0):
+    rng = np.random.default_rng(seed)
+    fR = lens.surfaces[0].semi_aperture
+    z0 = lens.surfaces[0].z - 20.0
+    ang = np.radians(rng.uniform(-max_deg, max_deg, (n, 2)))
+    d = np.stack([np.tan(ang[:, 0]), np.tan(ang[:, 1]), np.ones(n)], 1)
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    rays = np.zeros((n, 7), np.float32)
+    rays[:, :2] = rng.uniform(-1, 1, (n, 2)) * fR
+    rays[:, 2] = z0
+    rays[:, 3:6] = d
+    rays[:, 6] = rng.uniform(400, 700, n)
+    return rays
+
+
+@pytest.mark.parametrize("name", ["c1", "c2", "c4"])
+@pytest.mark.parametrize("t_ge0", [False, True])
+def test_single_ray_trace_bitexact(name, t_ge0):
+    lens = LENSES[name]
+    n_s = lens.num_surfaces()
+    rays = _random_rays(lens, 600, seed=hash(name) % 1000)
+    pairs = [(0, 1), (1, n_s - 1), (2, n_s // 2), (n_s // 3, n_s - 2), (n_s - 3, n_s - 1)]
+    for (a, b) in pairs + [(-1, -1)]:
+        pos, w, st, sg = trace_ghost_rays(lens, a, b, rays, t_ge0=t_ge0, fp64_radius_min=0.0)
+        opos, ow, ost, osg = pyoracle.trace_ghost(lens, a, b, rays, dbl=False, t_ge0=t_ge0)
+        print(f"{name} pair ({a},{b}): {int((st == 0).sum())} OK of {len(st)}", flush=True)
+        assert np.array_equal(st, ost), f"status mismatch for pair {(a, b)}"
+        assert np.array_equal(sg, osg), "segment counts differ"
+        ok = st == 0
+        assert np.array_equal(pos[ok, :2], opos[ok, :2].astype(np.float32)), "landing positions differ"
+        # Weights differ only through device vs libm cos/asin (a few ulp).
+        np.testing.assert_allclose(w[ok], ow[ok].astype(np.float32), rtol=2e-6, atol=1e-12)
+
+
+def test_fp64_sphere_path_matches_oracle():
+    lens = LENSES["c1"]
+    rays = _random_rays(lens, 400, seed=7)
+    pos, w, st, _ = trace_ghost_rays(lens, 2, 9, rays, t_ge0=False, fp64_radius_min=30.0)
+    opos, ow, ost, _ = pyoracle.trace_ghost(lens, 2, 9, rays, dbl=False, t_ge0=False, fp64_radius_min=30.0)
+    assert np.array_equal(st, ost)
+    ok = st == 0
+    assert np.array_equal(pos[ok, :2], opos[ok, :2].astype(np.float32))
+
+
+@pytest.mark.parametrize("name", ["c1", "c2", "c4"])
+def test_filter_ghost_pairs_matches_oracle(name):
+    lens = LENSES[name]
+    hw, hh = synthetic.sensor_for(lens, 1920, 1080)
+    pairs, boosts = filter_ghost_pairs(lens, hw, hh, FlareConfig())
+    opairs, oboosts = pyoracle.filter_ghost_pairs(lens, hw, hh)
+    assert pairs == opairs
+    assert np.array_equal(np.float32(boosts), np.float32(oboosts))
+
+
+def _render_pair(lens, cfg, W, H, src, pairs=None, boosts=None, fov=14.0):
+    hw, hh = synthetic.sensor_for(lens, W, H, fov)
+    if pairs is None:
+        pairs, boosts = pyoracle.filter_ghost_pairs(lens, hw, hh)
+    planes, stats = render_ghosts(lens, src, W, H, cfg, hw, hh, pairs=pairs, boosts=boosts, stats=True)
+    spec = hostprep.spectral_table(max(3, cfg.spectral_samples))
+    C = hostprep.spectral_correction(spec) if cfg.spectral_samples != 3 else None
+    p, _keep = to_params(cfg)
+    o32, info = pyoracle.render_ghosts(lens, pairs, boosts, src, spec, p, hw, hh, W, H, correction=C, want_masks=True)
+    return planes, stats, o32, info, (pairs, boosts, spec, C, p, hw, hh)
+
+
+@pytest.mark.parametrize("conc", [False, True])
+def test_render_c1_parity(conc):
+    lens = LENSES["c1"]
+    cfg = FlareConfig(ray_grid=64, spectral_samples=8, concentrate_samples=conc)
+    src = np.array([[math.radians(10.0), 0.0, 1.0, 1.0, 1.0]], np.float32)
+    W = H = 512
+    planes, stats, o32, info, (pairs, boosts, spec, C, p, hw, hh) = _render_pair(lens, cfg, W, H, src)
+    err = pyoracle.rel_l2(planes, o32)
+    print(f"C1 conc={conc}: {len(pairs)} pairs, rel-L2 vs FP32 oracle {err:.3e}, segments {stats['segments']}",
+          flush=True)
+    assert sum(float(x.sum()) for x in o32) > 0
+    assert err <= TOL_FP32
+    assert stats["segments"] == info["segments"], "executed segment counts differ"
+    if conc:
+        n_ps = len(pairs)
+        rects = np.array(stats["ps_rect"], np.float32).reshape(-1, 4)
+        assert np.array_equal(rects, info["rects"][:n_ps]), "concentration rects differ"
+        assert np.array_equal(np.array(stats["ps_budget"]), info["budgets"][:n_ps, 0]), "budgets differ"
+    o64, _ = pyoracle.render_ghosts(lens, pairs, boosts, src, spec, p, hw, hh, W, H, correction=C, dbl=True)
+    err64 = pyoracle.rel_l2(planes, o64)
+    print(f"C1 conc={conc}: rel-L2 vs FP64 oracle {err64:.3e}", flush=True)
+    assert err64 <= TOL_FP64
+
+
+def test_render_c2_small_parity():
+    """Spectral-angular (TMM) coatings, 24 glass surfaces, reduced grid."""
+    lens = LENSES["c2"]
+    cfg = FlareConfig(ray_grid=32, spectral_samples=8)
+    src = np.array([[math.radians(12.0), math.radians(4.0), 1.0, 1.0, 1.0]], np.float32)
+    planes, stats, o32, info, _ = _render_pair(lens, cfg, 320, 180, src, fov=20.0)
+    err = pyoracle.rel_l2(planes, o32)
+    print(f"C2 small: rel-L2 vs FP32 oracle {err:.3e}, segments {stats['segments']}", flush=True)
+    assert err <= TOL_FP32
+    assert stats["segments"] == info["segments"]
+
+
+def test_render_multi_source_and_format_window():
+    lens = LENSES["c4"]
+    cfg = FlareConfig(ray_grid=24, spectral_samples=4, pupil_jitter=2)
+    src = np.array([[math.radians(-6.0), math.radians(3.0), 1.0, 0.5, 0.2],
+                    [math.radians(8.0), math.radians(-2.0), 0.3, 1.0, 0.7]], np.float32)
+    W, H = 256, 108
+    hw, hh = synthetic.sensor_for(lens, W, H, 16.0)
+    pairs, boosts = pyoracle.filter_ghost_pairs(lens, hw, hh)
+    pairs, boosts = pairs[:40], boosts[:40]
+    fmt = (200, 90, 20, 9)
+    planes = render_ghosts(lens, src, W, H, cfg, hw, hh, pairs=pairs, boosts=boosts, fmt=fmt)
+    spec = hostprep.spectral_table(4)
+    p, _keep = to_params(cfg)
+    o32, _ = pyoracle.render_ghosts(lens, pairs, boosts, src, spec, p, hw, hh, W, H, fmt=fmt,
+                                    correction=hostprep.spectral_correction(spec))
+    assert pyoracle.rel_l2(planes, o32) <= TOL_FP32
+
+
+def test_cull_on_equals_cull_off():
+    """validate_culling.py:100-114: culling drops only pairs that never reach
+    the sensor, so the image is unchanged."""
+    lens = LENSES["c1"]
+    W = H = 256
+    hw, hh = synthetic.sensor_for(lens, W, H)
+    src = np.array([[math.radians(18.0), math.radians(6.0), 1.0, 1.0, 1.0]], np.float32)
+    pairs, boosts = filter_ghost_pairs(lens, hw, hh)
+    on = render_ghosts(lens, src, W, H, FlareConfig(ray_grid=48, spectral_samples=6, concentrate_samples=False),
+                       hw, hh, pairs=pairs, boosts=boosts)
+    off = render_ghosts(lens, src, W, H, FlareConfig(ray_grid=48, spectral_samples=6, concentrate_samples=False,
+                                                     cull_dead_pairs=False), hw, hh, pairs=pairs, boosts=boosts)
+    e_on, e_off = sum(float(x.sum()) for x in on), sum(float(x.sum()) for x in off)
+    assert e_off > 0
+    assert abs(e_on / e_off - 1.0) < 1e-5
+    assert pyoracle.rel_l2(on, off) < 1e-5
+
+
+def test_gain_linearity_and_zero_source():
+    lens = LENSES["c1"]
+    W = H = 128
+    hw, hh = synthetic.sensor_for(lens, W, H)
+    pairs, boosts = filter_ghost_pairs(lens, hw, hh)
+    src = np.array([[math.radians(9.0), 0.0, 1.0, 1.0, 1.0]], np.float32)
+    a = render_ghosts(lens, src, W, H, FlareConfig(ray_grid=32, spectral_samples=8, flare_gain=1000.0), hw, hh,
+                      pairs=pairs, boosts=boosts)
+    b = render_ghosts(lens, src, W, H, FlareConfig(ray_grid=32, spectral_samples=8, flare_gain=2000.0), hw, hh,
+                      pairs=pairs, boosts=boosts)
+    assert pyoracle.rel_l2([2 * x for x in a], b) < 1e-5
+    z = render_ghosts(lens, np.array([[0.1, 0.0, 0.0, 0.0, 0.0]], np.float32), W, H,
+                      FlareConfig(ray_grid=32, spectral_samples=8), hw, hh, pairs=pairs, boosts=boosts)
+    assert all(float(np.abs(x).max()) == 0.0 for x in z)
