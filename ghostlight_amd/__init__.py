@@ -13,6 +13,7 @@ from .runtime import Context, device_available
 from .render import (enumerate_ghost_pairs, filter_ghost_pairs, render_ghosts, render_point_flare,
                      render_source_flare, solve_source_angle, trace_ghost_rays)
 from .hostprep import build_spectral_lambdas
+from .diffraction import _render_starburst_debug, _render_veil_debug, fft2d
 
 
 def _cuda_available() -> bool:
@@ -20,4 +21,5 @@ def _cuda_available() -> bool:
     return device_available()
 
 
-__all__ = [n for n in dir() if not n.startswith("_")] + ["_cuda_available"]
+__all__ = [n for n in dir() if not n.startswith("_")] + ["_cuda_available", "_render_starburst_debug",
+                                                         "_render_veil_debug"]

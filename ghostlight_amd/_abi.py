@@ -109,6 +109,40 @@ class gl_ghost_stats(C.Structure):
     ]
 
 
+class gl_calibration(C.Structure):
+    _fields_ = [(n, C.c_float) for n in (
+        "sensor_half_w", "sensor_half_h", "max_half_angle_h", "max_half_angle_v", "image_circle_semi_w",
+        "image_circle_semi_h", "focal_length_x", "focal_length_y", "entrance_pupil_semi_x", "entrance_pupil_semi_y",
+        "f_number_x", "f_number_y", "pupil_area_frac")]
+
+
+class gl_diffraction_params(C.Structure):
+    _fields_ = [
+        ("starburst_engine", C.c_int32), ("starburst_grid", C.c_int32), ("starburst_grid_cap", C.c_int32),
+        ("pupil_fill", C.c_float), ("starburst_gain", C.c_float), ("scale_trim", C.c_float),
+        ("spectral_samples", C.c_int32), ("use_survivor_mask", C.c_int32), ("survivor_grid", C.c_int32),
+        ("veil_gain", C.c_float), ("veil_spread", C.c_float), ("veil_falloff", C.c_float),
+        ("render_spectral_samples", C.c_int32), ("sensor_model", C.c_int32), ("output_cs", C.c_int32),
+        ("custom_xyz_to_output", C.c_float * 9), ("aperture_blades", C.c_int32), ("aperture_rotation", C.c_float),
+        ("mdft_precision", C.c_int32),
+    ]
+
+
+class gl_starburst_debug(C.Structure):
+    _fields_ = [
+        ("grid", C.c_int32), ("grid_cap", C.c_int32),
+        ("psf", C.POINTER(C.c_float)), ("sprite_rgb", C.POINTER(C.c_float)), ("pupil", C.POINTER(C.c_float)),
+    ] + [(n, C.c_float) for n in (
+        "dx_mm_x", "dx_mm_y", "f_number_x", "f_number_y", "focal_length_x", "focal_length_y",
+        "entrance_pupil_semi_x", "entrance_pupil_semi_y", "lambda_ref_nm", "field_angle_x", "field_angle_y",
+        "throughput", "chief_offset_x", "chief_offset_y")]
+
+
+class gl_veil_debug(C.Structure):
+    _fields_ = [("grid", C.c_int32), ("reference", C.POINTER(C.c_float)), ("sprite_rgb", C.POINTER(C.c_float))] + [
+        (n, C.c_float) for n in ("dx_mm_x", "dx_mm_y", "core_mm", "falloff", "core_texels")]
+
+
 F32P = C.POINTER(C.c_float)
 I32P = C.POINTER(C.c_int32)
 VP = C.c_void_p
@@ -147,6 +181,20 @@ SIGNATURES = {
                                            C.c_int32, I32P, C.POINTER(gl_ghost_stats)]),
     "gl_trace_ghost_rays": (C.c_int, [VP, VP, C.c_int32, C.c_int32, F32P, C.c_int32, C.c_int32, C.c_float,
                                       F32P, F32P, I32P, I32P]),
+    "gl_fft2d_c2c_forward_inplace": (C.c_int, [VP, VP, C.c_int32]),
+    "gl_fft2d_c2c_forward_host": (C.c_int, [VP, F32P, F32P, C.c_int32]),
+    "gl_render_starburst": (C.c_int, [VP, VP, C.POINTER(gl_calibration), C.POINTER(gl_source), F32P, F32P, C.c_int32,
+                                      C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_float, C.c_float,
+                                      C.POINTER(gl_diffraction_params), F32P, F32P, F32P,
+                                      C.POINTER(gl_starburst_debug)]),
+    "gl_render_veil": (C.c_int, [VP, C.POINTER(gl_source), F32P, F32P, C.c_int32, C.c_int32, C.c_int32, C.c_int32,
+                                 C.c_int32, C.c_float, C.c_float, C.POINTER(gl_diffraction_params), F32P, F32P, F32P,
+                                 C.POINTER(gl_veil_debug)]),
+    "gl_cgemm_host": (C.c_int, [VP, C.c_int32, C.c_int32, C.c_int32, F32P, F32P, F32P, C.c_int32]),
+    "gl_mft_psf": (C.c_int, [VP, F32P, C.c_int32, C.c_int32, C.c_float, C.c_float, F32P, C.c_int32, C.c_int32, F32P,
+                             C.POINTER(F32P), F32P]),
+    "gl_primary_survival": (C.c_int, [VP, VP, F32P, C.c_int32, C.c_float, C.c_float, C.c_float, C.c_float,
+                                      C.c_float, C.POINTER(C.c_uint8)]),
 }
 
 _lib = None

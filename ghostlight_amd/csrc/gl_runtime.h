@@ -55,6 +55,8 @@ struct gl_ctx {
     float spec_C[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
     bool spec_do_correct = false;
     bool grid_uploaded = false, spec_uploaded = false;
+    float veil_baked_p = -1.0f;          // falloff the cached veil reference holds
+    int max_lds = 0;                     // per-workgroup LDS bytes (device attribute)
     gl::DevBuf& buf(const char* name) { return bufs[name]; }
 };
 
@@ -84,6 +86,16 @@ void ior_tables(const gl_surface* s, int n, const gl_spec* spec, int ns, float* 
 float ior_at(const gl_surface& s, float lambda_nm);
 int spectral_table(int n, int output_cs, const float* custom9, std::vector<gl_spec>& out);
 int spectral_correction(const std::vector<gl_spec>& t, int output_cs, const float* custom9, float C[9]);
+int spectral_table_cmf(int n, int output_cs, const float* custom9, std::vector<gl_spec>& out);
+bool mask_contains_host(const DevMask& m, float u, float v);
+
+// A source as the diffraction passes see it: buffer-pixel position + colour.
+struct StarSource { float px, py, r, g, b; };
+
+// Matrix-DFT starburst window (mdft.hip).
+int mdft_render(gl_ctx* c, const float2* d_pupil, int N, float fill, const std::vector<float4>& spec, float lambda_ref,
+                float dx_x, float dx_y, float px_mm_x, float px_mm_y, const std::vector<StarSource>& src, float gain,
+                float* o_r, float* o_g, float* o_b, int width, int height, int precision);
 void poly_normals(int n_blades, float rot_rad, float scale, float* c, float* s, float* apothem);
 double now_ms();
 }  // namespace gl

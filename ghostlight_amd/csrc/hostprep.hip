@@ -142,6 +142,14 @@ int spectral_table(int n, int output_cs, const float* custom9, std::vector<gl_sp
     return GL_OK;
 }
 
+int spectral_table_cmf(int n, int output_cs, const float* custom9, std::vector<gl_spec>& out) {
+    if (n <= 0 || n > GL_MAX_SPECTRAL) return set_error(GL_ERR_INVALID, "spectral_samples %d outside [1, %d]", n, GL_MAX_SPECTRAL);
+    float M[9];
+    output_matrix(output_cs, custom9, M);
+    out = table_of(n, M);
+    return GL_OK;
+}
+
 static void gram(const std::vector<gl_spec>& t, float O[9]) {
     for (int i = 0; i < 9; ++i) O[i] = 0.0f;
     for (const auto& s : t) {
@@ -250,7 +258,7 @@ DevMask build_mask(const gl_surface* s, int n, int bo, float rdeg) {
     return m;
 }
 
-static bool mask_host(const DevMask& m, float u, float v) {
+bool mask_contains_host(const DevMask& m, float u, float v) {
     const float ux = u / m.aspect;
     const float r2 = ux * ux + v * v;
     if (r2 > 1.0f) return false;
@@ -295,7 +303,7 @@ std::vector<gl_sample> build_samples(const gl_surface* s, int n, int N, int jitt
             u = (((float)gx + ju) / (float)N) * 2.0f - 1.0f;
             v = (((float)gy + jv) / (float)N) * 2.0f - 1.0f;
         }
-        if (mask_host(m, u, v)) out.push_back({u, v});
+        if (mask_contains_host(m, u, v)) out.push_back({u, v});
     }
     return out;
 }

@@ -232,12 +232,12 @@ def render_source_flare(offsets, width: int, height: int, lens: OpticalSystem, c
         C.byref(aov_n), C.byref(st.s) if st else None), "render_source_flare")
     out["ghost_r"], out["ghost_g"], out["ghost_b"] = planes
     if config.diffraction.starburst:
-        from . import starburst
-        out.update(starburst.render_starburst_layer(lens, calibration, sources, width, height, hw, hh, config,
-                                                    ctx=ctx))
+        from . import diffraction
+        out.update(diffraction.render_starburst_layer(lens, calibration, sources, width, height, hw, hh, config,
+                                                      ctx=ctx))
     if config.diffraction.veil:
-        from . import veil
-        out.update(veil.render_veil_layer(lens, calibration, sources, width, height, hw, hh, config, ctx=ctx))
+        from . import diffraction
+        out.update(diffraction.render_veil_layer(lens, calibration, sources, width, height, hw, hh, config, ctx=ctx))
     if aov_buf is not None:
         for i in range(aov_n.value):
             a, b = int(aov_pairs[2 * i]), int(aov_pairs[2 * i + 1])

@@ -210,6 +210,7 @@ def make_lens(seed: int, n_glass_surfaces: int, n_doublets: int = 0, stop_after_
         surfs.append(s)
     lens = OpticalSystem(surfs, name=name or f"synthetic-{n_glass_surfaces}s-seed{seed}")
     lens.focal_length = float(efl)
+    lens.entrance_pupil_semi = float(ep)
     lens.finalize()
     if decenter_max > 0 or tilt_max_deg > 0:
         rng = np.random.default_rng([seed, 7777])
@@ -230,6 +231,19 @@ def make_lens(seed: int, n_glass_surfaces: int, n_doublets: int = 0, stop_after_
             if not s.is_stop:
                 lens.set_coating_layers(i, layers)
     return lens
+
+
+def calibration_for(lens: OpticalSystem, width: int, height: int, half_field_deg: float = 14.0):
+    """First-order calibration record of a synthetic lens (paraxial focal
+    length and entrance pupil; the sensor covers `half_field_deg`)."""
+    from .lens import LensCalibration
+    hw, hh = sensor_for(lens, width, height, half_field_deg)
+    f = lens.focal_length
+    ep = getattr(lens, "entrance_pupil_semi", f / 5.6)
+    return LensCalibration(sensor_half_w=hw, sensor_half_h=hh, max_half_angle_h=math.atan(hw / f),
+                           max_half_angle_v=math.atan(hh / f), image_circle_semi_w=hw, image_circle_semi_h=hh,
+                           focal_length_x=f, focal_length_y=f, entrance_pupil_semi_x=ep, entrance_pupil_semi_y=ep,
+                           f_number_x=f / (2 * ep), f_number_y=f / (2 * ep), pupil_area_frac=1.0)
 
 
 def sensor_for(lens: OpticalSystem, width: int, height: int, half_field_deg: float = 14.0):

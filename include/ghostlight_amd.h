@@ -300,6 +300,127 @@ int  gl_trace_ghost_rays(gl_ctx* ctx, const gl_lens* lens, int32_t bounce_a,
                          float* out_pos, float* out_w, int32_t* out_status,
                          int32_t* out_segments);
 
+/* ---- diffraction: FFT seam, starburst, veil --------------------------------- */
+
+/* LensCalibration (lens_calibration.h:11-57), same field order. */
+typedef struct gl_calibration {
+    float sensor_half_w, sensor_half_h, max_half_angle_h, max_half_angle_v;
+    float image_circle_semi_w, image_circle_semi_h;
+    float focal_length_x, focal_length_y;
+    float entrance_pupil_semi_x, entrance_pupil_semi_y;
+    float f_number_x, f_number_y;
+    float pupil_area_frac;
+} gl_calibration;
+
+enum { GL_STARBURST_SPRITE = 0, GL_STARBURST_MDFT = 1 };
+enum { GL_MDFT_FP32 = 0, GL_MDFT_BF16X3 = 1 };
+
+/* DiffractionConfig (render_config.h:131-189) + the RenderConfig / FlareConfig
+ * fields the starburst and veil read. */
+typedef struct gl_diffraction_params {
+    int32_t starburst_engine;       /* GL_STARBURST_*                              */
+    int32_t starburst_grid;
+    int32_t starburst_grid_cap;
+    float   pupil_fill;
+    float   starburst_gain;
+    float   scale_trim;
+    int32_t spectral_samples;       /* 0 = inherit render_spectral_samples         */
+    int32_t use_survivor_mask;
+    int32_t survivor_grid;
+    float   veil_gain;
+    float   veil_spread;
+    float   veil_falloff;
+    int32_t render_spectral_samples;
+    int32_t sensor_model;
+    int32_t output_cs;
+    float   custom_xyz_to_output[9];
+    int32_t aperture_blades;
+    float   aperture_rotation;      /* degrees                                     */
+    int32_t mdft_precision;         /* GL_MDFT_*: build-specific (3xBF16 MFMA split) */
+} gl_diffraction_params;
+
+/* StarburstDebug (diffraction.h:22-39).  Arrays are caller-owned with room
+ * for grid_cap^2 (psf, pupil) and 3*grid_cap^2 (sprite_rgb); NULL skips. */
+typedef struct gl_starburst_debug {
+    int32_t grid;
+    int32_t grid_cap;
+    float*  psf;
+    float*  sprite_rgb;
+    float*  pupil;
+    float   dx_mm_x, dx_mm_y, f_number_x, f_number_y;
+    float   focal_length_x, focal_length_y;
+    float   entrance_pupil_semi_x, entrance_pupil_semi_y;
+    float   lambda_ref_nm, field_angle_x, field_angle_y;
+    float   throughput, chief_offset_x, chief_offset_y;
+} gl_starburst_debug;
+
+/* VeilDebug (veil_render.h:31-39); reference is grid^2, sprite 3*grid^2. */
+typedef struct gl_veil_debug {
+    int32_t grid;
+    float*  reference;
+    float*  sprite_rgb;
+    float   dx_mm_x, dx_mm_y, core_mm, falloff, core_texels;
+} gl_veil_debug;
+
+/* fft2d_c2c_forward_inplace (fft_backend.h:25): forward, unnormalised,
+ * in-place 2-D C2C FFT of an N x N interleaved complex64 device array
+ * (N a power of two, 2..16384), on ctx's stream. */
+int  gl_fft2d_c2c_forward_inplace(gl_ctx* ctx, void* d_data, int32_t N);
+
+/* Host-buffer convenience form of the same transform (tests, tools). */
+int  gl_fft2d_c2c_forward_host(gl_ctx* ctx, const float* in, float* out, int32_t N);
+
+/* render_starburst (diffraction.h:54-72): effective pupil (stop x cat's-eye
+ * envelope), LDS radix FFT, PSF, lambda bake, downsample and area resample
+ * (SPRITE) or the matrix-DFT window (MDFT).  px/py: each source's buffer-pixel
+ * position.  out planes: host, width*height, overwritten.  dbg may be NULL. */
+int  gl_render_starburst(gl_ctx* ctx, const gl_lens* lens, const gl_calibration* calib,
+                         const gl_source* sources, const float* px, const float* py,
+                         int32_t n_sources, int32_t width, int32_t height,
+                         int32_t fmt_w, int32_t fmt_h, float sensor_half_w,
+                         float sensor_half_h, const gl_diffraction_params* p,
+                         float* out_r, float* out_g, float* out_b,
+                         gl_starburst_debug* dbg);
+
+/* render_veil (veil_render.h:54-70): analytic Lorentzian glare-spread sprite
+ * (cached per falloff) resampled per source.  dbg may be NULL. */
+int  gl_render_veil(gl_ctx* ctx, const gl_source* sources, const float* px, const float* py,
+                    int32_t n_sources, int32_t width, int32_t height, int32_t fmt_w,
+                    int32_t fmt_h, float sensor_half_w, float sensor_half_h,
+                    const gl_diffraction_params* p, float* out_r, float* out_g,
+                    float* out_b, gl_veil_debug* dbg);
+
+/* Survival of primary rays launched from (x, y) on the spawn plane in one
+ * direction (the CPU tracer's t >= 0 rule, trace.cpp:243): the kernel behind
+ * the starburst's survivor centroid and cat's-eye envelope
+ * (starburst_render.cu:393-582).  launch: n x 2 floats; out: n bytes (1 = OK). */
+int  gl_primary_survival(gl_ctx* ctx, const gl_lens* lens, const float* launch, int32_t n,
+                         float dir_x, float dir_y, float dir_z, float z_spawn,
+                         float lambda_nm, unsigned char* out);
+
+/* ---- matrix DFT on MFMA ----------------------------------------------------- */
+
+/* Complex GEMM C = A . B on the matrix cores (row-major interleaved
+ * complex64, host buffers), for tests of the MFMA engine.  precision:
+ * GL_MDFT_FP32 = v_mfma_f32_32x32x2_f32, GL_MDFT_BF16X3 = 3-term BF16 split
+ * (hi*hi + hi*lo + lo*hi) on v_mfma_f32_32x32x16_bf16, FP32 accumulate.
+ * Build-specific; no counterpart in the reference. */
+int  gl_cgemm_host(gl_ctx* ctx, int32_t M, int32_t N, int32_t K, const float* A,
+                   const float* B, float* C, int32_t precision);
+
+/* Matrix-DFT PSFs of one N x N complex pupil (host buffer) at n_lambda
+ * wavelengths: E_l = A_y . P . A_x^T with A[k][n] = exp(-2 pi i (k - M/2)
+ * (n - N/2) / (q_l N)), q_l = q_ref * lambda_l / lambda_ref (the lambda-scaled
+ * sampling of the precision-starburst workload), PSF_l = |E_l|^2 (M x M).
+ * Replaces the per-(source, lambda) contract_x / accumulate_y loops
+ * (mdft_render.cu:137-175) with batched MFMA GEMMs.  out_psf: host,
+ * n_lambda * M * M floats (may be NULL with dev_out); dev_out receives the
+ * device array (owned by ctx).  ms_out (may be NULL): device time of the
+ * GEMMs. */
+int  gl_mft_psf(gl_ctx* ctx, const float* pupil, int32_t N, int32_t M, float q_ref,
+                float lambda_ref, const float* lambdas, int32_t n_lambda, int32_t precision,
+                float* out_psf, float** dev_out, float* ms_out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -314,16 +314,18 @@ int orc_trace_ghost(int dbl, const gl_surface* s, int n, const float* arena, int
 
 int orc_trace_primary(int dbl, const gl_surface* s, int n, const float* arena, const float* ray7,
                       const float* ior_at, int t_ge0, float fp64_rmin, double* out_pos, double* out_w,
-                      int* out_status) {
+                      int* out_status, int* out_segs) {
     Lens L; lens_init(L, s, n, arena, fp64_rmin);
     if (dbl) {
         Ray<double> r; r.o = V3<double>(ray7[0], ray7[1], ray7[2]); r.d = V3<double>(ray7[3], ray7[4], ray7[5]);
         TraceOut<double> o = trace_primary<double>(L, r, ray7[6], ior_at, t_ge0 != 0);
         out_pos[0] = o.pos.x; out_pos[1] = o.pos.y; out_pos[2] = o.pos.z; *out_w = o.weight; *out_status = o.status;
+        if (out_segs) *out_segs = o.segments;
     } else {
         Ray<float> r; r.o = V3<float>(ray7[0], ray7[1], ray7[2]); r.d = V3<float>(ray7[3], ray7[4], ray7[5]);
         TraceOut<float> o = trace_primary<float>(L, r, ray7[6], ior_at, t_ge0 != 0);
         out_pos[0] = o.pos.x; out_pos[1] = o.pos.y; out_pos[2] = o.pos.z; *out_w = o.weight; *out_status = o.status;
+        if (out_segs) *out_segs = o.segments;
     }
     return 0;
 }

@@ -70,7 +70,7 @@ def trace_ghost(lens, a, b, rays, dbl=False, t_ge0=True, fp64_radius_min=0.0):
                               C.c_float(fp64_radius_min), op, C.byref(ow), C.byref(ost), C.byref(seg))
         else:
             L.orc_trace_primary(int(dbl), surfs, n, _f(arena), _f(rays[i]), _f(at), int(t_ge0),
-                                C.c_float(fp64_radius_min), op, C.byref(ow), C.byref(ost))
+                                C.c_float(fp64_radius_min), op, C.byref(ow), C.byref(ost), C.byref(seg))
         pos[i] = op[:]; w[i] = ow.value; st[i] = ost.value; sg[i] = seg.value
     return pos, w, st, sg
 
