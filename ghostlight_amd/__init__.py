@@ -10,8 +10,9 @@ from .config import (BaffleShape, Baffle, DiffractionConfig, FlareConfig, GateCo
 from .lens import (ApertureShape, Coating, CoatingModel, CylinderAxis, DispersionModel, LensCalibration,
                    OpticalSystem, Surface, SurfaceForm, TraceStatus)
 from .runtime import Context, device_available
-from .render import (enumerate_ghost_pairs, filter_ghost_pairs, render_ghosts, render_point_flare,
-                     render_source_flare, solve_source_angle, trace_ghost_rays)
+from .render import (GhostPair, Ray, TraceResult, enumerate_ghost_pairs, filter_ghost_pairs, render_ghosts,
+                     render_point_flare, render_source_flare, solve_source_angle, trace_ghost_ray, trace_ghost_rays,
+                     trace_primary_ray)
 from .hostprep import build_spectral_lambdas
 from .diffraction import _render_starburst_debug, _render_veil_debug, fft2d
 

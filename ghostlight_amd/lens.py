@@ -294,6 +294,36 @@ class OpticalSystem:
             self._dev_key = key
         return self._dev
 
+    # ---- renderer wrappers (optical_system.py:334-429) -------------------
+    def calibration(self) -> "LensCalibration":
+        """The lens's calibration record: the one attached to it (lens files
+        and the synthetic generators attach theirs), else a first-order
+        record from the paraxial focal length and entrance pupil."""
+        cal = getattr(self, "_calibration", None)
+        if cal is not None:
+            return cal
+        from .synthetic import calibration_from_paraxial
+        return calibration_from_paraxial(self)
+
+    def set_calibration(self, calib: "LensCalibration") -> None:
+        self._calibration = calib
+
+    def render_point_flare(self, width: int, height: int, config, *, calib: Optional["LensCalibration"] = None,
+                           ctx=None) -> dict:
+        from .render import render_point_flare
+        return render_point_flare(width, height, self, calib if calib is not None else self.calibration(), config,
+                                  ctx=ctx)
+
+    def render_source_flare(self, offsets, width: int, height: int, config, *,
+                            calib: Optional["LensCalibration"] = None, ctx=None) -> dict:
+        from .render import render_source_flare
+        arr = np.ascontiguousarray(offsets, dtype=np.float32)
+        if arr.ndim != 2 or arr.shape[1] != 3 or arr.shape[0] < 1:
+            raise ValueError("render_source_flare: offsets must be a (N, 3) float32 array "
+                             "of [d_angle_x, d_angle_y, weight] rows with N >= 1")
+        return render_source_flare(arr, width, height, self, calib if calib is not None else self.calibration(),
+                                   config, ctx=ctx)
+
     def __repr__(self) -> str:
         return f"OpticalSystem(name={self.name!r}, surfaces={len(self.surfaces)})"
 

@@ -9,6 +9,8 @@ field-angle sources, used by the benchmark configurations.
 """
 from __future__ import annotations
 
+from dataclasses import dataclass
+
 import ctypes as C
 import math
 from typing import Optional, Sequence
@@ -17,7 +19,7 @@ import numpy as np
 
 from . import _abi, hostprep
 from .config import FlareConfig, PointFlareConfig, to_params
-from .lens import LensCalibration, OpticalSystem
+from .lens import LensCalibration, OpticalSystem, TraceStatus
 from .runtime import ctx_handle, lib
 
 
@@ -163,6 +165,52 @@ def trace_ghost_rays(lens: OpticalSystem, bounce_a: int, bounce_b: int, rays: np
                                          int(bool(t_ge0)), float(fp64_radius_min), _abi.fptr(pos), _abi.fptr(w),
                                          _abi.iptr(st), _abi.iptr(sg)), "trace_ghost_rays")
     return pos, w, st, sg
+
+
+@dataclass
+class Ray:
+    """A ray (vec3.h Ray): origin, unit direction, wavelength in nm."""
+    origin: tuple = (0.0, 0.0, 0.0)
+    direction: tuple = (0.0, 0.0, 1.0)
+    wavelength: float = 550.0
+
+
+@dataclass
+class TraceResult:
+    """Landing of one traced ray (trace.h TraceResult)."""
+    position: tuple
+    weight: float
+    status: TraceStatus
+
+
+@dataclass
+class GhostPair:
+    """Two-bounce ghost (ghost.h GhostPair): reflect at surf_b, then at surf_a."""
+    surf_a: int
+    surf_b: int
+
+
+def _one_ray(ray: Ray) -> np.ndarray:
+    r = np.zeros((1, 7), np.float32)
+    r[0, :3] = ray.origin
+    r[0, 3:6] = ray.direction
+    r[0, 6] = ray.wavelength
+    return r
+
+
+def trace_ghost_ray(ray: Ray, lens: OpticalSystem, bounce_a: int, bounce_b: int, ctx=None) -> TraceResult:
+    """One ghost trace (bind_trace.cpp:57-63; trace.cpp:171): the device
+    tracer in FP32 with the CPU tracer's t >= 0 landing rule."""
+    if bounce_a < 0 or bounce_b < 0 or bounce_a >= bounce_b or bounce_b >= lens.num_surfaces():
+        raise ValueError("bounce indices must satisfy 0 <= bounce_a < bounce_b < num_surfaces")
+    pos, w, st, _ = trace_ghost_rays(lens, bounce_a, bounce_b, _one_ray(ray), t_ge0=True, ctx=ctx)
+    return TraceResult(tuple(float(x) for x in pos[0]), float(w[0]), TraceStatus(int(st[0])))
+
+
+def trace_primary_ray(ray: Ray, lens: OpticalSystem, ctx=None) -> TraceResult:
+    """One primary trace (bind_trace.cpp:65-67; trace.cpp:243)."""
+    pos, w, st, _ = trace_ghost_rays(lens, -1, -1, _one_ray(ray), t_ge0=True, ctx=ctx)
+    return TraceResult(tuple(float(x) for x in pos[0]), float(w[0]), TraceStatus(int(st[0])))
 
 
 def solve_source_angle(lens: OpticalSystem, calib: LensCalibration, ndc_x: float, ndc_y: float,

@@ -246,6 +246,35 @@ def calibration_for(lens: OpticalSystem, width: int, height: int, half_field_deg
                            f_number_x=f / (2 * ep), f_number_y=f / (2 * ep), pupil_area_frac=1.0)
 
 
+def calibration_from_paraxial(lens: OpticalSystem, sensor_half_w: float = 18.0, sensor_half_h: float = 12.0):
+    """First-order calibration of any lens at the d line: paraxial focal
+    length from a unit-height ray from infinity, entrance pupil = stop
+    semi-aperture over that ray's height at the stop, and a full-frame sensor."""
+    from .lens import LensCalibration
+    y, u, n = 1.0, 0.0, 1.0
+    y_stop = None
+    surfs = [s for s in lens.surfaces if s.is_active or s.is_stop]
+    for i, s in enumerate(surfs):
+        n2 = s.ior_at(587.56) if s.is_active else n
+        c = 0.0 if abs(s.radius) < 1e-9 else 1.0 / s.radius
+        if s.is_stop and y_stop is None:
+            y_stop = abs(y)
+        u = (n * u - y * (n2 - n) * c) / n2
+        n = n2
+        if i + 1 < len(surfs):
+            y = y + u * (surfs[i + 1].z - s.z)
+    if abs(u) < 1e-12:
+        raise ValueError("calibration: afocal lens (no paraxial focal length)")
+    f = -1.0 / u
+    stop = next((s for s in lens.surfaces if s.is_stop), None)
+    ep = stop.semi_aperture / y_stop if (stop is not None and y_stop) else lens.surfaces[0].semi_aperture
+    return LensCalibration(sensor_half_w=sensor_half_w, sensor_half_h=sensor_half_h,
+                           max_half_angle_h=math.atan(sensor_half_w / f), max_half_angle_v=math.atan(sensor_half_h / f),
+                           image_circle_semi_w=sensor_half_w, image_circle_semi_h=sensor_half_h,
+                           focal_length_x=f, focal_length_y=f, entrance_pupil_semi_x=ep, entrance_pupil_semi_y=ep,
+                           f_number_x=f / (2 * ep), f_number_y=f / (2 * ep), pupil_area_frac=1.0)
+
+
 def sensor_for(lens: OpticalSystem, width: int, height: int, half_field_deg: float = 14.0):
     """Sensor half extents (mm) so the frame's half-height covers `half_field_deg`."""
     hh = lens.focal_length * math.tan(math.radians(half_field_deg))

@@ -201,3 +201,22 @@ def test_splat_energy_conservation():
     want = [1000.0 * R * R * (1 - R) ** 2 * w_ray * len(samples)] * 3
     got = [float(x.sum()) for x in planes]
     assert got == pytest.approx(want, rel=1e-5)
+
+
+def test_paraxial_calibration_thick_lens_closed_form():
+    """Thick biconvex singlet: EFL from the thick-lens equation
+    1/f = (n-1)(1/R1 - 1/R2 + (n-1) t / (n R1 R2)); the stop in front of the
+    lens makes the entrance pupil equal to the stop semi-aperture."""
+    from ghostlight_amd import OpticalSystem, Surface
+    from ghostlight_amd.synthetic import calibration_from_paraxial
+    n, R1, R2, t = 1.6, 40.0, -60.0, 5.0
+    stop = Surface(radius=0.0, thickness=2.0, ior=1.0, semi_aperture=4.0, is_stop=True, is_active=False)
+    s1 = Surface(radius=R1, thickness=t, ior=n, semi_aperture=10.0)
+    s2 = Surface(radius=R2, thickness=30.0, ior=1.0, semi_aperture=10.0)
+    lens = OpticalSystem([stop, s1, s2])
+    lens.finalize()
+    cal = calibration_from_paraxial(lens)
+    f = 1.0 / ((n - 1) * (1 / R1 - 1 / R2 + (n - 1) * t / (n * R1 * R2)))
+    assert abs(cal.focal_length_x - f) <= 1e-4 * f
+    assert abs(cal.entrance_pupil_semi_x - 4.0) <= 1e-6
+    assert abs(cal.f_number_x - f / 8.0) <= 1e-4 * f
