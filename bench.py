@@ -101,12 +101,51 @@ def cpu_baseline(lens, cfg, src, pairs, boosts, W, H, hw, hh, target_s=15.0):
                       f"({info['segments']} segments in {dt:.2f} s, {platform.processor() or 'x86_64'})"}
 
 
+def precision_sample(lens, cfg, src, pairs, boosts, W, H, hw, hh, k, ctx):
+    """Error of the device render against the FP32 and FP64 oracles on the
+    first k pairs of the workload (the same launcher semantics on the CPU)."""
+    from oracle import pyoracle
+    from ghostlight_amd import hostprep, render_ghosts
+    from ghostlight_amd.config import to_params
+    spec = hostprep.spectral_table(cfg.spectral_samples)
+    C = hostprep.spectral_correction(spec)
+    p, _keep = to_params(cfg)
+    gpu = render_ghosts(lens, src, W, H, cfg, hw, hh, pairs=pairs[:k], boosts=boosts[:k], ctx=ctx)
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    r32, _ = pyoracle.render_ghosts(lens, pairs[:k], boosts[:k], src, spec, p, hw, hh, W, H, correction=C,
+                                    threads=threads)
+    r64, _ = pyoracle.render_ghosts(lens, pairs[:k], boosts[:k], src, spec, p, hw, hh, W, H, correction=C,
+                                    dbl=True, threads=threads)
+    return {"rel_l2_vs_fp32_oracle": pyoracle.rel_l2(gpu, r32), "rel_l2_vs_fp64_oracle": pyoracle.rel_l2(gpu, r64),
+            "fp32_oracle_vs_fp64_oracle": pyoracle.rel_l2(r32, r64),
+            "sample": f"ghost image of the first {k} of {len(pairs)} pairs of the same workload"}
+
+
+def serving_leg(n_requests, rate_hz=40.0, slots=4):
+    """Mixed-request serving stream (seeded lenses of 8-32 surfaces, 512^2 to
+    3840x2160, 8-31 wavelengths, 5-11 blades; ghosts + starburst + veil)."""
+    from ghostlight_amd.serving import (FlareServer, MixSpec, make_mix, run_one_at_a_time, run_stream,
+                                        serving_lens_pool)
+    pool = serving_lens_pool(5)
+    mix = make_mix(MixSpec(n_requests=n_requests, rate_hz=rate_hz, seed=5), pool)
+    warm = make_mix(MixSpec(n_requests=2 * slots, rate_hz=1e6, seed=99), pool)
+    with FlareServer(slots=slots) as srv:
+        run_stream(srv, warm)
+        served = run_stream(srv, mix)
+    plain = run_one_at_a_time(mix)
+    r = lambda d: {k: (round(v, 3) if isinstance(v, float) else v) for k, v in d.items()}  # noqa: E731
+    return {"requests": n_requests, "rate_hz": rate_hz, "slots": slots, "served": r(served),
+            "one_at_a_time": r(plain)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline and precision legs")
+    ap.add_argument("--serving-requests", type=int, default=200,
+                    help="requests in the serving leg (0 skips it)")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {a.gpus} is not supported "
@@ -126,11 +165,12 @@ def main():
     for _ in range(a.warmup):
         step()
     ctx.synchronize()
-    kms = []
+    kms, sms = [], []
     t0 = time.perf_counter()
     for _ in range(a.steps):
         step()
         kms.append(ctx.last_kernel_ms())
+        sms.append(ctx.last_splat_ms())
     ctx.synchronize()
     t1 = time.perf_counter()
     ms_per_step = (t1 - t0) * 1e3 / a.steps
@@ -141,8 +181,10 @@ def main():
     live_pairs = sum(1 for x in st["pair_on_sensor"] if x > 0)
     value = segs / (ms_per_step * 1e-3)
     kernel_ms = float(np.mean(kms))
-    print(f"bench: {segs} segments/step, {ms_per_step:.3f} ms/step, main kernel {kernel_ms:.3f} ms, "
-          f"{live_pairs} pairs reach the sensor", file=sys.stderr, flush=True)
+    splat_ms = float(np.mean(sms))
+    print(f"bench: {segs} segments/step, {ms_per_step:.3f} ms/step, trace kernel {kernel_ms:.3f} ms, "
+          f"tiled splat {splat_ms:.3f} ms, {live_pairs} pairs reach the sensor, "
+          f"{st.get('traces_on_sensor', 0)} on-sensor traces", file=sys.stderr, flush=True)
 
     roof = None
     try:
@@ -150,17 +192,26 @@ def main():
         achieved = segs * ops / (kernel_ms * 1e-3) / 1e12
         roof = {"bound": "valu", "achieved": round(achieved, 3), "peak": FP32_VECTOR_PEAK_TFLOPS,
                 "unit": "TFLOP/s", "frac": round(achieved / FP32_VECTOR_PEAK_TFLOPS, 4), "traffic": None,
-                "kernel": "ghost_kernel<false,true,false> (trace + splat)", "kernel_ms": round(kernel_ms, 4),
+                "kernel": "ghost_kernel<false,true,false> (trace + deposit records)", "kernel_ms": round(kernel_ms, 4),
+                "splat_resolve_ms": round(splat_ms, 4),
                 "ops_per_segment": round(ops, 2), "segments_per_launch": segs}
     except Exception as e:  # the oracle library is test infrastructure; report its absence
         print(f"bench: roofline op count unavailable ({e})", file=sys.stderr)
 
-    cpu = None
+    cpu = prec = None
     if not a.no_cpu:
         try:
             cpu = cpu_baseline(lens, cfg, src, pairs, boosts, W, H, hw, hh)
         except Exception as e:
             print(f"bench: cpu baseline unavailable ({e})", file=sys.stderr)
+        try:
+            prec = precision_sample(lens, cfg, src, pairs, boosts, W, H, hw, hh, 24, ctx)
+        except Exception as e:
+            print(f"bench: precision sample unavailable ({e})", file=sys.stderr)
+    serving = None
+    if a.serving_requests > 0:
+        print(f"bench: serving leg, {a.serving_requests} requests", file=sys.stderr, flush=True)
+        serving = serving_leg(a.serving_requests)
 
     out = {
         "metric": "ghost-ray segments/s",
@@ -181,6 +232,8 @@ def main():
                    "pairs_on_sensor": live_pairs, "segments_per_step": segs, "parallelism": "single GPU"},
         "roofline": roof,
         "cpu_baseline": cpu,
+        "precision": prec,
+        "serving": serving,
     }
     print(json.dumps(out), flush=True)
 

@@ -155,6 +155,10 @@ SIGNATURES = {
     "gl_ctx_destroy": (None, [VP]),
     "gl_ctx_synchronize": (C.c_int, [VP]),
     "gl_ctx_last_kernel_ms": (C.c_float, [VP]),
+    "gl_ctx_last_splat_ms": (C.c_float, [VP]),
+    "gl_device_memory": (C.c_int, [C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
+    "gl_ctx_memory_bytes": (C.c_uint64, [VP]),
+    "gl_ctx_trim": (C.c_int, [VP]),
     "gl_lens_create": (C.c_int, [C.POINTER(gl_surface), C.c_int32, F32P, C.c_int32, C.POINTER(VP)]),
     "gl_lens_destroy": (None, [VP]),
     "gl_flare_params_init": (None, [C.POINTER(gl_flare_params)]),

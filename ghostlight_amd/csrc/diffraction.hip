@@ -406,7 +406,7 @@ __global__ __launch_bounds__(256) void survive_kernel(const DevSurf* __restrict_
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n_launch) return;
     const float2 l = launch[i];
-    TraceRes r = trace_primary(as_const(S), n, as_const(poly), arena, f3(l.x, l.y, z0), f3(dx, dy, dz), lam,
+    TraceRes r = trace_primary<false>(as_const(S), n, as_const(poly), arena, f3(l.x, l.y, z0), f3(dx, dy, dz), lam,
                                as_const(iat), true, nullptr);
     out[i] = (r.status == GL_TRACE_OK && isfinite(r.px) && isfinite(r.py)) ? 1 : 0;
 }
@@ -441,7 +441,8 @@ static int survive(gl_ctx* c, gl_lens* L, const std::vector<float2>& launch, flo
     out.assign(launch.size(), 0);
     if (launch.empty()) return GL_OK;
     int st;
-    if ((st = lens_sync(L, 0.0f, c->stream))) return st;
+    LensDev D;
+    if ((st = lens_sync(L, 0.0f, c->stream, &D))) return st;
     const int n = (int)L->host.size();
     const float bx = std::tan(ax), by = std::tan(ay);   // float tangent, as the reference's host code
     float l = std::sqrt(bx * bx + by * by + 1.0f * 1.0f);
@@ -456,8 +457,8 @@ static int survive(gl_ctx* c, gl_lens* L, const std::vector<float2>& launch, flo
         return st;
     GL_HIP(hipMemcpyAsync(bl.p, launch.data(), launch.size() * sizeof(float2), hipMemcpyHostToDevice, c->stream));
     GL_HIP(hipMemcpyAsync(bi.p, at.data(), n * sizeof(float), hipMemcpyHostToDevice, c->stream));
-    hipLaunchKernelGGL(survive_kernel, dim3((unsigned)((launch.size() + 255) / 256)), dim3(256), 0, c->stream, L->d_surf,
-                       L->d_poly, L->d_arena, n, bl.as<float2>(), (int)launch.size(), dx, dy, dz,
+    hipLaunchKernelGGL(survive_kernel, dim3((unsigned)((launch.size() + 255) / 256)), dim3(256), 0, c->stream, D.surf,
+                       D.poly, D.arena, n, bl.as<float2>(), (int)launch.size(), dx, dy, dz,
                        L->host[0].z - 20.0f, lam, bi.as<float>(), bo.as<unsigned char>());
     GL_HIP(hipGetLastError());
     GL_HIP(hipMemcpyAsync(out.data(), bo.p, launch.size(), hipMemcpyDeviceToHost, c->stream));
@@ -711,9 +712,9 @@ int gl_primary_survival(gl_ctx* ctx, const gl_lens* lens, const float* launch, i
     if (!c) return st;
     std::lock_guard<std::mutex> g(c->mu);
     gl_lens* L = const_cast<gl_lens*>(lens);
-    std::lock_guard<std::mutex> gl_(L->mu);
     if (n == 0) return GL_OK;
-    if ((st = lens_sync(L, 0.0f, c->stream))) return st;
+    LensDev D;
+    if ((st = lens_sync(L, 0.0f, c->stream, &D))) return st;
     const int ns = (int)L->host.size();
     gl_spec sp{lam, 0, 0, 0};
     std::vector<float> at(ns), bf(ns);
@@ -723,8 +724,8 @@ int gl_primary_survival(gl_ctx* ctx, const gl_lens* lens, const float* launch, i
         return st;
     GL_HIP(hipMemcpyAsync(bl.p, launch, (size_t)n * sizeof(float2), hipMemcpyHostToDevice, c->stream));
     GL_HIP(hipMemcpyAsync(bi.p, at.data(), ns * sizeof(float), hipMemcpyHostToDevice, c->stream));
-    hipLaunchKernelGGL(survive_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, L->d_surf, L->d_poly,
-                       L->d_arena, ns, bl.as<float2>(), n, dx, dy, dz, z_spawn, lam, bi.as<float>(),
+    hipLaunchKernelGGL(survive_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, D.surf, D.poly,
+                       D.arena, ns, bl.as<float2>(), n, dx, dy, dz, z_spawn, lam, bi.as<float>(),
                        bo.as<unsigned char>());
     GL_HIP(hipGetLastError());
     GL_HIP(hipMemcpyAsync(out, bo.p, n, hipMemcpyDeviceToHost, c->stream));
@@ -747,7 +748,6 @@ int gl_render_starburst(gl_ctx* ctx, const gl_lens* lens, const gl_calibration* 
     if (!c) return st;
     std::lock_guard<std::mutex> g(c->mu);
     gl_lens* L = const_cast<gl_lens*>(lens);
-    std::lock_guard<std::mutex> gl_(L->mu);
 
     // ---- first-order optics (starburst_render.cu:619-635)
     const float front_R = L->host[0].semi_aperture;

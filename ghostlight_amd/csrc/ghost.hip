@@ -9,6 +9,7 @@
 // samples (the reference's decomposition, ghost_render.cu:249-254,372), so a
 // wave shares its pair and walks the prescription with uniform indices.
 #include "gl_runtime.h"
+#include "gl_splat.h"
 
 #include <algorithm>
 #include <cmath>
@@ -25,6 +26,7 @@ static constexpr int kConcMinHits = 3;
 static constexpr int kRefineGrid = 128;
 static constexpr int kRefineHitsMax = 16;
 static constexpr int kRefineMaxEntries = 4096;
+static constexpr int kStage = 256;            // deposit records staged per wave in LDS
 
 struct SrcDev { float bx, by, dx, dy, dz, r, g, b; };
 
@@ -67,6 +69,11 @@ struct GhostArgs {
     int* __restrict__ bbox;                     // STATS: mins | maxs | hits
     unsigned long long* __restrict__ cnt;       // STATS: traces | survived | on-sensor (per pair)
     unsigned long long* __restrict__ segs;      // optional segment counter
+    const int2* __restrict__ lrow;              // SPLAT: per-(lambda, surface) coating lookups
+    int lam_group;                              // STATS: lanes per sample (one lambda each), 0 = loop
+    float4* __restrict__ rec;                   // SPLAT: deposit records
+    unsigned long long rec_cap;
+    unsigned long long* __restrict__ rec_count;
 };
 
 __device__ __forceinline__ int enc_pupil(float x) { return __float_as_int(x + 2.0f); }
@@ -114,8 +121,24 @@ __device__ __forceinline__ bool sample_at(const GhostArgs& A, int ps, int gi, fl
     return true;
 }
 
-__device__ __forceinline__ void splat_px(float* __restrict__ o, int p, float c, float w) {
-    if (fabsf(c) > 1e-14f) atomicAdd(o + p, c * w);
+// Flush a wave's staged deposit records to HBM: one 64-bit reservation, then
+// the 64 lanes copy the run out with consecutive (coalesced) 16-byte stores.
+// Records past the capacity are counted but not written; the host regrows
+// the buffer and renders again.
+__device__ __forceinline__ void flush_stage(const GhostArgs& A, float4* stage, int n, int lane) {
+    unsigned long long base = 0;
+    if (lane == 0) base = atomicAdd(A.rec_count, (unsigned long long)n);
+    base = __shfl(base, 0, 64);
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    for (int i = lane; i < n; i += 64) {
+        const unsigned long long d = base + (unsigned long long)i;
+        if (d < A.rec_cap) A.rec[d] = stage[i];
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
 // STATS = probe / diagnostic pass (counters, survivor bbox); otherwise the
@@ -124,7 +147,18 @@ template <bool STATS, bool SPLAT, bool COUNT>
 __global__ __launch_bounds__(kBlock) void ghost_kernel(GhostArgs A) {
     const int ps = blockIdx.x;
     const int pair_idx = ps / A.n_src, src_idx = ps % A.n_src;
-    const int gi = (int)blockIdx.y * kBlock + (int)threadIdx.x;
+    const int lane = threadIdx.x & 63;
+    // STATS with lam_group = G > 0: G lanes per pupil sample, one wavelength
+    // each (the probe passes are short; spreading lambda fills the device).
+    const int G = STATS ? A.lam_group : 0;
+    int gi, si0 = 0, si1 = A.n_spec;
+    if (STATS && G > 0) {
+        gi = (int)blockIdx.y * (kBlock / G) + (int)(threadIdx.x >> 6) * (64 / G) + lane / G;
+        si0 = lane % G;
+        si1 = si0 < A.n_spec ? si0 + 1 : si0;
+    } else {
+        gi = (int)blockIdx.y * kBlock + (int)threadIdx.x;
+    }
     // Whole-block exits are uniform; per-lane validity stays a flag so the
     // wave reductions below see every lane.
     if (A.pair_active != nullptr && A.pair_active[pair_idx] == 0) return;
@@ -137,55 +171,63 @@ __global__ __launch_bounds__(kBlock) void ghost_kernel(GhostArgs A) {
     if (live) live = sample_at(A, ps, gi, u, v, w_ray);
 
     unsigned long long n_tr = 0, n_sv = 0, n_on = 0, n_seg = 0;
-    if (live) {
-        const SrcDev sr = A.src[src_idx];
-        const F3 o0 = f3(u * A.front_R + (-20.0f * sr.bx), v * A.front_R + (-20.0f * sr.by), A.start_z);
-        const F3 d0 = f3(sr.dx, sr.dy, sr.dz);
-        for (int si = 0; si < A.n_spec; ++si) {
+    // SPLAT: per-wave LDS staging of deposit records (no image atomics here).
+    __shared__ float4 stage_all[SPLAT ? (kBlock / 64) : 1][SPLAT ? kStage : 1];
+    float4* stage = stage_all[SPLAT ? (threadIdx.x >> 6) : 0];
+    int staged = 0;                                  // wave-uniform
+    const SrcDev sr = A.src[src_idx];
+    const F3 o0 = f3(u * A.front_R + (-20.0f * sr.bx), v * A.front_R + (-20.0f * sr.by), A.start_z);
+    const F3 d0 = f3(sr.dx, sr.dy, sr.dz);
+    for (int si = si0; si < si1; ++si) {
+        bool dep = false;
+        float4 rec = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (live) do {
             const float GLC* spf = as_const(reinterpret_cast<const float*>(A.spec)) + 4 * si;
             const float4 sp = make_float4(spf[0], spf[1], spf[2], spf[3]);
             if (STATS) ++n_tr;
-            TraceRes res = trace_ghost(as_const(A.S), A.n, as_const(A.poly), A.arena, pr.surf_a, pr.surf_b, o0, d0,
+            TraceRes res = trace_ghost<SPLAT>(as_const(A.S), A.n, as_const(A.poly), A.arena, pr.surf_a, pr.surf_b, o0, d0,
                                        sp.x, as_const(A.iat) + (size_t)si * A.n, as_const(A.ibf) + (size_t)si * A.n,
-                                       false);
+                                       false, (SPLAT && A.lrow) ? as_const(A.lrow) + (size_t)si * A.n : nullptr);
             if (COUNT || STATS) n_seg += (unsigned long long)res.segs;
-            if (res.status != GL_TRACE_OK) continue;
-            if (!isfinite(res.px) || !isfinite(res.py)) continue;
+            if (res.status != GL_TRACE_OK) break;
+            if (!isfinite(res.px) || !isfinite(res.py)) break;
             if (STATS) {
                 ++n_sv;
                 float spx = (res.px / (2.0f * A.sw) + 0.5f) * A.fmt_w + A.x0;
                 float spy = (res.py / (2.0f * A.sh) + 0.5f) * A.fmt_h + A.y0;
                 if (isfinite(spx) && isfinite(spy) && spx >= 0.0f && spx < (float)A.width && spy >= 0.0f &&
                     spy < (float)A.height) ++n_on;
-                continue;
+                break;
             }
-            if (!SPLAT) continue;
+            if (!SPLAT) break;
             float base = res.w * w_ray * A.gain * pr.area_boost;
-            if (base < 1e-14f) continue;
+            if (base < 1e-14f) break;
             float px = (res.px / (2.0f * A.sw) + 0.5f) * A.fmt_w + A.x0;
             float py = (res.py / (2.0f * A.sh) + 0.5f) * A.fmt_h + A.y0;
-            if (!isfinite(px) || !isfinite(py)) continue;
+            if (!isfinite(px) || !isfinite(py)) break;
             px = fmaxf(-2.0e9f, fminf(2.0e9f, px));
             py = fmaxf(-2.0e9f, fminf(2.0e9f, py));
             const int x0 = (int)floorf(px - 0.5f), y0 = (int)floorf(py - 0.5f);
-            const float fx = (px - 0.5f) - (float)x0, fy = (py - 0.5f) - (float)y0;
-            const float w00 = (1.0f - fx) * (1.0f - fy), w10 = fx * (1.0f - fy);
-            const float w01 = (1.0f - fx) * fy, w11 = fx * fy;
+            if (x0 + 1 < 0 || x0 >= A.width || y0 + 1 < 0 || y0 >= A.height) break;   // footprint off the image
             const float P = sr.r * sp.y + sr.g * sp.z + sr.b * sp.w;
-            const float power = P * base;
-            const float cr = power * sp.y, cg = power * sp.z, cb = power * sp.w;
-            const int xs[4] = {x0, x0 + 1, x0, x0 + 1}, ys[4] = {y0, y0, y0 + 1, y0 + 1};
-            const float ws[4] = {w00, w10, w01, w11};
-#pragma unroll
-            for (int k = 0; k < 4; ++k) {
-                if (xs[k] < 0 || xs[k] >= A.width || ys[k] < 0 || ys[k] >= A.height) continue;
-                const int p = ys[k] * A.width + xs[k];
-                splat_px(A.out_r, p, cr, ws[k]);
-                splat_px(A.out_g, p, cg, ws[k]);
-                splat_px(A.out_b, p, cb, ws[k]);
+            rec = make_float4(px, py, P * base, __int_as_float(si));
+            dep = true;
+        } while (false);
+        if (SPLAT) {
+            const unsigned long long bal = __ballot(dep);
+            const int cnt = (int)__popcll(bal);
+            if (cnt) {
+                if (staged + cnt > kStage) { flush_stage(A, stage, staged, lane); staged = 0; }
+                if (dep) {
+                    const int slot = staged + (int)__builtin_amdgcn_mbcnt_hi((unsigned)(bal >> 32),
+                                                                            __builtin_amdgcn_mbcnt_lo((unsigned)bal, 0u));
+                    stage[slot] = rec;
+                }
+                staged += cnt;
             }
         }
     }
+    if (SPLAT && staged) flush_stage(A, stage, staged, lane);
     if (STATS) {
         const unsigned long long t = wave_sum_u64(n_tr), s = wave_sum_u64(n_sv), o = wave_sum_u64(n_on);
         const unsigned long long sg = wave_sum_u64(n_seg);
@@ -199,6 +241,13 @@ __global__ __launch_bounds__(kBlock) void ghost_kernel(GhostArgs A) {
         if (A.bbox != nullptr) {
             const bool on = n_on > 0;
             const unsigned long long bal = __ballot(on);
+            // hits = pupil samples with any wavelength on the sensor
+            int hits = (int)__popcll(bal);
+            if (G > 0) {
+                hits = 0;
+                const unsigned long long gm = G == 64 ? ~0ull : ((1ull << G) - 1ull);
+                for (int g0 = 0; g0 < 64; g0 += G) hits += ((bal >> g0) & gm) != 0ull ? 1 : 0;
+            }
             const float umn = wave_min(on ? u : 3.0f), vmn = wave_min(on ? v : 3.0f);
             const float umx = wave_max(on ? u : -3.0f), vmx = wave_max(on ? v : -3.0f);
             if (lane0 && bal) {
@@ -207,13 +256,18 @@ __global__ __launch_bounds__(kBlock) void ghost_kernel(GhostArgs A) {
                 atomicMin(A.bbox + 2 * ps + 1, enc_pupil(vmn));
                 atomicMax(A.bbox + 2 * n_ps + 2 * ps, enc_pupil(umx));
                 atomicMax(A.bbox + 2 * n_ps + 2 * ps + 1, enc_pupil(vmx));
-                atomicAdd(A.bbox + 4 * n_ps + ps, (int)__popcll(bal));
+                atomicAdd(A.bbox + 4 * n_ps + ps, hits);
             }
         }
     } else if (COUNT) {
         const unsigned long long sg = wave_sum_u64(n_seg);
         if ((threadIdx.x & 63) == 0 && sg) atomicAdd(A.segs, sg);
     }
+}
+
+__global__ void fill3_kernel(float* __restrict__ a, float* __restrict__ b, float* __restrict__ c, size_t n) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) { a[i] = 0.f; b[i] = 0.f; c[i] = 0.f; }
 }
 
 // Output-space 3x3 colour correction (ghost_render.cu:610-622); contraction
@@ -309,10 +363,22 @@ static SrcDev make_src(const gl_source& s) {
     return d;
 }
 
+// Lanes per sample for the lambda-parallel probe: the smallest power of two
+// holding every wavelength, or 0 (per-lane loop) above 64.
+static int lam_group(int ns) {
+    if (ns > 64) return 0;
+    int g = 1;
+    while (g < ns) g <<= 1;
+    return g;
+}
+
 template <bool STATS, bool SPLAT, bool COUNT>
-static int launch(const GhostArgs& A, int n_ps, int n_samples, hipStream_t s) {
+static int launch(const GhostArgs& A_in, int n_ps, int n_samples, hipStream_t s) {
     if (n_ps <= 0 || n_samples <= 0) return GL_OK;
-    dim3 grid((unsigned)n_ps, (unsigned)((n_samples + kBlock - 1) / kBlock), 1);
+    GhostArgs A = A_in;
+    A.lam_group = STATS ? lam_group(A.n_spec) : 0;
+    const int per_block = A.lam_group > 0 ? kBlock / A.lam_group : kBlock;
+    dim3 grid((unsigned)n_ps, (unsigned)((n_samples + per_block - 1) / per_block), 1);
     hipLaunchKernelGGL((ghost_kernel<STATS, SPLAT, COUNT>), grid, dim3(kBlock), 0, s, A);
     GL_HIP(hipGetLastError());
     return GL_OK;
@@ -379,9 +445,13 @@ static int do_launch(Launch& Lc, float* out_r, float* out_g, float* out_b, float
     const size_t npx = (size_t)Lc.width * Lc.height;
     DevBuf &bo_r = c->buf("out_r"), &bo_g = c->buf("out_g"), &bo_b = c->buf("out_b");
     if ((st = bo_r.ensure(npx * 4)) || (st = bo_g.ensure(npx * 4)) || (st = bo_b.ensure(npx * 4))) return st;
-    GL_HIP(hipMemsetAsync(bo_r.p, 0, npx * 4, c->stream));
-    GL_HIP(hipMemsetAsync(bo_g.p, 0, npx * 4, c->stream));
-    GL_HIP(hipMemsetAsync(bo_b.p, 0, npx * 4, c->stream));
+    auto zero_out = [&]() -> int {
+        hipLaunchKernelGGL(fill3_kernel, dim3((unsigned)((npx + 255) / 256)), dim3(256), 0, c->stream, bo_r.as<float>(),
+                           bo_g.as<float>(), bo_b.as<float>(), npx);
+        GL_HIP(hipGetLastError());
+        return GL_OK;
+    };
+    if ((st = zero_out())) return st;
     if (dev_out) { dev_out[0] = bo_r.as<float>(); dev_out[1] = bo_g.as<float>(); dev_out[2] = bo_b.as<float>(); }
     if (stats) {
         stats->n_pairs = n_pairs; stats->n_sources = n_src; stats->n_grid = n_grid; stats->n_spec = ns;
@@ -399,12 +469,17 @@ static int do_launch(Launch& Lc, float* out_r, float* out_g, float* out_b, float
     if (n_pairs == 0 || n_src == 0 || n_grid == 0) return finish_empty();
 
     const double t_up0 = now_ms();
-    if ((st = lens_sync(L, p->fp64_radius_min, c->stream))) return st;
+    LensDev D;
+    if ((st = lens_sync(L, p->fp64_radius_min, c->stream, &D))) return st;
     void *d_pairs, *d_src, *d_iat, *d_ibf, *d_grid = c->buf("grid").p, *d_spec = c->buf("spec").p;
     if ((st = upload(c, "pairs", Lc.pairs.data(), n_pairs * sizeof(gl_pair), &d_pairs))) return st;
     if ((st = upload(c, "src", Lc.src.data(), n_src * sizeof(SrcDev), &d_src))) return st;
     if ((st = upload(c, "iat", iat.data(), iat.size() * 4, &d_iat))) return st;
     if ((st = upload(c, "ibf", ibf.data(), ibf.size() * 4, &d_ibf))) return st;
+    std::vector<int2> lrows;
+    lambda_rows(L->host.data(), n, L->arena.data(), c->spec.data(), ns, lrows);
+    void* d_lrow;
+    if ((st = upload(c, "lrow", lrows.data(), lrows.size() * sizeof(int2), &d_lrow))) return st;
     if (!c->grid_uploaded || !d_grid || c->buf("grid").cap < n_grid * sizeof(gl_sample)) {
         if ((st = upload(c, "grid", c->grid.data(), n_grid * sizeof(gl_sample), &d_grid))) return st;
         c->grid_uploaded = true;
@@ -416,7 +491,7 @@ static int do_launch(Launch& Lc, float* out_r, float* out_g, float* out_b, float
     const double t_up1 = now_ms();
 
     GhostArgs A{};
-    A.S = L->d_surf; A.poly = L->d_poly; A.arena = L->d_arena; A.n = n;
+    A.S = D.surf; A.poly = D.poly; A.arena = D.arena; A.n = n;
     A.pairs = static_cast<const gl_pair*>(d_pairs);
     A.n_src = n_src; A.src = static_cast<const SrcDev*>(d_src);
     A.grid = static_cast<const gl_sample*>(d_grid); A.n_grid = n_grid;
@@ -427,6 +502,7 @@ static int do_launch(Launch& Lc, float* out_r, float* out_g, float* out_b, float
     A.gain = p->flare_gain; A.ray_weight = 1.0f / (float)n_grid;
     A.spec = static_cast<const float4*>(d_spec); A.n_spec = ns;
     A.iat = static_cast<const float*>(d_iat); A.ibf = static_cast<const float*>(d_ibf);
+    A.lrow = static_cast<const int2*>(d_lrow);
     A.conc.rects = nullptr;
 
     // ---- probe / cull / concentrate (ghost_render.cu:969-1238)
@@ -604,26 +680,59 @@ static int do_launch(Launch& Lc, float* out_r, float* out_g, float* out_b, float
                     n_ps, (n_grid + kBlock - 1) / kBlock);
         std::fflush(stdout);
     }
-    GL_HIP(hipEventRecord(c->ev0, c->stream));
-    if ((st = launch<false, true, false>(A, n_ps, n_grid, c->stream))) return st;
-    GL_HIP(hipEventRecord(c->ev1, c->stream));
-    if (c->spec_do_correct) {
-        const float* C = c->spec_C;
-        hipLaunchKernelGGL(spectral_correct_kernel, dim3((unsigned)((npx + 255) / 256)), dim3(256), 0, c->stream,
-                           A.out_r, A.out_g, A.out_b, npx, C[0], C[1], C[2], C[3], C[4], C[5], C[6], C[7], C[8]);
-        GL_HIP(hipGetLastError());
+    // Deposit-record capacity: the previous render's count with headroom,
+    // bounded by the number of (sample, lambda) traces this launch can make.
+    unsigned long long max_rec = 0;
+    for (int i = 0; i < n_ps; ++i) {
+        unsigned long long ns_i = (unsigned long long)n_grid;
+        if (rect_ready && rects[i].x <= rects[i].z) ns_i = budgets.empty() ? (unsigned long long)A.conc.n_r
+                                                                           : (unsigned long long)budgets[i].x;
+        max_rec += ns_i;
     }
+    max_rec *= (unsigned long long)ns;
+    const SplatGeom sg = splat_geom(Lc.width, Lc.height);
+    unsigned long long want = std::max<unsigned long long>(c->splat_hint + c->splat_hint / 4, 1ull << 22);
+    want = std::max<unsigned long long>(1, std::min(want, max_rec));
     const double t_k0 = now_ms();
-    if (out_r) {
-        GL_HIP(hipMemcpyAsync(out_r, A.out_r, npx * 4, hipMemcpyDeviceToHost, c->stream));
-        GL_HIP(hipMemcpyAsync(out_g, A.out_g, npx * 4, hipMemcpyDeviceToHost, c->stream));
-        GL_HIP(hipMemcpyAsync(out_b, A.out_b, npx * 4, hipMemcpyDeviceToHost, c->stream));
+    for (int attempt = 0; attempt < 2; ++attempt) {
+        SplatBufs sb{};
+        if ((st = splat_prepare(c, sg, want, sb))) return st;
+        if (attempt > 0 && (st = zero_out())) return st;
+        if ((st = splat_clear(c->stream, sg, sb))) return st;
+        A.rec = sb.rec; A.rec_cap = sb.rec_cap; A.rec_count = sb.rec_count;
+        GL_HIP(hipEventRecord(c->ev0, c->stream));
+        if ((st = launch<false, true, false>(A, n_ps, n_grid, c->stream))) return st;
+        GL_HIP(hipEventRecord(c->ev1, c->stream));
+        if ((st = splat_resolve(c->stream, sg, sb, A.spec, A.out_r, A.out_g, A.out_b))) return st;
+        GL_HIP(hipEventRecord(c->ev2, c->stream));
+        if (c->spec_do_correct) {
+            const float* C = c->spec_C;
+            hipLaunchKernelGGL(spectral_correct_kernel, dim3((unsigned)((npx + 255) / 256)), dim3(256), 0, c->stream,
+                               A.out_r, A.out_g, A.out_b, npx, C[0], C[1], C[2], C[3], C[4], C[5], C[6], C[7], C[8]);
+            GL_HIP(hipGetLastError());
+        }
+        unsigned long long counts[2] = {0, 0};
+        GL_HIP(hipMemcpyAsync(&counts[0], sb.rec_count, 8, hipMemcpyDeviceToHost, c->stream));
+        GL_HIP(hipMemcpyAsync(&counts[1], sb.bin_total, 8, hipMemcpyDeviceToHost, c->stream));
+        if (out_r) {
+            GL_HIP(hipMemcpyAsync(out_r, A.out_r, npx * 4, hipMemcpyDeviceToHost, c->stream));
+            GL_HIP(hipMemcpyAsync(out_g, A.out_g, npx * 4, hipMemcpyDeviceToHost, c->stream));
+            GL_HIP(hipMemcpyAsync(out_b, A.out_b, npx * 4, hipMemcpyDeviceToHost, c->stream));
+        }
+        GL_HIP(hipStreamSynchronize(c->stream));
+        c->splat_hint = counts[0];
+        const bool overflow = counts[0] > sb.rec_cap || counts[1] > sb.bin_cap;
+        if (!overflow) break;
+        if (attempt == 1) return set_error(GL_ERR_DEVICE, "splat record buffer overflow after regrow");
+        // The binned copy holds each record once per touched tile (at most 4).
+        want = std::max<unsigned long long>(counts[0], (counts[1] * 2 + 2) / 3) + (1ull << 20);
     }
-    GL_HIP(hipStreamSynchronize(c->stream));
     const double t_d1 = now_ms();
-    float kms = 0.f;
+    float kms = 0.f, sms = 0.f;
     GL_HIP(hipEventElapsedTime(&kms, c->ev0, c->ev1));
+    GL_HIP(hipEventElapsedTime(&sms, c->ev1, c->ev2));
     c->last_kernel_ms = kms;
+    c->last_splat_ms = sms;
 
     if (stats) {
         stats->ms_upload = (float)(t_up1 - t_up0);
@@ -696,7 +805,6 @@ int gl_launch_ghost_render(gl_ctx* ctx, const gl_lens* lens, const gl_pair* pair
     if (!c) return st;
     std::lock_guard<std::mutex> g(c->mu);
     gl_lens* L = const_cast<gl_lens*>(lens);
-    std::lock_guard<std::mutex> gl_(L->mu);
     Launch Lc{c, L, p, std::vector<gl_pair>(pairs, pairs + n_pairs), {}, n_sources, width, height,
               fmt_w, fmt_h, fx0, fy0, sw, sh};
     Lc.src.reserve(n_sources);
@@ -730,8 +838,8 @@ int gl_filter_ghost_pairs(gl_ctx* ctx, const gl_lens* lens, float hw, float hh, 
     if (!c) return st;
     std::lock_guard<std::mutex> g(c->mu);
     gl_lens* L = const_cast<gl_lens*>(lens);
-    std::lock_guard<std::mutex> gl_(L->mu);
-    if ((st = lens_sync(L, 0.0f, c->stream))) return st;
+    LensDev D;
+    if ((st = lens_sync(L, 0.0f, c->stream, &D))) return st;
     const gl_spec s3[3] = {{650.f, 0, 0, 0}, {550.f, 0, 0, 0}, {450.f, 0, 0, 0}};
     std::vector<float> a3(3 * (size_t)n), b3(3 * (size_t)n);
     ior_tables(s.data(), n, s3, 3, a3.data(), b3.data());
@@ -742,8 +850,8 @@ int gl_filter_ghost_pairs(gl_ctx* ctx, const gl_lens* lens, float hw, float hh, 
         return st;
     DevBuf& bo = c->buf("f_out");
     if ((st = bo.ensure(cand.size() * 67 * sizeof(float4)))) return st;
-    hipLaunchKernelGGL(filter_probe_kernel, dim3((unsigned)cand.size()), dim3(128), 0, c->stream, L->d_surf, L->d_poly,
-                       L->d_arena, n, static_cast<const int2*>(d_c), static_cast<const float*>(d_a),
+    hipLaunchKernelGGL(filter_probe_kernel, dim3((unsigned)cand.size()), dim3(128), 0, c->stream, D.surf, D.poly,
+                       D.arena, n, static_cast<const int2*>(d_c), static_cast<const float*>(d_a),
                        static_cast<const float*>(d_b), s[0].semi_aperture, s[0].z - 20.0f, bo.as<float4>());
     GL_HIP(hipGetLastError());
     std::vector<float4> res(cand.size() * 67);
@@ -844,8 +952,8 @@ int gl_trace_ghost_rays(gl_ctx* ctx, const gl_lens* lens, int32_t a, int32_t b, 
     if (!c) return st;
     std::lock_guard<std::mutex> g(c->mu);
     gl_lens* L = const_cast<gl_lens*>(lens);
-    std::lock_guard<std::mutex> gl_(L->mu);
-    if ((st = lens_sync(L, fp64_rmin, c->stream))) return st;
+    LensDev D;
+    if ((st = lens_sync(L, fp64_rmin, c->stream, &D))) return st;
     // Per-ray IOR slices (each ray carries its own wavelength).
     std::vector<float> iat((size_t)n_rays * n), ibf((size_t)n_rays * n);
     for (int i = 0; i < n_rays; ++i) {
@@ -861,8 +969,8 @@ int gl_trace_ghost_rays(gl_ctx* ctx, const gl_lens* lens, int32_t a, int32_t b, 
     if ((st = bp.ensure((size_t)n_rays * 12)) || (st = bw.ensure((size_t)n_rays * 4)) ||
         (st = bs.ensure((size_t)n_rays * 4)) || (st = bg.ensure((size_t)n_rays * 4)))
         return st;
-    hipLaunchKernelGGL(ray_kernel, dim3((unsigned)((n_rays + 255) / 256)), dim3(256), 0, c->stream, L->d_surf, L->d_poly,
-                       L->d_arena, n, a, b, static_cast<const float*>(d_r), n_rays, static_cast<const float*>(d_a),
+    hipLaunchKernelGGL(ray_kernel, dim3((unsigned)((n_rays + 255) / 256)), dim3(256), 0, c->stream, D.surf, D.poly,
+                       D.arena, n, a, b, static_cast<const float*>(d_r), n_rays, static_cast<const float*>(d_a),
                        static_cast<const float*>(d_b), t_ge0, bp.as<float>(), bw.as<float>(), bs.as<int>(), bg.as<int>());
     GL_HIP(hipGetLastError());
     GL_HIP(hipMemcpyAsync(out_pos, bp.p, (size_t)n_rays * 12, hipMemcpyDeviceToHost, c->stream));

@@ -33,6 +33,8 @@ template <class T> __host__ __device__ inline const T GLC* as_const(const T* p) 
 enum : int32_t {
     SF_ACTIVE = 1, SF_TRANSFORMED = 2, SF_FLAT = 4, SF_POLYGON = 8, SF_FP64 = 16,
     SF_ASPECT = 32, SF_CYL_Y = 64,
+    SF_TAB_SORTED = 128,      // 1-D coating table keys non-decreasing (binary search allowed)
+    SF_SA_SORTED = 256,       // spectral-angular wavelength and angle axes non-decreasing
 };
 
 // One surface as the tracer reads it (built on the host from gl_surface).
@@ -111,10 +113,30 @@ GLD float ref_angle_deg(float cos_i, float n1, float aref) {   // trace_core.h:6
     if (s >= 1.0f) return 90.0f;
     return asinf(s) * (180.0f / 3.14159265358979323846f);
 }
-GLD float lerp1d(const float* t, int n, float key, bool disc, bool* out_disc) {  // trace_core.h:575-602
+GLD int2 ld_i2(const int2 GLC* p) {
+    const int GLC* q = reinterpret_cast<const int GLC*>(p);
+    return make_int2(q[0], q[1]);
+}
+
+// First i in [lo, hi] with key <= v[i * stride], for v non-decreasing and
+// key <= v[hi * stride]: the same cell the reference's forward scan stops at.
+GLD int lower_cell(const float* v, int stride, int lo, int hi, float key) {
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (key <= v[mid * stride]) hi = mid; else lo = mid + 1;
+    }
+    return lo;
+}
+GLD float lerp1d(const float* t, int n, float key, bool disc, bool* out_disc, bool sorted) {  // trace_core.h:575-602
     if (n <= 0) return 0.0f;
     if (key <= t[0]) { if (disc && key < t[0]) *out_disc = true; return t[1]; }
     if (key >= t[2 * (n - 1)]) { if (disc && key > t[2 * (n - 1)]) *out_disc = true; return t[2 * (n - 1) + 1]; }
+    if (sorted) {
+        const int i = lower_cell(t, 2, 1, n - 1, key);
+        float span = t[2 * i] - t[2 * (i - 1)];
+        float f = (span > 0.0f) ? (key - t[2 * (i - 1)]) / span : 0.0f;
+        return t[2 * (i - 1) + 1] + f * (t[2 * i + 1] - t[2 * (i - 1) + 1]);
+    }
     for (int i = 1; i < n; ++i)
         if (key <= t[2 * i]) {
             float span = t[2 * i] - t[2 * (i - 1)];
@@ -125,8 +147,11 @@ GLD float lerp1d(const float* t, int n, float key, bool disc, bool* out_disc) { 
 }
 
 // surface_weight (trace_core.h:635-783).
+// lr: this surface's entry of the per-lambda lookup row (the lambda-only part
+// of the weight, precomputed on the host with the same float operations), or
+// nullptr to evaluate it here.
 GLD float surface_weight(F3 dir, F3 nrm, float n1, float n2, const DevSurf GLC& s, const float* __restrict__ arena,
-                         float lam, bool reflecting) {
+                         float lam, bool reflecting, const int2 GLC* lr) {
     float cos_i = fabsf(dot3(nrm, dir));
     float R;
     const int m = s.coat_model;
@@ -139,6 +164,8 @@ GLD float surface_weight(F3 dir, F3 nrm, float n1, float n2, const DevSurf GLC& 
             for (int i = 1; i < s.ar_layers; ++i) R *= 0.25f;
             R = clamp01(R);
         }
+    } else if (m == GL_COAT_ARTIST && lr) {
+        R = __int_as_float(ld_i2(lr).y);
     } else if (m == GL_COAT_ARTIST) {
         const float inv_2s2 = 1.0f / (2.0f * 65.0f * 65.0f);
         float dr = lam - 650.0f, dg = lam - 550.0f, db = lam - 450.0f;
@@ -147,10 +174,14 @@ GLD float surface_weight(F3 dir, F3 nrm, float n1, float n2, const DevSurf GLC& 
         R = clamp01(s.tint_strength * (s.tint_r * br + s.tint_g * bg + s.tint_b * bb) / sum);
     } else if (m == GL_COAT_SPECTRAL || m == GL_COAT_ANGULAR) {
         if (s.tab_n <= 0) R = fresnel_R(cos_i, n1, n2);
-        else {
+        else if (m == GL_COAT_SPECTRAL && lr) {
+            const int2 e = ld_i2(lr);
+            if (e.x == -1) return 0.0f;
+            R = __int_as_float(e.y);
+        } else {
             float key = (m == GL_COAT_SPECTRAL) ? lam : ref_angle_deg(cos_i, n1, s.angle_ref_ior);
             bool d = false;
-            R = lerp1d(arena + s.tab_off, s.tab_n, key, s.discard != 0, &d);
+            R = lerp1d(arena + s.tab_off, s.tab_n, key, s.discard != 0, &d, (s.flags & SF_TAB_SORTED) != 0);
             if (d) return 0.0f;
         }
     } else if (m == GL_COAT_SPECTRAL_ANGULAR) {
@@ -163,18 +194,29 @@ GLD float surface_weight(F3 dir, F3 nrm, float n1, float n2, const DevSurf GLC& 
             float ang = ref_angle_deg(cos_i, n1, s.angle_ref_ior);
             bool disc = false;
             const bool oob = s.discard != 0;
+            const bool sorted = (s.flags & SF_SA_SORTED) != 0;
             int iw = 0; float fw = 0.0f;
-            if (lam <= wl[0]) { if (oob && lam < wl[0]) disc = true; }
+            if (lr) {
+                const int2 e = ld_i2(lr);
+                if (e.x == -1) return 0.0f;
+                iw = e.x; fw = __int_as_float(e.y);
+            } else if (lam <= wl[0]) { if (oob && lam < wl[0]) disc = true; }
             else if (lam >= wl[nw - 1]) { iw = nw > 1 ? nw - 2 : 0; fw = nw > 1 ? 1.0f : 0.0f; if (oob && lam > wl[nw - 1]) disc = true; }
-            else for (int i = 1; i < nw; ++i) if (lam <= wl[i]) {
-                iw = i - 1; float span = wl[i] - wl[i - 1];
-                fw = span > 0.0f ? (lam - wl[i - 1]) / span : 0.0f; break; }
+            else {
+                const int i = sorted ? lower_cell(wl, 1, 1, nw - 1, lam) : 1;
+                for (int j = i; j < nw; ++j) if (lam <= wl[j]) {
+                    iw = j - 1; float span = wl[j] - wl[j - 1];
+                    fw = span > 0.0f ? (lam - wl[j - 1]) / span : 0.0f; break; }
+            }
             int ia = 0; float fa = 0.0f;
             if (ang <= an[0]) { if (oob && ang < an[0]) disc = true; }
             else if (ang >= an[na - 1]) { ia = na > 1 ? na - 2 : 0; fa = na > 1 ? 1.0f : 0.0f; if (oob && ang > an[na - 1]) disc = true; }
-            else for (int i = 1; i < na; ++i) if (ang <= an[i]) {
-                ia = i - 1; float span = an[i] - an[i - 1];
-                fa = span > 0.0f ? (ang - an[i - 1]) / span : 0.0f; break; }
+            else {
+                const int i = sorted ? lower_cell(an, 1, 1, na - 1, ang) : 1;
+                for (int j = i; j < na; ++j) if (ang <= an[j]) {
+                    ia = j - 1; float span = an[j] - an[j - 1];
+                    fa = span > 0.0f ? (ang - an[j - 1]) / span : 0.0f; break; }
+            }
             if (disc) return 0.0f;
             const int iw1 = iw + 1 < nw ? iw + 1 : iw, ia1 = ia + 1 < na ? ia + 1 : ia;
             float r00 = rr[iw * na + ia], r01 = rr[iw * na + ia1], r10 = rr[iw1 * na + ia], r11 = rr[iw1 * na + ia1];
@@ -394,8 +436,12 @@ GLD F3 reflect(F3 dir, F3 nrm) { return normalized(sub(dir, mul(nrm, 2.0f * dot3
 
 // One surface interaction (trace_cuda.h:226-361 loop bodies).  Returns false
 // when the ray died; res.status is set to TIR for total internal reflection.
+// W = false skips the Fresnel / coating / attenuator weights (callers that
+// need only landings, statuses and segment counts).
+template <bool W>
 GLD bool step(const DevSurf GLC& s, const float GLC* poly, const float* __restrict__ arena, F3& o, F3& d,
-              float lam, float n2, bool reflecting, float reflect_ior, float& cur, TraceRes& res) {
+              float lam, float n2, bool reflecting, float reflect_ior, float& cur, TraceRes& res,
+              const int2 GLC* lr) {
     float t; F3 nrm;
     ++res.segs;
     if (!intersect(o, d, s, poly, t, nrm)) return false;
@@ -403,15 +449,19 @@ GLD bool step(const DevSurf GLC& s, const float GLC* poly, const float* __restri
     o = hit;
     const float n1 = cur;
     if (reflecting) {
-        res.w *= surface_weight(d, nrm, n1, n2, s, arena, lam, true);
-        res.w *= attenuator(hit, s);
+        if (W) {
+            res.w *= surface_weight(d, nrm, n1, n2, s, arena, lam, true, lr);
+            res.w *= attenuator(hit, s);
+        }
         d = reflect(d, nrm);
         cur = reflect_ior;
     } else {
         F3 nd;
         if (!refract(d, nrm, n1, n2, nd)) { res.status = GL_TRACE_TIR; return false; }
-        res.w *= surface_weight(d, nrm, n1, n2, s, arena, lam, false);
-        res.w *= attenuator(hit, s);
+        if (W) {
+            res.w *= surface_weight(d, nrm, n1, n2, s, arena, lam, false, lr);
+            res.w *= attenuator(hit, s);
+        }
         d = nd;
         cur = n2;
     }
@@ -431,32 +481,34 @@ GLD void to_sensor(F3 o, F3 d, bool t_ge0, TraceRes& res) {   // trace_cuda.h:36
 // d_trace_ghost_ray (trace_cuda.h:195-382) with per-lambda IOR slices.  The
 // surface record is copied from the constant address space per step; only
 // the fields a branch reads are loaded (scalar loads, uniform index).
+template <bool W = true>
 GLD TraceRes trace_ghost(const DevSurf GLC* S, int n, const float GLC* poly, const float* __restrict__ arena,
                          int a, int b, F3 o, F3 d, float lam, const float GLC* iat, const float GLC* ibf,
-                         bool t_ge0) {
+                         bool t_ge0, const int2 GLC* lrow = nullptr) {
     TraceRes res; res.status = GL_TRACE_VIGNETTED; res.w = 1.0f; res.segs = 0; res.px = res.py = res.pz = 0.f;
     float cur = 1.0f;
     for (int k = 0; k <= b; ++k) {
         const DevSurf GLC& s = S[k];
         if (!(s.flags & SF_ACTIVE)) continue;
-        if (!step(s, poly, arena, o, d, lam, iat[k], k == b, cur, cur, res)) return res;
+        if (!step<W>(s, poly, arena, o, d, lam, iat[k], k == b, cur, cur, res, lrow ? lrow + k : nullptr)) return res;
     }
     const float refl_a = iat[a];
     for (int k = b - 1; k >= a; --k) {
         const DevSurf GLC& s = S[k];
         if (!(s.flags & SF_ACTIVE)) continue;
-        if (!step(s, poly, arena, o, d, lam, ibf[k], k == a, refl_a, cur, res)) return res;
+        if (!step<W>(s, poly, arena, o, d, lam, ibf[k], k == a, refl_a, cur, res, lrow ? lrow + k : nullptr)) return res;
     }
     for (int k = a + 1; k < n; ++k) {
         const DevSurf GLC& s = S[k];
         if (!(s.flags & SF_ACTIVE)) continue;
-        if (!step(s, poly, arena, o, d, lam, iat[k], false, 0.0f, cur, res)) return res;
+        if (!step<W>(s, poly, arena, o, d, lam, iat[k], false, 0.0f, cur, res, lrow ? lrow + k : nullptr)) return res;
     }
     to_sensor(o, d, t_ge0, res);
     return res;
 }
 
 // d_trace_primary_ray (trace_cuda.h:111-172).
+template <bool W = true>
 GLD TraceRes trace_primary(const DevSurf GLC* S, int n, const float GLC* poly, const float* __restrict__ arena,
                            F3 o, F3 d, float lam, const float GLC* iat, bool t_ge0, F3* out_dir) {
     TraceRes res; res.status = GL_TRACE_VIGNETTED; res.w = 1.0f; res.segs = 0; res.px = res.py = res.pz = 0.f;
@@ -464,7 +516,7 @@ GLD TraceRes trace_primary(const DevSurf GLC* S, int n, const float GLC* poly, c
     for (int k = 0; k < n; ++k) {
         const DevSurf GLC& s = S[k];
         if (!(s.flags & SF_ACTIVE)) continue;
-        if (!step(s, poly, arena, o, d, lam, iat[k], false, 0.0f, cur, res)) return res;
+        if (!step<W>(s, poly, arena, o, d, lam, iat[k], false, 0.0f, cur, res, nullptr)) return res;
     }
     to_sensor(o, d, t_ge0, res);
     if (out_dir) *out_dir = d;

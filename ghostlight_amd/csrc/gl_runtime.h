@@ -45,8 +45,10 @@ struct gl_ctx {
     int device = 0;
     std::mutex mu;                       // one render at a time per context
     std::map<std::string, gl::DevBuf> bufs;
-    hipEvent_t ev0 = nullptr, ev1 = nullptr;
-    float last_kernel_ms = 0.0f;         // main trace+splat kernel, event-timed
+    hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr;
+    float last_kernel_ms = 0.0f;         // main trace kernel, event-timed
+    float last_splat_ms = 0.0f;          // tiled splat resolve after it, event-timed
+    unsigned long long splat_hint = 0;   // deposit records the last render produced
     // Host caches (pupil grid, spectral table) keyed like the reference's
     // GridKey / SpecKey (ghost_render.cu:515-556).
     std::vector<uint8_t> grid_key, spec_key;
@@ -60,24 +62,41 @@ struct gl_ctx {
     gl::DevBuf& buf(const char* name) { return bufs[name]; }
 };
 
-// Device-resident prescription.
+namespace gl {
+// Device view of a lens as the kernels read it.
+struct LensDev {
+    const DevSurf* surf = nullptr;
+    const float* poly = nullptr;             // per polygon surface: c[32] | s[32]
+    const float* arena = nullptr;
+};
+}  // namespace gl
+
+// Device-resident prescription.  The host copy never changes after creation;
+// device variants (one per FP64-radius threshold in use) are created on first
+// use and never modified, so concurrent renders on one lens need the mutex
+// only while a variant is looked up or created.
 struct gl_lens {
     std::vector<gl_surface> host;
     std::vector<float> arena;
-    std::vector<gl::DevSurf> dsurf;
-    std::vector<float> poly;             // per polygon surface: c[32] | s[32]
-    float fp64_rmin = -1.0f;             // value the device copy was built with
     int device = -1;
-    gl::DevSurf* d_surf = nullptr;
     float* d_arena = nullptr;
-    float* d_poly = nullptr;
+    struct Variant {
+        float fp64_rmin;
+        gl::DevSurf* d_surf;
+        float* d_poly;
+    };
+    std::vector<Variant> variants;
     std::mutex mu;
 };
 
 namespace gl {
 gl_ctx* resolve_ctx(gl_ctx* ctx, int* status);
-int lens_sync(gl_lens* lens, float fp64_rmin, hipStream_t stream);
-void build_devsurfs(const gl_surface* s, int n, float fp64_rmin, std::vector<DevSurf>& out, std::vector<float>& poly);
+int lens_sync(gl_lens* lens, float fp64_rmin, hipStream_t stream, LensDev* out);
+void build_devsurfs(const gl_surface* s, int n, float fp64_rmin, const float* arena, int arena_len,
+                    std::vector<DevSurf>& out, std::vector<float>& poly);
+// Per-lambda coating lookups of the lambda-only part of surface_weight, one
+// int2 per (lambda, surface): see LambdaRow in gl_device.h.
+void lambda_rows(const gl_surface* s, int n, const float* arena, const gl_spec* spec, int ns, std::vector<int2>& out);
 DevMask build_mask(const gl_surface* s, int n, int blades_override, float rotation_deg);
 
 // host preparation (hostprep.cpp)

@@ -316,7 +316,14 @@ static bool rot_identity(const float m[9]) {
             std::fabs(m[6]) < eps && std::fabs(m[7]) < eps && std::fabs(m[8] - 1.f) < eps);
 }
 
-void build_devsurfs(const gl_surface* s, int n, float fp64_rmin, std::vector<DevSurf>& out, std::vector<float>& poly) {
+static bool nondecreasing(const float* v, int n, int stride) {
+    for (int i = 1; i < n; ++i)
+        if (!(v[(size_t)i * stride] >= v[(size_t)(i - 1) * stride])) return false;
+    return true;
+}
+
+void build_devsurfs(const gl_surface* s, int n, float fp64_rmin, const float* arena, int arena_len,
+                    std::vector<DevSurf>& out, std::vector<float>& poly) {
     out.assign(n, DevSurf{});
     poly.clear();
     for (int i = 0; i < n; ++i) {
@@ -344,6 +351,14 @@ void build_devsurfs(const gl_surface* s, int n, float fp64_rmin, std::vector<Dev
             poly_normals(g.aperture_blades, g.aperture_rotation_rad, g.semi_aperture,
                          &poly[d.poly_off], &poly[d.poly_off + GL_MAX_BLADES], &d.apothem);
         }
+        if (g.table_count > 0 && g.table_offset >= 0 && g.table_offset + 2 * g.table_count <= arena_len &&
+            nondecreasing(arena + g.table_offset, g.table_count, 2))
+            f |= SF_TAB_SORTED;
+        if (g.sa_n_wavelengths > 0 && g.sa_n_angles > 0 && g.sa_offset >= 0 &&
+            g.sa_offset + g.sa_n_wavelengths + g.sa_n_angles <= arena_len &&
+            nondecreasing(arena + g.sa_offset, g.sa_n_wavelengths, 1) &&
+            nondecreasing(arena + g.sa_offset + g.sa_n_wavelengths, g.sa_n_angles, 1))
+            f |= SF_SA_SORTED;
         d.flags = f;
         d.coat_model = g.coating_model; d.ar_layers = g.ar_layers; d.discard = g.out_of_range_discard;
         d.angle_ref_ior = g.angle_ref_ior;
@@ -357,6 +372,61 @@ void build_devsurfs(const gl_surface* s, int n, float fp64_rmin, std::vector<Dev
         d.g_dx = g.gauss_decenter_x; d.g_dy = g.gauss_decenter_y;
     }
     if (poly.empty()) poly.assign(2 * GL_MAX_BLADES, 0.0f);
+}
+
+// The lambda-only part of surface_weight for every (lambda, surface), with the
+// device's float operation order (trace_core.h:575-602, 657-770):
+//   SPECTRAL_ANGULAR  x = wavelength cell iw, y = bits of its fraction fw
+//   SPECTRAL          x = -2, y = bits of the interpolated reflectance
+//   ARTIST            x = -2, y = bits of the tinted reflectance
+//   out of range with discard: x = -1 (weight 0);  other models: x = -3.
+void lambda_rows(const gl_surface* s, int n, const float* arena, const gl_spec* spec, int ns, std::vector<int2>& out) {
+    out.assign((size_t)ns * n, make_int2(-3, 0));
+    auto bits = [](float f) { int i; std::memcpy(&i, &f, 4); return i; };
+    for (int si = 0; si < ns; ++si) {
+        const float lam = spec[si].lambda;
+        for (int k = 0; k < n; ++k) {
+            const gl_surface& g = s[k];
+            int2& e = out[(size_t)si * n + k];
+            const bool oob = g.out_of_range_discard != 0;
+            if (g.coating_model == GL_COAT_SPECTRAL_ANGULAR && g.sa_n_wavelengths > 0 && g.sa_n_angles > 0) {
+                const int nw = g.sa_n_wavelengths;
+                const float* wl = arena + g.sa_offset;
+                int iw = 0; float fw = 0.0f; bool disc = false;
+                if (lam <= wl[0]) { if (oob && lam < wl[0]) disc = true; }
+                else if (lam >= wl[nw - 1]) { iw = nw > 1 ? nw - 2 : 0; fw = nw > 1 ? 1.0f : 0.0f; if (oob && lam > wl[nw - 1]) disc = true; }
+                else for (int i = 1; i < nw; ++i) if (lam <= wl[i]) {
+                    iw = i - 1; const float span = wl[i] - wl[i - 1];
+                    fw = span > 0.0f ? (lam - wl[i - 1]) / span : 0.0f; break; }
+                e = disc ? make_int2(-1, 0) : make_int2(iw, bits(fw));
+            } else if (g.coating_model == GL_COAT_SPECTRAL && g.table_count > 0) {
+                const float* t = arena + g.table_offset;
+                const int tn = g.table_count;
+                bool disc = false; float R;
+                if (lam <= t[0]) { if (oob && lam < t[0]) disc = true; R = t[1]; }
+                else if (lam >= t[2 * (tn - 1)]) { if (oob && lam > t[2 * (tn - 1)]) disc = true; R = t[2 * (tn - 1) + 1]; }
+                else {
+                    R = t[2 * (tn - 1) + 1];
+                    for (int i = 1; i < tn; ++i)
+                        if (lam <= t[2 * i]) {
+                            const float span = t[2 * i] - t[2 * (i - 1)];
+                            const float f = (span > 0.0f) ? (lam - t[2 * (i - 1)]) / span : 0.0f;
+                            R = t[2 * (i - 1) + 1] + f * (t[2 * i + 1] - t[2 * (i - 1) + 1]);
+                            break;
+                        }
+                }
+                e = disc ? make_int2(-1, 0) : make_int2(-2, bits(R));
+            } else if (g.coating_model == GL_COAT_ARTIST) {
+                const float inv_2s2 = 1.0f / (2.0f * 65.0f * 65.0f);
+                const float dr = lam - 650.0f, dg = lam - 550.0f, db = lam - 450.0f;
+                const float br = std::exp(-dr * dr * inv_2s2), bg = std::exp(-dg * dg * inv_2s2), bb = std::exp(-db * db * inv_2s2);
+                const float sum = br + bg + bb;
+                float R = g.tint_strength * (g.tint_r * br + g.tint_g * bg + g.tint_b * bb) / sum;
+                R = std::fmin(std::fmax(R, 0.0f), 1.0f);
+                e = make_int2(-2, bits(R));
+            }
+        }
+    }
 }
 
 double now_ms() {

@@ -2,6 +2,8 @@
 // device-resident lenses.
 #include "gl_runtime.h"
 
+#include <algorithm>
+
 #include <cstdio>
 
 namespace gl {
@@ -54,6 +56,7 @@ static int ctx_init(gl_ctx* c) {
     GL_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
     GL_HIP(hipEventCreate(&c->ev0));
     GL_HIP(hipEventCreate(&c->ev1));
+    GL_HIP(hipEventCreate(&c->ev2));
     return GL_OK;
 }
 
@@ -74,25 +77,37 @@ gl_ctx* resolve_ctx(gl_ctx* ctx, int* status) {
     return def;
 }
 
-int lens_sync(gl_lens* L, float fp64_rmin, hipStream_t stream) {
+int lens_sync(gl_lens* L, float fp64_rmin, hipStream_t stream, LensDev* out) {
     int dev = 0;
     GL_HIP(hipGetDevice(&dev));
-    if (L->d_surf && L->device == dev && L->fp64_rmin == fp64_rmin) return GL_OK;
-    build_devsurfs(L->host.data(), (int)L->host.size(), fp64_rmin, L->dsurf, L->poly);
+    std::lock_guard<std::mutex> g(L->mu);
     if (L->device != dev) {
-        if (L->d_surf) { (void)hipFree(L->d_surf); (void)hipFree(L->d_arena); (void)hipFree(L->d_poly); }
-        L->d_surf = nullptr; L->d_arena = nullptr; L->d_poly = nullptr;
-        GL_HIP(hipMalloc(&L->d_surf, GL_MAX_SURFACES * sizeof(DevSurf)));
-        GL_HIP(hipMalloc(&L->d_arena, L->arena.size() * sizeof(float)));
-        GL_HIP(hipMalloc(&L->d_poly, GL_MAX_SURFACES * 2 * GL_MAX_BLADES * sizeof(float)));
-        GL_HIP(hipMemcpyAsync(L->d_arena, L->arena.data(), L->arena.size() * sizeof(float),
-                              hipMemcpyHostToDevice, stream));
+        if (L->device >= 0) return set_error(GL_ERR_INVALID, "lens was created for device %d, used on device %d",
+                                             L->device, dev);
+        GL_HIP(hipMalloc(&L->d_arena, std::max<size_t>(1, L->arena.size()) * sizeof(float)));
+        GL_HIP(hipMemcpyAsync(L->d_arena, L->arena.data(), L->arena.size() * sizeof(float), hipMemcpyHostToDevice,
+                              stream));
         L->device = dev;
     }
-    GL_HIP(hipMemcpyAsync(L->d_surf, L->dsurf.data(), L->dsurf.size() * sizeof(DevSurf), hipMemcpyHostToDevice, stream));
-    GL_HIP(hipMemcpyAsync(L->d_poly, L->poly.data(), L->poly.size() * sizeof(float), hipMemcpyHostToDevice, stream));
-    GL_HIP(hipStreamSynchronize(stream));
-    L->fp64_rmin = fp64_rmin;
+    const gl_lens::Variant* v = nullptr;
+    for (const auto& x : L->variants)
+        if (x.fp64_rmin == fp64_rmin) { v = &x; break; }
+    if (!v) {
+        std::vector<DevSurf> ds;
+        std::vector<float> poly;
+        build_devsurfs(L->host.data(), (int)L->host.size(), fp64_rmin, L->arena.data(), (int)L->arena.size(), ds, poly);
+        gl_lens::Variant nv{fp64_rmin, nullptr, nullptr};
+        GL_HIP(hipMalloc(&nv.d_surf, std::max<size_t>(1, ds.size()) * sizeof(DevSurf)));
+        GL_HIP(hipMalloc(&nv.d_poly, std::max<size_t>(1, poly.size()) * sizeof(float)));
+        GL_HIP(hipMemcpyAsync(nv.d_surf, ds.data(), ds.size() * sizeof(DevSurf), hipMemcpyHostToDevice, stream));
+        GL_HIP(hipMemcpyAsync(nv.d_poly, poly.data(), poly.size() * sizeof(float), hipMemcpyHostToDevice, stream));
+        GL_HIP(hipStreamSynchronize(stream));
+        L->variants.push_back(nv);
+        v = &L->variants.back();
+    }
+    out->surf = v->d_surf;
+    out->poly = v->d_poly;
+    out->arena = L->d_arena;
     return GL_OK;
 }
 
@@ -127,6 +142,7 @@ void gl_ctx_destroy(gl_ctx* c) {
         for (auto& kv : c->bufs) kv.second.release();
         if (c->ev0) (void)hipEventDestroy(c->ev0);
         if (c->ev1) (void)hipEventDestroy(c->ev1);
+        if (c->ev2) (void)hipEventDestroy(c->ev2);
         if (c->stream) (void)hipStreamDestroy(c->stream);
     }
     delete c;
@@ -138,6 +154,44 @@ int gl_ctx_synchronize(gl_ctx* ctx) {
     if (!c) return st;
     GL_HIP(hipStreamSynchronize(c->stream));
     return GL_OK;
+}
+
+int gl_device_memory(uint64_t* free_bytes, uint64_t* total_bytes) {
+    size_t f = 0, t = 0;
+    GL_HIP(hipMemGetInfo(&f, &t));
+    if (free_bytes) *free_bytes = f;
+    if (total_bytes) *total_bytes = t;
+    return GL_OK;
+}
+
+uint64_t gl_ctx_memory_bytes(gl_ctx* ctx) {
+    int st;
+    gl_ctx* c = gl::resolve_ctx(ctx, &st);
+    if (!c) return 0;
+    std::lock_guard<std::mutex> g(c->mu);
+    uint64_t n = 0;
+    for (const auto& kv : c->bufs) n += kv.second.cap;
+    return n;
+}
+
+int gl_ctx_trim(gl_ctx* ctx) {
+    int st;
+    gl_ctx* c = gl::resolve_ctx(ctx, &st);
+    if (!c) return st;
+    std::lock_guard<std::mutex> g(c->mu);
+    GL_HIP(hipStreamSynchronize(c->stream));
+    for (auto& kv : c->bufs) kv.second.release();
+    c->bufs.clear();
+    c->grid_uploaded = false;
+    c->spec_uploaded = false;
+    c->veil_baked_p = -1.0f;
+    return GL_OK;
+}
+
+float gl_ctx_last_splat_ms(gl_ctx* ctx) {
+    int st;
+    gl_ctx* c = gl::resolve_ctx(ctx, &st);
+    return c ? c->last_splat_ms : -1.0f;
 }
 
 float gl_ctx_last_kernel_ms(gl_ctx* ctx) {
@@ -172,7 +226,8 @@ int gl_lens_create(const gl_surface* s, int32_t n, const float* arena, int32_t a
 
 void gl_lens_destroy(gl_lens* L) {
     if (!L) return;
-    if (L->d_surf) { (void)hipFree(L->d_surf); (void)hipFree(L->d_arena); (void)hipFree(L->d_poly); }
+    for (auto& v : L->variants) { (void)hipFree(v.d_surf); (void)hipFree(v.d_poly); }
+    if (L->d_arena) (void)hipFree(L->d_arena);
     delete L;
 }
 

@@ -1,0 +1,301 @@
+// splat.hip — the ghost splat without global float atomics in the hot loop.
+//
+// The trace kernel does not add into the image.  Each surviving (ray, lambda)
+// becomes one 16-byte deposit record {px, py, power, lambda index}, staged per
+// wave in LDS and flushed to HBM in dense runs (one 64-bit atomic reservation
+// per ~256 records).  The image is then built tile by tile:
+//
+//   bin_count    per-workgroup LDS histogram of records over screen tiles
+//                (a bilinear footprint touches 1, 2 or 4 tiles), merged into
+//                per-tile counts
+//   bin_scan     one workgroup: exclusive scan -> tile starts / cursors and the
+//                work-chunk list (a tile with many records is split into chunks)
+//   bin_scatter  copies each record into its tiles' ranges
+//   resolve      persistent workgroups pull chunks from a queue, accumulate the
+//                chunk's bilinear contributions in an LDS tile (LDS atomics),
+//                then write the tile out: plain read-add-write when the tile has
+//                one chunk (its pixels belong to that workgroup alone), one
+//                atomic per pixel otherwise.
+//
+// Each contribution is the same product the reference's splat adds
+// (ghost_render.cu:430-462): colour = power * cmf_c(lambda), weight = the
+// bilinear cell overlap, skipped when |colour| <= 1e-14.
+#include "gl_runtime.h"
+#include "gl_splat.h"
+
+#include <algorithm>
+#include <cmath>
+
+namespace gl {
+
+static constexpr int kBinBlocks = 1024;
+static constexpr int kBinThreads = 256;
+static constexpr int kChunk = 8192;          // records per resolve work item
+static constexpr int kResolveBlocks = 1024;
+static constexpr int kMaxTiles = 8192;       // LDS: 12 B a tile in bin_scatter
+
+// Bilinear footprint of a record: pixels x0..x0+1, y0..y0+1 clipped to the
+// image (the splat's cell arithmetic, ghost_render.cu:430-441).
+struct Foot {
+    int x0, y0;
+    float fx, fy;
+};
+
+__device__ __forceinline__ Foot foot_of(float px, float py) {
+    Foot f;
+    f.x0 = (int)floorf(px - 0.5f);
+    f.y0 = (int)floorf(py - 0.5f);
+    f.fx = (px - 0.5f) - (float)f.x0;
+    f.fy = (py - 0.5f) - (float)f.y0;
+    return f;
+}
+
+// Tiles touched by a record (clipped footprint), as a [tx0, tx1] x [ty0, ty1] box.
+__device__ __forceinline__ bool tiles_of(const float4& r, const SplatGeom& g, int& tx0, int& tx1, int& ty0, int& ty1) {
+    const Foot f = foot_of(r.x, r.y);
+    const int xa = max(f.x0, 0), xb = min(f.x0 + 1, g.width - 1);
+    const int ya = max(f.y0, 0), yb = min(f.y0 + 1, g.height - 1);
+    if (xa > xb || ya > yb) return false;
+    tx0 = xa >> g.tile_shift; tx1 = xb >> g.tile_shift;
+    ty0 = ya >> g.tile_shift; ty1 = yb >> g.tile_shift;
+    return true;
+}
+
+__device__ __forceinline__ unsigned long long n_records(const SplatBufs& B) {
+    const unsigned long long n = *B.rec_count;
+    return n < B.rec_cap ? n : B.rec_cap;
+}
+
+__global__ __launch_bounds__(256) void splat_clear_kernel(SplatBufs B, int n_tiles) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n_tiles) { B.tile_count[i] = 0u; }
+    if (i == 0) { *B.rec_count = 0ull; *B.queue = 0u; *B.bin_total = 0ull; }
+}
+
+__global__ __launch_bounds__(kBinThreads) void bin_count_kernel(SplatBufs B, SplatGeom g) {
+    extern __shared__ unsigned hist[];
+    for (int t = threadIdx.x; t < g.n_tiles; t += blockDim.x) hist[t] = 0u;
+    __syncthreads();
+    const unsigned long long n = n_records(B);
+    const unsigned long long per = (n + gridDim.x - 1) / gridDim.x;
+    const unsigned long long r0 = per * blockIdx.x, r1 = min(n, r0 + per);
+    for (unsigned long long i = r0 + threadIdx.x; i < r1; i += blockDim.x) {
+        int tx0, tx1, ty0, ty1;
+        if (!tiles_of(B.rec[i], g, tx0, tx1, ty0, ty1)) continue;
+        for (int ty = ty0; ty <= ty1; ++ty)
+            for (int tx = tx0; tx <= tx1; ++tx) atomicAdd(&hist[ty * g.tiles_x + tx], 1u);
+    }
+    __syncthreads();
+    for (int t = threadIdx.x; t < g.n_tiles; t += blockDim.x)
+        if (hist[t]) atomicAdd(&B.tile_count[t], hist[t]);
+}
+
+// Exclusive scan of tile counts (one workgroup of 1024 lanes, 8 tiles a lane
+// at most): tile_start, tile_cursor, chunk_start, and the totals.
+__global__ __launch_bounds__(1024) void bin_scan_kernel(SplatBufs B, SplatGeom g) {
+    __shared__ unsigned long long s_rec[1024];
+    __shared__ unsigned s_chk[1024];
+    const int per = (g.n_tiles + 1023) / 1024;
+    const int t0 = threadIdx.x * per, t1 = min(g.n_tiles, t0 + per);
+    unsigned long long rs = 0;
+    unsigned cs = 0;
+    for (int t = t0; t < t1; ++t) {
+        const unsigned c = B.tile_count[t];
+        rs += c;
+        cs += (c + kChunk - 1) / kChunk;
+    }
+    s_rec[threadIdx.x] = rs;
+    s_chk[threadIdx.x] = cs;
+    __syncthreads();
+    // Hillis-Steele inclusive scan over the 1024 lane totals.
+    for (int off = 1; off < 1024; off <<= 1) {
+        const unsigned long long a = threadIdx.x >= (unsigned)off ? s_rec[threadIdx.x - off] : 0ull;
+        const unsigned b = threadIdx.x >= (unsigned)off ? s_chk[threadIdx.x - off] : 0u;
+        __syncthreads();
+        s_rec[threadIdx.x] += a;
+        s_chk[threadIdx.x] += b;
+        __syncthreads();
+    }
+    unsigned long long run = threadIdx.x ? s_rec[threadIdx.x - 1] : 0ull;
+    unsigned crun = threadIdx.x ? s_chk[threadIdx.x - 1] : 0u;
+    for (int t = t0; t < t1; ++t) {
+        const unsigned c = B.tile_count[t];
+        B.tile_start[t] = run;
+        B.tile_cursor[t] = run;
+        B.chunk_start[t] = crun;
+        run += c;
+        crun += (c + kChunk - 1) / kChunk;
+    }
+    if (threadIdx.x == 1023) {
+        *B.bin_total = s_rec[1023];
+        B.chunk_start[g.n_tiles] = s_chk[1023];
+    }
+}
+
+__global__ __launch_bounds__(kBinThreads) void bin_scatter_kernel(SplatBufs B, SplatGeom g) {
+    extern __shared__ unsigned smem[];
+    unsigned* hist = smem;                       // counts, then per-block base offsets
+    unsigned long long* base = reinterpret_cast<unsigned long long*>(smem + ((g.n_tiles + 1) & ~1));
+    for (int t = threadIdx.x; t < g.n_tiles; t += blockDim.x) hist[t] = 0u;
+    __syncthreads();
+    const unsigned long long n = n_records(B);
+    const unsigned long long per = (n + gridDim.x - 1) / gridDim.x;
+    const unsigned long long r0 = per * blockIdx.x, r1 = min(n, r0 + per);
+    for (unsigned long long i = r0 + threadIdx.x; i < r1; i += blockDim.x) {
+        int tx0, tx1, ty0, ty1;
+        if (!tiles_of(B.rec[i], g, tx0, tx1, ty0, ty1)) continue;
+        for (int ty = ty0; ty <= ty1; ++ty)
+            for (int tx = tx0; tx <= tx1; ++tx) atomicAdd(&hist[ty * g.tiles_x + tx], 1u);
+    }
+    __syncthreads();
+    for (int t = threadIdx.x; t < g.n_tiles; t += blockDim.x) {
+        const unsigned c = hist[t];
+        base[t] = c ? atomicAdd(&B.tile_cursor[t], (unsigned long long)c) : 0ull;
+        hist[t] = 0u;
+    }
+    __syncthreads();
+    for (unsigned long long i = r0 + threadIdx.x; i < r1; i += blockDim.x) {
+        const float4 r = B.rec[i];
+        int tx0, tx1, ty0, ty1;
+        if (!tiles_of(r, g, tx0, tx1, ty0, ty1)) continue;
+        for (int ty = ty0; ty <= ty1; ++ty)
+            for (int tx = tx0; tx <= tx1; ++tx) {
+                const int t = ty * g.tiles_x + tx;
+                const unsigned long long slot = base[t] + atomicAdd(&hist[t], 1u);
+                if (slot < B.bin_cap) B.binned[slot] = r;
+            }
+    }
+}
+
+// One work item = up to kChunk records of one tile.
+__global__ __launch_bounds__(256) void resolve_kernel(SplatBufs B, SplatGeom g, const float4* __restrict__ spec,
+                                                      float* __restrict__ out_r, float* __restrict__ out_g,
+                                                      float* __restrict__ out_b) {
+    extern __shared__ float tile[];               // 3 planes of TS x TS
+    __shared__ unsigned s_item;
+    const int TS = 1 << g.tile_shift, TT = TS * TS;
+    const unsigned n_items = B.chunk_start[g.n_tiles];
+    const unsigned long long bin_n = min(*B.bin_total, B.bin_cap);
+    for (;;) {
+        if (threadIdx.x == 0) s_item = atomicAdd(B.queue, 1u);
+        __syncthreads();
+        const unsigned item = s_item;
+        __syncthreads();
+        if (item >= n_items) break;
+        // Tile of this item: last t with chunk_start[t] <= item.
+        int lo = 0, hi = g.n_tiles - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (B.chunk_start[mid] <= item) lo = mid; else hi = mid - 1;
+        }
+        const int t = lo;
+        const unsigned k = item - B.chunk_start[t];
+        const unsigned long long rb = B.tile_start[t] + (unsigned long long)k * kChunk;
+        const unsigned long long re = min(min(B.tile_start[t] + B.tile_count[t], rb + kChunk), bin_n);
+        const int nchunks = (int)(B.chunk_start[t + 1] - B.chunk_start[t]);
+        for (int i = threadIdx.x; i < 3 * TT; i += blockDim.x) tile[i] = 0.0f;
+        __syncthreads();
+        const int ox = (t % g.tiles_x) << g.tile_shift, oy = (t / g.tiles_x) << g.tile_shift;
+        for (unsigned long long i = rb + threadIdx.x; i < re; i += blockDim.x) {
+            const float4 r = B.binned[i];
+            const Foot f = foot_of(r.x, r.y);
+            const float4 s = spec[__float_as_int(r.w)];
+            const float cr = r.z * s.y, cg = r.z * s.z, cb = r.z * s.w;
+            const float w00 = (1.0f - f.fx) * (1.0f - f.fy), w10 = f.fx * (1.0f - f.fy);
+            const float w01 = (1.0f - f.fx) * f.fy, w11 = f.fx * f.fy;
+            const int xs[4] = {f.x0, f.x0 + 1, f.x0, f.x0 + 1}, ys[4] = {f.y0, f.y0, f.y0 + 1, f.y0 + 1};
+            const float ws[4] = {w00, w10, w01, w11};
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int lx = xs[q] - ox, ly = ys[q] - oy;
+                if (xs[q] < 0 || xs[q] >= g.width || ys[q] < 0 || ys[q] >= g.height) continue;
+                if (lx < 0 || lx >= TS || ly < 0 || ly >= TS) continue;
+                const int p = ly * TS + lx;
+                if (fabsf(cr) > 1e-14f) atomicAdd(&tile[p], cr * ws[q]);
+                if (fabsf(cg) > 1e-14f) atomicAdd(&tile[TT + p], cg * ws[q]);
+                if (fabsf(cb) > 1e-14f) atomicAdd(&tile[2 * TT + p], cb * ws[q]);
+            }
+        }
+        __syncthreads();
+        for (int p = threadIdx.x; p < TT; p += blockDim.x) {
+            const int x = ox + (p & (TS - 1)), y = oy + (p >> g.tile_shift);
+            if (x >= g.width || y >= g.height) continue;
+            const size_t o = (size_t)y * g.width + x;
+            const float vr = tile[p], vg = tile[TT + p], vb = tile[2 * TT + p];
+            if (nchunks == 1) {
+                if (vr != 0.0f) out_r[o] += vr;
+                if (vg != 0.0f) out_g[o] += vg;
+                if (vb != 0.0f) out_b[o] += vb;
+            } else {
+                if (vr != 0.0f) atomicAdd(out_r + o, vr);
+                if (vg != 0.0f) atomicAdd(out_g + o, vg);
+                if (vb != 0.0f) atomicAdd(out_b + o, vb);
+            }
+        }
+        __syncthreads();
+    }
+}
+
+SplatGeom splat_geom(int width, int height) {
+    SplatGeom g{};
+    g.width = width;
+    g.height = height;
+    g.tile_shift = 5;
+    for (;;) {
+        const int TS = 1 << g.tile_shift;
+        g.tiles_x = (width + TS - 1) / TS;
+        g.tiles_y = (height + TS - 1) / TS;
+        const long long nt = (long long)g.tiles_x * g.tiles_y;
+        if (nt <= kMaxTiles || g.tile_shift >= 6) { g.n_tiles = (int)nt; break; }
+        ++g.tile_shift;
+    }
+    return g;
+}
+
+int splat_prepare(gl_ctx* c, const SplatGeom& g, unsigned long long rec_cap, SplatBufs& B) {
+    if (g.n_tiles > kMaxTiles) return set_error(GL_ERR_INVALID, "image too large for the tiled splat (%d tiles)", g.n_tiles);
+    DevBuf &brec = c->buf("splat_rec"), &bbin = c->buf("splat_bin"), &bt = c->buf("splat_tiles");
+    int st;
+    const unsigned long long bin_cap = rec_cap + rec_cap / 2;
+    if ((st = brec.ensure((size_t)rec_cap * sizeof(float4))) || (st = bbin.ensure((size_t)bin_cap * sizeof(float4))))
+        return st;
+    // tile arrays: count (u32), start/cursor (u64), chunk_start (u32, n+1), scalars
+    const size_t nt = (size_t)g.n_tiles;
+    const size_t bytes = nt * 4 + nt * 8 * 2 + (nt + 1) * 4 + 64;
+    if ((st = bt.ensure(bytes + 256))) return st;
+    char* p = bt.as<char>();
+    B.rec = brec.as<float4>();
+    B.rec_cap = brec.cap / sizeof(float4);
+    B.binned = bbin.as<float4>();
+    B.bin_cap = bbin.cap / sizeof(float4);
+    B.tile_start = reinterpret_cast<unsigned long long*>(p);
+    B.tile_cursor = B.tile_start + nt;
+    B.rec_count = B.tile_cursor + nt;
+    B.bin_total = B.rec_count + 1;
+    B.tile_count = reinterpret_cast<unsigned*>(B.bin_total + 1);
+    B.chunk_start = B.tile_count + nt;
+    B.queue = B.chunk_start + nt + 1;
+    return GL_OK;
+}
+
+int splat_clear(hipStream_t s, const SplatGeom& g, const SplatBufs& B) {
+    hipLaunchKernelGGL(splat_clear_kernel, dim3((unsigned)((g.n_tiles + 255) / 256)), dim3(256), 0, s, B, g.n_tiles);
+    GL_HIP(hipGetLastError());
+    return GL_OK;
+}
+
+int splat_resolve(hipStream_t s, const SplatGeom& g, const SplatBufs& B, const float4* spec, float* out_r, float* out_g,
+                  float* out_b) {
+    const size_t hist_bytes = (size_t)g.n_tiles * 4;
+    const size_t scat_bytes = (size_t)((g.n_tiles + 1) & ~1) * 4 + (size_t)g.n_tiles * 8;
+    hipLaunchKernelGGL(bin_count_kernel, dim3(kBinBlocks), dim3(kBinThreads), hist_bytes, s, B, g);
+    hipLaunchKernelGGL(bin_scan_kernel, dim3(1), dim3(1024), 0, s, B, g);
+    hipLaunchKernelGGL(bin_scatter_kernel, dim3(kBinBlocks), dim3(kBinThreads), scat_bytes, s, B, g);
+    const int TS = 1 << g.tile_shift;
+    hipLaunchKernelGGL(resolve_kernel, dim3(kResolveBlocks), dim3(256), (size_t)3 * TS * TS * 4, s, B, g, spec, out_r,
+                       out_g, out_b);
+    GL_HIP(hipGetLastError());
+    return GL_OK;
+}
+
+}  // namespace gl

@@ -20,6 +20,13 @@ def device_available() -> bool:
         return False
 
 
+def device_memory():
+    """(free, total) bytes of the current device."""
+    f, t = C.c_uint64(0), C.c_uint64(0)
+    _abi.check(lib().gl_device_memory(C.byref(f), C.byref(t)), "gl_device_memory")
+    return int(f.value), int(t.value)
+
+
 class Context:
     """One HIP stream and its grow-only device buffers.  Renders on different
     contexts may run concurrently on the device."""
@@ -37,6 +44,17 @@ class Context:
 
     def last_kernel_ms(self) -> float:
         return float(lib().gl_ctx_last_kernel_ms(self._h))
+
+    def last_splat_ms(self) -> float:
+        return float(lib().gl_ctx_last_splat_ms(self._h))
+
+    def memory_bytes(self) -> int:
+        """Device bytes this context's buffer cache holds."""
+        return int(lib().gl_ctx_memory_bytes(self._h))
+
+    def trim(self) -> None:
+        """Release the context's cached device buffers."""
+        _abi.check(lib().gl_ctx_trim(self._h), "gl_ctx_trim")
 
     def close(self):
         if self._h:

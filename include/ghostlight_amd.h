@@ -187,9 +187,20 @@ void gl_ctx_destroy(gl_ctx* ctx);
 int  gl_ctx_synchronize(gl_ctx* ctx);
 
 /* Device time (ms, HIP events on the context's stream) of the last main
- * ghost trace + splat kernel run on ctx.  Build-specific instrumentation for
- * the roofline report; no counterpart in the reference. */
+ * ghost trace kernel run on ctx, and of the tiled splat resolve that follows
+ * it.  Build-specific instrumentation for the roofline report; no
+ * counterpart in the reference. */
 float gl_ctx_last_kernel_ms(gl_ctx* ctx);
+float gl_ctx_last_splat_ms(gl_ctx* ctx);
+
+/* HBM accounting for admission control (build-specific; the reference has
+ * no serving layer).  gl_device_memory: free / total bytes of the current
+ * device.  gl_ctx_memory_bytes: device bytes a context's buffer cache holds.
+ * gl_ctx_trim: release that cache (the context stays usable; buffers regrow
+ * on demand).  Trimming waits for the context's stream first. */
+int      gl_device_memory(uint64_t* free_bytes, uint64_t* total_bytes);
+uint64_t gl_ctx_memory_bytes(gl_ctx* ctx);
+int      gl_ctx_trim(gl_ctx* ctx);
 
 /* Device-resident lens (replaces the per-launch surface + coating-arena
  * upload, ghost_render.cu:919-954, coating_upload.cu:46-148). */
