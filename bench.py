@@ -27,6 +27,10 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 FP32_VECTOR_PEAK_TFLOPS = 157.3   # MI355X published FP32 vector peak
+# HBM bytes per launch of the trace kernel on this workload, from rocprofv3
+# FETCH_SIZE / WRITE_SIZE runs scaled by a 1 GiB copy (profiles/ghost_persist_kernel_c2.md):
+# 5.4 MiB read (lens, tables, pupil grid) + 849 MiB of deposit records written.
+TRACE_TRAFFIC_BYTES = 896e6
 HBM_PEAK_GBS = 8000.0
 
 
@@ -121,6 +125,44 @@ def precision_sample(lens, cfg, src, pairs, boosts, W, H, hw, hh, k, ctx):
             "sample": f"ghost image of the first {k} of {len(pairs)} pairs of the same workload"}
 
 
+def precision_starburst_leg(ghost_planes, W, H, hw, hh, check=(400.0, 550.0, 700.0)):
+    """Matrix-DFT of the 1024^2 precision pupil (9-blade rounded polygon,
+    Perlin dust, 20 scratches) to a 2048^2 PSF at 31 wavelengths with
+    lambda-scaled sampling, FP32 MFMA and 3xBF16 MFMA; per-lambda error vs
+    the FP64 (numpy complex128) matrix DFT on `check`; then the FFT veil
+    convolution of the 1920x1080 ghost image."""
+    from ghostlight_amd import FlareConfig, diffraction as dfr, hostprep, synthetic
+    from oracle import diffraction_oracle as dor
+    N, M, q = 1024, 2048, 2.0
+    P = synthetic.precision_pupil(N, seed=3)
+    lams = np.array(hostprep.build_spectral_lambdas(31), np.float32)
+    flops = 8.0 * len(lams) * (N * N * M + M * M * N)
+    out = {"pupil": f"{N}x{N} complex64, 9-blade rounded polygon, Perlin dust 0.9-1.0, 20 scratches",
+           "psf": f"{M}x{M} per lambda, {len(lams)} lambda, q = {q} x lambda / 550 nm", "gflop": round(flops / 1e9, 1)}
+    idx = [int(np.argmin(np.abs(lams - c))) for c in check]
+    refs = {}
+    for i in idx:
+        refs[i] = np.abs(dor.mft(P, M, q * float(lams[i]) / 550.0)) ** 2
+    for name, prec, peak in (("fp32", dfr.MDFT_FP32, 157.3), ("bf16x3", dfr.MDFT_BF16X3, 2500.0)):
+        dfr.mft_psf(P, M, q, lams[:2], precision=prec)                 # warm the buffers
+        psf, ms = dfr.mft_psf(P, M, q, lams, precision=prec, timing=True)
+        tf = flops / (ms * 1e-3) / 1e12
+        issued = tf * (3.0 if prec == dfr.MDFT_BF16X3 else 1.0)       # hi*hi + hi*lo + lo*hi products
+        out[name] = {"ms": round(ms, 3), "tflops": round(tf, 2), "mfma_tflops_issued": round(issued, 2),
+                     "peak": peak, "frac": round(issued / peak, 4),
+                     "rel_l2_vs_fp64": {f"{float(lams[i]):.1f}": float(dor.rel_l2(psf[i], refs[i])) for i in idx}}
+        del psf
+    cfg = FlareConfig()
+    vo, vms = dfr.veil_convolve(ghost_planes, hw, hh, cfg, timing=True)
+    d = cfg.diffraction
+    vref = dor.veil_convolve([np.asarray(x, np.float64) for x in ghost_planes], 2 * hw / W, 2 * hh / H,
+                             d.veil_spread * hh, d.veil_falloff, d.veil_gain)
+    P2 = dor.veil_pad(W, H)
+    out["veil_fft"] = {"image": f"{W}x{H} ghost image, 3 planes", "fft": f"{P2}x{P2}", "ms": round(vms, 3),
+                       "rel_l2_vs_fp64": float(dor.rel_l2(np.stack(vo), np.stack(vref)))}
+    return out
+
+
 def serving_leg(n_requests, rate_hz=40.0, slots=4):
     """Mixed-request serving stream (seeded lenses of 8-32 surfaces, 512^2 to
     3840x2160, 8-31 wavelengths, 5-11 blades; ghosts + starburst + veil)."""
@@ -129,13 +171,15 @@ def serving_leg(n_requests, rate_hz=40.0, slots=4):
     pool = serving_lens_pool(5)
     mix = make_mix(MixSpec(n_requests=n_requests, rate_hz=rate_hz, seed=5), pool)
     warm = make_mix(MixSpec(n_requests=2 * slots, rate_hz=1e6, seed=99), pool)
+    burst = [(0.0, req) for _, req in mix]
     with FlareServer(slots=slots) as srv:
         run_stream(srv, warm)
         served = run_stream(srv, mix)
+        saturated = run_stream(srv, burst)
     plain = run_one_at_a_time(mix)
     r = lambda d: {k: (round(v, 3) if isinstance(v, float) else v) for k, v in d.items()}  # noqa: E731
     return {"requests": n_requests, "rate_hz": rate_hz, "slots": slots, "served": r(served),
-            "one_at_a_time": r(plain)}
+            "served_all_at_once": r(saturated), "one_at_a_time": r(plain)}
 
 
 def main():
@@ -144,6 +188,7 @@ def main():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline and precision legs")
+    ap.add_argument("--no-starburst", action="store_true", help="skip the precision-starburst leg")
     ap.add_argument("--serving-requests", type=int, default=200,
                     help="requests in the serving leg (0 skips it)")
     a = ap.parse_args()
@@ -164,7 +209,14 @@ def main():
 
     for _ in range(a.warmup):
         step()
+    # The first renders of a workload shape try the launch-schedule
+    # candidates; warm up until the autotuner has settled.
+    extra = 0
+    while ctx.last_schedule()["tuning"] and extra < 16:
+        step()
+        extra += 1
     ctx.synchronize()
+    sched = ctx.last_schedule()
     kms, sms = [], []
     t0 = time.perf_counter()
     for _ in range(a.steps):
@@ -180,9 +232,20 @@ def main():
     segs = st["segments"]
     live_pairs = sum(1 for x in st["pair_on_sensor"] if x > 0)
     value = segs / (ms_per_step * 1e-3)
-    kernel_ms = float(np.mean(kms))
+    graph_kernel_ms = float(np.mean(kms))
     splat_ms = float(np.mean(sms))
-    print(f"bench: {segs} segments/step, {ms_per_step:.3f} ms/step, trace kernel {kernel_ms:.3f} ms, "
+    # Roofline timing: the same kernel with the tuned schedule issued eagerly,
+    # HIP events around the launch on the context's stream.
+    import dataclasses
+    cfg_ev = dataclasses.replace(cfg, batch_mode=1, schedule=sched["candidate"])
+    kev = []
+    for _ in range(3):
+        render_ghosts(lens, src, W, H, cfg_ev, hw, hh, pairs=pairs, boosts=boosts, ctx=ctx, device_out=True)
+        kev.append(ctx.last_kernel_ms())
+    kernel_ms = float(np.mean(kev))
+    print(f"bench: schedule {sched}, {extra} autotune warmup renders", file=sys.stderr, flush=True)
+    print(f"bench: {segs} segments/step, {ms_per_step:.3f} ms/step, trace kernel {kernel_ms:.3f} ms "
+          f"(graph replay wall clock {graph_kernel_ms:.3f} ms), "
           f"tiled splat {splat_ms:.3f} ms, {live_pairs} pairs reach the sensor, "
           f"{st.get('traces_on_sensor', 0)} on-sensor traces", file=sys.stderr, flush=True)
 
@@ -191,8 +254,10 @@ def main():
         ops = ops_per_segment(lens, pairs, cfg, src)
         achieved = segs * ops / (kernel_ms * 1e-3) / 1e12
         roof = {"bound": "valu", "achieved": round(achieved, 3), "peak": FP32_VECTOR_PEAK_TFLOPS,
-                "unit": "TFLOP/s", "frac": round(achieved / FP32_VECTOR_PEAK_TFLOPS, 4), "traffic": None,
-                "kernel": "ghost_kernel<false,true,false> (trace + deposit records)", "kernel_ms": round(kernel_ms, 4),
+                "unit": "TFLOP/s", "frac": round(achieved / FP32_VECTOR_PEAK_TFLOPS, 4),
+                "traffic": TRACE_TRAFFIC_BYTES,
+                "kernel": "ghost_persist_kernel (trace + deposit records)", "kernel_ms": round(kernel_ms, 4),
+                "kernel_ms_in_graph": round(graph_kernel_ms, 4), "schedule": sched,
                 "splat_resolve_ms": round(splat_ms, 4),
                 "ops_per_segment": round(ops, 2), "segments_per_launch": segs}
     except Exception as e:  # the oracle library is test infrastructure; report its absence
@@ -208,6 +273,11 @@ def main():
             prec = precision_sample(lens, cfg, src, pairs, boosts, W, H, hw, hh, 24, ctx)
         except Exception as e:
             print(f"bench: precision sample unavailable ({e})", file=sys.stderr)
+    starburst = None
+    if not a.no_starburst:
+        print("bench: precision-starburst leg", file=sys.stderr, flush=True)
+        ghost = render_ghosts(lens, src, W, H, cfg, hw, hh, pairs=pairs, boosts=boosts, ctx=ctx)
+        starburst = precision_starburst_leg(ghost, W, H, hw, hh)
     serving = None
     if a.serving_requests > 0:
         print(f"bench: serving leg, {a.serving_requests} requests", file=sys.stderr, flush=True)
@@ -233,6 +303,7 @@ def main():
         "roofline": roof,
         "cpu_baseline": cpu,
         "precision": prec,
+        "precision_starburst": starburst,
         "serving": serving,
     }
     print(json.dumps(out), flush=True)

@@ -89,6 +89,8 @@ class gl_flare_params(C.Structure):
         ("ghost_filter_mode", C.c_int32), ("ghost_filter_count", C.c_int32),
         ("ghost_filter_pairs", C.POINTER(C.c_int32)),
         ("fp64_radius_min", C.c_float),
+        ("batch_mode", C.c_int32),
+        ("schedule", C.c_int32),
     ]
 
 
@@ -159,6 +161,7 @@ SIGNATURES = {
     "gl_device_memory": (C.c_int, [C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
     "gl_ctx_memory_bytes": (C.c_uint64, [VP]),
     "gl_ctx_trim": (C.c_int, [VP]),
+    "gl_ctx_last_schedule": (C.c_int, [VP, C.POINTER(C.c_int32)]),
     "gl_lens_create": (C.c_int, [C.POINTER(gl_surface), C.c_int32, F32P, C.c_int32, C.POINTER(VP)]),
     "gl_lens_destroy": (None, [VP]),
     "gl_flare_params_init": (None, [C.POINTER(gl_flare_params)]),
@@ -194,6 +197,8 @@ SIGNATURES = {
     "gl_render_veil": (C.c_int, [VP, C.POINTER(gl_source), F32P, F32P, C.c_int32, C.c_int32, C.c_int32, C.c_int32,
                                  C.c_int32, C.c_float, C.c_float, C.POINTER(gl_diffraction_params), F32P, F32P, F32P,
                                  C.POINTER(gl_veil_debug)]),
+    "gl_veil_convolve": (C.c_int, [VP, VP, VP, VP, C.c_int32, C.c_int32, C.c_float, C.c_float,
+                                   C.POINTER(gl_diffraction_params), C.c_int32, VP, VP, VP, F32P]),
     "gl_cgemm_host": (C.c_int, [VP, C.c_int32, C.c_int32, C.c_int32, F32P, F32P, F32P, C.c_int32]),
     "gl_mft_psf": (C.c_int, [VP, F32P, C.c_int32, C.c_int32, C.c_float, C.c_float, F32P, C.c_int32, C.c_int32, F32P,
                              C.POINTER(F32P), F32P]),

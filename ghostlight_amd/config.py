@@ -203,6 +203,10 @@ class FlareConfig(RenderConfig):
     # Build-specific precision knob: spheres with |R| >= this (mm) use an FP64
     # ray-sphere solve; <= 0 disables it.
     fp64_radius_min: float = 1000.0
+    # Build-specific launch form of the main trace: 0 grid, 1 persistent
+    # (autotuned schedule), 2 persistent replayed from a hipGraph.
+    batch_mode: int = 2
+    schedule: int = -1          # -1 autotune, else a fixed candidate index
 
 
 @dataclass
@@ -274,4 +278,6 @@ def to_params(cfg: FlareConfig):
     p.ghost_filter_count = len(gf.pairs)
     p.ghost_filter_pairs = _abi.iptr(buf)
     p.fp64_radius_min = float(cfg.fp64_radius_min)
+    p.batch_mode = int(getattr(cfg, "batch_mode", 2))
+    p.schedule = int(getattr(cfg, "schedule", -1))
     return p, buf

@@ -28,6 +28,12 @@ static constexpr int kRefineHitsMax = 16;
 static constexpr int kRefineMaxEntries = 4096;
 static constexpr int kStage = 256;            // deposit records staged per wave in LDS
 
+// Persistent-launch schedule candidates the autotuner times: lambda range
+// per work item (ns / lam_div), workgroups per CU, longest-first order.
+struct Sched { int lam_div, blocks_per_cu, longest_first; };
+static constexpr Sched kSched[] = {{1, 4, 1}, {1, 8, 1}, {2, 8, 1}, {4, 8, 1}, {1, 8, 0}, {2, 4, 1}};
+static constexpr int kNumSched = (int)(sizeof(kSched) / sizeof(kSched[0]));
+
 struct SrcDev { float bx, by, dx, dy, dz, r, g, b; };
 
 struct ConcDev {
@@ -141,40 +147,17 @@ __device__ __forceinline__ void flush_stage(const GhostArgs& A, float4* stage, i
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// STATS = probe / diagnostic pass (counters, survivor bbox); otherwise the
-// render (SPLAT) and/or the segment count (COUNT).
-template <bool STATS, bool SPLAT, bool COUNT>
-__global__ __launch_bounds__(kBlock) void ghost_kernel(GhostArgs A) {
-    const int ps = blockIdx.x;
+// Trace one lane's pupil sample `gi` of (pair, source) slot `ps` over the
+// wavelengths [si0, si1) and stage a deposit record for every landing
+// (ghost_render.cu:372-462).  The wave must be converged: the lambda loop is
+// uniform and the staging uses wave ballots.
+__device__ __forceinline__ void deposit_range(const GhostArgs& A, int ps, int gi, int si0, int si1, float4* stage,
+                                              int& staged, int lane) {
     const int pair_idx = ps / A.n_src, src_idx = ps % A.n_src;
-    const int lane = threadIdx.x & 63;
-    // STATS with lam_group = G > 0: G lanes per pupil sample, one wavelength
-    // each (the probe passes are short; spreading lambda fills the device).
-    const int G = STATS ? A.lam_group : 0;
-    int gi, si0 = 0, si1 = A.n_spec;
-    if (STATS && G > 0) {
-        gi = (int)blockIdx.y * (kBlock / G) + (int)(threadIdx.x >> 6) * (64 / G) + lane / G;
-        si0 = lane % G;
-        si1 = si0 < A.n_spec ? si0 + 1 : si0;
-    } else {
-        gi = (int)blockIdx.y * kBlock + (int)threadIdx.x;
-    }
-    // Whole-block exits are uniform; per-lane validity stays a flag so the
-    // wave reductions below see every lane.
-    if (A.pair_active != nullptr && A.pair_active[pair_idx] == 0) return;
-    if (A.ps_refine != nullptr && A.ps_refine[ps] == 0) return;
     const gl_pair pr = A.pairs[pair_idx];
-    if (pr.surf_a < 0 || pr.surf_b <= pr.surf_a || pr.surf_b >= A.n) return;
-
     float u = 0.f, v = 0.f, w_ray = 0.f;
     bool live = gi < A.n_grid;
     if (live) live = sample_at(A, ps, gi, u, v, w_ray);
-
-    unsigned long long n_tr = 0, n_sv = 0, n_on = 0, n_seg = 0;
-    // SPLAT: per-wave LDS staging of deposit records (no image atomics here).
-    __shared__ float4 stage_all[SPLAT ? (kBlock / 64) : 1][SPLAT ? kStage : 1];
-    float4* stage = stage_all[SPLAT ? (threadIdx.x >> 6) : 0];
-    int staged = 0;                                  // wave-uniform
     const SrcDev sr = A.src[src_idx];
     const F3 o0 = f3(u * A.front_R + (-20.0f * sr.bx), v * A.front_R + (-20.0f * sr.by), A.start_z);
     const F3 d0 = f3(sr.dx, sr.dy, sr.dz);
@@ -184,22 +167,12 @@ __global__ __launch_bounds__(kBlock) void ghost_kernel(GhostArgs A) {
         if (live) do {
             const float GLC* spf = as_const(reinterpret_cast<const float*>(A.spec)) + 4 * si;
             const float4 sp = make_float4(spf[0], spf[1], spf[2], spf[3]);
-            if (STATS) ++n_tr;
-            TraceRes res = trace_ghost<SPLAT>(as_const(A.S), A.n, as_const(A.poly), A.arena, pr.surf_a, pr.surf_b, o0, d0,
-                                       sp.x, as_const(A.iat) + (size_t)si * A.n, as_const(A.ibf) + (size_t)si * A.n,
-                                       false, (SPLAT && A.lrow) ? as_const(A.lrow) + (size_t)si * A.n : nullptr);
-            if (COUNT || STATS) n_seg += (unsigned long long)res.segs;
+            TraceRes res = trace_ghost<true>(as_const(A.S), A.n, as_const(A.poly), A.arena, pr.surf_a, pr.surf_b, o0,
+                                             d0, sp.x, as_const(A.iat) + (size_t)si * A.n,
+                                             as_const(A.ibf) + (size_t)si * A.n, false,
+                                             A.lrow ? as_const(A.lrow) + (size_t)si * A.n : nullptr);
             if (res.status != GL_TRACE_OK) break;
             if (!isfinite(res.px) || !isfinite(res.py)) break;
-            if (STATS) {
-                ++n_sv;
-                float spx = (res.px / (2.0f * A.sw) + 0.5f) * A.fmt_w + A.x0;
-                float spy = (res.py / (2.0f * A.sh) + 0.5f) * A.fmt_h + A.y0;
-                if (isfinite(spx) && isfinite(spy) && spx >= 0.0f && spx < (float)A.width && spy >= 0.0f &&
-                    spy < (float)A.height) ++n_on;
-                break;
-            }
-            if (!SPLAT) break;
             float base = res.w * w_ray * A.gain * pr.area_boost;
             if (base < 1e-14f) break;
             float px = (res.px / (2.0f * A.sw) + 0.5f) * A.fmt_w + A.x0;
@@ -213,21 +186,117 @@ __global__ __launch_bounds__(kBlock) void ghost_kernel(GhostArgs A) {
             rec = make_float4(px, py, P * base, __int_as_float(si));
             dep = true;
         } while (false);
-        if (SPLAT) {
-            const unsigned long long bal = __ballot(dep);
-            const int cnt = (int)__popcll(bal);
-            if (cnt) {
-                if (staged + cnt > kStage) { flush_stage(A, stage, staged, lane); staged = 0; }
-                if (dep) {
-                    const int slot = staged + (int)__builtin_amdgcn_mbcnt_hi((unsigned)(bal >> 32),
-                                                                            __builtin_amdgcn_mbcnt_lo((unsigned)bal, 0u));
-                    stage[slot] = rec;
-                }
-                staged += cnt;
+        const unsigned long long bal = __ballot(dep);
+        const int cnt = (int)__popcll(bal);
+        if (cnt) {
+            if (staged + cnt > kStage) { flush_stage(A, stage, staged, lane); staged = 0; }
+            if (dep) {
+                const int slot = staged + (int)__builtin_amdgcn_mbcnt_hi((unsigned)(bal >> 32),
+                                                                        __builtin_amdgcn_mbcnt_lo((unsigned)bal, 0u));
+                stage[slot] = rec;
             }
+            staged += cnt;
         }
     }
-    if (SPLAT && staged) flush_stage(A, stage, staged, lane);
+}
+
+// Persistent main trace: a fixed set of workgroups pulls (slot, sample
+// chunk, lambda range) items from a device work list, longest first when the
+// schedule says so; deposit staging carries over between items.  clk[0] /
+// clk[1] receive the device wall clock at the first start / last finish.
+struct PersistQ {
+    const int4* __restrict__ work;      // {ps, chunk, si0, si1}
+    unsigned* __restrict__ ctl;         // [0] queue head, [1] item count
+    unsigned long long* __restrict__ clk;
+};
+
+__global__ __launch_bounds__(kBlock) void ghost_persist_kernel(GhostArgs A, PersistQ Q) {
+    __shared__ float4 stage_all[kBlock / 64][kStage];
+    __shared__ unsigned s_item;
+    const int lane = threadIdx.x & 63;
+    float4* stage = stage_all[threadIdx.x >> 6];
+    int staged = 0;
+    if (threadIdx.x == 0) atomicMin(Q.clk, (unsigned long long)wall_clock64());
+    const unsigned n_items = Q.ctl[1];
+    for (;;) {
+        if (threadIdx.x == 0) s_item = atomicAdd(Q.ctl, 1u);
+        __syncthreads();
+        const unsigned item = s_item;
+        __syncthreads();
+        if (item >= n_items) break;
+        const int4 w = Q.work[item];
+        deposit_range(A, w.x, w.y * kBlock + (int)threadIdx.x, w.z, w.w, stage, staged, lane);
+    }
+    if (staged) flush_stage(A, stage, staged, lane);
+    __syncthreads();
+    if (threadIdx.x == 0) atomicMax(Q.clk + 1, (unsigned long long)wall_clock64());
+}
+
+__global__ void persist_reset_kernel(PersistQ Q) {
+    Q.ctl[0] = 0u;
+    Q.clk[0] = ~0ull;
+    Q.clk[1] = 0ull;
+}
+
+// STATS = probe / diagnostic pass (counters, survivor bbox); otherwise the
+// render (SPLAT) and/or the segment count (COUNT).
+template <bool STATS, bool SPLAT, bool COUNT>
+__global__ __launch_bounds__(kBlock) void ghost_kernel(GhostArgs A) {
+    const int ps = blockIdx.x;
+    const int pair_idx = ps / A.n_src, src_idx = ps % A.n_src;
+    const int lane = threadIdx.x & 63;
+    // STATS with lam_group = G > 0: G lanes per pupil sample, one wavelength
+    // each (the probe passes are short; spreading lambda fills the device).
+    const int G = STATS ? A.lam_group : 0;
+    int gi, si0 = 0, si1 = A.n_spec;
+    if (STATS && G > 0) {
+        const int Gs = G > 0 ? G : 1;                // keeps the non-STATS instantiations free of a zero divisor
+        gi = (int)blockIdx.y * (kBlock / Gs) + (int)(threadIdx.x >> 6) * (64 / Gs) + lane / Gs;
+        si0 = lane % Gs;
+        si1 = si0 < A.n_spec ? si0 + 1 : si0;
+    } else {
+        gi = (int)blockIdx.y * kBlock + (int)threadIdx.x;
+    }
+    // Whole-block exits are uniform; per-lane validity stays a flag so the
+    // wave reductions below see every lane.
+    if (A.pair_active != nullptr && A.pair_active[pair_idx] == 0) return;
+    if (A.ps_refine != nullptr && A.ps_refine[ps] == 0) return;
+    const gl_pair pr = A.pairs[pair_idx];
+    if (pr.surf_a < 0 || pr.surf_b <= pr.surf_a || pr.surf_b >= A.n) return;
+
+    if (SPLAT) {
+        __shared__ float4 stage_all[kBlock / 64][kStage];
+        float4* stage = stage_all[threadIdx.x >> 6];
+        int staged = 0;                              // wave-uniform
+        deposit_range(A, ps, gi, 0, A.n_spec, stage, staged, lane);
+        if (staged) flush_stage(A, stage, staged, lane);
+        return;
+    }
+    float u = 0.f, v = 0.f, w_ray = 0.f;
+    bool live = gi < A.n_grid;
+    if (live) live = sample_at(A, ps, gi, u, v, w_ray);
+
+    unsigned long long n_tr = 0, n_sv = 0, n_on = 0, n_seg = 0;
+    const SrcDev sr = A.src[src_idx];
+    const F3 o0 = f3(u * A.front_R + (-20.0f * sr.bx), v * A.front_R + (-20.0f * sr.by), A.start_z);
+    const F3 d0 = f3(sr.dx, sr.dy, sr.dz);
+    if (live)
+        for (int si = si0; si < si1; ++si) {
+            const float GLC* spf = as_const(reinterpret_cast<const float*>(A.spec)) + 4 * si;
+            if (STATS) ++n_tr;
+            TraceRes res = trace_ghost<false>(as_const(A.S), A.n, as_const(A.poly), A.arena, pr.surf_a, pr.surf_b, o0,
+                                              d0, spf[0], as_const(A.iat) + (size_t)si * A.n,
+                                              as_const(A.ibf) + (size_t)si * A.n, false);
+            n_seg += (unsigned long long)res.segs;
+            if (!STATS) continue;
+            if (res.status != GL_TRACE_OK) continue;
+            if (!isfinite(res.px) || !isfinite(res.py)) continue;
+            ++n_sv;
+            float spx = (res.px / (2.0f * A.sw) + 0.5f) * A.fmt_w + A.x0;
+            float spy = (res.py / (2.0f * A.sh) + 0.5f) * A.fmt_h + A.y0;
+            if (isfinite(spx) && isfinite(spy) && spx >= 0.0f && spx < (float)A.width && spy >= 0.0f &&
+                spy < (float)A.height) ++n_on;
+        }
     if (STATS) {
         const unsigned long long t = wave_sum_u64(n_tr), s = wave_sum_u64(n_sv), o = wave_sum_u64(n_on);
         const unsigned long long sg = wave_sum_u64(n_seg);
@@ -490,7 +559,8 @@ static int do_launch(Launch& Lc, float* out_r, float* out_g, float* out_b, float
     }
     const double t_up1 = now_ms();
 
-    GhostArgs A{};
+    GhostArgs A;
+    std::memset(&A, 0, sizeof A);           // padding too: the bytes key the captured graph
     A.S = D.surf; A.poly = D.poly; A.arena = D.arena; A.n = n;
     A.pairs = static_cast<const gl_pair*>(d_pairs);
     A.n_src = n_src; A.src = static_cast<const SrcDev*>(d_src);
@@ -693,27 +763,151 @@ static int do_launch(Launch& Lc, float* out_r, float* out_g, float* out_b, float
     const SplatGeom sg = splat_geom(Lc.width, Lc.height);
     unsigned long long want = std::max<unsigned long long>(c->splat_hint + c->splat_hint / 4, 1ull << 22);
     want = std::max<unsigned long long>(1, std::min(want, max_rec));
-    const double t_k0 = now_ms();
-    for (int attempt = 0; attempt < 2; ++attempt) {
-        SplatBufs sb{};
-        if ((st = splat_prepare(c, sg, want, sb))) return st;
-        if (attempt > 0 && (st = zero_out())) return st;
-        if ((st = splat_clear(c->stream, sg, sb))) return st;
-        A.rec = sb.rec; A.rec_cap = sb.rec_cap; A.rec_count = sb.rec_count;
-        GL_HIP(hipEventRecord(c->ev0, c->stream));
-        if ((st = launch<false, true, false>(A, n_ps, n_grid, c->stream))) return st;
-        GL_HIP(hipEventRecord(c->ev1, c->stream));
-        if ((st = splat_resolve(c->stream, sg, sb, A.spec, A.out_r, A.out_g, A.out_b))) return st;
-        GL_HIP(hipEventRecord(c->ev2, c->stream));
-        if (c->spec_do_correct) {
-            const float* C = c->spec_C;
-            hipLaunchKernelGGL(spectral_correct_kernel, dim3((unsigned)((npx + 255) / 256)), dim3(256), 0, c->stream,
-                               A.out_r, A.out_g, A.out_b, npx, C[0], C[1], C[2], C[3], C[4], C[5], C[6], C[7], C[8]);
-            GL_HIP(hipGetLastError());
+    // ---- launch form: grid, or persistent over a cost-ordered work list
+    // (schedule from the autotuner), optionally replayed from a hipGraph.
+    const bool persistent = p->batch_mode != GL_BATCH_GRID;
+    const bool use_graph = p->batch_mode == GL_BATCH_GRAPH;
+    int cand = -1;
+    bool trial = false;
+    gl_ctx::TuneState* ts = nullptr;
+    PersistQ Q;
+    std::memset(&Q, 0, sizeof Q);
+    int workers = 0, lam_grp = ns;
+    if (persistent) {
+        if (c->n_cu <= 0) {
+            GL_HIP(hipDeviceGetAttribute(&c->n_cu, hipDeviceAttributeMultiprocessorCount, 0));
+            GL_HIP(hipDeviceGetAttribute(&c->wall_khz, hipDeviceAttributeWallClockRate, 0));
+            if (c->n_cu <= 0) c->n_cu = 1;
+            if (c->wall_khz <= 0) c->wall_khz = 100000;
         }
-        unsigned long long counts[2] = {0, 0};
+        std::vector<uint8_t> tk;
+        key_append(tk, n_ps); key_append(tk, ns); key_append(tk, n_grid); key_append(tk, Lc.width);
+        key_append(tk, Lc.height); key_append(tk, n); key_append(tk, rect_ready);
+        ts = &c->tune[tk];
+        if (ts->ms.empty()) ts->ms.assign(kNumSched, -1.0f);
+        if (p->schedule >= 0 && p->schedule < kNumSched) cand = p->schedule;
+        else {
+            for (int k = 0; k < kNumSched && cand < 0; ++k) if (ts->ms[k] < 0.0f) { cand = k; trial = true; }
+            if (cand < 0) cand = (int)(std::min_element(ts->ms.begin(), ts->ms.end()) - ts->ms.begin());
+        }
+        const Sched& sc = kSched[cand];
+        lam_grp = (ns + sc.lam_div - 1) / sc.lam_div;
+        workers = c->n_cu * sc.blocks_per_cu;
+        // Work items {slot, chunk, lambda range}; cost ~ surfaces per trace x
+        // rays.  Rebuilt only when its inputs change.
+        std::vector<uint8_t> wk;
+        key_append(wk, cand); key_append(wk, ns); key_append(wk, n_grid); key_append(wk, n_src); key_append(wk, n);
+        key_append(wk, cull_ready); key_append(wk, rect_ready); key_append(wk, A.conc.n_r);
+        wk.insert(wk.end(), active.begin(), active.end());
+        for (const auto& pq : Lc.pairs) { key_append(wk, pq.surf_a); key_append(wk, pq.surf_b); }
+        if (rect_ready) for (const auto& r : rects) key_append(wk, r);
+        for (const auto& b : budgets) key_append(wk, b);
+        DevBuf &bw = c->buf("persist_work"), &bq = c->buf("persist_ctl");
+        if ((st = bq.ensure(64))) return st;
+        if (wk != c->work_key || !bw.p) {
+            std::vector<int4> work;
+            std::vector<float> cost;
+            for (int q = 0; q < n_pairs; ++q) {
+                if (cull_ready && !active[q]) continue;
+                const gl_pair& pq = Lc.pairs[q];
+                if (pq.surf_a < 0 || pq.surf_b <= pq.surf_a || pq.surf_b >= n) continue;
+                for (int si = 0; si < n_src; ++si) {
+                    const int ps = q * n_src + si;
+                    int ns_ps = n_grid;
+                    if (rect_ready && rects[ps].x <= rects[ps].z) ns_ps = budgets.empty() ? A.conc.n_r : budgets[ps].x;
+                    const int chunks = (ns_ps + kBlock - 1) / kBlock;
+                    for (int ch = 0; ch < chunks; ++ch)
+                        for (int l0 = 0; l0 < ns; l0 += lam_grp) {
+                            const int l1 = std::min(ns, l0 + lam_grp);
+                            work.push_back(make_int4(ps, ch, l0, l1));
+                            const int rays = std::min(kBlock, ns_ps - ch * kBlock);
+                            cost.push_back((float)(n + 2 * (pq.surf_b - pq.surf_a)) * (float)rays * (float)(l1 - l0));
+                        }
+                }
+            }
+            if (sc.longest_first) {
+                std::vector<int> idx(work.size());
+                for (size_t i = 0; i < idx.size(); ++i) idx[i] = (int)i;
+                std::stable_sort(idx.begin(), idx.end(), [&](int x, int y) { return cost[x] > cost[y]; });
+                std::vector<int4> w2(work.size());
+                for (size_t i = 0; i < idx.size(); ++i) w2[i] = work[idx[i]];
+                work.swap(w2);
+            }
+            if ((st = bw.ensure(std::max<size_t>(1, work.size()) * sizeof(int4)))) return st;
+            if (!work.empty())
+                GL_HIP(hipMemcpyAsync(bw.p, work.data(), work.size() * sizeof(int4), hipMemcpyHostToDevice, c->stream));
+            const unsigned ctl[2] = {0u, (unsigned)work.size()};
+            GL_HIP(hipMemcpyAsync(bq.p, ctl, sizeof ctl, hipMemcpyHostToDevice, c->stream));
+            c->work_host.swap(work);
+            c->work_key.swap(wk);
+        }
+        Q.work = bw.as<int4>();
+        Q.ctl = bq.as<unsigned>();
+        Q.clk = reinterpret_cast<unsigned long long*>(bq.as<char>() + 16);
+        c->last_sched[5] = (int)c->work_host.size();
+    }
+
+    const double t_k0 = now_ms();
+    bool graphed = false;
+    for (int attempt = 0; attempt < 2; ++attempt) {
+        SplatBufs sb;
+        std::memset(&sb, 0, sizeof sb);
+        if ((st = splat_prepare(c, sg, want, sb))) return st;
+        A.rec = sb.rec; A.rec_cap = sb.rec_cap; A.rec_count = sb.rec_count;
+        // The whole device sequence of one render after the probes: no host
+        // synchronisation and no memset nodes, so it can be captured.
+        auto enqueue = [&](bool events) -> int {
+            int e;
+            if ((e = zero_out())) return e;
+            if ((e = splat_clear(c->stream, sg, sb))) return e;
+            if (persistent) hipLaunchKernelGGL(persist_reset_kernel, dim3(1), dim3(1), 0, c->stream, Q);
+            if (events) GL_HIP(hipEventRecord(c->ev0, c->stream));
+            if (persistent) {
+                hipLaunchKernelGGL(ghost_persist_kernel, dim3((unsigned)workers), dim3(kBlock), 0, c->stream, A, Q);
+                GL_HIP(hipGetLastError());
+            } else if ((e = launch<false, true, false>(A, n_ps, n_grid, c->stream))) {
+                return e;
+            }
+            if (events) GL_HIP(hipEventRecord(c->ev1, c->stream));
+            if ((e = splat_resolve(c->stream, sg, sb, A.spec, A.out_r, A.out_g, A.out_b))) return e;
+            if (events) GL_HIP(hipEventRecord(c->ev2, c->stream));
+            if (c->spec_do_correct) {
+                const float* C = c->spec_C;
+                hipLaunchKernelGGL(spectral_correct_kernel, dim3((unsigned)((npx + 255) / 256)), dim3(256), 0, c->stream,
+                                   A.out_r, A.out_g, A.out_b, npx, C[0], C[1], C[2], C[3], C[4], C[5], C[6], C[7], C[8]);
+                GL_HIP(hipGetLastError());
+            }
+            return GL_OK;
+        };
+        if (use_graph) {
+            std::vector<uint8_t> gk;
+            key_append(gk, A); key_append(gk, sb); key_append(gk, Q); key_append(gk, workers); key_append(gk, sg);
+            key_append(gk, c->spec_do_correct); key_append(gk, c->spec_C); key_append(gk, npx);
+            if (!c->graph || gk != c->graph_key) {
+                if (c->graph) { (void)hipGraphExecDestroy(c->graph); c->graph = nullptr; }
+                c->graph_key.clear();
+                GL_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+                const int e = enqueue(false);
+                hipGraph_t gph = nullptr;
+                const hipError_t ec = hipStreamEndCapture(c->stream, &gph);
+                if (e) { if (gph) (void)hipGraphDestroy(gph); return e; }
+                if (ec != hipSuccess) return set_error(GL_ERR_DEVICE, "hipStreamEndCapture -- %s", hipGetErrorString(ec));
+                const hipError_t ei = hipGraphInstantiate(&c->graph, gph, nullptr, nullptr, 0);
+                (void)hipGraphDestroy(gph);
+                if (ei != hipSuccess) { c->graph = nullptr; return set_error(GL_ERR_DEVICE, "hipGraphInstantiate -- %s", hipGetErrorString(ei)); }
+                c->graph_key = gk;
+            }
+            GL_HIP(hipEventRecord(c->ev0, c->stream));
+            GL_HIP(hipGraphLaunch(c->graph, c->stream));
+            GL_HIP(hipEventRecord(c->ev2, c->stream));
+            graphed = true;
+        } else if ((st = enqueue(true))) {
+            return st;
+        }
+        unsigned long long counts[4] = {0, 0, 0, 0};
         GL_HIP(hipMemcpyAsync(&counts[0], sb.rec_count, 8, hipMemcpyDeviceToHost, c->stream));
         GL_HIP(hipMemcpyAsync(&counts[1], sb.bin_total, 8, hipMemcpyDeviceToHost, c->stream));
+        if (persistent) GL_HIP(hipMemcpyAsync(&counts[2], Q.clk, 16, hipMemcpyDeviceToHost, c->stream));
         if (out_r) {
             GL_HIP(hipMemcpyAsync(out_r, A.out_r, npx * 4, hipMemcpyDeviceToHost, c->stream));
             GL_HIP(hipMemcpyAsync(out_g, A.out_g, npx * 4, hipMemcpyDeviceToHost, c->stream));
@@ -721,22 +915,48 @@ static int do_launch(Launch& Lc, float* out_r, float* out_g, float* out_b, float
         }
         GL_HIP(hipStreamSynchronize(c->stream));
         c->splat_hint = counts[0];
+        // Trace-kernel time: HIP events around the launch when it is issued
+        // eagerly; inside a replayed graph, the device wall clock its first
+        // and last workgroups read.
+        float kms = 0.f, sms = 0.f;
+        if (graphed) {
+            kms = counts[3] > counts[2] ? (float)((double)(counts[3] - counts[2]) / (double)c->wall_khz) : 0.0f;
+        } else {
+            GL_HIP(hipEventElapsedTime(&kms, c->ev0, c->ev1));
+        }
+        if (graphed) {
+            float tot = 0.f;
+            GL_HIP(hipEventElapsedTime(&tot, c->ev0, c->ev2));
+            sms = std::max(0.0f, tot - kms);
+        } else {
+            GL_HIP(hipEventElapsedTime(&sms, c->ev1, c->ev2));
+        }
+        c->last_kernel_ms = kms;
+        c->last_splat_ms = sms;
         const bool overflow = counts[0] > sb.rec_cap || counts[1] > sb.bin_cap;
-        if (!overflow) break;
+        if (!overflow) {
+            if (ts && cand >= 0) {
+                if (ts->ms[cand] < 0.0f || kms < ts->ms[cand]) ts->ms[cand] = kms;
+            }
+            break;
+        }
         if (attempt == 1) return set_error(GL_ERR_DEVICE, "splat record buffer overflow after regrow");
         // The binned copy holds each record once per touched tile (at most 4).
         want = std::max<unsigned long long>(counts[0], (counts[1] * 2 + 2) / 3) + (1ull << 20);
     }
     const double t_d1 = now_ms();
-    float kms = 0.f, sms = 0.f;
-    GL_HIP(hipEventElapsedTime(&kms, c->ev0, c->ev1));
-    GL_HIP(hipEventElapsedTime(&sms, c->ev1, c->ev2));
-    c->last_kernel_ms = kms;
-    c->last_splat_ms = sms;
+    if (persistent) {
+        const Sched& sc = kSched[cand];
+        c->last_sched[0] = lam_grp; c->last_sched[1] = workers; c->last_sched[2] = sc.longest_first;
+        c->last_sched[3] = cand; c->last_sched[4] = trial ? 1 : 0; c->last_sched[6] = graphed ? 1 : 0;
+        c->last_sched[7] = kNumSched;
+    } else {
+        for (int i = 0; i < 8; ++i) c->last_sched[i] = 0;
+    }
 
     if (stats) {
         stats->ms_upload = (float)(t_up1 - t_up0);
-        stats->ms_kernel = kms;
+        stats->ms_kernel = c->last_kernel_ms;
         stats->ms_download = (float)(t_d1 - t_k0);
         // Segment count of the main render (same inputs, no splat), then the
         // pre-cull per-pair picture over the full grid (ghost_render.cu:1361-1438).

@@ -49,6 +49,17 @@ struct gl_ctx {
     float last_kernel_ms = 0.0f;         // main trace kernel, event-timed
     float last_splat_ms = 0.0f;          // tiled splat resolve after it, event-timed
     unsigned long long splat_hint = 0;   // deposit records the last render produced
+    // Persistent-launch autotuner: per workload shape, the measured kernel
+    // time of each schedule candidate (-1 = not tried yet).
+    struct TuneState { std::vector<float> ms; };
+    std::map<std::vector<uint8_t>, TuneState> tune;
+    std::vector<int4> work_host;         // work list currently on the device
+    std::vector<uint8_t> work_key;       // inputs it was built from
+    hipGraphExec_t graph = nullptr;      // captured trace + splat sequence
+    std::vector<uint8_t> graph_key;
+    int last_sched[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    int n_cu = 0, wall_khz = 0;
+    std::vector<uint8_t> vconv_key;      // geometry of the cached veil-kernel transform
     // Host caches (pupil grid, spectral table) keyed like the reference's
     // GridKey / SpecKey (ghost_render.cu:515-556).
     std::vector<uint8_t> grid_key, spec_key;

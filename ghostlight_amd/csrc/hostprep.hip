@@ -480,6 +480,8 @@ void gl_flare_params_init(gl_flare_params* p) {
     p->aov_mode = GL_AOV_NONE; p->aov_max_pairs = -1;
     p->ghost_filter_mode = GL_FILTER_ALL;
     p->fp64_radius_min = 1000.0f;
+    p->batch_mode = GL_BATCH_GRAPH;
+    p->schedule = -1;
 }
 
 int gl_build_spectral_table(int32_t n, int32_t output_cs, const float* custom9, gl_spec* out) {

@@ -143,6 +143,7 @@ void gl_ctx_destroy(gl_ctx* c) {
         if (c->ev0) (void)hipEventDestroy(c->ev0);
         if (c->ev1) (void)hipEventDestroy(c->ev1);
         if (c->ev2) (void)hipEventDestroy(c->ev2);
+        if (c->graph) (void)hipGraphExecDestroy(c->graph);
         if (c->stream) (void)hipStreamDestroy(c->stream);
     }
     delete c;
@@ -182,9 +183,23 @@ int gl_ctx_trim(gl_ctx* ctx) {
     GL_HIP(hipStreamSynchronize(c->stream));
     for (auto& kv : c->bufs) kv.second.release();
     c->bufs.clear();
+    if (c->graph) { (void)hipGraphExecDestroy(c->graph); c->graph = nullptr; }
+    c->graph_key.clear();
+    c->work_host.clear();
+    c->work_key.clear();
+    c->vconv_key.clear();
     c->grid_uploaded = false;
     c->spec_uploaded = false;
     c->veil_baked_p = -1.0f;
+    return GL_OK;
+}
+
+int gl_ctx_last_schedule(gl_ctx* ctx, int32_t out[8]) {
+    if (!out) return gl::set_error(GL_ERR_INVALID, "null out");
+    int st;
+    gl_ctx* c = gl::resolve_ctx(ctx, &st);
+    if (!c) return st;
+    for (int i = 0; i < 8; ++i) out[i] = c->last_sched[i];
     return GL_OK;
 }
 

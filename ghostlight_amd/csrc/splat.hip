@@ -25,6 +25,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstring>
 
 namespace gl {
 
@@ -237,7 +238,8 @@ __global__ __launch_bounds__(256) void resolve_kernel(SplatBufs B, SplatGeom g, 
 }
 
 SplatGeom splat_geom(int width, int height) {
-    SplatGeom g{};
+    SplatGeom g;
+    std::memset(&g, 0, sizeof g);
     g.width = width;
     g.height = height;
     g.tile_shift = 5;

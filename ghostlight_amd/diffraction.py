@@ -252,3 +252,20 @@ def primary_survival(lens: OpticalSystem, launch_xy, direction, z_spawn: float, 
                                          float(d[0]), float(d[1]), float(d[2]), float(z_spawn), float(lambda_nm),
                                          out.ctypes.data_as(C.POINTER(C.c_ubyte))), "primary_survival")
     return out.astype(bool)
+
+
+def veil_convolve(planes, sensor_half_w: float, sensor_half_h: float, config: FlareConfig, ctx=None,
+                  timing: bool = False):
+    """Veil applied to a whole image by FFT convolution with the unit-sum
+    glare-spread kernel (the extended-source form of render_veil).  planes:
+    three (H, W) float32 arrays; returns three arrays (and the device ms)."""
+    ins = [np.ascontiguousarray(x, np.float32) for x in planes]
+    H, W = ins[0].shape
+    outs = [np.empty_like(ins[0]) for _ in range(3)]
+    p = diffraction_params(config)
+    ms = C.c_float(0.0)
+    vp = lambda a: a.ctypes.data_as(C.c_void_p)  # noqa: E731
+    _abi.check(lib().gl_veil_convolve(ctx_handle(ctx), vp(ins[0]), vp(ins[1]), vp(ins[2]), W, H,
+                                      float(sensor_half_w), float(sensor_half_h), C.byref(p), 0, vp(outs[0]),
+                                      vp(outs[1]), vp(outs[2]), C.byref(ms)), "veil_convolve")
+    return (outs, ms.value) if timing else outs

@@ -48,6 +48,13 @@ class Context:
     def last_splat_ms(self) -> float:
         return float(lib().gl_ctx_last_splat_ms(self._h))
 
+    def last_schedule(self) -> dict:
+        """Schedule the last ghost render on this context used."""
+        a = (C.c_int32 * 8)()
+        _abi.check(lib().gl_ctx_last_schedule(self._h, a), "gl_ctx_last_schedule")
+        return {"lambda_group": a[0], "workers": a[1], "longest_first": bool(a[2]), "candidate": a[3],
+                "tuning": bool(a[4]), "work_items": a[5], "graph": bool(a[6]), "candidates": a[7]}
+
     def memory_bytes(self) -> int:
         """Device bytes this context's buffer cache holds."""
         return int(lib().gl_ctx_memory_bytes(self._h))

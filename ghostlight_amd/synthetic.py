@@ -301,3 +301,76 @@ def config4_lens() -> OpticalSystem:
     tilt <= 0.3 deg."""
     return make_lens(seed=4, n_glass_surfaces=30, n_doublets=2, blades=9, coating_layers=1,
                      n_cyl=4, squeeze=1.5, decenter_max=0.2, tilt_max_deg=0.3, name="C4-anamorphic-30")
+
+
+# ---- precision-starburst pupil ----------------------------------------------
+
+def _perlin(shape, cells, rng):
+    """2-D gradient noise (K. Perlin, "Improving Noise", SIGGRAPH 2002:
+    random unit gradients on a lattice, quintic fade 6t^5 - 15t^4 + 10t^3),
+    values roughly in [-0.7, 0.7]."""
+    H, W = shape
+    ang = rng.uniform(0.0, 2.0 * np.pi, (cells + 1, cells + 1))
+    gx, gy = np.cos(ang), np.sin(ang)
+    y = np.linspace(0.0, cells, H, endpoint=False)
+    x = np.linspace(0.0, cells, W, endpoint=False)
+    X, Y = np.meshgrid(x, y)
+    x0, y0 = np.floor(X).astype(int), np.floor(Y).astype(int)
+    fx, fy = X - x0, Y - y0
+    fade = lambda t: t * t * t * (t * (t * 6.0 - 15.0) + 10.0)  # noqa: E731
+    def dot(ix, iy, dx, dy):
+        return gx[iy, ix] * dx + gy[iy, ix] * dy
+    n00 = dot(x0, y0, fx, fy)
+    n10 = dot(x0 + 1, y0, fx - 1.0, fy)
+    n01 = dot(x0, y0 + 1, fx, fy - 1.0)
+    n11 = dot(x0 + 1, y0 + 1, fx - 1.0, fy - 1.0)
+    u, v = fade(fx), fade(fy)
+    return (n00 * (1 - u) + n10 * u) * (1 - v) + (n01 * (1 - u) + n11 * u) * v
+
+
+def rounded_polygon_mask(N, blades, curvature, rotation=0.0, fill=0.9, ss=4):
+    """ss x ss supersampled coverage of an n-blade stop whose edges bow
+    outward: per half-sector 1/r = A cos(phi - phi_w) + C with 1/r(waist) =
+    1/r_w and r(tip) = 1, r_w = apothem + curvature (1 - apothem) (curvature
+    0: the regular polygon, 1: the circle)."""
+    apo = math.cos(math.pi / blades)
+    r_w = apo + curvature * (1.0 - apo)
+    half = math.pi / blades
+    A = (1.0 / r_w - 1.0) / (1.0 - math.cos(half)) if half > 0 else 0.0
+    Cc = 1.0 / r_w - A
+    acc = np.zeros((N, N))
+    i = np.arange(N, dtype=np.float64)
+    for sy in range(ss):
+        v = ((i + (sy + 0.5) / ss) / N * 2.0 - 1.0) / fill
+        for sx in range(ss):
+            u = ((i + (sx + 0.5) / ss) / N * 2.0 - 1.0) / fill
+            U, V = np.meshgrid(u, v)
+            r = np.hypot(U, V)
+            phi = np.mod(np.arctan2(V, U) - rotation, 2.0 * half)   # 0 at a tip, waist at `half`
+            rb = 1.0 / (A * np.cos(phi - half) + Cc)
+            acc += r <= rb
+    return acc / (ss * ss)
+
+
+def precision_pupil(N=1024, blades=9, curvature=0.35, seed=3, n_scratches=20):
+    """The precision-starburst pupil: a 9-blade rounded polygon, seeded
+    Perlin dust (transmission 0.9..1.0) and thin scratch lines
+    (transmission 0.2), as complex64."""
+    rng = np.random.default_rng(seed)
+    mask = rounded_polygon_mask(N, blades, curvature, rotation=float(rng.uniform(0, 2 * math.pi / blades)))
+    n = _perlin((N, N), 24, rng) + 0.5 * _perlin((N, N), 96, rng)
+    n = (n - n.min()) / max(n.max() - n.min(), 1e-12)
+    dust = 0.9 + 0.1 * n
+    scr = np.ones((N, N))
+    yy, xx = np.mgrid[0:N, 0:N].astype(np.float64)
+    for _ in range(n_scratches):
+        x0, y0 = rng.uniform(0.2 * N, 0.8 * N, 2)
+        th = rng.uniform(0, math.pi)
+        L = rng.uniform(0.05 * N, 0.3 * N)
+        w = rng.uniform(0.6, 1.8)
+        dx, dy = math.cos(th), math.sin(th)
+        t = (xx - x0) * dx + (yy - y0) * dy
+        d = np.abs(-(xx - x0) * dy + (yy - y0) * dx)
+        hit = (np.abs(t) <= L / 2) & (d <= w / 2)
+        scr[hit] = 0.2
+    return (mask * dust * scr).astype(np.complex64)

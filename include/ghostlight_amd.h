@@ -143,7 +143,20 @@ typedef struct gl_flare_params {
     /* Build-specific precision knob: spherical surfaces with |R| at or above
      * this use an FP64 ray-sphere solve (sag cancellation); <= 0 disables. */
     float   fp64_radius_min;
+    /* Build-specific launch form of the main trace (no reference
+     * counterpart): GL_BATCH_GRID = one workgroup per (pair-source, 256
+     * samples); GL_BATCH_PERSISTENT = persistent workgroups over a
+     * cost-ordered work list whose lambda split / worker count / order the
+     * autotuner picks per workload shape; GL_BATCH_GRAPH = the same, with the
+     * trace + splat + correction sequence captured once in a hipGraph and
+     * replayed.  schedule: -1 = autotune, else a fixed candidate index. */
+    int32_t batch_mode;
+    int32_t schedule;
 } gl_flare_params;
+
+#define GL_BATCH_GRID        0
+#define GL_BATCH_PERSISTENT  1
+#define GL_BATCH_GRAPH       2
 
 /* Per-render diagnostics (GhostRenderStats, flare_buffers.h:21-74).  Arrays
  * are caller-owned; *_cap gives their capacity, n_pairs / n_ps the fill. */
@@ -199,6 +212,13 @@ float gl_ctx_last_splat_ms(gl_ctx* ctx);
  * gl_ctx_trim: release that cache (the context stays usable; buffers regrow
  * on demand).  Trimming waits for the context's stream first. */
 int      gl_device_memory(uint64_t* free_bytes, uint64_t* total_bytes);
+
+/* Schedule of the last ghost render on ctx (build-specific): out[0] lambda
+ * values per work item, out[1] persistent workgroups, out[2] order (1 =
+ * longest-first), out[3] candidate index, out[4] 1 while the autotuner is
+ * still trying candidates for this workload shape, out[5] work items,
+ * out[6] 1 if replayed from a captured hipGraph, out[7] candidates. */
+int      gl_ctx_last_schedule(gl_ctx* ctx, int32_t out[8]);
 uint64_t gl_ctx_memory_bytes(gl_ctx* ctx);
 int      gl_ctx_trim(gl_ctx* ctx);
 
@@ -400,6 +420,20 @@ int  gl_render_veil(gl_ctx* ctx, const gl_source* sources, const float* px, cons
                     int32_t fmt_h, float sensor_half_w, float sensor_half_h,
                     const gl_diffraction_params* p, float* out_r, float* out_g,
                     float* out_b, gl_veil_debug* dbg);
+
+/* Veil applied to a whole image by FFT convolution (the extended-source form
+ * of render_veil, veil_render.h:54-70): out_c = veil_gain * (in_c (*) g) with
+ * the unit-sum glare-spread kernel g(r) = (a^2/(r^2+a^2))^p of
+ * veil_render.cu:94-106 (a = veil_spread * sensor_half_h mm, p =
+ * veil_falloff), linear convolution over a zero-padded power-of-two grid.
+ * device_ptrs = 1: in_* / out_* are device pointers on ctx's device; else
+ * host buffers of width*height floats.  ms_out (may be NULL): device time of
+ * the transform work, HIP events.  Build-specific: the reference splats the
+ * kernel per point source instead. */
+int  gl_veil_convolve(gl_ctx* ctx, const float* in_r, const float* in_g, const float* in_b,
+                      int32_t width, int32_t height, float sensor_half_w, float sensor_half_h,
+                      const gl_diffraction_params* p, int32_t device_ptrs,
+                      float* out_r, float* out_g, float* out_b, float* ms_out);
 
 /* Survival of primary rays launched from (x, y) on the spawn plane in one
  * direction (the CPU tracer's t >= 0 rule, trace.cpp:243): the kernel behind

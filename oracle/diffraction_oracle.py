@@ -7,6 +7,8 @@ numpy FP64 restatement of the reference's diffraction passes:
   splat_resample_kernel     ghostlight/src/starburst_render.cu:316-384
   veil reference + splat    ghostlight/src/veil_render.cu:94-212, 243-414
   mdft_starburst            ghostlight/src/mdft_render.cu:77-340
+  veil glare-spread kernel  ghostlight/src/veil_render.cu:94-106, applied to a
+                            whole image by linear FFT convolution
   the pupil -> PSF matrix DFT (MFT) of the precision starburst workload.
 The FFT is numpy's (pocketfft, complex128); the reference's own numpy
 mirrors (validation/validate_starburst.py:49-60,
@@ -207,3 +209,49 @@ def mdft_window(C, mc, spec_w, Sx, Sy, px_mm_x, px_mm_y, src, width, height, kno
     for ch, col in enumerate((r, g, b)):
         out[ch, ky0:ky1, kx0:kx1] += win[ch] * knorm_gain * col
     return out
+
+
+def veil_pad(W, H):
+    P = 2
+    while P < 2 * max(W, H):
+        P *= 2
+    return P
+
+
+def veil_kernel(P, px_mm_x, px_mm_y, a_mm, p):
+    """Unit-sum g(r) = (a^2/(r^2+a^2))^p at integer pixel offsets, origin at
+    [0, 0] with negative offsets wrapped (FP64)."""
+    d = np.arange(P)
+    d = np.where(d < P // 2, d, d - P).astype(np.float64)
+    rx, ry = d * px_mm_x, d * px_mm_y
+    g = (a_mm ** 2 / (rx[None, :] ** 2 + ry[:, None] ** 2 + a_mm ** 2)) ** p
+    return g / g.sum()
+
+
+def veil_convolve(planes, px_mm_x, px_mm_y, a_mm, p, gain):
+    """Linear convolution of each plane with veil_kernel over the zero-padded
+    P x P grid (numpy FFT, complex128)."""
+    H, W = planes[0].shape
+    P = veil_pad(W, H)
+    G = np.fft.fft2(veil_kernel(P, px_mm_x, px_mm_y, a_mm, p))
+    out = []
+    for x in planes:
+        z = np.zeros((P, P))
+        z[:H, :W] = x
+        out.append(gain * np.real(np.fft.ifft2(np.fft.fft2(z) * G))[:H, :W])
+    return out
+
+
+def veil_convolve_direct(plane, px_mm_x, px_mm_y, a_mm, p, gain):
+    """Brute-force sum over every input pixel (tiny images only)."""
+    H, W = plane.shape
+    P = veil_pad(W, H)
+    g = veil_kernel(P, px_mm_x, px_mm_y, a_mm, p)
+    out = np.zeros((H, W))
+    for y in range(H):
+        for x in range(W):
+            if plane[y, x] != 0.0:
+                dy = (np.arange(H)[:, None] - y) % P
+                dx = (np.arange(W)[None, :] - x) % P
+                out += plane[y, x] * g[dy, dx]
+    return gain * out

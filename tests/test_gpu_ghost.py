@@ -177,3 +177,81 @@ def test_gain_linearity_and_zero_source():
     z = render_ghosts(lens, np.array([[0.1, 0.0, 0.0, 0.0, 0.0]], np.float32), W, H,
                       FlareConfig(ray_grid=32, spectral_samples=8), hw, hh, pairs=pairs, boosts=boosts)
     assert all(float(np.abs(x).max()) == 0.0 for x in z)
+
+
+def _c2_setup(ray_grid=48, ns=12):
+    lens = LENSES["c2"]
+    W, H = 480, 270
+    hw, hh = synthetic.sensor_for(lens, W, H, half_field_deg=20.0)
+    src = np.array([[math.radians(12.0), math.radians(4.0), 1.0, 1.0, 1.0]], np.float32)
+    base = FlareConfig(ray_grid=ray_grid, spectral_samples=ns)
+    pairs, boosts = pyoracle.filter_ghost_pairs(lens, hw, hh)
+    return lens, W, H, hw, hh, src, base, pairs, boosts
+
+
+def _cfg(base, **kw):
+    import dataclasses
+    return dataclasses.replace(base, **kw)
+
+
+def test_launch_forms_agree_with_the_oracle():
+    """Grid launch, persistent launch under every schedule candidate, and the
+    graph-replayed form trace the same (sample, lambda) set: images agree to
+    float summation order and each matches the FP32 oracle."""
+    from ghostlight_amd import Context
+    lens, W, H, hw, hh, src, base, pairs, boosts = _c2_setup()
+    spec = hostprep.spectral_table(base.spectral_samples)
+    C = hostprep.spectral_correction(spec)
+    p, _k = to_params(base)
+    o32, _ = pyoracle.render_ghosts(lens, pairs, boosts, src, spec, p, hw, hh, W, H, correction=C)
+    ctx = Context()
+    ref = render_ghosts(lens, src, W, H, _cfg(base, batch_mode=0), hw, hh, pairs=pairs, boosts=boosts, ctx=ctx)
+    assert pyoracle.rel_l2(ref, o32) <= TOL_FP32
+    n_sched = ctx.last_schedule()["candidates"] or 6
+    for k in range(n_sched):
+        img = render_ghosts(lens, src, W, H, _cfg(base, batch_mode=1, schedule=k), hw, hh, pairs=pairs, boosts=boosts,
+                            ctx=ctx)
+        sch = ctx.last_schedule()
+        assert sch["candidate"] == k and not sch["graph"] and sch["work_items"] > 0
+        e = pyoracle.rel_l2(img, ref)
+        print(f"schedule {k} {sch}: rel-L2 vs grid launch {e:.2e}", flush=True)
+        assert e <= 1e-6
+    ctx.close()
+
+
+def test_graph_replays_match_eager():
+    """The captured sequence replayed several times, with eager renders and
+    synchronisations between the replays: every replay equals the eager result."""
+    from ghostlight_amd import Context
+    lens, W, H, hw, hh, src, base, pairs, boosts = _c2_setup(ray_grid=40, ns=8)
+    ctx = Context()
+    eager = render_ghosts(lens, src, W, H, _cfg(base, batch_mode=1, schedule=1), hw, hh, pairs=pairs, boosts=boosts,
+                          ctx=ctx)
+    for rep in range(3):
+        g = render_ghosts(lens, src, W, H, _cfg(base, batch_mode=2, schedule=1), hw, hh, pairs=pairs, boosts=boosts,
+                          ctx=ctx)
+        assert ctx.last_schedule()["graph"]
+        e = pyoracle.rel_l2(g, eager)
+        print(f"graph replay {rep}: rel-L2 vs eager {e:.2e}", flush=True)
+        assert e <= 1e-6
+        mid = render_ghosts(lens, src, W, H, _cfg(base, batch_mode=1, schedule=1), hw, hh, pairs=pairs,
+                            boosts=boosts, ctx=ctx)
+        ctx.synchronize()
+        assert pyoracle.rel_l2(mid, eager) <= 1e-6
+    ctx.close()
+
+
+def test_autotuner_settles_on_the_fastest_candidate():
+    from ghostlight_amd import Context
+    lens, W, H, hw, hh, src, base, pairs, boosts = _c2_setup(ray_grid=40, ns=8)
+    ctx = Context()
+    seen = []
+    for _ in range(8):
+        render_ghosts(lens, src, W, H, _cfg(base, batch_mode=2), hw, hh, pairs=pairs, boosts=boosts, ctx=ctx)
+        s = ctx.last_schedule()
+        seen.append((s["candidate"], s["tuning"]))
+    n = ctx.last_schedule()["candidates"]
+    print(f"autotune sequence {seen}", flush=True)
+    assert [c for c, t in seen[:n]] == list(range(n)) and all(t for c, t in seen[:n])
+    assert not seen[-1][1] and seen[-1][0] == seen[-2][0]
+    ctx.close()
