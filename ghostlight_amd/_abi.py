@@ -94,6 +94,17 @@ class gl_flare_params(C.Structure):
     ]
 
 
+class gl_psf_source(C.Structure):
+    _fields_ = [("angle_x", C.c_float), ("angle_y", C.c_float), ("r", C.c_float), ("g", C.c_float),
+                ("b", C.c_float), ("chief_x_mm", C.c_float), ("chief_y_mm", C.c_float), ("tile_x0", C.c_int32),
+                ("tile_y0", C.c_int32)]
+
+
+class gl_psf_params(C.Structure):
+    _fields_ = [("grid_nx", C.c_int32), ("grid_ny", C.c_int32), ("tile_w", C.c_int32), ("tile_h", C.c_int32),
+                ("tile_extent_mm", C.c_float), ("monochromatic", C.c_int32), ("splat_sigma_um", C.c_float)]
+
+
 class gl_ghost_stats(C.Structure):
     _fields_ = [
         ("ms_grid_build", C.c_float), ("ms_upload", C.c_float), ("ms_kernel", C.c_float),
@@ -197,6 +208,8 @@ SIGNATURES = {
     "gl_render_veil": (C.c_int, [VP, C.POINTER(gl_source), F32P, F32P, C.c_int32, C.c_int32, C.c_int32, C.c_int32,
                                  C.c_int32, C.c_float, C.c_float, C.POINTER(gl_diffraction_params), F32P, F32P, F32P,
                                  C.POINTER(gl_veil_debug)]),
+    "gl_render_psf_tiles": (C.c_int, [VP, VP, C.POINTER(gl_psf_source), C.c_int32, C.POINTER(gl_flare_params),
+                                      C.POINTER(gl_psf_params), F32P, F32P, F32P, F32P]),
     "gl_veil_convolve": (C.c_int, [VP, VP, VP, VP, C.c_int32, C.c_int32, C.c_float, C.c_float,
                                    C.POINTER(gl_diffraction_params), C.c_int32, VP, VP, VP, F32P]),
     "gl_cgemm_host": (C.c_int, [VP, C.c_int32, C.c_int32, C.c_int32, F32P, F32P, F32P, C.c_int32]),

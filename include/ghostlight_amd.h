@@ -421,6 +421,36 @@ int  gl_render_veil(gl_ctx* ctx, const gl_source* sources, const float* px, cons
                     const gl_diffraction_params* p, float* out_r, float* out_g,
                     float* out_b, gl_veil_debug* dbg);
 
+/* ---- geometric PSF grid --------------------------------------------------- */
+
+/* PSFGpuSource (psf_render.h:17-24): a field-point source paired with its
+ * chief landing (computed by the host pre-pass) and its tile origin. */
+typedef struct gl_psf_source {
+    float   angle_x, angle_y;           /* ray-spawn direction (radians)              */
+    float   r, g, b;                    /* source weights                             */
+    float   chief_x_mm, chief_y_mm;     /* chief landing on the sensor                */
+    int32_t tile_x0, tile_y0;           /* tile origin in the composite (pixels)      */
+} gl_psf_source;
+
+/* PSFConfig's grid fields (render_config.h:364-383); the RenderConfig part
+ * (ray grid, spectra, colour space, stop override) comes in gl_flare_params. */
+typedef struct gl_psf_params {
+    int32_t grid_nx, grid_ny, tile_w, tile_h;
+    float   tile_extent_mm;
+    int32_t monochromatic;
+    float   splat_sigma_um;             /* <= 0: bilinear splat                       */
+} gl_psf_params;
+
+/* launch_psf_render (psf_render.h:30-42, psf_render.cu:225-465): primary
+ * traces of every (source, pupil sample, wavelength) splatted relative to
+ * the chief landing into the source's tile; composite of grid_nx*tile_w x
+ * grid_ny*tile_h.  out_*: host, overwritten.  ms_out (may be NULL): kernel
+ * time, HIP events. */
+int  gl_render_psf_tiles(gl_ctx* ctx, const gl_lens* lens, const gl_psf_source* sources,
+                         int32_t n_sources, const gl_flare_params* render,
+                         const gl_psf_params* psf, float* out_r, float* out_g, float* out_b,
+                         float* ms_out);
+
 /* Veil applied to a whole image by FFT convolution (the extended-source form
  * of render_veil, veil_render.h:54-70): out_c = veil_gain * (in_c (*) g) with
  * the unit-sum glare-spread kernel g(r) = (a^2/(r^2+a^2))^p of
