@@ -163,6 +163,62 @@ def precision_starburst_leg(ghost_planes, W, H, hw, hh, check=(400.0, 550.0, 700
     return out
 
 
+def sweep_leg(n_sources=120):
+    """Anamorphic source sweep: the 30-surface lens with 4 cylindrical
+    surfaces (1.5x squeeze), seeded decenter/tilt, 192^2 pupil, 16 lambda,
+    sources on a diagonal, each into its own 2048x858 frame, one launch."""
+    import dataclasses
+    from ghostlight_amd import Context, FlareConfig, filter_ghost_pairs, render_ghosts, synthetic
+    lens = synthetic.config4_lens()
+    W, H = 2048, 858
+    hw, hh = synthetic.sensor_for(lens, W, H, half_field_deg=12.0)
+    t = np.radians(np.linspace(-14.0, 14.0, n_sources))
+    src = np.stack([t, t * (H / W), np.ones(n_sources), np.ones(n_sources), np.ones(n_sources)], 1).astype(np.float32)
+    cfg = FlareConfig(ray_grid=192, spectral_samples=16, frame_per_source=True, batch_mode=1, schedule=3)
+    ctx = Context()
+    pairs, boosts = filter_ghost_pairs(lens, hw, hh, cfg, ctx=ctx)
+    t0 = time.perf_counter()
+    render_ghosts(lens, src, W, H, cfg, hw, hh, pairs=pairs, boosts=boosts, ctx=ctx, device_out=True)
+    ctx.synchronize()
+    first = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    render_ghosts(lens, src, W, H, cfg, hw, hh, pairs=pairs, boosts=boosts, ctx=ctx, device_out=True)
+    ctx.synchronize()
+    dt = time.perf_counter() - t0
+    kms = ctx.last_kernel_ms()
+    _, st = render_ghosts(lens, src, W, H, cfg, hw, hh, pairs=pairs, boosts=boosts, ctx=ctx, device_out=True,
+                          stats=True)
+    ctx.close()
+    return {"sources": n_sources, "frame": f"{W}x{H}", "pairs_prefiltered": len(pairs), "render_ms": round(dt * 1e3, 2),
+            "first_render_ms": round(first * 1e3, 2), "trace_kernel_ms": round(kms, 2),
+            "segments": st["segments"], "segments_per_s": st["segments"] / dt, "frames_per_s": n_sources / dt,
+            "output_bytes": 3 * 4 * W * H * n_sources}
+
+
+def psf_grid_leg():
+    """Geometric PSF grid at the validation shape: 7 x 7 field points on the
+    12-surface lens, 96^2 tiles, 256^2 pupil grid, 16 lambda, 4 um Gaussian
+    splat, chief-centroid anchoring."""
+    from ghostlight_amd import PSFConfig, synthetic
+    from ghostlight_amd.psf import render_psf
+    lens = synthetic.config1_lens()
+    cfg = PSFConfig(grid_nx=7, grid_ny=7, tile_w=96, tile_h=96, tile_extent_mm=0.08, ray_grid=256,
+                    spectral_samples=16, splat_sigma_um=4.0)
+    a = np.radians(np.linspace(-10.0, 10.0, 7))
+    AX, AY = np.meshgrid(a, a)
+    src = np.stack([AX.ravel(), AY.ravel(), np.ones(49), np.ones(49), np.ones(49)], 1).astype(np.float32)
+    render_psf(src, lens, cfg)                                     # warm
+    t0 = time.perf_counter()
+    out = render_psf(src, lens, cfg, timing=True)
+    wall = (time.perf_counter() - t0) * 1e3
+    from ghostlight_amd import hostprep
+    n_grid = len(hostprep.aperture_samples(lens, 256))
+    traces = 49 * n_grid * 16
+    return {"sources": 49, "tile": "96x96", "pupil_samples": n_grid, "lambda": 16, "splat_sigma_um": 4.0,
+            "kernel_ms": round(out["kernel_ms"], 3), "render_ms": round(wall, 3),
+            "traces_per_s": traces / (out["kernel_ms"] * 1e-3), "dark_cells": int((out["status"] == 2).sum())}
+
+
 def serving_leg(n_requests, rate_hz=40.0, slots=4):
     """Mixed-request serving stream (seeded lenses of 8-32 surfaces, 512^2 to
     3840x2160, 8-31 wavelengths, 5-11 blades; ghosts + starburst + veil)."""
@@ -188,7 +244,8 @@ def main():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline and precision legs")
-    ap.add_argument("--no-starburst", action="store_true", help="skip the precision-starburst leg")
+    ap.add_argument("--no-starburst", action="store_true", help="skip the precision-starburst and PSF legs")
+    ap.add_argument("--sweep-sources", type=int, default=120, help="sources in the sweep leg (0 skips it)")
     ap.add_argument("--serving-requests", type=int, default=200,
                     help="requests in the serving leg (0 skips it)")
     a = ap.parse_args()
@@ -278,6 +335,14 @@ def main():
         print("bench: precision-starburst leg", file=sys.stderr, flush=True)
         ghost = render_ghosts(lens, src, W, H, cfg, hw, hh, pairs=pairs, boosts=boosts, ctx=ctx)
         starburst = precision_starburst_leg(ghost, W, H, hw, hh)
+    psf = None
+    if not a.no_starburst:
+        print("bench: PSF grid leg", file=sys.stderr, flush=True)
+        psf = psf_grid_leg()
+    sweep = None
+    if a.sweep_sources > 0:
+        print(f"bench: source-sweep leg, {a.sweep_sources} sources", file=sys.stderr, flush=True)
+        sweep = sweep_leg(a.sweep_sources)
     serving = None
     if a.serving_requests > 0:
         print(f"bench: serving leg, {a.serving_requests} requests", file=sys.stderr, flush=True)
@@ -304,6 +369,8 @@ def main():
         "cpu_baseline": cpu,
         "precision": prec,
         "precision_starburst": starburst,
+        "psf_grid": psf,
+        "source_sweep": sweep,
         "serving": serving,
     }
     print(json.dumps(out), flush=True)

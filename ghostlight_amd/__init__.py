@@ -15,6 +15,7 @@ from .render import (GhostPair, Ray, TraceResult, enumerate_ghost_pairs, filter_
                      trace_primary_ray)
 from .hostprep import build_spectral_lambdas
 from .diffraction import _render_starburst_debug, _render_veil_debug, fft2d
+from .psf import render_psf
 
 
 def _cuda_available() -> bool:

@@ -91,6 +91,7 @@ class gl_flare_params(C.Structure):
         ("fp64_radius_min", C.c_float),
         ("batch_mode", C.c_int32),
         ("schedule", C.c_int32),
+        ("frame_per_source", C.c_int32),
     ]
 
 

@@ -207,6 +207,9 @@ class FlareConfig(RenderConfig):
     # (autotuned schedule), 2 persistent replayed from a hipGraph.
     batch_mode: int = 2
     schedule: int = -1          # -1 autotune, else a fixed candidate index
+    # Build-specific: every source renders into a frame of its own (render_ghosts
+    # then returns (n_sources, H, W) planes).
+    frame_per_source: bool = False
 
 
 @dataclass
@@ -237,6 +240,29 @@ class PSFConfig(RenderConfig):
 def _flat9(m) -> list:
     a = np.asarray(m, np.float32).reshape(-1)
     return [float(x) for x in a[:9]] + [0.0] * (9 - min(9, a.size))
+
+
+def render_params(cfg: RenderConfig):
+    """The RenderConfig part of gl_flare_params (PSF renders), the ghost
+    fields at their gl_flare_params_init defaults."""
+    p = _abi.gl_flare_params()
+    _abi.load_library().gl_flare_params_init(_abi.C.byref(p))
+    p.ray_grid = int(cfg.ray_grid)
+    p.spectral_samples = int(cfg.spectral_samples)
+    p.pupil_jitter = int(cfg.pupil_jitter)
+    p.jitter_seed = int(cfg.jitter_seed)
+    p.sensor_model = int(cfg.sensor_model)
+    p.input_cs = int(cfg.input_cs)
+    p.output_cs = int(cfg.output_cs)
+    for k, v in enumerate(_flat9(cfg.custom_input_to_xyz)):
+        p.custom_input_to_xyz[k] = v
+    for k, v in enumerate(_flat9(cfg.custom_xyz_to_output)):
+        p.custom_xyz_to_output[k] = v
+    p.verbose = int(bool(cfg.verbose))
+    p.aperture_blades = int(getattr(cfg, "aperture_blades", 0))
+    p.aperture_rotation = float(getattr(cfg, "aperture_rotation", 0.0))
+    p.fp64_radius_min = float(getattr(cfg, "fp64_radius_min", 1000.0))
+    return p
 
 
 def to_params(cfg: FlareConfig):
@@ -280,4 +306,5 @@ def to_params(cfg: FlareConfig):
     p.fp64_radius_min = float(cfg.fp64_radius_min)
     p.batch_mode = int(getattr(cfg, "batch_mode", 2))
     p.schedule = int(getattr(cfg, "schedule", -1))
+    p.frame_per_source = int(bool(getattr(cfg, "frame_per_source", False)))
     return p, buf

@@ -77,6 +77,7 @@ struct GhostArgs {
     unsigned long long* __restrict__ segs;      // optional segment counter
     const int2* __restrict__ lrow;              // SPLAT: per-(lambda, surface) coating lookups
     int lam_group;                              // STATS: lanes per sample (one lambda each), 0 = loop
+    int frame_mode;                             // SPLAT: records carry their source as the frame
     float4* __restrict__ rec;                   // SPLAT: deposit records
     unsigned long long rec_cap;
     unsigned long long* __restrict__ rec_count;
@@ -183,7 +184,7 @@ __device__ __forceinline__ void deposit_range(const GhostArgs& A, int ps, int gi
             const int x0 = (int)floorf(px - 0.5f), y0 = (int)floorf(py - 0.5f);
             if (x0 + 1 < 0 || x0 >= A.width || y0 + 1 < 0 || y0 >= A.height) break;   // footprint off the image
             const float P = sr.r * sp.y + sr.g * sp.z + sr.b * sp.w;
-            rec = make_float4(px, py, P * base, __int_as_float(si));
+            rec = make_float4(px, py, P * base, __int_as_float(si | (A.frame_mode ? (src_idx << 8) : 0)));
             dep = true;
         } while (false);
         const unsigned long long bal = __ballot(dep);
@@ -511,7 +512,9 @@ static int do_launch(Launch& Lc, float* out_r, float* out_g, float* out_b, float
     ior_tables(L->host.data(), n, c->spec.data(), ns, iat.data(), ibf.data());
     const double t_prep1 = now_ms();
 
-    const size_t npx = (size_t)Lc.width * Lc.height;
+    // Frame mode: one output frame per source, frame-major planes.
+    const int frames = p->frame_per_source ? std::max(1, n_src) : 1;
+    const size_t npx = (size_t)Lc.width * Lc.height * (size_t)frames;
     DevBuf &bo_r = c->buf("out_r"), &bo_g = c->buf("out_g"), &bo_b = c->buf("out_b");
     if ((st = bo_r.ensure(npx * 4)) || (st = bo_g.ensure(npx * 4)) || (st = bo_b.ensure(npx * 4))) return st;
     auto zero_out = [&]() -> int {
@@ -760,7 +763,8 @@ static int do_launch(Launch& Lc, float* out_r, float* out_g, float* out_b, float
         max_rec += ns_i;
     }
     max_rec *= (unsigned long long)ns;
-    const SplatGeom sg = splat_geom(Lc.width, Lc.height);
+    const SplatGeom sg = splat_geom(Lc.width, Lc.height, frames);
+    A.frame_mode = frames > 1 ? 1 : 0;
     unsigned long long want = std::max<unsigned long long>(c->splat_hint + c->splat_hint / 4, 1ull << 22);
     want = std::max<unsigned long long>(1, std::min(want, max_rec));
     // ---- launch form: grid, or persistent over a cost-ordered work list
@@ -1115,6 +1119,8 @@ int gl_render_ghost_pipeline(gl_ctx* ctx, const gl_lens* lens, const gl_source* 
                              int32_t* aov_n, gl_ghost_stats* stats) {
     int st = validate(lens, p, width, height);
     if (st) return st;
+    if (p && p->frame_per_source)
+        return set_error(GL_ERR_INVALID, "frame_per_source renders go through gl_launch_ghost_render");
     if (aov_n) *aov_n = 0;
     const size_t npx = (size_t)width * height;
     if (out_r) { std::fill(out_r, out_r + npx, 0.0f); std::fill(out_g, out_g + npx, 0.0f); std::fill(out_b, out_b + npx, 0.0f); }

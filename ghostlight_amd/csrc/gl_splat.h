@@ -10,7 +10,11 @@ namespace gl {
 struct SplatGeom {
     int width, height;
     int tile_shift;          // tile side = 1 << tile_shift pixels
-    int tiles_x, tiles_y, n_tiles;
+    int tiles_x, tiles_y;
+    int frames;              // output frames (one per source in frame mode, else 1)
+    int tiles_per_frame;
+    int n_tiles;             // tiles_per_frame * frames
+    int lds_hist;            // 1: per-workgroup LDS histograms in the binning passes
 };
 
 struct SplatBufs {
@@ -27,7 +31,7 @@ struct SplatBufs {
     unsigned* queue;                      // resolve work-queue head
 };
 
-SplatGeom splat_geom(int width, int height);
+SplatGeom splat_geom(int width, int height, int frames = 1);
 int splat_prepare(gl_ctx* c, const SplatGeom& g, unsigned long long rec_cap, SplatBufs& B);
 int splat_clear(hipStream_t s, const SplatGeom& g, const SplatBufs& B);
 int splat_resolve(hipStream_t s, const SplatGeom& g, const SplatBufs& B, const float4* spec, float* out_r,

@@ -17,6 +17,12 @@
 //                one chunk (its pixels belong to that workgroup alone), one
 //                atomic per pixel otherwise.
 //
+// Frame mode: a record's lambda word also carries its frame (one frame per
+// source, rec.w bits 8..), tiles are numbered frame-major and the resolve
+// writes into that frame's planes (out + frame * width * height).  With more
+// tiles than an LDS histogram holds, the binning passes count straight into
+// the global tile counters.
+//
 // Each contribution is the same product the reference's splat adds
 // (ghost_render.cu:430-462): colour = power * cmf_c(lambda), weight = the
 // bilinear cell overlap, skipped when |colour| <= 1e-14.
@@ -51,12 +57,17 @@ __device__ __forceinline__ Foot foot_of(float px, float py) {
     return f;
 }
 
-// Tiles touched by a record (clipped footprint), as a [tx0, tx1] x [ty0, ty1] box.
-__device__ __forceinline__ bool tiles_of(const float4& r, const SplatGeom& g, int& tx0, int& tx1, int& ty0, int& ty1) {
+// Tiles touched by a record (clipped footprint), as a [tx0, tx1] x [ty0, ty1]
+// box within frame `base / tiles_per_frame` (base = first tile id of the frame).
+__device__ __forceinline__ bool tiles_of(const float4& r, const SplatGeom& g, int& tx0, int& tx1, int& ty0, int& ty1,
+                                         int& base) {
     const Foot f = foot_of(r.x, r.y);
     const int xa = max(f.x0, 0), xb = min(f.x0 + 1, g.width - 1);
     const int ya = max(f.y0, 0), yb = min(f.y0 + 1, g.height - 1);
     if (xa > xb || ya > yb) return false;
+    const int frame = __float_as_int(r.w) >> 8;
+    if (frame >= g.frames) return false;
+    base = frame * g.tiles_per_frame;
     tx0 = xa >> g.tile_shift; tx1 = xb >> g.tile_shift;
     ty0 = ya >> g.tile_shift; ty1 = yb >> g.tile_shift;
     return true;
@@ -75,24 +86,29 @@ __global__ __launch_bounds__(256) void splat_clear_kernel(SplatBufs B, int n_til
 
 __global__ __launch_bounds__(kBinThreads) void bin_count_kernel(SplatBufs B, SplatGeom g) {
     extern __shared__ unsigned hist[];
-    for (int t = threadIdx.x; t < g.n_tiles; t += blockDim.x) hist[t] = 0u;
-    __syncthreads();
+    if (g.lds_hist) {
+        for (int t = threadIdx.x; t < g.n_tiles; t += blockDim.x) hist[t] = 0u;
+        __syncthreads();
+    }
+    unsigned* h = g.lds_hist ? hist : B.tile_count;
     const unsigned long long n = n_records(B);
     const unsigned long long per = (n + gridDim.x - 1) / gridDim.x;
     const unsigned long long r0 = per * blockIdx.x, r1 = min(n, r0 + per);
     for (unsigned long long i = r0 + threadIdx.x; i < r1; i += blockDim.x) {
-        int tx0, tx1, ty0, ty1;
-        if (!tiles_of(B.rec[i], g, tx0, tx1, ty0, ty1)) continue;
+        int tx0, tx1, ty0, ty1, base;
+        if (!tiles_of(B.rec[i], g, tx0, tx1, ty0, ty1, base)) continue;
         for (int ty = ty0; ty <= ty1; ++ty)
-            for (int tx = tx0; tx <= tx1; ++tx) atomicAdd(&hist[ty * g.tiles_x + tx], 1u);
+            for (int tx = tx0; tx <= tx1; ++tx) atomicAdd(&h[base + ty * g.tiles_x + tx], 1u);
     }
-    __syncthreads();
-    for (int t = threadIdx.x; t < g.n_tiles; t += blockDim.x)
-        if (hist[t]) atomicAdd(&B.tile_count[t], hist[t]);
+    if (g.lds_hist) {
+        __syncthreads();
+        for (int t = threadIdx.x; t < g.n_tiles; t += blockDim.x)
+            if (hist[t]) atomicAdd(&B.tile_count[t], hist[t]);
+    }
 }
 
-// Exclusive scan of tile counts (one workgroup of 1024 lanes, 8 tiles a lane
-// at most): tile_start, tile_cursor, chunk_start, and the totals.
+// Exclusive scan of tile counts (one workgroup of 1024 lanes, a contiguous
+// span of tiles each): tile_start, tile_cursor, chunk_start, and the totals.
 __global__ __launch_bounds__(1024) void bin_scan_kernel(SplatBufs B, SplatGeom g) {
     __shared__ unsigned long long s_rec[1024];
     __shared__ unsigned s_chk[1024];
@@ -135,18 +151,32 @@ __global__ __launch_bounds__(1024) void bin_scan_kernel(SplatBufs B, SplatGeom g
 
 __global__ __launch_bounds__(kBinThreads) void bin_scatter_kernel(SplatBufs B, SplatGeom g) {
     extern __shared__ unsigned smem[];
+    const unsigned long long n = n_records(B);
+    const unsigned long long per = (n + gridDim.x - 1) / gridDim.x;
+    const unsigned long long r0 = per * blockIdx.x, r1 = min(n, r0 + per);
+    if (!g.lds_hist) {
+        // Global form: one cursor reservation per (record, tile).
+        for (unsigned long long i = r0 + threadIdx.x; i < r1; i += blockDim.x) {
+            const float4 r = B.rec[i];
+            int tx0, tx1, ty0, ty1, fb;
+            if (!tiles_of(r, g, tx0, tx1, ty0, ty1, fb)) continue;
+            for (int ty = ty0; ty <= ty1; ++ty)
+                for (int tx = tx0; tx <= tx1; ++tx) {
+                    const unsigned long long slot = atomicAdd(&B.tile_cursor[fb + ty * g.tiles_x + tx], 1ull);
+                    if (slot < B.bin_cap) B.binned[slot] = r;
+                }
+        }
+        return;
+    }
     unsigned* hist = smem;                       // counts, then per-block base offsets
     unsigned long long* base = reinterpret_cast<unsigned long long*>(smem + ((g.n_tiles + 1) & ~1));
     for (int t = threadIdx.x; t < g.n_tiles; t += blockDim.x) hist[t] = 0u;
     __syncthreads();
-    const unsigned long long n = n_records(B);
-    const unsigned long long per = (n + gridDim.x - 1) / gridDim.x;
-    const unsigned long long r0 = per * blockIdx.x, r1 = min(n, r0 + per);
     for (unsigned long long i = r0 + threadIdx.x; i < r1; i += blockDim.x) {
-        int tx0, tx1, ty0, ty1;
-        if (!tiles_of(B.rec[i], g, tx0, tx1, ty0, ty1)) continue;
+        int tx0, tx1, ty0, ty1, fb;
+        if (!tiles_of(B.rec[i], g, tx0, tx1, ty0, ty1, fb)) continue;
         for (int ty = ty0; ty <= ty1; ++ty)
-            for (int tx = tx0; tx <= tx1; ++tx) atomicAdd(&hist[ty * g.tiles_x + tx], 1u);
+            for (int tx = tx0; tx <= tx1; ++tx) atomicAdd(&hist[fb + ty * g.tiles_x + tx], 1u);
     }
     __syncthreads();
     for (int t = threadIdx.x; t < g.n_tiles; t += blockDim.x) {
@@ -157,11 +187,11 @@ __global__ __launch_bounds__(kBinThreads) void bin_scatter_kernel(SplatBufs B, S
     __syncthreads();
     for (unsigned long long i = r0 + threadIdx.x; i < r1; i += blockDim.x) {
         const float4 r = B.rec[i];
-        int tx0, tx1, ty0, ty1;
-        if (!tiles_of(r, g, tx0, tx1, ty0, ty1)) continue;
+        int tx0, tx1, ty0, ty1, fb;
+        if (!tiles_of(r, g, tx0, tx1, ty0, ty1, fb)) continue;
         for (int ty = ty0; ty <= ty1; ++ty)
             for (int tx = tx0; tx <= tx1; ++tx) {
-                const int t = ty * g.tiles_x + tx;
+                const int t = fb + ty * g.tiles_x + tx;
                 const unsigned long long slot = base[t] + atomicAdd(&hist[t], 1u);
                 if (slot < B.bin_cap) B.binned[slot] = r;
             }
@@ -196,11 +226,13 @@ __global__ __launch_bounds__(256) void resolve_kernel(SplatBufs B, SplatGeom g, 
         const int nchunks = (int)(B.chunk_start[t + 1] - B.chunk_start[t]);
         for (int i = threadIdx.x; i < 3 * TT; i += blockDim.x) tile[i] = 0.0f;
         __syncthreads();
-        const int ox = (t % g.tiles_x) << g.tile_shift, oy = (t / g.tiles_x) << g.tile_shift;
+        const int frame = t / g.tiles_per_frame, tf = t % g.tiles_per_frame;
+        const int ox = (tf % g.tiles_x) << g.tile_shift, oy = (tf / g.tiles_x) << g.tile_shift;
+        const size_t fo = (size_t)frame * g.width * g.height;
         for (unsigned long long i = rb + threadIdx.x; i < re; i += blockDim.x) {
             const float4 r = B.binned[i];
             const Foot f = foot_of(r.x, r.y);
-            const float4 s = spec[__float_as_int(r.w)];
+            const float4 s = spec[__float_as_int(r.w) & 0xFF];
             const float cr = r.z * s.y, cg = r.z * s.z, cb = r.z * s.w;
             const float w00 = (1.0f - f.fx) * (1.0f - f.fy), w10 = f.fx * (1.0f - f.fy);
             const float w01 = (1.0f - f.fx) * f.fy, w11 = f.fx * f.fy;
@@ -221,7 +253,7 @@ __global__ __launch_bounds__(256) void resolve_kernel(SplatBufs B, SplatGeom g, 
         for (int p = threadIdx.x; p < TT; p += blockDim.x) {
             const int x = ox + (p & (TS - 1)), y = oy + (p >> g.tile_shift);
             if (x >= g.width || y >= g.height) continue;
-            const size_t o = (size_t)y * g.width + x;
+            const size_t o = fo + (size_t)y * g.width + x;
             const float vr = tile[p], vg = tile[TT + p], vb = tile[2 * TT + p];
             if (nchunks == 1) {
                 if (vr != 0.0f) out_r[o] += vr;
@@ -237,25 +269,30 @@ __global__ __launch_bounds__(256) void resolve_kernel(SplatBufs B, SplatGeom g, 
     }
 }
 
-SplatGeom splat_geom(int width, int height) {
+SplatGeom splat_geom(int width, int height, int frames) {
     SplatGeom g;
     std::memset(&g, 0, sizeof g);
     g.width = width;
     g.height = height;
+    g.frames = frames < 1 ? 1 : frames;
     g.tile_shift = 5;
     for (;;) {
         const int TS = 1 << g.tile_shift;
         g.tiles_x = (width + TS - 1) / TS;
         g.tiles_y = (height + TS - 1) / TS;
         const long long nt = (long long)g.tiles_x * g.tiles_y;
-        if (nt <= kMaxTiles || g.tile_shift >= 6) { g.n_tiles = (int)nt; break; }
+        if (nt <= kMaxTiles || g.tile_shift >= 6) { g.tiles_per_frame = (int)nt; break; }
         ++g.tile_shift;
     }
+    const long long all = (long long)g.tiles_per_frame * g.frames;
+    g.n_tiles = all > (long long)INT32_MAX / 2 ? INT32_MAX / 2 : (int)all;
+    g.lds_hist = g.n_tiles <= kMaxTiles ? 1 : 0;
     return g;
 }
 
 int splat_prepare(gl_ctx* c, const SplatGeom& g, unsigned long long rec_cap, SplatBufs& B) {
-    if (g.n_tiles > kMaxTiles) return set_error(GL_ERR_INVALID, "image too large for the tiled splat (%d tiles)", g.n_tiles);
+    if (g.tiles_per_frame > kMaxTiles)
+        return set_error(GL_ERR_INVALID, "image too large for the tiled splat (%d tiles a frame)", g.tiles_per_frame);
     DevBuf &brec = c->buf("splat_rec"), &bbin = c->buf("splat_bin"), &bt = c->buf("splat_tiles");
     int st;
     const unsigned long long bin_cap = rec_cap + rec_cap / 2;
@@ -288,8 +325,8 @@ int splat_clear(hipStream_t s, const SplatGeom& g, const SplatBufs& B) {
 
 int splat_resolve(hipStream_t s, const SplatGeom& g, const SplatBufs& B, const float4* spec, float* out_r, float* out_g,
                   float* out_b) {
-    const size_t hist_bytes = (size_t)g.n_tiles * 4;
-    const size_t scat_bytes = (size_t)((g.n_tiles + 1) & ~1) * 4 + (size_t)g.n_tiles * 8;
+    const size_t hist_bytes = g.lds_hist ? (size_t)g.n_tiles * 4 : 0;
+    const size_t scat_bytes = g.lds_hist ? (size_t)((g.n_tiles + 1) & ~1) * 4 + (size_t)g.n_tiles * 8 : 0;
     hipLaunchKernelGGL(bin_count_kernel, dim3(kBinBlocks), dim3(kBinThreads), hist_bytes, s, B, g);
     hipLaunchKernelGGL(bin_scan_kernel, dim3(1), dim3(1024), 0, s, B, g);
     hipLaunchKernelGGL(bin_scatter_kernel, dim3(kBinBlocks), dim3(kBinThreads), scat_bytes, s, B, g);

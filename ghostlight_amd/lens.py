@@ -324,6 +324,25 @@ class OpticalSystem:
         return render_source_flare(arr, width, height, self, calib if calib is not None else self.calibration(),
                                    config, ctx=ctx)
 
+    def render_psf(self, sources, config, *, weights=None, targets_mm=None, ctx=None) -> dict:
+        """Geometric PSF grid (optical_system.py:431-481): sources (N, 2)
+        field angles or (N, 5) rows; weights scale the source colour;
+        targets_mm (N, 2) switches on FIXED_TARGET anchoring."""
+        from .psf import render_psf
+        s = np.asarray(sources, np.float32)
+        if s.ndim != 2 or s.shape[1] not in (2, 5, 7):
+            raise ValueError("render_psf: sources must be (N, 2), (N, 5) or (N, 7)")
+        if s.shape[1] == 2:
+            s = np.concatenate([s, np.ones((len(s), 3), np.float32)], 1)
+        if weights is not None:
+            w = np.asarray(weights, np.float32).reshape(-1, 1)
+            s = s.copy()
+            s[:, 2:5] *= w
+        if targets_mm is not None:
+            t = np.asarray(targets_mm, np.float32).reshape(-1, 2)
+            s = np.concatenate([s[:, :5], t], 1)
+        return render_psf(s, self, config, ctx=ctx)
+
     def __repr__(self) -> str:
         return f"OpticalSystem(name={self.name!r}, surfaces={len(self.surfaces)})"
 

@@ -139,7 +139,8 @@ def render_ghosts(lens: OpticalSystem, sources, width: int, height: int, config:
                                                 C.byref(st.s) if st else None), "render_ghosts")
         out = tuple(C.cast(ptrs[i], C.c_void_p).value for i in range(3))
     else:
-        planes = [np.zeros((height, width), np.float32) for _ in range(3)]
+        shape = (n_src, height, width) if getattr(config, "frame_per_source", False) else (height, width)
+        planes = [np.zeros(shape, np.float32) for _ in range(3)]
         _abi.check(lib().gl_launch_ghost_render(ctx_handle(ctx), dl.handle, parr, len(pairs), sarr, n_src,
                                                 float(sensor_half_w), float(sensor_half_h), width, height, fw, fh, x0,
                                                 y0, C.byref(p), _abi.fptr(planes[0]), _abi.fptr(planes[1]),

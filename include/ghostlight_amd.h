@@ -152,6 +152,10 @@ typedef struct gl_flare_params {
      * replayed.  schedule: -1 = autotune, else a fixed candidate index. */
     int32_t batch_mode;
     int32_t schedule;
+    /* Build-specific: 1 = every source renders into a frame of its own; the
+     * output planes then hold n_sources x height x width floats each (the
+     * reference needs one call per source for this). */
+    int32_t frame_per_source;
 } gl_flare_params;
 
 #define GL_BATCH_GRID        0

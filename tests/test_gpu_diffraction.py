@@ -371,3 +371,59 @@ def test_precision_pupil_mft(prec, tol):
         err = dor.rel_l2(got[li], ref)
         print(f"precision pupil MFT prec={prec} lambda={lam}: rel-L2 vs FP64 {err:.2e}", flush=True)
         assert err <= tol
+
+
+@pytest.mark.parametrize("sigma_um,mono", [(0.0, False), (4.0, False), (4.0, True)])
+def test_psf_grid_matches_oracle(c1, sigma_um, mono):
+    from ghostlight_amd import PSFConfig, hostprep
+    from ghostlight_amd.psf import render_psf
+    from oracle import psf_oracle
+    lens, _cal = c1
+    cfg = PSFConfig(grid_nx=3, grid_ny=2, tile_w=48, tile_h=40, tile_extent_mm=0.06, ray_grid=32, spectral_samples=6,
+                    splat_sigma_um=sigma_um, monochromatic=mono)
+    ang = np.radians([[0.0, 0.0], [5.0, 0.0], [10.0, 2.0], [-7.0, 4.0], [12.0, -6.0]])
+    src = np.concatenate([ang, np.array([[1.0, 0.8, 0.6]] * len(ang))], 1).astype(np.float32)
+    out = render_psf(src, lens, cfg)
+    assert out["r"].shape == (80, 144) and out["status"].dtype == np.uint8
+    # chief pre-pass (7 x 7 pupil mean, CPU t >= 0 rule)
+    for i in range(len(src)):
+        hits, mx, my = psf_oracle.pupil_probe_mean(lens, src[i, 0], src[i, 1])
+        assert (out["status"][i] == 0) == (hits > 0)
+        assert abs(out["chief_x_mm"][i] - mx) <= 1e-5 and abs(out["chief_y_mm"][i] - my) <= 1e-5
+    grid = pyoracle.build_samples(lens, 32)
+    if mono:
+        spec = [(587.56, 0.0, 1.0, 0.0)]
+    else:
+        t = hostprep.spectral_table(6).astype(np.float64)
+        t[:, 1:] *= 300.0 / 6
+        spec = [tuple(r) for r in t]
+    mmpp = 0.06 / 48
+    sig = sigma_um / (mmpp * 1000.0)
+    rad = min(16, max(1, math.ceil(2.5 * sig))) if sig > 0 else 0
+    chief = np.stack([out["chief_x_mm"], out["chief_y_mm"]], 1).astype(np.float64)
+    ref = psf_oracle.psf_splat(lens, src, chief, grid, spec, 48, 40, 3, 2, mmpp, sig, rad, mono=mono)
+    got = np.stack([out["r"], out["g"], out["b"]])
+    err = dor.rel_l2(got, ref)
+    print(f"PSF grid sigma={sigma_um} mono={mono}: rel-L2 vs oracle {err:.2e}", flush=True)
+    assert err <= 1e-4
+    if mono:
+        assert not out["g"].any() and not out["b"].any()
+
+
+def test_psf_fixed_target_aims_the_chief(c1):
+    from ghostlight_amd import PSFConfig
+    from ghostlight_amd.psf import render_psf
+    lens, cal = c1
+    f = lens.focal_length
+    cfg = PSFConfig(grid_nx=2, grid_ny=2, tile_w=32, tile_h=32, ray_grid=24, spectral_samples=4,
+                    center_mode=1, splat_sigma_um=0.0)
+    targets = np.array([[0.0, 0.0], [1.5, 0.0], [-2.0, 1.0], [3.0, -2.5]])
+    seeds = np.arctan(targets / f)               # first-order seed
+    src = np.concatenate([seeds, np.ones((4, 3)), targets], 1).astype(np.float32)
+    out = render_psf(src, lens, cfg)
+    ok = out["status"] == 0
+    print(f"aim residuals (mm) {out['aim_residual_mm']}, status {out['status']}", flush=True)
+    assert ok.all()
+    assert np.all(out["aim_residual_mm"][ok] < 1e-3)
+    assert np.allclose(out["chief_x_mm"], targets[:, 0], atol=1e-3)
+    assert np.allclose(out["chief_y_mm"], targets[:, 1], atol=1e-3)

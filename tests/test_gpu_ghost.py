@@ -255,3 +255,26 @@ def test_autotuner_settles_on_the_fastest_candidate():
     assert [c for c, t in seen[:n]] == list(range(n)) and all(t for c, t in seen[:n])
     assert not seen[-1][1] and seen[-1][0] == seen[-2][0]
     ctx.close()
+
+
+def test_frame_per_source_matches_single_renders():
+    """Frame mode (one output frame per source, one launch) equals rendering
+    each source alone, up to float summation order (cull off, so the traced
+    pair set is the same either way)."""
+    lens = LENSES["c4"]
+    W, H = 256, 108
+    hw, hh = synthetic.sensor_for(lens, W, H, half_field_deg=12.0)
+    cfg = FlareConfig(ray_grid=32, spectral_samples=6, cull_dead_pairs=False)
+    pairs, boosts = pyoracle.filter_ghost_pairs(lens, hw, hh)
+    t = np.radians(np.linspace(-8.0, 8.0, 3))
+    src = np.stack([t, 0.4 * t, np.ones(3), np.ones(3) * 0.7, np.ones(3) * 0.4], 1).astype(np.float32)
+    import dataclasses
+    fr = render_ghosts(lens, src, W, H, dataclasses.replace(cfg, frame_per_source=True), hw, hh, pairs=pairs,
+                       boosts=boosts)
+    assert fr[0].shape == (3, H, W)
+    for i in range(3):
+        one = render_ghosts(lens, src[i:i + 1], W, H, cfg, hw, hh, pairs=pairs, boosts=boosts)
+        e = pyoracle.rel_l2([fr[c][i] for c in range(3)], one)
+        print(f"frame {i}: rel-L2 vs single-source render {e:.2e}", flush=True)
+        assert float(sum(x.sum() for x in one)) > 0
+        assert e <= 1e-6

@@ -74,3 +74,25 @@ def test_precision_pupil_ranges():
     core = a[96:160, 96:160]
     assert core.max() <= 1.0 and np.median(core) >= 0.9       # dust transmission 0.9..1.0 inside the stop
     assert np.array_equal(P, synthetic.precision_pupil(256, seed=3))   # seeded
+
+
+def test_psf_oracle_splat_conserves_weight():
+    """Each landed ray deposits its full weight (bilinear weights and the
+    normalised Gaussian both sum to one) when its footprint is inside the
+    tile: the composite total equals the sum of landed weights x colour."""
+    from ghostlight_amd import synthetic
+    from oracle import psf_oracle, pyoracle
+    lens = synthetic.config1_lens()
+    grid = pyoracle.build_samples(lens, 16)
+    src = np.array([[0.0, 0.0, 1.0, 1.0, 1.0]], np.float32)
+    hits, mx, my = psf_oracle.pupil_probe_mean(lens, 0.0, 0.0)
+    assert hits > 0
+    chief = np.array([[mx, my]])
+    spec = [(550.0, 0.0, 1.0, 0.0)]
+    r = psf_oracle._rays(lens, 0.0, 0.0, grid[:, 0], grid[:, 1], 550.0)
+    pos, w, st, _ = pyoracle.trace_ghost(lens, -1, -1, r, dbl=False, t_ge0=False, fp64_radius_min=1000.0)
+    ok = (st == 0)
+    expect = float(w[ok].sum()) / len(grid)
+    for sigma, rad in ((0.0, 0), (2.0, 5)):
+        out = psf_oracle.psf_splat(lens, src, chief, grid, spec, 512, 512, 1, 1, 0.01, sigma, rad)
+        assert math.isclose(out[1].sum(), expect, rel_tol=1e-5)
