@@ -11,7 +11,7 @@ from .lens import (ApertureShape, Coating, CoatingModel, CylinderAxis, Dispersio
                    OpticalSystem, Surface, SurfaceForm, TraceStatus)
 from .runtime import Context, device_available
 from .render import (GhostPair, Ray, TraceResult, enumerate_ghost_pairs, filter_ghost_pairs, render_ghosts,
-                     render_point_flare, render_source_flare, solve_source_angle, trace_ghost_ray, trace_ghost_rays,
+                     render_point_flare, render_source_flare, render_sources, solve_source_angle, trace_ghost_ray, trace_ghost_rays,
                      trace_primary_ray)
 from .hostprep import build_spectral_lambdas
 from .diffraction import _render_starburst_debug, _render_veil_debug, fft2d

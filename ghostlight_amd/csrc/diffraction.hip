@@ -934,8 +934,8 @@ int gl_render_starburst(gl_ctx* ctx, const gl_lens* lens, const gl_calibration* 
     if (dbg) {
         dbg->dx_mm_x = sx; dbg->dx_mm_y = sy;
         if (dbg->grid_cap >= N) {
-            if (dbg->psf) GL_HIP(hipMemcpy(dbg->psf, bpsf.p, NN * 4, hipMemcpyDeviceToHost));
-            if (dbg->sprite_rgb) GL_HIP(hipMemcpy(dbg->sprite_rgb, bsp.p, NN * 12, hipMemcpyDeviceToHost));
+            if (dbg->psf) GL_HIP(d2h_sync(c, dbg->psf, bpsf.p, NN * 4));
+            if (dbg->sprite_rgb) GL_HIP(d2h_sync(c, dbg->sprite_rgb, bsp.p, NN * 12));
         }
     }
     return GL_OK;
@@ -1006,7 +1006,7 @@ int gl_render_veil(gl_ctx* ctx, const gl_source* sources, const float* src_px, c
         dbg->core_mm = core_mm; dbg->falloff = fp; dbg->core_texels = a_texels;
         if (dbg->reference || dbg->sprite_rgb) {
             std::vector<float> h(NN);
-            GL_HIP(hipMemcpy(h.data(), bref.p, NN * 4, hipMemcpyDeviceToHost));
+            GL_HIP(d2h_sync(c, h.data(), bref.p, NN * 4));
             if (dbg->reference) std::copy(h.begin(), h.end(), dbg->reference);
             if (dbg->sprite_rgb)
                 for (size_t k = 0; k < NN; ++k) { dbg->sprite_rgb[3 * k] = h[k]; dbg->sprite_rgb[3 * k + 1] = h[k]; dbg->sprite_rgb[3 * k + 2] = h[k]; }

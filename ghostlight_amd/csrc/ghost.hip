@@ -618,7 +618,7 @@ static int do_launch(Launch& Lc, float* out_r, float* out_g, float* out_b, float
                 GL_HIP(hipStreamSynchronize(c->stream));
                 pass0_ok = true;
                 if (!want_conc) {
-                    GL_HIP(hipMemcpy(cnt.data(), bc.p, 3 * n_pairs * 8, hipMemcpyDeviceToHost));
+                    GL_HIP(d2h_sync(c, cnt.data(), bc.p, 3 * n_pairs * 8));
                     int nc = 0;
                     for (int q = 0; q < n_pairs; ++q)
                         if (cnt[q] > 0 && cnt[2 * n_pairs + q] == 0) { cand[q] = 1; ++nc; }
@@ -670,7 +670,7 @@ static int do_launch(Launch& Lc, float* out_r, float* out_g, float* out_b, float
             for (int q = 0; q < n_pairs; ++q) active[q] = cnt[2 * n_pairs + q] > 0 ? 1 : 0;
             if (want_conc) {
                 std::vector<int> bbh((size_t)n_ps * 5);
-                GL_HIP(hipMemcpy(bbh.data(), bb.p, (size_t)n_ps * 5 * 4, hipMemcpyDeviceToHost));
+                GL_HIP(d2h_sync(c, bbh.data(), bb.p, (size_t)n_ps * 5 * 4));
                 rects.assign(n_ps, make_float4(2.f, 2.f, -2.f, -2.f));
                 const float dil = kConcDilateCells * (2.0f / (float)kProbeGrid);
                 auto dec = [](int i) { float f; std::memcpy(&f, &i, 4); return f - 2.0f; };
@@ -888,12 +888,17 @@ static int do_launch(Launch& Lc, float* out_r, float* out_g, float* out_b, float
             key_append(gk, A); key_append(gk, sb); key_append(gk, Q); key_append(gk, workers); key_append(gk, sg);
             key_append(gk, c->spec_do_correct); key_append(gk, c->spec_C); key_append(gk, npx);
             if (!c->graph || gk != c->graph_key) {
-                if (c->graph) { (void)hipGraphExecDestroy(c->graph); c->graph = nullptr; }
                 c->graph_key.clear();
-                GL_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-                const int e = enqueue(false);
                 hipGraph_t gph = nullptr;
-                const hipError_t ec = hipStreamEndCapture(c->stream, &gph);
+                int e = 0;
+                hipError_t ec;
+                {
+                    std::lock_guard<std::mutex> gm(device_mem_mutex());
+                    if (c->graph) { (void)hipGraphExecDestroy(c->graph); c->graph = nullptr; }
+                    GL_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+                    e = enqueue(false);
+                    ec = hipStreamEndCapture(c->stream, &gph);
+                }
                 if (e) { if (gph) (void)hipGraphDestroy(gph); return e; }
                 if (ec != hipSuccess) return set_error(GL_ERR_DEVICE, "hipStreamEndCapture -- %s", hipGetErrorString(ec));
                 const hipError_t ei = hipGraphInstantiate(&c->graph, gph, nullptr, nullptr, 0);

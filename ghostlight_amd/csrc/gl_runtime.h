@@ -74,6 +74,19 @@ struct gl_ctx {
 };
 
 namespace gl {
+// Device-to-host copy ordered on the context's stream.  The library never
+// touches the legacy (null) stream: a synchronous hipMemcpy there would have to
+// wait on every blocking stream, including one another thread is capturing.
+inline hipError_t d2h_sync(gl_ctx* c, void* dst, const void* src, size_t bytes) {
+    const hipError_t e = hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, c->stream);
+    return e != hipSuccess ? e : hipStreamSynchronize(c->stream);
+}
+
+// Held around every hipMalloc/hipFree of the library and around each graph
+// capture, so that no allocation or release (hipFree synchronises the device)
+// falls inside another thread's capture sequence.
+std::mutex& device_mem_mutex();
+
 // Device view of a lens as the kernels read it.
 struct LensDev {
     const DevSurf* surf = nullptr;

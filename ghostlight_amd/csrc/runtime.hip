@@ -20,8 +20,14 @@ int set_error(int code, const char* fmt, ...) {
     return code;
 }
 
+std::mutex& device_mem_mutex() {
+    static std::mutex m;
+    return m;
+}
+
 int DevBuf::ensure(size_t bytes) {
     if (bytes <= cap && p) return GL_OK;
+    std::lock_guard<std::mutex> g(device_mem_mutex());
     if (p) (void)hipFree(p);
     p = nullptr; cap = 0;
     size_t want = bytes < 256 ? 256 : bytes;
@@ -36,7 +42,9 @@ int DevBuf::ensure(size_t bytes) {
 }
 
 void DevBuf::release() {
-    if (p) (void)hipFree(p);
+    if (!p) return;
+    std::lock_guard<std::mutex> g(device_mem_mutex());
+    (void)hipFree(p);
     p = nullptr; cap = 0;
 }
 
@@ -84,7 +92,10 @@ int lens_sync(gl_lens* L, float fp64_rmin, hipStream_t stream, LensDev* out) {
     if (L->device != dev) {
         if (L->device >= 0) return set_error(GL_ERR_INVALID, "lens was created for device %d, used on device %d",
                                              L->device, dev);
-        GL_HIP(hipMalloc(&L->d_arena, std::max<size_t>(1, L->arena.size()) * sizeof(float)));
+        {
+            std::lock_guard<std::mutex> ga(device_mem_mutex());
+            GL_HIP(hipMalloc(&L->d_arena, std::max<size_t>(1, L->arena.size()) * sizeof(float)));
+        }
         GL_HIP(hipMemcpyAsync(L->d_arena, L->arena.data(), L->arena.size() * sizeof(float), hipMemcpyHostToDevice,
                               stream));
         L->device = dev;
@@ -97,8 +108,11 @@ int lens_sync(gl_lens* L, float fp64_rmin, hipStream_t stream, LensDev* out) {
         std::vector<float> poly;
         build_devsurfs(L->host.data(), (int)L->host.size(), fp64_rmin, L->arena.data(), (int)L->arena.size(), ds, poly);
         gl_lens::Variant nv{fp64_rmin, nullptr, nullptr};
-        GL_HIP(hipMalloc(&nv.d_surf, std::max<size_t>(1, ds.size()) * sizeof(DevSurf)));
-        GL_HIP(hipMalloc(&nv.d_poly, std::max<size_t>(1, poly.size()) * sizeof(float)));
+        {
+            std::lock_guard<std::mutex> ga(device_mem_mutex());
+            GL_HIP(hipMalloc(&nv.d_surf, std::max<size_t>(1, ds.size()) * sizeof(DevSurf)));
+            GL_HIP(hipMalloc(&nv.d_poly, std::max<size_t>(1, poly.size()) * sizeof(float)));
+        }
         GL_HIP(hipMemcpyAsync(nv.d_surf, ds.data(), ds.size() * sizeof(DevSurf), hipMemcpyHostToDevice, stream));
         GL_HIP(hipMemcpyAsync(nv.d_poly, poly.data(), poly.size() * sizeof(float), hipMemcpyHostToDevice, stream));
         GL_HIP(hipStreamSynchronize(stream));
@@ -143,8 +157,11 @@ void gl_ctx_destroy(gl_ctx* c) {
         if (c->ev0) (void)hipEventDestroy(c->ev0);
         if (c->ev1) (void)hipEventDestroy(c->ev1);
         if (c->ev2) (void)hipEventDestroy(c->ev2);
-        if (c->graph) (void)hipGraphExecDestroy(c->graph);
-        if (c->stream) (void)hipStreamDestroy(c->stream);
+        {
+            std::lock_guard<std::mutex> gm(gl::device_mem_mutex());
+            if (c->graph) (void)hipGraphExecDestroy(c->graph);
+            if (c->stream) (void)hipStreamDestroy(c->stream);
+        }
     }
     delete c;
 }
@@ -183,7 +200,11 @@ int gl_ctx_trim(gl_ctx* ctx) {
     GL_HIP(hipStreamSynchronize(c->stream));
     for (auto& kv : c->bufs) kv.second.release();
     c->bufs.clear();
-    if (c->graph) { (void)hipGraphExecDestroy(c->graph); c->graph = nullptr; }
+    if (c->graph) {
+        std::lock_guard<std::mutex> gm(gl::device_mem_mutex());
+        (void)hipGraphExecDestroy(c->graph);
+        c->graph = nullptr;
+    }
     c->graph_key.clear();
     c->work_host.clear();
     c->work_key.clear();
@@ -241,8 +262,11 @@ int gl_lens_create(const gl_surface* s, int32_t n, const float* arena, int32_t a
 
 void gl_lens_destroy(gl_lens* L) {
     if (!L) return;
-    for (auto& v : L->variants) { (void)hipFree(v.d_surf); (void)hipFree(v.d_poly); }
-    if (L->d_arena) (void)hipFree(L->d_arena);
+    {
+        std::lock_guard<std::mutex> g(gl::device_mem_mutex());
+        for (auto& v : L->variants) { (void)hipFree(v.d_surf); (void)hipFree(v.d_poly); }
+        if (L->d_arena) (void)hipFree(L->d_arena);
+    }
     delete L;
 }
 

@@ -13,6 +13,8 @@ import math
 from dataclasses import dataclass, field
 from typing import List, Optional
 
+import ctypes as C
+
 import numpy as np
 
 from . import _abi
@@ -277,6 +279,43 @@ class OpticalSystem:
                 g.sa_offset, g.sa_n_wavelengths, g.sa_n_angles = len(arena), len(wl), len(an)
                 arena.extend(wl.tolist()); arena.extend(an.tolist()); arena.extend(r.ravel().tolist())
         return arr, np.asarray(arena if arena else [0.0], np.float32)
+
+    @classmethod
+    def from_packed(cls, surfaces, arena=None, name: str = "") -> "OpticalSystem":
+        """The inverse of pack(): a lens from the C ABI's gl_surface array
+        (a ctypes array or its raw bytes) and coating arena."""
+        if isinstance(surfaces, (bytes, bytearray, memoryview, np.ndarray)):
+            raw = bytes(np.asarray(surfaces, np.uint8)) if isinstance(surfaces, np.ndarray) else bytes(surfaces)
+            n = len(raw) // C.sizeof(_abi.gl_surface)
+            surfaces = (_abi.gl_surface * n).from_buffer_copy(raw[: n * C.sizeof(_abi.gl_surface)])
+        ar = np.asarray(arena if arena is not None else [0.0], np.float32)
+        out = []
+        for g in surfaces:
+            c = Coating(model=CoatingModel(g.coating_model), ar_layers=int(g.ar_layers),
+                        out_of_range_discard=bool(g.out_of_range_discard), gauss_sigma=g.gauss_sigma,
+                        gauss_background=g.gauss_background, gauss_peak=g.gauss_peak,
+                        gauss_decenter_x=g.gauss_decenter_x, gauss_decenter_y=g.gauss_decenter_y,
+                        angle_ref_ior=g.angle_ref_ior, tint_r=g.tint_r, tint_g=g.tint_g, tint_b=g.tint_b,
+                        tint_strength=g.tint_strength)
+            if g.table_count > 0:
+                c.table = ar[g.table_offset:g.table_offset + 2 * g.table_count].reshape(-1, 2).copy()
+            if g.sa_n_wavelengths > 0 and g.sa_n_angles > 0:
+                o, nw, na = g.sa_offset, g.sa_n_wavelengths, g.sa_n_angles
+                c.sa_wavelengths = ar[o:o + nw].copy()
+                c.sa_angles = ar[o + nw:o + nw + na].copy()
+                c.sa_r = ar[o + nw + na:o + nw + na + nw * na].reshape(nw, na).copy()
+            out.append(Surface(radius=g.radius, thickness=g.thickness, ior=g.ior, abbe_v=g.abbe_v,
+                               semi_aperture=g.semi_aperture, coating=c, is_stop=bool(g.is_stop),
+                               is_active=bool(g.is_active), z=g.z, aperture_shape=ApertureShape(g.aperture_shape),
+                               aperture_blades=int(g.aperture_blades), aperture_rotation_rad=g.aperture_rotation_rad,
+                               aperture_aspect=g.aperture_aspect, decenter_x=g.decenter_x, decenter_y=g.decenter_y,
+                               rot=tuple(g.rot[k] for k in range(9)), form=SurfaceForm(g.form), conic_k=g.conic_k,
+                               asphere_terms=tuple(g.asphere_terms[k] for k in range(g.n_asphere_terms)),
+                               cyl_axis=CylinderAxis(g.cyl_axis), disp_model=DispersionModel(g.disp_model),
+                               sellmeier_B=tuple(g.sellmeier_B[k] for k in range(3)),
+                               sellmeier_C=tuple(g.sellmeier_C[k] for k in range(3))))
+        lens = cls(out, name=name)
+        return lens
 
     def pack_key(self) -> bytes:
         arr, arena = self.pack()

@@ -228,6 +228,17 @@ def solve_source_angle(lens: OpticalSystem, calib: LensCalibration, ndc_x: float
     return np.float32(ax), np.float32(ay)
 
 
+def source_rgb(config: PointFlareConfig):
+    """The source colour in the output space: Mout . Min . (r, g, b) in float32
+    with the reference's summation order (SourceFlareRenderer.cpp:76-86)."""
+    Min, Mout = hostprep.color_matrices(config.input_cs, config.output_cs, config.custom_input_to_xyz,
+                                        config.custom_xyz_to_output)
+    f = np.float32
+    rgb = [f(config.source_r), f(config.source_g), f(config.source_b)]
+    xyz = [f(f(f(Min[i, 0] * rgb[0]) + f(Min[i, 1] * rgb[1])) + f(Min[i, 2] * rgb[2])) for i in range(3)]
+    return [f(f(f(Mout[i, 0] * xyz[0]) + f(Mout[i, 1] * xyz[1])) + f(Mout[i, 2] * xyz[2])) for i in range(3)]
+
+
 def render_source_flare(offsets, width: int, height: int, lens: OpticalSystem, calibration: LensCalibration,
                         config: PointFlareConfig, ctx=None) -> dict:
     """Extended-source flare (SourceFlareRenderer.cpp:24-208): offsets is an
@@ -247,17 +258,31 @@ def render_source_flare(offsets, width: int, height: int, lens: OpticalSystem, c
     hh = config.sensor_half_h if config.sensor_half_h > 0 else calibration.sensor_half_h
     ax0, ay0 = solve_source_angle(lens, calibration, (config.source_x - 0.5) * 2.0, (config.source_y - 0.5) * 2.0,
                                   hw, hh)
-    Min, Mout = hostprep.color_matrices(config.input_cs, config.output_cs, config.custom_input_to_xyz,
-                                        config.custom_xyz_to_output)
-    f = np.float32
-    rgb = [f(config.source_r), f(config.source_g), f(config.source_b)]
-    xyz = [f(f(f(Min[i, 0] * rgb[0]) + f(Min[i, 1] * rgb[1])) + f(Min[i, 2] * rgb[2])) for i in range(3)]
-    src_rgb = [f(f(f(Mout[i, 0] * xyz[0]) + f(Mout[i, 1] * xyz[1])) + f(Mout[i, 2] * xyz[2])) for i in range(3)]
+    src_rgb = source_rgb(config)
     sources = np.zeros((len(off), 5), np.float32)
     sources[:, 0] = ax0 + off[:, 0]
     sources[:, 1] = ay0 + off[:, 1]
     for c in range(3):
         sources[:, 2 + c] = src_rgb[c] * off[:, 2]
+    return render_sources(sources, width, height, lens, calibration, config, ctx=ctx)
+
+
+def render_sources(sources, width: int, height: int, lens: OpticalSystem, calibration: LensCalibration,
+                   config: PointFlareConfig, ctx=None) -> dict:
+    """The flare of (N, 5) [angle_x, angle_y, r, g, b] sources whose field
+    angles are already solved and whose colour is already in the output space
+    (the body of SourceFlareRenderer.cpp:24-208 after the source map and the
+    colour conversion).  Same output dict as render_source_flare."""
+    _dims("render_sources", width, height)
+    sources = np.ascontiguousarray(sources, np.float32)
+    if sources.ndim != 2 or sources.shape[1] != 5 or sources.shape[0] < 1:
+        raise ValueError("render_sources: sources must be a (N, 5) float32 array with N >= 1")
+    if not np.all(np.isfinite(sources)):
+        raise RuntimeError("render_sources: HIP error: source rows must be finite")
+    if config.gate.enabled:
+        raise NotImplementedError("render_sources: the film-gate pass is not part of this build")
+    hw = config.sensor_half_w if config.sensor_half_w > 0 else calibration.sensor_half_w
+    hh = config.sensor_half_h if config.sensor_half_h > 0 else calibration.sensor_half_h
     out = {"width": int(width), "height": int(height)}
     p, _keep = to_params(config)
     sarr, n_src = _sources_array(sources)

@@ -11,6 +11,7 @@
 #include "aperture_sampler.h"
 #include "tmm.h"
 #include "lens_calibration.h"
+#include "source_map.h"
 
 #include "../include/ghostlight_amd.h"
 
@@ -65,6 +66,64 @@ OpticalSystem make_lens(const gl_surface* s, int n, const float* arena) {
     }
     L.sync_coating_pointers();
     return L;
+}
+
+// The reverse of make_lens: a reference OpticalSystem as the C ABI's surface
+// array and coating arena.
+int export_lens(const OpticalSystem& L, gl_surface* out, int cap, float* arena, int arena_cap, int* arena_len) {
+    const int n = (int)L.surfaces.size();
+    if (n > cap) return -1;
+    std::vector<float> ar;
+    for (int i = 0; i < n; ++i) {
+        const Surface& d = L.surfaces[i];
+        gl_surface& g = out[i];
+        std::memset(&g, 0, sizeof g);
+        g.radius = d.radius; g.thickness = d.thickness; g.ior = d.ior; g.abbe_v = d.abbe_v;
+        g.semi_aperture = d.semi_aperture; g.z = d.z;
+        g.is_stop = d.is_stop ? 1 : 0; g.is_active = d.is_active ? 1 : 0;
+        g.aperture_shape = d.aperture_shape; g.aperture_blades = d.aperture_blades;
+        g.aperture_rotation_rad = d.aperture_rotation_rad; g.aperture_aspect = d.aperture_aspect;
+        g.decenter_x = d.decenter_x; g.decenter_y = d.decenter_y;
+        std::memcpy(g.rot, d.rot, sizeof g.rot);
+        g.form = d.form; g.conic_k = d.conic_k;
+        std::memcpy(g.asphere_terms, d.asphere_terms, sizeof g.asphere_terms);
+        g.n_asphere_terms = d.n_asphere_terms; g.cyl_axis = d.cyl_axis; g.disp_model = d.disp_model;
+        std::memcpy(g.sellmeier_B, d.sellmeier_B, sizeof g.sellmeier_B);
+        std::memcpy(g.sellmeier_C, d.sellmeier_C, sizeof g.sellmeier_C);
+        const Coating& c = d.coating;
+        g.coating_model = (int)c.model; g.ar_layers = c.ar_layers;
+        g.out_of_range_discard = c.out_of_range_discard ? 1 : 0; g.angle_ref_ior = c.angle_ref_ior;
+        g.tint_r = c.tint_r; g.tint_g = c.tint_g; g.tint_b = c.tint_b; g.tint_strength = c.tint_strength;
+        g.gauss_sigma = c.gauss_sigma; g.gauss_background = c.gauss_background; g.gauss_peak = c.gauss_peak;
+        g.gauss_decenter_x = c.gauss_decenter_x; g.gauss_decenter_y = c.gauss_decenter_y;
+        if (i < (int)L.coating_tables.size()) {
+            const CoatingTables& t = L.coating_tables[i];
+            if (!t.table.empty()) {
+                g.table_offset = (int)ar.size(); g.table_count = (int)t.table.size();
+                for (const auto& e : t.table) { ar.push_back(e.key); ar.push_back(e.r); }
+            }
+            if (!t.sa_wavelengths.empty() && !t.sa_angles.empty()) {
+                g.sa_offset = (int)ar.size();
+                g.sa_n_wavelengths = (int)t.sa_wavelengths.size(); g.sa_n_angles = (int)t.sa_angles.size();
+                ar.insert(ar.end(), t.sa_wavelengths.begin(), t.sa_wavelengths.end());
+                ar.insert(ar.end(), t.sa_angles.begin(), t.sa_angles.end());
+                ar.insert(ar.end(), t.sa_r.begin(), t.sa_r.end());
+            }
+        }
+    }
+    if ((int)ar.size() > arena_cap) return -2;
+    std::copy(ar.begin(), ar.end(), arena);
+    *arena_len = (int)ar.size();
+    return n;
+}
+
+LensCalibration calib_of(const float* v) {
+    LensCalibration c{};
+    c.sensor_half_w = v[0]; c.sensor_half_h = v[1]; c.max_half_angle_h = v[2]; c.max_half_angle_v = v[3];
+    c.image_circle_semi_w = v[4]; c.image_circle_semi_h = v[5]; c.focal_length_x = v[6]; c.focal_length_y = v[7];
+    c.entrance_pupil_semi_x = v[8]; c.entrance_pupil_semi_y = v[9]; c.f_number_x = v[10]; c.f_number_y = v[11];
+    c.pupil_area_frac = v[12];
+    return c;
 }
 
 void put(const TraceResult& r, float* pos, float* w, int* st) {
@@ -178,6 +237,23 @@ int ref_calibrate_lens(const gl_surface* s, int n, const float* arena, float* ou
                          c.entrance_pupil_semi_x, c.entrance_pupil_semi_y, c.f_number_x, c.f_number_y,
                          c.pupil_area_frac};
     std::memcpy(out13, v, sizeof v);
+    return 0;
+}
+
+// OpticalSystem::load (optical_system.cpp:1299) of a .lens file, exported.
+// Returns the surface count, or a negative value on failure.
+int ref_load_lens(const char* path, gl_surface* out, int cap, float* arena, int arena_cap, int* arena_len) {
+    OpticalSystem L;
+    if (!L.load(path)) return -3;
+    return export_lens(L, out, cap, arena, arena_cap, arena_len);
+}
+
+// solve_source_angle (source_map.cpp:182) for a screen NDC position.
+int ref_solve_source_angle(const gl_surface* s, int n, const float* arena, const float* calib13, float ndc_x,
+                           float ndc_y, float eff_half_w, float eff_half_h, float* out2) {
+    OpticalSystem L = make_lens(s, n, arena);
+    const SourceMapSolve r = solve_source_angle(L, calib_of(calib13), ndc_x, ndc_y, eff_half_w, eff_half_h);
+    out2[0] = r.angle_x; out2[1] = r.angle_y;
     return 0;
 }
 

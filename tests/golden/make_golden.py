@@ -122,5 +122,67 @@ def main():
     print(f"wrote {path} ({os.path.getsize(path) / 1024:.0f} KiB, {len(out)} arrays)")
 
 
+REF_ROOT = "/root/reference"
+PF_CASES = [("double_gauss", "lenses/DoubleGauss.lens"),
+            ("sirui", "lenses/anamorphic/Sirui_75mm_F1.8_1.33x_EP3936919NWA1.lens")]
+# (source_x, source_y, blades override) of validation/point_flare_sampling_baseline.py:25
+PF_CONFIGS = [(0.5, 0.5, 0), (1.3, 0.5, 0), (1.6, 0.9, 0), (1.3, 0.5, 6)]
+PF_SH = 14.0
+PF_DS = 8
+
+
+def box(img, k):
+    H, W, Cc = img.shape
+    return img.reshape(H // k, k, W // k, k, Cc).sum(axis=(1, 3))
+
+
+def point_flare_goldens():
+    """The reference's point-flare goldens (validation/goldens/point_flare_sampling,
+    320x320x3 float32, produced by the reference's GPU renderer) in reduced
+    form -- 8x8 box sums, per-channel totals, peak -- together with what it takes
+    to re-render them: each lens as the C ABI's surface array (OpticalSystem::load,
+    optical_system.cpp:1299), its calibrate_lens record and the solve_source_angle
+    angles (source_map.cpp:182) of every configuration, from the reference's CPU
+    code."""
+    ref = pyoracle.ref()
+    out = {}
+    gold = os.path.join(REF_ROOT, "validation", "goldens", "point_flare_sampling")
+    for name, rel in PF_CASES:
+        arr = (_abi.gl_surface * 64)()
+        arena = np.zeros(1 << 20, np.float32)
+        al = C.c_int()
+        n = ref.ref_load_lens(os.path.join(REF_ROOT, rel).encode(), arr, 64,
+                              arena.ctypes.data_as(C.POINTER(C.c_float)), len(arena), C.byref(al))
+        if n <= 0:
+            raise SystemExit(f"could not load {rel}")
+        raw = bytes(arr)[: n * C.sizeof(_abi.gl_surface)]
+        out[f"pf_{name}_surfaces"] = np.frombuffer(raw, np.uint8).copy()
+        out[f"pf_{name}_arena"] = arena[: max(1, al.value)].copy()
+        cal = np.zeros(13, np.float32)
+        ref.ref_calibrate_lens(arr, n, arena.ctypes.data_as(C.POINTER(C.c_float)),
+                               cal.ctypes.data_as(C.POINTER(C.c_float)))
+        out[f"pf_{name}_calib"] = cal
+        for sx, sy, blades in PF_CONFIGS:
+            ang = np.zeros(2, np.float32)
+            ref.ref_solve_source_angle(arr, n, arena.ctypes.data_as(C.POINTER(C.c_float)),
+                                       cal.ctypes.data_as(C.POINTER(C.c_float)), C.c_float((sx - 0.5) * 2.0),
+                                       C.c_float((sy - 0.5) * 2.0), C.c_float(PF_SH), C.c_float(PF_SH),
+                                       ang.ctypes.data_as(C.POINTER(C.c_float)))
+            for cull in (0, 1):
+                key = f"{name}_{sx}_{sy}_b{blades}_c{cull}"
+                img = np.load(os.path.join(gold, key + ".npy"))          # allow_pickle=False (default)
+                out[f"pf_{key}_angles"] = ang
+                out[f"pf_{key}_ds"] = box(img.astype(np.float64), PF_DS).astype(np.float32)
+                out[f"pf_{key}_sum"] = img.astype(np.float64).sum(axis=(0, 1))
+                out[f"pf_{key}_peak"] = np.array([float(img.sum(-1).max())])
+    path = os.path.join(HERE, "point_flare_goldens.npz")
+    np.savez_compressed(path, **out)
+    print(f"wrote {path} ({os.path.getsize(path)} bytes, {len(out)} arrays)")
+
+
 if __name__ == "__main__":
-    main()
+    if len(sys.argv) > 1 and sys.argv[1] == "point_flare":
+        point_flare_goldens()
+    else:
+        main()
+        point_flare_goldens()
