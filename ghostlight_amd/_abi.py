@@ -130,6 +130,13 @@ class gl_calibration(C.Structure):
         "f_number_x", "f_number_y", "pupil_area_frac")]
 
 
+class gl_source_map(C.Structure):
+    _fields_ = [("angle_x", C.c_float), ("angle_y", C.c_float), ("status", C.c_int32), ("residual_mm", C.c_float),
+                ("anchor_ax", C.c_float), ("anchor_ay", C.c_float), ("anchor_x", C.c_float), ("anchor_y", C.c_float),
+                ("J", C.c_float * 4), ("eff_half_w", C.c_float), ("eff_half_h", C.c_float),
+                ("inv_tan_w", C.c_float), ("inv_tan_h", C.c_float), ("probe_evals", C.c_int32)]
+
+
 class gl_diffraction_params(C.Structure):
     _fields_ = [
         ("starburst_engine", C.c_int32), ("starburst_grid", C.c_int32), ("starburst_grid_cap", C.c_int32),
@@ -209,6 +216,12 @@ SIGNATURES = {
     "gl_render_veil": (C.c_int, [VP, C.POINTER(gl_source), F32P, F32P, C.c_int32, C.c_int32, C.c_int32, C.c_int32,
                                  C.c_int32, C.c_float, C.c_float, C.POINTER(gl_diffraction_params), F32P, F32P, F32P,
                                  C.POINTER(gl_veil_debug)]),
+    "gl_calibrate_lens": (C.c_int, [VP, VP, C.c_float, C.c_float, C.POINTER(gl_calibration)]),
+    "gl_solve_source_angle": (C.c_int, [VP, VP, C.POINTER(gl_calibration), C.c_float, C.c_float, C.c_float,
+                                        C.c_float, C.POINTER(gl_source_map)]),
+    "gl_source_map_screen": (None, [C.POINTER(gl_source_map), C.c_float, C.c_float, F32P, F32P]),
+    "gl_source_map_landing": (C.c_int, [VP, VP, C.POINTER(gl_calibration), C.c_float, C.c_float, F32P, F32P,
+                                        I32P]),
     "gl_render_psf_tiles": (C.c_int, [VP, VP, C.POINTER(gl_psf_source), C.c_int32, C.POINTER(gl_flare_params),
                                       C.POINTER(gl_psf_params), F32P, F32P, F32P, F32P]),
     "gl_veil_convolve": (C.c_int, [VP, VP, VP, VP, C.c_int32, C.c_int32, C.c_float, C.c_float,

@@ -58,7 +58,8 @@ def _check_unsupported(cfg: FlareConfig):
 
 def source_pixels(lens: OpticalSystem, calib: LensCalibration, sources, width, height, half_w, half_h):
     """Buffer-pixel positions of field-angle sources through the first-order
-    map (the inverse of render.solve_source_angle)."""
+    map y' = f tan(theta) with the calibrated focal lengths (used where no
+    source-map solve is at hand)."""
     fx = calib.focal_length_x or lens.focal_length
     fy = calib.focal_length_y or lens.focal_length
     s = np.asarray(sources, np.float32).reshape(-1, 5)
@@ -158,28 +159,32 @@ def fft2d(a: np.ndarray, ctx=None) -> np.ndarray:
     return out
 
 
-def render_starburst_layer(lens, calib, sources, width, height, hw, hh, config, ctx=None):
-    px, py = source_pixels(lens, calib, sources, width, height, hw, hh)
+def render_starburst_layer(lens, calib, sources, width, height, hw, hh, config, px=None, py=None, ctx=None):
+    if px is None or py is None:
+        px, py = source_pixels(lens, calib, sources, width, height, hw, hh)
     r, g, b = render_starburst(lens, calib, sources, px, py, width, height, hw, hh, config, ctx=ctx)
     return {"starburst_r": r, "starburst_g": g, "starburst_b": b}
 
 
-def render_veil_layer(lens, calib, sources, width, height, hw, hh, config, ctx=None):
-    px, py = source_pixels(lens, calib, sources, width, height, hw, hh)
+def render_veil_layer(lens, calib, sources, width, height, hw, hh, config, px=None, py=None, ctx=None):
+    if px is None or py is None:
+        px, py = source_pixels(lens, calib, sources, width, height, hw, hh)
     r, g, b = render_veil(sources, px, py, width, height, hw, hh, config, ctx=ctx)
     return {"veil_r": r, "veil_g": g, "veil_b": b}
 
 
 def _debug_source(lens, calib, cfg: PointFlareConfig, w, h):
-    from . import hostprep
-    from .render import solve_source_angle
+    from .render import source_rgb
+    from .source_map import solve_source_map
     hw = cfg.sensor_half_w if cfg.sensor_half_w > 0 else calib.sensor_half_w
     hh = cfg.sensor_half_h if cfg.sensor_half_h > 0 else calib.sensor_half_h
-    ax, ay = solve_source_angle(lens, calib, (cfg.source_x - 0.5) * 2.0, (cfg.source_y - 0.5) * 2.0, hw, hh)
-    Mi, Mo = hostprep.color_matrices(cfg.input_cs, cfg.output_cs, cfg.custom_input_to_xyz, cfg.custom_xyz_to_output)
-    rgb = Mo @ (Mi @ np.array([cfg.source_r, cfg.source_g, cfg.source_b], np.float32))
+    sm = solve_source_map(lens, calib, (cfg.source_x - 0.5) * 2.0, (cfg.source_y - 0.5) * 2.0, hw, hh)
+    ax, ay = np.float32(sm.angle_x), np.float32(sm.angle_y)
+    rgb = source_rgb(cfg)
     src = np.array([[ax, ay, rgb[0], rgb[1], rgb[2]]], np.float32)
-    px, py = source_pixels(lens, calib, src, w, h, hw, hh)
+    sx, sy = sm.screen(ax, ay)
+    px = np.array([np.float32(sx) * np.float32(w)], np.float32)
+    py = np.array([np.float32(sy) * np.float32(h)], np.float32)
     return src, px, py, hw, hh
 
 

@@ -334,15 +334,21 @@ class OpticalSystem:
         return self._dev
 
     # ---- renderer wrappers (optical_system.py:334-429) -------------------
-    def calibration(self) -> "LensCalibration":
-        """The lens's calibration record: the one attached to it (lens files
-        and the synthetic generators attach theirs), else a first-order
-        record from the paraxial focal length and entrance pupil."""
+    def calibration(self, d_line_nm: float = 587.56) -> "LensCalibration":
+        """The lens's calibration record (optical_system.py:334-346): the one
+        attached with set_calibration, else calibrate_lens traced on the
+        device, cached until the prescription changes."""
         cal = getattr(self, "_calibration", None)
         if cal is not None:
             return cal
-        from .synthetic import calibration_from_paraxial
-        return calibration_from_paraxial(self)
+        key = (self.pack_key(), float(d_line_nm))
+        cached = getattr(self, "_calib_cache", None)
+        if cached is not None and cached[0] == key:
+            return cached[1]
+        from .source_map import calibrate_lens
+        cal = calibrate_lens(self, d_line_nm)
+        self._calib_cache = (key, cal)
+        return cal
 
     def set_calibration(self, calib: "LensCalibration") -> None:
         self._calibration = calib

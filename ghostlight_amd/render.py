@@ -215,17 +215,13 @@ def trace_primary_ray(ray: Ray, lens: OpticalSystem, ctx=None) -> TraceResult:
 
 
 def solve_source_angle(lens: OpticalSystem, calib: LensCalibration, ndc_x: float, ndc_y: float,
-                       half_w: float, half_h: float):
-    """Screen NDC -> field angle.  First-order map tan(theta) = y' / f through
-    the calibrated focal lengths (the reference refines this with a traced
-    Newton solve, source_map.cpp:182; that solve is not part of this build)."""
-    fx = calib.focal_length_x or lens.focal_length
-    fy = calib.focal_length_y or lens.focal_length
-    if not (fx > 0 and fy > 0):
-        raise ValueError("calibration has no focal length; set LensCalibration.focal_length_x/y")
-    ax = math.atan(ndc_x * half_w / fx)
-    ay = math.atan(ndc_y * half_h / fy)
-    return np.float32(ax), np.float32(ay)
+                       half_w: float, half_h: float, ctx=None):
+    """Screen NDC -> field angle by the traced source-map solve
+    (source_map.cpp:182): the angle whose spawn-disc survivor mean lands at
+    (ndc_x * half_w, ndc_y * half_h) mm."""
+    from .source_map import solve_source_map
+    sm = solve_source_map(lens, calib, ndc_x, ndc_y, half_w, half_h, ctx=ctx)
+    return np.float32(sm.angle_x), np.float32(sm.angle_y)
 
 
 def source_rgb(config: PointFlareConfig):
@@ -256,23 +252,35 @@ def render_source_flare(offsets, width: int, height: int, lens: OpticalSystem, c
         raise RuntimeError("render_source_flare: HIP error: lens has no surfaces")
     hw = config.sensor_half_w if config.sensor_half_w > 0 else calibration.sensor_half_w
     hh = config.sensor_half_h if config.sensor_half_h > 0 else calibration.sensor_half_h
-    ax0, ay0 = solve_source_angle(lens, calibration, (config.source_x - 0.5) * 2.0, (config.source_y - 0.5) * 2.0,
-                                  hw, hh)
+    from .source_map import solve_source_map
+    smap = solve_source_map(lens, calibration, (config.source_x - 0.5) * 2.0, (config.source_y - 0.5) * 2.0, hw, hh,
+                            ctx=ctx)
+    ax0, ay0 = np.float32(smap.angle_x), np.float32(smap.angle_y)
     src_rgb = source_rgb(config)
     sources = np.zeros((len(off), 5), np.float32)
     sources[:, 0] = ax0 + off[:, 0]
     sources[:, 1] = ay0 + off[:, 1]
     for c in range(3):
         sources[:, 2 + c] = src_rgb[c] * off[:, 2]
-    return render_sources(sources, width, height, lens, calibration, config, ctx=ctx)
+    # Each source's buffer position from the solve's forward map
+    # (SourceFlareRenderer.cpp:95-120), for the starburst and veil splats.
+    src_px = np.zeros(len(off), np.float32)
+    src_py = np.zeros(len(off), np.float32)
+    for i in range(len(off)):
+        sx, sy = smap.screen(sources[i, 0], sources[i, 1])
+        src_px[i] = np.float32(sx) * np.float32(width)
+        src_py[i] = np.float32(sy) * np.float32(height)
+    return render_sources(sources, width, height, lens, calibration, config, ctx=ctx, src_px=src_px, src_py=src_py)
 
 
 def render_sources(sources, width: int, height: int, lens: OpticalSystem, calibration: LensCalibration,
-                   config: PointFlareConfig, ctx=None) -> dict:
+                   config: PointFlareConfig, ctx=None, src_px=None, src_py=None) -> dict:
     """The flare of (N, 5) [angle_x, angle_y, r, g, b] sources whose field
     angles are already solved and whose colour is already in the output space
     (the body of SourceFlareRenderer.cpp:24-208 after the source map and the
-    colour conversion).  Same output dict as render_source_flare."""
+    colour conversion).  src_px/src_py: each source's buffer-pixel position
+    for the starburst and veil splats (default: the first-order map through
+    the calibrated focal lengths).  Same output dict as render_source_flare."""
     _dims("render_sources", width, height)
     sources = np.ascontiguousarray(sources, np.float32)
     if sources.ndim != 2 or sources.shape[1] != 5 or sources.shape[0] < 1:
@@ -308,10 +316,12 @@ def render_sources(sources, width: int, height: int, lens: OpticalSystem, calibr
     if config.diffraction.starburst:
         from . import diffraction
         out.update(diffraction.render_starburst_layer(lens, calibration, sources, width, height, hw, hh, config,
+                                                      px=src_px, py=src_py,
                                                       ctx=ctx))
     if config.diffraction.veil:
         from . import diffraction
-        out.update(diffraction.render_veil_layer(lens, calibration, sources, width, height, hw, hh, config, ctx=ctx))
+        out.update(diffraction.render_veil_layer(lens, calibration, sources, width, height, hw, hh, config,
+                                                 px=src_px, py=src_py, ctx=ctx))
     if aov_buf is not None:
         for i in range(aov_n.value):
             a, b = int(aov_pairs[2 * i]), int(aov_pairs[2 * i + 1])

@@ -425,6 +425,54 @@ int  gl_render_veil(gl_ctx* ctx, const gl_source* sources, const float* px, cons
                     const gl_diffraction_params* p, float* out_r, float* out_g,
                     float* out_b, gl_veil_debug* dbg);
 
+/* ---- calibration and source map ------------------------------------------- */
+
+/* calibrate_lens (lens_calibration.cpp:344-443; Python binding
+ * bind_calibration.cpp:40): paraxial focal lengths at a 1-degree field,
+ * entrance-pupil support (marginal-ray bisection per azimuth), covered field
+ * at 90% of the axial spawn-disc throughput and image circle at 5%, each with
+ * its survivor-mean landing.  Every probe ray is traced on the device.
+ * file_focal_length: the lens file's focal length, used only when the
+ * covered field resolves to nothing (0 = none). */
+int  gl_calibrate_lens(gl_ctx* ctx, const gl_lens* lens, float d_line_nm,
+                       float file_focal_length, gl_calibration* out);
+
+enum { GL_SMAP_SOLVED = 0, GL_SMAP_CONTINUED = 1, GL_SMAP_FALLBACK = 2 };
+
+/* SourceMapSolve (source_map.h:20-52), same field order. */
+typedef struct gl_source_map {
+    float   angle_x, angle_y;           /* radians: the answer                        */
+    int32_t status;                     /* GL_SMAP_*                                  */
+    float   residual_mm;
+    float   anchor_ax, anchor_ay;       /* angle whose landing was measured           */
+    float   anchor_x, anchor_y;         /* its landing (mm)                           */
+    float   J[4];                       /* d(landing)/d(angle), row-major             */
+    float   eff_half_w, eff_half_h;
+    float   inv_tan_w, inv_tan_h;       /* FALLBACK only                              */
+    int32_t probe_evals;
+} gl_source_map;
+
+/* solve_source_angle (source_map.cpp:182-452): screen NDC (-1..1, 0 = axis)
+ * of a frame with half-extents eff_half_w/h mm to the field angle whose
+ * spawn-disc survivor mean lands there (Newton on device-traced probes),
+ * with the arctangent continuation past the image circle and the
+ * constant-focal-length fallback.  One-entry memo as the reference. */
+int  gl_solve_source_angle(gl_ctx* ctx, const gl_lens* lens, const gl_calibration* calib,
+                           float ndc_x, float ndc_y, float eff_half_w, float eff_half_h,
+                           gl_source_map* out);
+
+/* source_map_screen (source_map.cpp:456-484): the forward map of a solve at
+ * any angle, as [0, 1] frame fractions.  Pure arithmetic, no device work. */
+void gl_source_map_screen(const gl_source_map* base, float angle_x, float angle_y,
+                          float* out_sx, float* out_sy);
+
+/* source_map_landing (source_map.cpp:487-519): survivor-mean landing of the
+ * spawn disc at one angle on the calibrated probe grid; *out_ok = 0 when no
+ * ray survives. */
+int  gl_source_map_landing(gl_ctx* ctx, const gl_lens* lens, const gl_calibration* calib,
+                           float angle_x, float angle_y, float* out_x_mm, float* out_y_mm,
+                           int32_t* out_ok);
+
 /* ---- geometric PSF grid --------------------------------------------------- */
 
 /* PSFGpuSource (psf_render.h:17-24): a field-point source paired with its

@@ -257,4 +257,40 @@ int ref_solve_source_angle(const gl_surface* s, int n, const float* arena, const
     return 0;
 }
 
+// solve_source_angle (source_map.cpp:182), the whole SourceMapSolve in
+// gl_source_map field order: angle_x, angle_y, status, residual_mm,
+// anchor_ax, anchor_ay, anchor_x, anchor_y, J[4], eff_half_w, eff_half_h,
+// inv_tan_w, inv_tan_h, probe_evals (status and probe_evals as floats).
+int ref_solve_source_map(const gl_surface* s, int n, const float* arena, const float* calib13, float ndc_x,
+                         float ndc_y, float eff_half_w, float eff_half_h, float* out17) {
+    OpticalSystem L = make_lens(s, n, arena);
+    const SourceMapSolve r = solve_source_angle(L, calib_of(calib13), ndc_x, ndc_y, eff_half_w, eff_half_h);
+    const float v[17] = {r.angle_x, r.angle_y, (float)(int)r.status, r.residual_mm, r.anchor_ax, r.anchor_ay,
+                         r.anchor_x, r.anchor_y, r.J[0], r.J[1], r.J[2], r.J[3], r.eff_half_w, r.eff_half_h,
+                         r.inv_tan_w, r.inv_tan_h, (float)r.probe_evals};
+    std::memcpy(out17, v, sizeof v);
+    return 0;
+}
+
+// source_map_screen (source_map.cpp:456) of a solve given as ref_solve_source_map's 17 floats.
+int ref_source_map_screen(const float* solve17, float angle_x, float angle_y, float* out2) {
+    SourceMapSolve b;
+    b.angle_x = solve17[0]; b.angle_y = solve17[1];
+    b.status = (SourceMapStatus)(int)solve17[2];
+    b.residual_mm = solve17[3];
+    b.anchor_ax = solve17[4]; b.anchor_ay = solve17[5]; b.anchor_x = solve17[6]; b.anchor_y = solve17[7];
+    for (int i = 0; i < 4; ++i) b.J[i] = solve17[8 + i];
+    b.eff_half_w = solve17[12]; b.eff_half_h = solve17[13];
+    b.inv_tan_w = solve17[14]; b.inv_tan_h = solve17[15];
+    source_map_screen(b, angle_x, angle_y, &out2[0], &out2[1]);
+    return 0;
+}
+
+// source_map_landing (source_map.cpp:487): returns 1 with the landing, 0 when nothing survives.
+int ref_source_map_landing(const gl_surface* s, int n, const float* arena, const float* calib13, float angle_x,
+                           float angle_y, float* out2) {
+    OpticalSystem L = make_lens(s, n, arena);
+    return source_map_landing(L, calib_of(calib13), angle_x, angle_y, &out2[0], &out2[1]) ? 1 : 0;
+}
+
 }  // extern "C"

@@ -180,9 +180,104 @@ def point_flare_goldens():
     print(f"wrote {path} ({os.path.getsize(path)} bytes, {len(out)} arrays)")
 
 
+SM_LENSES = [("double_gauss", "lenses/DoubleGauss.lens"), ("helios44", "lenses/Helios44.lens"),
+             ("atlas_40_f3", "lenses/anamorphic/AtlasLensCo_40mm_F3.0_1.5x_US12379576B2_Table1.lens"),
+             ("atlas_40_f35", "lenses/anamorphic/AtlasLensCo_40mm_F3.5_1.5x_US20240053589A1_Table2_InfFocus.lens"),
+             ("cooke_50_2x", "lenses/anamorphic/Cooke_50mm_F3_2x_US9341827B2.lens"),
+             ("sirui_24", "lenses/anamorphic/Sirui_24mm_F1.8_1.33x_US10969568B1.lens"),
+             ("sirui", "lenses/anamorphic/Sirui_75mm_F1.8_1.33x_EP3936919NWA1.lens"),
+             ("sirui_venus", "lenses/anamorphic/Sirui_Venus_50mm_T2.9_1.6X_US20220196995A1.lens"),
+             ("exp_helios_2x", "lenses/experiments/Experimental_Helios44SphericalBlock_60mm_2x_"
+                               "DoubletsCounterRotatedDeAstigmatiser.lens"),
+             ("exp_panatar", "lenses/experiments/Experimental_PanavisionPanatarInspired_50mm_2x_"
+                             "CounterRotatingDeAstigmatiser.lens"),
+             ("artisans_50", "lenses/spherical/7Artisans_50mm_F0.95.lens"),
+             ("canon_100", "lenses/spherical/Canon_100mm.lens"),
+             ("canon_24", "lenses/spherical/Canon_24mm_F1.4.lens"),
+             ("cooke_pancro", "lenses/spherical/Cooke_SpeedPancro_50mm_F2_US2959102A.lens"),
+             ("summicron_100", "lenses/spherical/Leitz_Summicron_100mm.lens"),
+             ("hex_stop", "validation/goldens/aperture_baseline/hex_stop.lens"),
+             ("hex_rot30", "validation/goldens/aperture_baseline/hex_stop_rot30.lens")]
+# Screen NDC requests; None in the last two slots = the calibrated half extents.
+SM_REQUESTS = [(0.0, 0.0, None, None), (0.5, 0.0, None, None), (0.0, -0.5, None, None), (0.7, -0.6, None, None),
+               (-0.95, 0.9, None, None), (1.6, 0.2, None, None), (-2.5, -1.5, None, None), (0.3, 0.3, 14.0, 14.0)]
+
+
+def load_ref_lens(ref, rel):
+    arr = (_abi.gl_surface * 64)()
+    arena = np.zeros(1 << 20, np.float32)
+    al = C.c_int()
+    n = ref.ref_load_lens(os.path.join(REF_ROOT, rel).encode(), arr, 64,
+                          arena.ctypes.data_as(C.POINTER(C.c_float)), len(arena), C.byref(al))
+    if n <= 0:
+        raise SystemExit(f"could not load {rel}")
+    return arr, n, arena[: max(1, al.value)].copy()
+
+
+def source_map_goldens():
+    """calibrate_lens (lens_calibration.cpp:344), solve_source_angle
+    (source_map.cpp:182, the whole SourceMapSolve), source_map_landing (:487)
+    and source_map_screen (:456) from the reference's CPU code, for the
+    reference's own lens files; plus the calibration manifest the reference
+    keeps beside its aperture goldens."""
+    import json
+    ref = pyoracle.ref()
+    fp = lambda a: a.ctypes.data_as(C.POINTER(C.c_float))  # noqa: E731
+    out = {}
+    for name, rel in SM_LENSES:
+        arr, n, arena = load_ref_lens(ref, rel)
+        out[f"sm_{name}_surfaces"] = np.frombuffer(bytes(arr)[: n * C.sizeof(_abi.gl_surface)], np.uint8).copy()
+        out[f"sm_{name}_arena"] = arena
+        cal = np.zeros(13, np.float32)
+        ref.ref_calibrate_lens(arr, n, fp(arena), fp(cal))
+        out[f"sm_{name}_calib"] = cal
+        reqs, solves = [], []
+        for nx, ny, hw, hh in SM_REQUESTS:
+            hw = float(cal[0]) if hw is None else hw
+            hh = float(cal[1]) if hh is None else hh
+            r = np.zeros(17, np.float32)
+            ref.ref_solve_source_map(arr, n, fp(arena), fp(cal), C.c_float(nx), C.c_float(ny), C.c_float(hw),
+                                     C.c_float(hh), fp(r))
+            reqs.append((nx, ny, hw, hh))
+            solves.append(r)
+        out[f"sm_{name}_requests"] = np.array(reqs, np.float32)
+        out[f"sm_{name}_solves"] = np.array(solves, np.float32)
+        # The forward map of every solve at its own angle and at two offsets.
+        offs = np.array([[0.0, 0.0], [0.01, -0.02], [0.3, 0.1]], np.float32)
+        scr = np.zeros((len(solves), len(offs), 2), np.float32)
+        for i, r in enumerate(solves):
+            for j, (dx, dy) in enumerate(offs):
+                ref.ref_source_map_screen(fp(r), C.c_float(np.float32(r[0] + dx)), C.c_float(np.float32(r[1] + dy)),
+                                          fp(scr[i, j]))
+        out[f"sm_{name}_screen_offsets"] = offs
+        out[f"sm_{name}_screen"] = scr
+        mh, mv = float(cal[2]), float(cal[3])
+        angs = np.array([[0.0, 0.0], [0.5 * mh, 0.0], [0.0, -0.8 * mv], [0.6 * mh, 0.5 * mv], [1.3 * mh, 0.4 * mv]],
+                        np.float32)
+        lands = np.zeros((len(angs), 3), np.float32)
+        for i, (ax, ay) in enumerate(angs):
+            xy = np.zeros(2, np.float32)
+            ok = ref.ref_source_map_landing(arr, n, fp(arena), fp(cal), C.c_float(ax), C.c_float(ay), fp(xy))
+            lands[i] = (ok, xy[0], xy[1])
+        out[f"sm_{name}_landing_angles"] = angs
+        out[f"sm_{name}_landings"] = lands
+        print(f"  {name}: {n} surfaces, calib sensor_half ({cal[0]:.4f}, {cal[1]:.4f}), statuses "
+              f"{[int(r[2]) for r in solves]}", flush=True)
+    man = json.load(open(os.path.join(REF_ROOT, "validation", "goldens", "aperture_baseline", "manifest.json")))
+    fields = [f for f in _abi.gl_calibration._fields_]
+    for key in ("double_gauss", "sirui", "hex_stop", "hex_rot30"):
+        out[f"manifest_{key}"] = np.array([man[key][f[0]] for f in fields], np.float32)
+    path = os.path.join(HERE, "source_map_goldens.npz")
+    np.savez_compressed(path, **out)
+    print(f"wrote {path} ({os.path.getsize(path)} bytes, {len(out)} arrays)")
+
+
 if __name__ == "__main__":
     if len(sys.argv) > 1 and sys.argv[1] == "point_flare":
         point_flare_goldens()
+    elif len(sys.argv) > 1 and sys.argv[1] == "source_map":
+        source_map_goldens()
     else:
         main()
         point_flare_goldens()
+        source_map_goldens()
