@@ -16,6 +16,7 @@ from .render import (GhostPair, Ray, TraceResult, enumerate_ghost_pairs, filter_
 from .hostprep import build_spectral_lambdas
 from .diffraction import _render_starburst_debug, _render_veil_debug, fft2d
 from .psf import render_psf
+from .lens_file import LensFileError, load_lens
 from .source_map import (SourceMapSolve, _solve_source_map, _source_map_landing, _source_map_screen, calibrate_lens,
                          solve_source_map)
 

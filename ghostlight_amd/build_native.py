@@ -2,7 +2,8 @@
 
 Every HIP translation unit is compiled with -ffp-contract=off and IEEE fp32
 division / square root (the tracer's bit-exact contract with its CPU oracle)
-and with packed-FP32 instruction selection disabled.  Usage:
+and with packed-FP32 instruction selection disabled.  Host-only .cpp units
+are compiled by the system C++ compiler.  Usage:
     python -m ghostlight_amd.build_native [--jobs N]
 """
 from __future__ import annotations
@@ -28,8 +29,13 @@ HIP_FLAGS = [
 ]
 
 
+# Host-only translation units, built by the system C++ compiler (see the
+# header of each file for why it is not a HIP unit).
+CXX_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-ffp-contract=off", "-fno-fast-math", "-Wall"]
+
+
 def sources():
-    return sorted(f for f in os.listdir(CSRC) if f.endswith(".hip"))
+    return sorted(f for f in os.listdir(CSRC) if f.endswith(".hip") or f.endswith(".cpp"))
 
 
 def hipcc() -> str:
@@ -40,8 +46,11 @@ def hipcc() -> str:
 
 
 def _compile(src: str) -> str:
-    obj = os.path.join(OBJ, src.replace(".hip", ".o"))
-    cmd = [hipcc()] + HIP_FLAGS + ["-c", os.path.join(CSRC, src), "-o", obj]
+    obj = os.path.join(OBJ, os.path.splitext(src)[0] + ".o")
+    if src.endswith(".cpp"):
+        cmd = [os.environ.get("CXX", "g++")] + CXX_FLAGS + ["-c", os.path.join(CSRC, src), "-o", obj]
+    else:
+        cmd = [hipcc()] + HIP_FLAGS + ["-c", os.path.join(CSRC, src), "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"hipcc failed for {src}:\n{r.stderr[-4000:]}")
@@ -53,7 +62,9 @@ def build(jobs: int = 8) -> str:
     with ThreadPoolExecutor(max_workers=max(1, jobs)) as ex:
         objs = list(ex.map(_compile, sources()))
     tmp = OUT + ".tmp"
-    cmd = [hipcc(), f"--offload-arch={ARCH}", "-shared", "-o", tmp] + objs
+    # Host runtime helpers (complex float division and the like) come from
+    # libgcc, as for code built by the system compiler.
+    cmd = [hipcc(), f"--offload-arch={ARCH}", "-shared", "--rtlib=libgcc", "-o", tmp] + objs
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stderr[-4000:]}")

@@ -7,7 +7,6 @@
 
 #include <chrono>
 #include <cmath>
-#include <complex>
 
 namespace gl {
 
@@ -433,36 +432,6 @@ double now_ms() {
     return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
 }
 
-// ---- thin-film stack (transfer-matrix method, Macleod "Thin-Film Optical
-// Filters" ch. 2; tmm.h:20-111 in the reference) ------------------------------
-namespace {
-using cplx = std::complex<float>;
-cplx cos_in(const cplx& n, float inv) {
-    cplx s = cplx(inv, 0.0f) / n;
-    cplx c = std::sqrt(cplx(1.0f, 0.0f) - s * s);
-    if ((n * c).imag() < 0.0f) c = -c;
-    return c;
-}
-cplx stack_r(int nl, const cplx* ln, const float* th, cplx n0, cplx ns, float lam, float inv, bool s_pol) {
-    const cplx one(1.0f, 0.0f);
-    auto eta_of = [&](const cplx& n) { cplx c = cos_in(n, inv); return s_pol ? n * c : n / c; };
-    cplx e0 = eta_of(n0), es = eta_of(ns);
-    cplx m00 = one, m01(0.0f, 0.0f), m10(0.0f, 0.0f), m11 = one;
-    for (int i = 0; i < nl; ++i) {
-        cplx c = cos_in(ln[i], inv);
-        cplx eta = s_pol ? ln[i] * c : ln[i] / c;
-        cplx delta = cplx(2.0f * 3.14159265358979323846f * th[i] / lam, 0.0f) * ln[i] * c;
-        cplx cd = std::cos(delta), sd = std::sin(delta);
-        cplx isd(-sd.imag(), sd.real());
-        cplx l00 = cd, l01 = isd / eta, l10 = isd * eta, l11 = cd;
-        cplx t00 = m00 * l00 + m01 * l10, t01 = m00 * l01 + m01 * l11;
-        cplx t10 = m10 * l00 + m11 * l10, t11 = m10 * l01 + m11 * l11;
-        m00 = t00; m01 = t01; m10 = t10; m11 = t11;
-    }
-    cplx B = m00 + m01 * es, Cc = m10 + m11 * es;
-    return (e0 * B - Cc) / (e0 * B + Cc);
-}
-}  // namespace
 
 }  // namespace gl
 
@@ -520,26 +489,5 @@ int gl_build_ior_tables(const gl_surface* s, int32_t n, const gl_spec* spec, int
     return GL_OK;
 }
 
-int gl_bake_tmm_table(int32_t nl, const float* n_re, const float* n_im, const float* thick,
-                      const float* n_amb, const float* n_sub, float* out) {
-    if (nl <= 0 || nl > 64) return gl::set_error(GL_ERR_INVALID, "layer count %d", nl);
-    const int NW = 31, NA = 18;
-    for (int w = 0; w < NW; ++w) out[w] = 400.0f + (float)w * 10.0f;
-    for (int a = 0; a < NA; ++a) out[NW + a] = 0.0f + (float)a * 5.0f;
-    std::vector<gl::cplx> ln(nl);
-    for (int w = 0; w < NW; ++w) {
-        const float lam = out[w];
-        for (int l = 0; l < nl; ++l) ln[l] = gl::cplx(n_re[l * NW + w], n_im[l * NW + w]);
-        for (int a = 0; a < NA; ++a) {
-            const float th0 = out[NW + a] * (3.14159265358979323846f / 180.0f);
-            const float inv = gl::cplx(n_amb[w], 0.0f).real() * std::sin(th0);
-            gl::cplx rs = gl::stack_r(nl, ln.data(), thick, {n_amb[w], 0.0f}, {n_sub[w], 0.0f}, lam, inv, true);
-            gl::cplx rp = gl::stack_r(nl, ln.data(), thick, {n_amb[w], 0.0f}, {n_sub[w], 0.0f}, lam, inv, false);
-            float R = 0.5f * (std::norm(rs) + std::norm(rp));
-            out[NW + NA + w * NA + a] = R < 0.0f ? 0.0f : (R > 1.0f ? 1.0f : R);
-        }
-    }
-    return GL_OK;
-}
 
 }  // extern "C"

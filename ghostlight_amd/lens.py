@@ -103,6 +103,12 @@ class Surface:
     aperture_rotation_rad: float = 0.0
     aperture_aspect: float = 1.0
     aperture_semi_diameter: float = 0.0
+    # Blade-shape controls of a polygonal stop (optical_system.h:196-205);
+    # only the plain polygon (all zero) is rendered by this build.
+    aperture_curvature: float = 0.0
+    aperture_twist: float = 0.0
+    aperture_notch_rad: float = 0.0
+    aperture_notch_angle_rad: float = 0.0
     decenter_x: float = 0.0
     decenter_y: float = 0.0
     rot: tuple = _IDENT
@@ -184,15 +190,26 @@ class OpticalSystem:
         return len(self.surfaces)
 
     def finalize(self) -> bool:
-        """Relay z from the thickness chain so the chain ends at z = 0
-        (OpticalSystem::finalize, optical_system.cpp:1425)."""
+        """Lay the vertices from the thickness chain so the chain ends at
+        z = 0 (OpticalSystem::finalize, optical_system.cpp:1427-1453): total =
+        sum of thicknesses, z starts at -total and advances by each
+        thickness, float32 throughout."""
         if not self.surfaces:
             return False
-        z = np.float32(0.0)
-        for s in reversed(self.surfaces):
-            z = np.float32(z - np.float32(s.thickness))
+        total = np.float32(0.0)
+        for s in self.surfaces:
+            total = np.float32(total + np.float32(s.thickness))
+        z = np.float32(-total)
+        for s in self.surfaces:
             s.z = float(z)
+            z = np.float32(z + np.float32(s.thickness))
         return True
+
+    @classmethod
+    def load(cls, path) -> "OpticalSystem":
+        """Load a ``ghostlight-optical`` JSON lens file (optical_system.py:89-101)."""
+        from .lens_file import load_lens
+        return load_lens(path)
 
     def ior_before(self, idx: int) -> float:
         idx = min(idx, len(self.surfaces))
@@ -244,6 +261,10 @@ class OpticalSystem:
         arr = (_abi.gl_surface * max(n, 1))()
         arena: List[float] = []
         for i, s in enumerate(self.surfaces):
+            if (s.aperture_shape == ApertureShape.POLYGON and
+                    (s.aperture_curvature or s.aperture_twist or s.aperture_notch_rad)):
+                raise NotImplementedError(f"surface {i}: deformed aperture blades (curvature / twist / notch) "
+                                          "are not part of this build; only plain polygon stops render")
             g = arr[i]
             g.radius, g.thickness, g.ior, g.abbe_v = s.radius, s.thickness, s.ior, s.abbe_v
             g.semi_aperture, g.z = s.semi_aperture, s.z

@@ -272,12 +272,43 @@ def source_map_goldens():
     print(f"wrote {path} ({os.path.getsize(path)} bytes, {len(out)} arrays)")
 
 
+LENS_DIR = os.path.join(HERE, "lenses")
+
+
+def lens_file_goldens():
+    """What the reference's loader (OpticalSystem::load, optical_system.cpp:1299)
+    makes of every lens file under tests/golden/lenses (copies of the
+    reference's lens files and its Python tests' fixtures), exported as the C
+    ABI's surface bytes and coating arena; -1 marks a file it refuses."""
+    ref = pyoracle.ref()
+    out = {}
+    for fn in sorted(os.listdir(LENS_DIR)):
+        if not fn.endswith(".lens"):
+            continue
+        arr = (_abi.gl_surface * 64)()
+        arena = np.zeros(1 << 20, np.float32)
+        al = C.c_int()
+        n = ref.ref_load_lens(os.path.join(LENS_DIR, fn).encode(), arr, 64,
+                              arena.ctypes.data_as(C.POINTER(C.c_float)), len(arena), C.byref(al))
+        key = fn[:-5]
+        out[f"lf_{key}_n"] = np.array([n], np.int32)
+        if n > 0:
+            out[f"lf_{key}_surfaces"] = np.frombuffer(bytes(arr)[: n * C.sizeof(_abi.gl_surface)], np.uint8).copy()
+            out[f"lf_{key}_arena"] = arena[: max(1, al.value)].copy()
+    path = os.path.join(HERE, "lens_file_goldens.npz")
+    np.savez_compressed(path, **out)
+    print(f"wrote {path} ({os.path.getsize(path)} bytes, {len(out)} arrays)")
+
+
 if __name__ == "__main__":
     if len(sys.argv) > 1 and sys.argv[1] == "point_flare":
         point_flare_goldens()
     elif len(sys.argv) > 1 and sys.argv[1] == "source_map":
         source_map_goldens()
+    elif len(sys.argv) > 1 and sys.argv[1] == "lens_files":
+        lens_file_goldens()
     else:
         main()
         point_flare_goldens()
         source_map_goldens()
+        lens_file_goldens()

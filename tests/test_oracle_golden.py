@@ -128,7 +128,7 @@ def test_tmm_bake_matches_reference():
         tab = hostprep.bake_tmm_table(n_re, n_im, th, np.ones(31, np.float32), np.full(31, 1.5168, np.float32))
         wi, ai = int(round((lam - 400) / 10)), int(round(aoi / 5))
         got = tab[49 + wi * 18 + ai]
-        assert abs(got - want) <= 5e-7, (nl, lam, aoi, got, want)
+        assert got == np.float32(want), (nl, lam, aoi, got, want)
 
 
 @pytest.mark.skipif(pyoracle.ref() is None, reason="oracle/_ref not built (reference sources absent)")
