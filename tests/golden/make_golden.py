@@ -300,6 +300,31 @@ def lens_file_goldens():
     print(f"wrote {path} ({os.path.getsize(path)} bytes, {len(out)} arrays)")
 
 
+AB_CASES = ("double_gauss", "sirui", "hex_stop", "hex_rot30")
+AB_LAYERS = ("ghost_axial", "ghost_offaxis", "ghost_corner", "starburst_sprite", "starburst_mdft", "psf",
+             "source_chunk")
+
+
+def aperture_baseline_goldens():
+    """The reference's aperture-baseline goldens (validation/goldens/
+    aperture_baseline/<lens>__<layer>.npy: 192x192x3 renders, 144x144x3 PSF
+    grid) for the four lenses without deformed blades, reduced to 4x4 box
+    sums, per-channel totals and the peak."""
+    gold = os.path.join(REF_ROOT, "validation", "goldens", "aperture_baseline")
+    out = {}
+    for case in AB_CASES:
+        for layer in AB_LAYERS:
+            img = np.load(os.path.join(gold, f"{case}__{layer}.npy")).astype(np.float64)
+            k = f"ab_{case}__{layer}"
+            out[k + "_ds"] = box(img, 4).astype(np.float32)
+            out[k + "_sum"] = img.sum(axis=(0, 1))
+            out[k + "_peak"] = np.array([float(np.abs(img).max())])
+            out[k + "_shape"] = np.array(img.shape, np.int32)
+    path = os.path.join(HERE, "aperture_goldens.npz")
+    np.savez_compressed(path, **out)
+    print(f"wrote {path} ({os.path.getsize(path)} bytes, {len(out)} arrays)")
+
+
 if __name__ == "__main__":
     if len(sys.argv) > 1 and sys.argv[1] == "point_flare":
         point_flare_goldens()
@@ -307,8 +332,11 @@ if __name__ == "__main__":
         source_map_goldens()
     elif len(sys.argv) > 1 and sys.argv[1] == "lens_files":
         lens_file_goldens()
+    elif len(sys.argv) > 1 and sys.argv[1] == "aperture":
+        aperture_baseline_goldens()
     else:
         main()
         point_flare_goldens()
         source_map_goldens()
         lens_file_goldens()
+        aperture_baseline_goldens()
