@@ -479,29 +479,34 @@ GLD void to_sensor(F3 o, F3 d, bool t_ge0, TraceRes& res) {   // trace_cuda.h:36
 }
 
 // d_trace_ghost_ray (trace_cuda.h:195-382) with per-lambda IOR slices.  The
-// surface record is copied from the constant address space per step; only
-// the fields a branch reads are loaded (scalar loads, uniform index).
+// three legs -- forward 0..b reflecting at b, backward b-1..a against the
+// index before each surface reflecting at a, forward a+1..n-1 -- run as ONE
+// loop over a wave-uniform step counter, so the surface interaction is
+// instantiated once (a third of the code of three inlined legs, which keeps
+// the kernel inside the instruction cache).  The surface record is read
+// through the constant address space (scalar loads, uniform index).
 template <bool W = true>
 GLD TraceRes trace_ghost(const DevSurf GLC* S, int n, const float GLC* poly, const float* __restrict__ arena,
                          int a, int b, F3 o, F3 d, float lam, const float GLC* iat, const float GLC* ibf,
                          bool t_ge0, const int2 GLC* lrow = nullptr) {
     TraceRes res; res.status = GL_TRACE_VIGNETTED; res.w = 1.0f; res.segs = 0; res.px = res.py = res.pz = 0.f;
     float cur = 1.0f;
-    for (int k = 0; k <= b; ++k) {
-        const DevSurf GLC& s = S[k];
-        if (!(s.flags & SF_ACTIVE)) continue;
-        if (!step<W>(s, poly, arena, o, d, lam, iat[k], k == b, cur, cur, res, lrow ? lrow + k : nullptr)) return res;
-    }
     const float refl_a = iat[a];
-    for (int k = b - 1; k >= a; --k) {
+    const int end1 = b + 1, end2 = end1 + (b - a), end3 = end2 + (n - 1 - a);
+    for (int i = 0; i < end3; ++i) {
+        int k;
+        float n2, rior;
+        bool refl;
+        if (i < end1) {
+            k = i; n2 = iat[k]; refl = (k == b); rior = cur;
+        } else if (i < end2) {
+            k = b - 1 - (i - end1); n2 = ibf[k]; refl = (k == a); rior = refl_a;
+        } else {
+            k = a + 1 + (i - end2); n2 = iat[k]; refl = false; rior = 0.0f;
+        }
         const DevSurf GLC& s = S[k];
         if (!(s.flags & SF_ACTIVE)) continue;
-        if (!step<W>(s, poly, arena, o, d, lam, ibf[k], k == a, refl_a, cur, res, lrow ? lrow + k : nullptr)) return res;
-    }
-    for (int k = a + 1; k < n; ++k) {
-        const DevSurf GLC& s = S[k];
-        if (!(s.flags & SF_ACTIVE)) continue;
-        if (!step<W>(s, poly, arena, o, d, lam, iat[k], false, 0.0f, cur, res, lrow ? lrow + k : nullptr)) return res;
+        if (!step<W>(s, poly, arena, o, d, lam, n2, refl, rior, cur, res, lrow ? lrow + k : nullptr)) return res;
     }
     to_sensor(o, d, t_ge0, res);
     return res;
