@@ -130,6 +130,14 @@ class gl_calibration(C.Structure):
         "f_number_x", "f_number_y", "pupil_area_frac")]
 
 
+class gl_gate_params(C.Structure):
+    _fields_ = [("thickness_mm", C.c_float), ("standoff_mm", C.c_float), ("offset_left_mm", C.c_float),
+                ("offset_right_mm", C.c_float), ("offset_bottom_mm", C.c_float), ("offset_top_mm", C.c_float),
+                ("lobe", C.c_int32), ("roughness_rad", C.c_float), ("groove_aniso", C.c_float),
+                ("groove_angle_deg", C.c_float), ("max_kick_rad", C.c_float), ("reflectance_r0", C.c_float),
+                ("gain", C.c_float), ("scatter_samples", C.c_int32), ("spectral_samples", C.c_int32)]
+
+
 class gl_source_map(C.Structure):
     _fields_ = [("angle_x", C.c_float), ("angle_y", C.c_float), ("status", C.c_int32), ("residual_mm", C.c_float),
                 ("anchor_ax", C.c_float), ("anchor_ay", C.c_float), ("anchor_x", C.c_float), ("anchor_y", C.c_float),
@@ -222,6 +230,9 @@ SIGNATURES = {
     "gl_source_map_screen": (None, [C.POINTER(gl_source_map), C.c_float, C.c_float, F32P, F32P]),
     "gl_source_map_landing": (C.c_int, [VP, VP, C.POINTER(gl_calibration), C.c_float, C.c_float, F32P, F32P,
                                         I32P]),
+    "gl_render_gate": (C.c_int, [VP, VP, C.POINTER(gl_source), C.c_int32, C.c_int32, C.c_int32, C.c_int32,
+                                 C.c_int32, C.c_int32, C.c_int32, C.c_float, C.c_float, C.POINTER(gl_flare_params),
+                                 C.POINTER(gl_gate_params), F32P, F32P, F32P, C.POINTER(C.c_uint64)]),
     "gl_render_psf_tiles": (C.c_int, [VP, VP, C.POINTER(gl_psf_source), C.c_int32, C.POINTER(gl_flare_params),
                                       C.POINTER(gl_psf_params), F32P, F32P, F32P, F32P]),
     "gl_veil_convolve": (C.c_int, [VP, VP, VP, VP, C.c_int32, C.c_int32, C.c_float, C.c_float,

@@ -135,8 +135,7 @@ class DiffractionConfig:
 
 @dataclass
 class GateConfig:
-    """Film-gate pass settings (render_config.h:213-271).  The gate pass is
-    outside this build; enabling it raises NotImplementedError."""
+    """Film-gate pass settings (render_config.h:212-269)."""
     enabled: bool = False
     thickness_mm: float = 0.8
     standoff_mm: float = 5.0
@@ -308,3 +307,12 @@ def to_params(cfg: FlareConfig):
     p.schedule = int(getattr(cfg, "schedule", -1))
     p.frame_per_source = int(bool(getattr(cfg, "frame_per_source", False)))
     return p, buf
+
+
+def gate_params(gc: "GateConfig"):
+    """GateConfig -> gl_gate_params (same fields and units)."""
+    g = _abi.gl_gate_params()
+    for name, _t in _abi.gl_gate_params._fields_:
+        v = getattr(gc, name)
+        setattr(g, name, int(v) if name in ("lobe", "scatter_samples", "spectral_samples") else float(v))
+    return g

@@ -246,8 +246,6 @@ def render_source_flare(offsets, width: int, height: int, lens: OpticalSystem, c
                          "[d_angle_x, d_angle_y, weight] rows with N >= 1")
     if not np.all(np.isfinite(off)):
         raise RuntimeError("render_source_flare: HIP error: source offsets must be finite")
-    if config.gate.enabled:
-        raise NotImplementedError("render_source_flare: the film-gate pass is not part of this build")
     if lens.num_surfaces() <= 0:
         raise RuntimeError("render_source_flare: HIP error: lens has no surfaces")
     hw = config.sensor_half_w if config.sensor_half_w > 0 else calibration.sensor_half_w
@@ -287,8 +285,6 @@ def render_sources(sources, width: int, height: int, lens: OpticalSystem, calibr
         raise ValueError("render_sources: sources must be a (N, 5) float32 array with N >= 1")
     if not np.all(np.isfinite(sources)):
         raise RuntimeError("render_sources: HIP error: source rows must be finite")
-    if config.gate.enabled:
-        raise NotImplementedError("render_sources: the film-gate pass is not part of this build")
     hw = config.sensor_half_w if config.sensor_half_w > 0 else calibration.sensor_half_w
     hh = config.sensor_half_h if config.sensor_half_h > 0 else calibration.sensor_half_h
     out = {"width": int(width), "height": int(height)}
@@ -322,6 +318,8 @@ def render_sources(sources, width: int, height: int, lens: OpticalSystem, calibr
         from . import diffraction
         out.update(diffraction.render_veil_layer(lens, calibration, sources, width, height, hw, hh, config,
                                                  px=src_px, py=src_py, ctx=ctx))
+    if config.gate.enabled:
+        out.update(render_gate_layer(sources, width, height, lens, config, hw, hh, ctx=ctx))
     if aov_buf is not None:
         for i in range(aov_n.value):
             a, b = int(aov_pairs[2 * i]), int(aov_pairs[2 * i + 1])
@@ -329,6 +327,28 @@ def render_sources(sources, width: int, height: int, lens: OpticalSystem, calibr
                 out[f"ghost_s{a}_s{b}_{ch}"] = np.ascontiguousarray(aov_buf[i, c])
     if st is not None:
         out["stats"] = st.to_dict()
+    return out
+
+
+def render_gate_layer(sources, width: int, height: int, lens: OpticalSystem, config: PointFlareConfig,
+                      sensor_half_w: float, sensor_half_h: float, ctx=None, counters: bool = False) -> dict:
+    """Film-gate scatter layer (render_gate, gate_render.cu:280-514; called by
+    SourceFlareRenderer.cpp:183-200 with the render's effective half
+    extents): gate_r/g/b planes, plus [traces, scrapes] when counters."""
+    from .config import gate_params
+    src = np.ascontiguousarray(sources, np.float32).reshape(-1, 5)
+    sarr, n_src = _sources_array(src)
+    p, _keep = to_params(config)
+    gp = gate_params(config.gate)
+    planes = [np.zeros((height, width), np.float32) for _ in range(3)]
+    ctr = (C.c_uint64 * 2)()
+    _abi.check(lib().gl_render_gate(ctx_handle(ctx), lens.device_lens().handle, sarr, n_src, width, height, width,
+                                    height, 0, 0, float(sensor_half_w), float(sensor_half_h), C.byref(p),
+                                    C.byref(gp), _abi.fptr(planes[0]), _abi.fptr(planes[1]), _abi.fptr(planes[2]),
+                                    ctr), "render_gate")
+    out = {"gate_r": planes[0], "gate_g": planes[1], "gate_b": planes[2]}
+    if counters:
+        out["gate_counters"] = (int(ctr[0]), int(ctr[1]))
     return out
 
 

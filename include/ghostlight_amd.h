@@ -503,6 +503,36 @@ int  gl_render_psf_tiles(gl_ctx* ctx, const gl_lens* lens, const gl_psf_source* 
                          const gl_psf_params* psf, float* out_r, float* out_g, float* out_b,
                          float* ms_out);
 
+/* ---- film-gate scatter layer ---------------------------------------------- */
+
+enum { GL_GATE_CAUCHY_ISO = 0, GL_GATE_GROOVED = 1 };
+
+/* GateConfig (render_config.h:212-269), same fields and units. */
+typedef struct gl_gate_params {
+    float   thickness_mm, standoff_mm;
+    float   offset_left_mm, offset_right_mm, offset_bottom_mm, offset_top_mm;
+    int32_t lobe;                       /* GL_GATE_*                                  */
+    float   roughness_rad, groove_aniso, groove_angle_deg, max_kick_rad, reflectance_r0;
+    float   gain;
+    int32_t scatter_samples;
+    int32_t spectral_samples;           /* 0 = the render's spectral_samples          */
+} gl_gate_params;
+
+/* render_gate (gate_render.h:52-67, gate_render.cu:280-514): primary traces of
+ * every (source, pupil sample, wavelength); a ray landing just outside the
+ * frame opening that strikes the gate's cut wall (standoff_mm ahead of the
+ * sensor, thickness_mm deep) reflects with Schlick reflectance, scatters by
+ * the wall's lobe (scatter_samples draws) and is splatted back in frame.
+ * sensor_half_w/h: the render's effective half extents (the opening).
+ * render: ray grid, jitter, stop override, spectra, colour space, FP64
+ * threshold.  out_*: host, width*height, overwritten.  counters_out (may be
+ * NULL): [primary traces, wall scrapes]. */
+int  gl_render_gate(gl_ctx* ctx, const gl_lens* lens, const gl_source* sources, int32_t n_sources,
+                    int32_t width, int32_t height, int32_t fmt_w, int32_t fmt_h, int32_t fmt_x0,
+                    int32_t fmt_y0, float sensor_half_w, float sensor_half_h,
+                    const gl_flare_params* render, const gl_gate_params* gate,
+                    float* out_r, float* out_g, float* out_b, uint64_t* counters_out);
+
 /* Veil applied to a whole image by FFT convolution (the extended-source form
  * of render_veil, veil_render.h:54-70): out_c = veil_gain * (in_c (*) g) with
  * the unit-sum glare-spread kernel g(r) = (a^2/(r^2+a^2))^p of

@@ -561,7 +561,8 @@ inline TraceOut<S> trace_ghost(const Lens& L, int a, int b, Ray<S> ray, S lambda
 }
 
 template <class S>
-inline TraceOut<S> trace_primary(const Lens& L, Ray<S> ray, S lambda, const float* ior_at, bool t_ge0) {
+inline TraceOut<S> trace_primary(const Lens& L, Ray<S> ray, S lambda, const float* ior_at, bool t_ge0,
+                                 V3<S>* out_dir = nullptr) {
     TraceOut<S> res; res.status = GL_TRACE_VIGNETTED; res.weight = S(1); res.segments = 0;
     S cur = S(1);
     for (int k = 0; k < L.n; ++k) {
@@ -569,6 +570,7 @@ inline TraceOut<S> trace_primary(const Lens& L, Ray<S> ray, S lambda, const floa
         if (!step<S>(L, k, ray, lambda, S(ior_at[k]), false, S(0), cur, res)) return res;
     }
     to_sensor(ray, t_ge0, res);
+    if (out_dir) *out_dir = ray.d;   // final direction (trace_cuda.h:170-171)
     return res;
 }
 

@@ -163,6 +163,32 @@ def ghost_stats(lens, pairs, boosts, sources, spec, params, half_w, half_h, widt
     return out.reshape(3, -1)[:, :np_], int(segs.value)
 
 
+def render_gate(lens, sources, grid, spec, gate, jitter_seed, half_w, half_h, width, height, fp64_rmin=0.0,
+                fmt=None):
+    """The film-gate layer on the CPU: sources (N, 5), grid (K, 2) pupil
+    samples, spec (S, 4) table already times d_lambda, gate a gl_gate_params,
+    fmt = (fmt_w, fmt_h, x0, y0) frame window in the buffer (default: the
+    whole buffer).  Returns (r, g, b) planes and (traces, scrapes)."""
+    fw, fh, x0, y0 = fmt if fmt is not None else (width, height, 0, 0)
+    from ghostlight_amd import _abi
+    surfs, arena = lens.pack()
+    n = lens.num_surfaces()
+    src = np.ascontiguousarray(sources, np.float32).reshape(-1, 5)
+    sarr = (_abi.gl_source * max(1, len(src)))()
+    C.memmove(sarr, src.ctypes.data, src.nbytes)
+    g = np.ascontiguousarray(grid, np.float32).reshape(-1, 2)
+    garr = (_abi.gl_sample * max(1, len(g)))()
+    C.memmove(garr, g.ctypes.data, g.nbytes)
+    sp = np.ascontiguousarray(spec, np.float32).reshape(-1, 4)
+    planes = [np.zeros((height, width), np.float32) for _ in range(3)]
+    ctr = np.zeros(2, np.uint64)
+    lib().orc_render_gate(surfs, n, _f(arena), sarr, len(src), garr, len(g), _f(sp), len(sp), C.byref(gate),
+                          int(jitter_seed), C.c_float(fp64_rmin), C.c_float(half_w), C.c_float(half_h), width, height,
+                          fw, fh, x0, y0, _f(planes[0]), _f(planes[1]), _f(planes[2]),
+                          ctr.ctypes.data_as(C.POINTER(C.c_uint64)))
+    return tuple(planes), (int(ctr[0]), int(ctr[1]))
+
+
 def rel_l2(a, b) -> float:
     a = np.concatenate([np.asarray(x, np.float64).ravel() for x in a])
     b = np.concatenate([np.asarray(x, np.float64).ravel() for x in b])

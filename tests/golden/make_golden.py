@@ -302,7 +302,13 @@ def lens_file_goldens():
 
 AB_CASES = ("double_gauss", "sirui", "hex_stop", "hex_rot30")
 AB_LAYERS = ("ghost_axial", "ghost_offaxis", "ghost_corner", "starburst_sprite", "starburst_mdft", "psf",
-             "source_chunk")
+             "source_chunk", "gate")
+AB_LENSES = {"double_gauss": "lenses/DoubleGauss.lens",
+             "sirui": "lenses/anamorphic/Sirui_75mm_F1.8_1.33x_EP3936919NWA1.lens",
+             "hex_stop": "validation/goldens/aperture_baseline/hex_stop.lens",
+             "hex_rot30": "validation/goldens/aperture_baseline/hex_stop_rot30.lens"}
+# Source position of the gate layer per lens (validation/aperture_baseline.py:49-55).
+AB_GATE_SX = {"double_gauss": 1.03, "sirui": 1.05, "hex_stop": 1.03, "hex_rot30": 1.03}
 
 
 def aperture_baseline_goldens():
@@ -320,6 +326,19 @@ def aperture_baseline_goldens():
             out[k + "_sum"] = img.sum(axis=(0, 1))
             out[k + "_peak"] = np.array([float(np.abs(img).max())])
             out[k + "_shape"] = np.array(img.shape, np.int32)
+        # The gate layer's source: the reference's solve of (gate_sx, 0.5) on a
+        # frame of 0.7 x the calibrated half extents (aperture_baseline.py:103-111).
+        ref = pyoracle.ref()
+        fp = lambda a: a.ctypes.data_as(C.POINTER(C.c_float))  # noqa: E731
+        arr, n, arena = load_ref_lens(ref, AB_LENSES[case])
+        cal = np.zeros(13, np.float32)
+        ref.ref_calibrate_lens(arr, n, fp(arena), fp(cal))
+        hw, hh = np.float32(0.70 * float(cal[0])), np.float32(0.70 * float(cal[1]))
+        r = np.zeros(17, np.float32)
+        ref.ref_solve_source_map(arr, n, fp(arena), fp(cal), C.c_float((AB_GATE_SX[case] - 0.5) * 2.0),
+                                 C.c_float(0.0), C.c_float(hw), C.c_float(hh), fp(r))
+        out[f"ab_{case}__gate_angles"] = r[:2].copy()
+        out[f"ab_{case}__gate_half"] = np.array([hw, hh], np.float32)
     path = os.path.join(HERE, "aperture_goldens.npz")
     np.savez_compressed(path, **out)
     print(f"wrote {path} ({os.path.getsize(path)} bytes, {len(out)} arrays)")
