@@ -140,5 +140,16 @@ int mdft_render(gl_ctx* c, const float2* d_pupil, int N, float fill, const std::
                 float dx_x, float dx_y, float px_mm_x, float px_mm_y, const std::vector<StarSource>& src, float gain,
                 float* o_r, float* o_g, float* o_b, int width, int height, int precision);
 void poly_normals(int n_blades, float rot_rad, float scale, float* c, float* s, float* apothem);
+
+// 3xBF16 complex GEMM on pre-split operands (cgemm3.hip).  split_planes
+// writes the 4 bf16 planes (re_hi, re_lo, im_hi, im_lo) of a batch of R x C
+// complex matrices, transposed on request; cgemm_nt3 computes C = alpha A B
+// from A (M x K) and B^T (N x K) planes, epi 0: float2, 1: |.|^2, 2: planes.
+bool cgemm_nt3_fits(int M, int N, int K);
+size_t cgemm_nt3_lds_bytes();
+int split_planes(hipStream_t s, const float2* X, int ldx, long long sX, int R, int C, int transpose, int batch,
+                 __bf16* out);
+int cgemm_nt3(hipStream_t s, int epi, const __bf16* A, long long sA, const __bf16* Bt, long long sB, void* C, int ldc,
+              long long sC, int M, int N, int K, int batch, float alpha);
 double now_ms();
 }  // namespace gl

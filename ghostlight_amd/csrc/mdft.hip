@@ -369,6 +369,21 @@ int mdft_render(gl_ctx* c, const float2* d_pupil, int N, float fill_in, const st
 
 using namespace gl;
 
+// The pre-split 3xBF16 kernel applies to tile-aligned shapes on a device
+// whose workgroups may hold its 80 KB of LDS.
+static bool nt3_ok(int M, int N, int K) {
+    if (!cgemm_nt3_fits(M, N, K)) return false;
+    static int lds_max = -1;
+    if (lds_max < 0) {
+        int dev = 0, v = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess)
+            v = 0;
+        lds_max = v;
+    }
+    return (size_t)lds_max >= cgemm_nt3_lds_bytes();
+}
+
 extern "C" {
 
 int gl_cgemm_host(gl_ctx* ctx, int32_t M, int32_t N, int32_t K, const float* A, const float* B, float* C, int32_t prec) {
@@ -383,9 +398,18 @@ int gl_cgemm_host(gl_ctx* ctx, int32_t M, int32_t N, int32_t K, const float* A, 
         return st;
     GL_HIP(hipMemcpyAsync(ba.p, A, (size_t)M * K * 8, hipMemcpyHostToDevice, c->stream));
     GL_HIP(hipMemcpyAsync(bb.p, B, (size_t)K * N * 8, hipMemcpyHostToDevice, c->stream));
-    if ((st = cgemm(c->stream, prec, EPI_COMPLEX, ba.as<float2>(), K, 0, bb.as<float2>(), N, 0, bcc.p, N, 0, M, N, K, 1,
-                    1.0f)))
+    if (prec == GL_MDFT_BF16X3 && nt3_ok(M, N, K)) {
+        // Pre-split operands: A as is, B transposed to N x K.
+        DevBuf &bas = c->buf("gemm_As"), &bbs = c->buf("gemm_Bs");
+        if ((st = bas.ensure((size_t)M * K * 8)) || (st = bbs.ensure((size_t)K * N * 8))) return st;
+        if ((st = split_planes(c->stream, ba.as<float2>(), K, 0, M, K, 0, 1, bas.as<__bf16>())) ||
+            (st = split_planes(c->stream, bb.as<float2>(), N, 0, K, N, 1, 1, bbs.as<__bf16>())) ||
+            (st = cgemm_nt3(c->stream, 0, bas.as<__bf16>(), 0, bbs.as<__bf16>(), 0, bcc.p, N, 0, M, N, K, 1, 1.0f)))
+            return st;
+    } else if ((st = cgemm(c->stream, prec, EPI_COMPLEX, ba.as<float2>(), K, 0, bb.as<float2>(), N, 0, bcc.p, N, 0, M,
+                           N, K, 1, 1.0f))) {
         return st;
+    }
     GL_HIP(hipMemcpyAsync(C, bcc.p, (size_t)M * N * 8, hipMemcpyDeviceToHost, c->stream));
     GL_HIP(hipStreamSynchronize(c->stream));
     return GL_OK;
@@ -417,15 +441,32 @@ int gl_mft_psf(gl_ctx* ctx, const float* pupil, int32_t N, int32_t M, float q_re
     hipLaunchKernelGGL(mft_matrix_kernel, dim3(gb, 1, (unsigned)nl), dim3(256), 0, c->stream, bay.as<float2>(), M, N,
                        (long long)MN, (double)M / 2.0, (double)N / 2.0, N, bq.as<double>());
     GL_HIP(hipGetLastError());
-    GL_HIP(hipEventRecord(c->ev0, c->stream));
-    // T_l = P . A_x^T (N x M);  PSF_l = |A_y . T_l|^2 (M x M)
-    if ((st = cgemm(c->stream, prec, EPI_COMPLEX, bp.as<float2>(), N, 0, bax.as<float2>(), M, (long long)MN, btt.p, M,
-                    (long long)MN, N, M, N, nl, 1.0f)))
-        return st;
-    if ((st = cgemm(c->stream, prec, EPI_POWER, bay.as<float2>(), N, (long long)MN, btt.as<float2>(), M, (long long)MN,
-                    bo.p, M, (long long)MM, M, M, N, nl, 1.0f)))
-        return st;
-    GL_HIP(hipEventRecord(c->ev1, c->stream));
+    if (prec == GL_MDFT_BF16X3 && nt3_ok(M, N, N) && nt3_ok(M, M, N)) {
+        // Split once: P (shared), A_y per lambda.  T_l^T = A_y P^T (M x N)
+        // comes out split, ready to be the B^T operand of PSF_l = |A_y T_l|^2.
+        DevBuf &bps = c->buf("mft_Ps"), &bys = c->buf("mft_Ays"), &bts = c->buf("mft_Ts");
+        if ((st = bps.ensure(NN * 8)) || (st = bys.ensure((size_t)nl * MN * 8)) || (st = bts.ensure((size_t)nl * MN * 8)))
+            return st;
+        GL_HIP(hipEventRecord(c->ev0, c->stream));
+        if ((st = split_planes(c->stream, bp.as<float2>(), N, 0, N, N, 0, 1, bps.as<__bf16>())) ||
+            (st = split_planes(c->stream, bay.as<float2>(), N, (long long)MN, M, N, 0, nl, bys.as<__bf16>())) ||
+            (st = cgemm_nt3(c->stream, 2, bys.as<__bf16>(), 4 * (long long)MN, bps.as<__bf16>(), 0, bts.p, N,
+                            4 * (long long)MN, M, N, N, nl, 1.0f)) ||
+            (st = cgemm_nt3(c->stream, 1, bys.as<__bf16>(), 4 * (long long)MN, bts.as<__bf16>(), 4 * (long long)MN,
+                            bo.p, M, (long long)MM, M, M, N, nl, 1.0f)))
+            return st;
+        GL_HIP(hipEventRecord(c->ev1, c->stream));
+    } else {
+        GL_HIP(hipEventRecord(c->ev0, c->stream));
+        // T_l = P . A_x^T (N x M);  PSF_l = |A_y . T_l|^2 (M x M)
+        if ((st = cgemm(c->stream, prec, EPI_COMPLEX, bp.as<float2>(), N, 0, bax.as<float2>(), M, (long long)MN, btt.p,
+                        M, (long long)MN, N, M, N, nl, 1.0f)))
+            return st;
+        if ((st = cgemm(c->stream, prec, EPI_POWER, bay.as<float2>(), N, (long long)MN, btt.as<float2>(), M,
+                        (long long)MN, bo.p, M, (long long)MM, M, M, N, nl, 1.0f)))
+            return st;
+        GL_HIP(hipEventRecord(c->ev1, c->stream));
+    }
     if (out_psf) GL_HIP(hipMemcpyAsync(out_psf, bo.p, (size_t)nl * MM * 4, hipMemcpyDeviceToHost, c->stream));
     GL_HIP(hipStreamSynchronize(c->stream));
     if (ms_out) GL_HIP(hipEventElapsedTime(ms_out, c->ev0, c->ev1));

@@ -82,8 +82,11 @@ def test_cgemm_fp32(shape):
     assert err <= 2e-6
 
 
-@pytest.mark.parametrize("shape", [(64, 64, 64), (100, 70, 50), (257, 129, 200)])
+@pytest.mark.parametrize("shape", [(64, 64, 64), (100, 70, 50), (257, 129, 200), (128, 128, 32), (256, 384, 96),
+                                   (384, 128, 1024)])
 def test_cgemm_bf16x3(shape):
+    """Unaligned shapes run the split-on-load kernel, tile-aligned ones the
+    pre-split 128 x 128 kernel."""
     M, N, K = shape
     rng = np.random.default_rng(M * 11 + N)
     A, B = _crand(rng, M, K), _crand(rng, K, N)
@@ -107,6 +110,17 @@ def test_cgemm_structure():
         ref = np.zeros_like(A)
         ref[:, perm] = A * 1j
         assert np.array_equal(got, ref)
+    # The pre-split 3xBF16 kernel's placement: same product to within the
+    # split's 2^-16 relative rounding.
+    M = K = 256
+    A = _crand(rng, M, K)
+    perm = rng.permutation(K)
+    P = np.zeros((K, K), np.complex64)
+    P[np.arange(K), perm] = 1j
+    got = dfr.cgemm(A, P, dfr.MDFT_BF16X3)
+    ref = np.zeros_like(A)
+    ref[:, perm] = A * 1j
+    np.testing.assert_allclose(got, ref, rtol=0, atol=3e-5 * float(np.abs(A).max()))
 
 
 # ---------------------------------------------------------------- matrix DFT
@@ -117,9 +131,9 @@ def _pupil(N, fill=0.5, blades=7):
     return dor.build_pupil(N, fill, nrm).astype(np.complex64)
 
 
-@pytest.mark.parametrize("prec,tol", [(0, 2e-5), (1, 2e-4)])
-def test_mft_psf_matches_fp64(prec, tol):
-    N, M, q = 128, 96, 2.0
+@pytest.mark.parametrize("prec,tol,N,M", [(0, 2e-5, 128, 96), (1, 2e-4, 128, 96), (1, 2e-4, 256, 384)])
+def test_mft_psf_matches_fp64(prec, tol, N, M):
+    q = 2.0
     P = _pupil(N)
     lams = np.array([430.0, 550.0, 680.0], np.float32)
     got = dfr.mft_psf(P, M, q, lams, precision=prec)
