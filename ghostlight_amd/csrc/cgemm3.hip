@@ -13,10 +13,11 @@
 //
 // Tile: 128 x 128 per workgroup, 4 waves each owning a 64 x 64 quadrant (2 x 2
 // MFMA blocks of 32 x 32, re and im accumulators: 128 accumulator registers),
-// K step 32.  The next K step's 64 KB of operands are loaded into registers
-// while the current step's 96 MFMAs per wave run out of LDS (80 KB, rows
-// padded by 16 bytes against bank conflicts); one workgroup per CU, one wave
-// per SIMD with its ~300 registers.
+// K step 32.  Two LDS stages of 80 KB (rows padded by 16 bytes against bank
+// conflicts): while a step's 96 MFMAs per wave run out of one stage, the next
+// step's operands go from registers into the other and the step after that is
+// fetched from HBM.  One workgroup per CU, one wave per SIMD with its ~300
+// registers.
 // Epilogues: complex float2, |z|^2 float, or the result split again into bf16
 // planes (so a GEMM can feed the next one as its B operand).
 #include "gl_runtime.h"
@@ -69,8 +70,7 @@ __global__ __launch_bounds__(256) void cgemm_nt3_kernel(const __bf16* __restrict
                                                            void* __restrict__ Cv, int ldc, long long sC, int M, int N,
                                                            int K, float alpha) {
     extern __shared__ __attribute__((aligned(16))) __bf16 lds[];
-    __bf16* sa = lds;                          // [4][128][kRow]
-    __bf16* sb = lds + 4 * kT * kRow;          // [4][128][kRow]
+    constexpr int kStage = 2 * 4 * kT * kRow;  // A [4][128][kRow] then B, per stage
     const int bz = blockIdx.z;
     const __bf16* Ab = A + (size_t)bz * sA;
     const __bf16* Bb = Bt + (size_t)bz * sB;
@@ -100,7 +100,9 @@ __global__ __launch_bounds__(256) void cgemm_nt3_kernel(const __bf16* __restrict
             rb[j] = *reinterpret_cast<const u32x4_t*>(Bb + p * pb + (size_t)(n0 + row) * K + k0 + kq);
         }
     };
-    auto stash = [&]() {
+    auto stash = [&](int stage) {
+        __bf16* sa = lds + stage * kStage;
+        __bf16* sb = sa + 4 * kT * kRow;
 #pragma unroll
         for (int j = 0; j < 8; ++j) {
             const int v = tid + 256 * j;
@@ -110,13 +112,18 @@ __global__ __launch_bounds__(256) void cgemm_nt3_kernel(const __bf16* __restrict
         }
     };
 
+    // Two LDS stages: step k computes out of one while step k+1's operands
+    // (already in registers) are written to the other and step k+2's are
+    // fetched; one barrier per step.
     fetch(0);
+    stash(0);
+    __syncthreads();
+    if (kBK < K) fetch(kBK);
     const int r = lane & 31, kh = lane >> 5;
-    for (int k0 = 0; k0 < K; k0 += kBK) {
-        __syncthreads();          // previous step's fragments are read
-        stash();
-        __syncthreads();
-        if (k0 + kBK < K) fetch(k0 + kBK);   // overlaps the MFMAs below
+    int stage = 0;
+    for (int k0 = 0; k0 < K; k0 += kBK, stage ^= 1) {
+        const __bf16* sa = lds + stage * kStage;
+        const __bf16* sb = sa + 4 * kT * kRow;
 #pragma unroll
         for (int ks = 0; ks < kBK; ks += 16) {
             bf16x8_t a[2][4], b[2][4];
@@ -150,6 +157,11 @@ __global__ __launch_bounds__(256) void cgemm_nt3_kernel(const __bf16* __restrict
                 }
             }
         }
+        if (k0 + kBK < K) {
+            stash(stage ^ 1);
+            if (k0 + 2 * kBK < K) fetch(k0 + 2 * kBK);
+        }
+        __syncthreads();
     }
 
     // 32 x 32 accumulator layout: lane l holds column l % 32 and rows
@@ -182,7 +194,7 @@ __global__ __launch_bounds__(256) void cgemm_nt3_kernel(const __bf16* __restrict
 
 }  // namespace
 
-size_t cgemm_nt3_lds_bytes() { return (size_t)2 * 4 * kT * kRow * sizeof(__bf16); }
+size_t cgemm_nt3_lds_bytes() { return (size_t)2 * 2 * 4 * kT * kRow * sizeof(__bf16); }
 
 bool cgemm_nt3_fits(int M, int N, int K) { return M > 0 && N > 0 && K > 0 && M % kT == 0 && N % kT == 0 && K % kBK == 0; }
 
