@@ -227,6 +227,7 @@ SIGNATURES = {
     "gl_calibrate_lens": (C.c_int, [VP, VP, C.c_float, C.c_float, C.POINTER(gl_calibration)]),
     "gl_solve_source_angle": (C.c_int, [VP, VP, C.POINTER(gl_calibration), C.c_float, C.c_float, C.c_float,
                                         C.c_float, C.POINTER(gl_source_map)]),
+    "gl_psf_deep_pupil": (C.c_int, [VP, VP, C.c_int32, F32P, F32P, C.c_int32, F32P]),
     "gl_source_map_screen": (None, [C.POINTER(gl_source_map), C.c_float, C.c_float, F32P, F32P]),
     "gl_source_map_landing": (C.c_int, [VP, VP, C.POINTER(gl_calibration), C.c_float, C.c_float, F32P, F32P,
                                         I32P]),

@@ -968,6 +968,21 @@ int solve(Prober& Pr, const gl_calibration& calib, float ndc_x, float ndc_y, flo
     return GL_OK;
 }
 
+// ---- PSF pre-pass deep-pupil retries (PSFRenderer.cpp:109-122, 203-247, 383-410)
+
+constexpr float kChiefLambdaNm = 587.56f;
+
+// The axial transmitting window: one full-disc probe on axis; its survivor
+// box seeds the tracked window of the deep-pupil retries.
+int measure_axial(Prober& P, Window* win, bool* usable) {
+    DiscArea d;
+    int st;
+    if ((st = probe_window(P, 0.0f, 0.0f, Window{0.0f, 0.0f, 1.0f}, 0, 0, &d))) return st;
+    *usable = d.area > 0.0f;
+    if (*usable) *win = d.fit;
+    return GL_OK;
+}
+
 }  // namespace
 }  // namespace gl
 
@@ -1004,6 +1019,82 @@ int gl_solve_source_angle(gl_ctx* ctx, const gl_lens* lens, const gl_calibration
     if (!lens->host.empty() && (st = P.init(c, lens, kLambdaNm))) return st;
     P.L = lens;
     return solve(P, *calib, ndc_x, ndc_y, eff_half_w, eff_half_h, out);
+}
+
+int gl_psf_deep_pupil(gl_ctx* ctx, const gl_lens* lens, int32_t n, const float* angles, const float* targets,
+                      int32_t target_mode, float* out6) {
+    if (!lens || n < 0 || (n && (!angles || !out6)) || (target_mode && n && !targets))
+        return set_error(GL_ERR_INVALID, "null argument");
+    for (int i = 0; i < n; ++i) {
+        float* o = out6 + 6 * (size_t)i;
+        o[0] = angles[2 * i]; o[1] = angles[2 * i + 1];
+        o[2] = o[3] = 0.0f; o[4] = 1e30f; o[5] = 0.0f;
+    }
+    if (n == 0 || lens->host.empty()) return GL_OK;
+    int st;
+    gl_ctx* c = resolve_ctx(ctx, &st);
+    if (!c) return st;
+    std::lock_guard<std::mutex> g(c->mu);
+    Prober P;
+    if ((st = P.init(c, lens, kChiefLambdaNm))) return st;
+    Window axial_win;
+    bool usable = false;
+    if ((st = measure_axial(P, &axial_win, &usable))) return st;
+    if (!usable) return GL_OK;
+    for (int i = 0; i < n; ++i) {
+        float* o = out6 + 6 * (size_t)i;
+        const float sax = angles[2 * i], say = angles[2 * i + 1];
+        const float mag = std::sqrt(sax * sax + say * say);
+        Window track = axial_win;
+        DiscArea d;
+        if (!target_mode) {
+            // Walk the window out to the angle, then the survivor mean there.
+            if (track.r < 1.0f && mag > 0.0f) {
+                const int steps = std::max(3, std::min(16, (int)std::ceil(mag / 0.10f)));
+                for (int k = 1; k <= steps; ++k) {
+                    const float f = (float)k / (float)steps;
+                    if ((st = probe_disc(P, sax * f, say * f, &track, 0, 0, &d))) return st;
+                }
+            }
+            if ((st = probe_disc(P, sax, say, &track, 0, 0, &d))) return st;
+            if (d.area > 0.0f && std::isfinite(d.mean_x) && std::isfinite(d.mean_y)) {
+                o[2] = d.mean_x; o[3] = d.mean_y; o[4] = 0.0f; o[5] = 1.0f;
+            }
+            continue;
+        }
+        // Aim the tracked window's survivor mean at the target.
+        int err = GL_OK;
+        auto landing = [&](float ax, float ay, float& x, float& y) -> bool {
+            if (err) return false;
+            Window w = track;
+            DiscArea dd;
+            if ((err = probe_disc(P, ax, ay, &w, 0, 0, &dd))) return false;
+            if (!(dd.area > 0.0f) || !std::isfinite(dd.mean_x) || !std::isfinite(dd.mean_y)) return false;
+            track = w;
+            x = dd.mean_x; y = dd.mean_y;
+            return true;
+        };
+        if (track.r < 1.0f && mag > 0.0f) {
+            const int steps = std::max(3, std::min(16, (int)std::ceil(mag / 0.10f)));
+            float tx, ty;
+            for (int k = 1; k <= steps; ++k) {
+                const float f = (float)k / (float)steps;
+                landing(sax * f, say * f, tx, ty);
+            }
+        }
+        float ax = sax, ay = say, tx, ty;
+        if (landing(ax, ay, tx, ty)) {
+            float res = 0.0f;
+            const float cap = std::max(0.02f, 0.5f * mag);
+            newton_aim(landing, targets[2 * i], targets[2 * i + 1], ax, ay, res, 8, 1e-3f, cap, 5e-3f);
+            float x, y;
+            if (landing(ax, ay, x, y)) {
+                o[0] = ax; o[1] = ay; o[2] = x; o[3] = y; o[4] = res; o[5] = 1.0f;
+            }
+        }
+        if (err) return err;
+    }
+    return GL_OK;
 }
 
 void gl_source_map_screen(const gl_source_map* base, float angle_x, float angle_y, float* out_sx, float* out_sy) {

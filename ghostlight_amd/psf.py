@@ -172,6 +172,15 @@ def chief_prepass(lens, sources, cfg: PSFConfig, ctx=None):
         chief_x, chief_y = mx, my
         status = np.where(hits > 0, PSF_CELL_OK, PSF_CELL_DARK).astype(np.uint8)
         frac = fr.astype(np.float32)
+        # A deep pupil between the probe's samples: retry in the tracked axial
+        # window (PSFRenderer.cpp:383-410); frac stays the full-disc measure.
+        dark = np.nonzero(hits == 0)[0]
+        if len(dark):
+            found = deep_pupil(lens, ax[dark], ay[dark], None, ctx)
+            for j, i in enumerate(dark):
+                if found[j, 5] > 0:
+                    chief_x[i], chief_y[i] = found[j, 2], found[j, 3]
+                    status[i] = PSF_CELL_OK
         return chief_x, chief_y, status, frac, resid, beam_ax, beam_ay
     tx = sources[:, 5].astype(np.float64)
     ty = sources[:, 6].astype(np.float64)
@@ -211,7 +220,32 @@ def chief_prepass(lens, sources, cfg: PSFConfig, ctx=None):
                 status[i] = PSF_CELL_DARK
                 resid[i] = 1e30
                 frac[i] = 0.0
+        # Deep pupil: aim the tracked axial window's survivor mean instead
+        # (PSFRenderer.cpp:203-247).
+        dark = np.array([i for i in rest if status[i] == PSF_CELL_DARK], np.int64)
+        if len(dark):
+            found = deep_pupil(lens, ax[dark], ay[dark], np.stack([tx[dark], ty[dark]], 1), ctx)
+            for j, i in enumerate(dark):
+                if found[j, 5] > 0:
+                    beam_ax[i], beam_ay[i] = found[j, 0], found[j, 1]
+                    chief_x[i], chief_y[i] = found[j, 2], found[j, 3]
+                    resid[i] = found[j, 4]
+                    status[i] = PSF_CELL_CHIEF_VIGNETTED
+                    frac[i] = pupil_probe(lens, beam_ax[i:i + 1], beam_ay[i:i + 1], 17, ctx, fp64)[1][0]
     return chief_x, chief_y, status, frac, resid, beam_ax, beam_ay
+
+
+def deep_pupil(lens, ax, ay, targets=None, ctx=None) -> np.ndarray:
+    """gl_psf_deep_pupil: per source [beam ax, ay, landing x, y, residual,
+    found] from the axial window tracked out to the source angle (targets
+    None: survivor mean at the angle; else the angle aimed at the target)."""
+    ang = np.ascontiguousarray(np.stack([np.asarray(ax, np.float32), np.asarray(ay, np.float32)], 1))
+    out = np.zeros((len(ang), 6), np.float32)
+    tg = None if targets is None else np.ascontiguousarray(targets, np.float32).reshape(-1, 2)
+    _abi.check(lib().gl_psf_deep_pupil(ctx_handle(ctx), lens.device_lens().handle, len(ang), _abi.fptr(ang),
+                                       _abi.fptr(tg) if tg is not None else None, int(tg is not None),
+                                       _abi.fptr(out)), "render_psf")
+    return out
 
 
 def render_psf(sources, lens: OpticalSystem, config: PSFConfig, ctx=None, timing: bool = False) -> dict:

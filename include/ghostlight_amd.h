@@ -473,6 +473,17 @@ int  gl_source_map_landing(gl_ctx* ctx, const gl_lens* lens, const gl_calibratio
                            float angle_x, float angle_y, float* out_x_mm, float* out_y_mm,
                            int32_t* out_ok);
 
+/* The PSF pre-pass's deep-pupil retries (PSFRenderer.cpp:109-122, 203-247,
+ * 383-410): for sources whose full-front-disc probes find nothing (a stop
+ * much smaller than the front element), track the axial transmitting window
+ * out to each source's field angle with the spawn-disc probe.
+ * target_mode 0: the survivor-mean landing at the source angle;
+ * target_mode 1: the field angle whose survivor mean lands on targets[2i..]
+ * (Newton, 1 mrad finite differences, 5 um tolerance).
+ * out6 per source: beam angle x, y, landing x, y (mm), residual, found (1/0). */
+int  gl_psf_deep_pupil(gl_ctx* ctx, const gl_lens* lens, int32_t n, const float* angles,
+                       const float* targets, int32_t target_mode, float* out6);
+
 /* ---- geometric PSF grid --------------------------------------------------- */
 
 /* PSFGpuSource (psf_render.h:17-24): a field-point source paired with its
