@@ -98,6 +98,31 @@ __device__ __forceinline__ float wave_max(float v) {
     return v;
 }
 
+// Concentration lattice index -> lattice index walked in 8 x 8 tiles, so the
+// 64 lanes of a wave trace a compact pupil patch instead of a 64-sample strip:
+// a vignetting edge then leaves fewer waves partly dead.  A bijection on
+// [0, lim): full 8-row bands go tile by tile (the last tile of a band is
+// nr % 8 wide), the rows after them keep row-major order.  The sample set,
+// and so the image, is unchanged; only the lane assignment moves.  `lim` is
+// the number of lattice indices traced (a budget lattice larger than the pupil
+// grid is cut at the grid size): only the 8-row bands wholly below it are
+// reordered, so the traced set stays [0, lim).
+__device__ __forceinline__ int tile_order(int idx, int nr, int lim) {
+    const int band_len = 8 * nr, full_bands = lim / band_len;
+    if (idx >= full_bands * band_len) return idx;
+    const int band = idx / band_len, r = idx - band * band_len, full = nr >> 3;
+    int gx, gy;
+    if (r < full * 64) {
+        gx = ((r >> 6) << 3) + (r & 7);
+        gy = (r >> 3) & 7;
+    } else {
+        const int rem = nr - 8 * full, q = r - full * 64;
+        gy = q / rem;
+        gx = 8 * full + (q - gy * rem);
+    }
+    return (band * 8 + gy) * nr + gx;
+}
+
 // Sample position and estimator weight (ghost_render.cu:280-335).
 __device__ __forceinline__ bool sample_at(const GhostArgs& A, int ps, int gi, float& u, float& v, float& w_ray) {
     w_ray = A.ray_weight;
@@ -110,6 +135,7 @@ __device__ __forceinline__ bool sample_at(const GhostArgs& A, int ps, int gi, fl
             float fu, fv;
             if (A.conc.jitter == 2) { fu = halton2((uint32_t)gi); fv = halton3((uint32_t)gi); }
             else {
+                gi = tile_order(gi, nr_lat, min(n_r_ps, A.n_grid));
                 const int gx = gi % nr_lat, gy = gi / nr_lat, nn = nr_lat * nr_lat;
                 const float ju = A.conc.jitter == 1 ? (float)wang_hash((uint32_t)gi + A.conc.seed_offset) / 4294967296.0f : 0.5f;
                 const float jv = A.conc.jitter == 1 ? (float)wang_hash((uint32_t)gi + (uint32_t)nn + A.conc.seed_offset) / 4294967296.0f : 0.5f;
