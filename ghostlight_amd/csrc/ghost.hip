@@ -237,7 +237,10 @@ struct PersistQ {
     unsigned long long* __restrict__ clk;
 };
 
-__global__ __launch_bounds__(kBlock) void ghost_persist_kernel(GhostArgs A, PersistQ Q) {
+// Eight waves per SIMD: the trace is VALU-bound with long dependent chains,
+// and the extra waves hide more latency than the few spilled registers cost.
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8, 8)))
+void ghost_persist_kernel(GhostArgs A, PersistQ Q) {
     __shared__ float4 stage_all[kBlock / 64][kStage];
     __shared__ unsigned s_item;
     const int lane = threadIdx.x & 63;
