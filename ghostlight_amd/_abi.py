@@ -50,7 +50,15 @@ class gl_surface(C.Structure):
         ("table_offset", C.c_int32), ("table_count", C.c_int32),
         ("sa_offset", C.c_int32), ("sa_n_wavelengths", C.c_int32),
         ("sa_n_angles", C.c_int32),
+        ("aperture_curvature", C.c_float), ("aperture_twist", C.c_float),
+        ("aperture_notch_rad", C.c_float), ("aperture_notch_angle_rad", C.c_float),
     ]
+
+
+class gl_blade_profile(C.Structure):
+    _fields_ = [("blades", C.c_int32), ("plain", C.c_int32), ("facets", C.c_int32)] + [
+        (n, C.c_float) for n in ("rotation", "sigma", "half", "r_w", "phi_w", "u_w", "u_rise", "qR", "qL", "p",
+                                 "beta", "eA", "eB", "area_frac")]
 
 
 class gl_sample(C.Structure):
@@ -201,6 +209,9 @@ SIGNATURES = {
     "gl_build_ior_tables": (C.c_int, [C.POINTER(gl_surface), C.c_int32, C.POINTER(gl_spec), C.c_int32,
                                       F32P, F32P]),
     "gl_bake_tmm_table": (C.c_int, [C.c_int32, F32P, F32P, F32P, F32P, F32P, F32P]),
+    "gl_blade_profile_build": (C.c_int, [C.c_int32, C.c_float, C.c_float, C.c_float, C.c_float, C.c_float,
+                                         C.POINTER(gl_blade_profile)]),
+    "gl_blade_radius_at": (C.c_float, [C.POINTER(gl_blade_profile), C.c_float]),
     "gl_enumerate_ghost_pairs": (C.c_int, [VP, C.POINTER(gl_pair), C.c_int32, I32P]),
     "gl_filter_ghost_pairs": (C.c_int, [VP, VP, C.c_float, C.c_float, C.POINTER(gl_flare_params),
                                         C.POINTER(gl_pair), C.c_int32, I32P]),

@@ -35,6 +35,14 @@ enum : int32_t {
     SF_ASPECT = 32, SF_CYL_Y = 64,
     SF_TAB_SORTED = 128,      // 1-D coating table keys non-decreasing (binary search allowed)
     SF_SA_SORTED = 256,       // spectral-angular wavelength and angle axes non-decreasing
+    SF_DEFORMED = 512,        // polygon with authored blade shape: test r <= sa * r(theta)
+};
+
+// Blade silhouette r(theta) of a deformed polygon (gl_blade_profile as the
+// device evaluates it, aperture_profile.h:116-181).
+struct DevBlade {
+    float rotation, sigma, half, phi_w, u_w, u_rise, qR, qL, p, beta, eA, eB;
+    int32_t facets;
 };
 
 // One surface as the tracer reads it (built on the host from gl_surface).
@@ -51,6 +59,7 @@ struct DevSurf {
     float tint_r, tint_g, tint_b, tint_strength;
     float g_sigma, g_bg, g_peak, g_dx, g_dy;
     float asph[GL_MAX_ASPHERE_TERMS];
+    DevBlade blade;                  // SF_DEFORMED only
 };
 
 struct TraceRes {
@@ -248,6 +257,35 @@ GLD float attenuator(F3 hit, const DevSurf GLC& s) {   // trace_core.h:791-809
     return s.g_bg + s.g_peak * expf(-0.5f * r2 / s2);
 }
 
+// ---- deformed blades (ApertureProfile::u_edge / fold / u_raw /
+// sector_angle / radius_at, aperture_profile.h:116-181) ---------------------
+template <class B> __host__ __device__ inline float blade_radius_at(const B& b, float theta) {
+    float phi = fmodf(theta - b.rotation, b.sigma);
+    if (phi < 0.0f) phi += b.sigma;
+    phi -= b.half;
+    const float psi = phi - b.phi_w;
+    const float sh = sinf(0.5f * psi);
+    float u = b.u_w - b.u_rise * (sh * sh) * (psi >= 0.0f ? b.qR : b.qL);
+    if (b.facets) {
+        float w = fmodf(phi - b.half + b.half, b.sigma);
+        if (w < 0.0f) w += b.sigma;
+        w -= b.half;
+        if (w >= -b.eA && w <= b.eB) {
+            const float uf = cosf(w + b.half - b.beta) / b.p;
+            if (uf > u) u = uf;
+        }
+    }
+    return 1.0f / u;
+}
+
+// Kept out of line: only a deformed stop takes it, and inlined its
+// trigonometry would weigh on every surface step's registers.
+__device__ __noinline__ bool blade_inside(const DevSurf GLC& s, float hx, float hy, float h2) {
+    const float r = sqrtf(h2);
+    if (r < 1e-9f) return true;
+    return r <= s.sa * blade_radius_at(s.blade, atan2f(hy, hx));
+}
+
 // check_aperture (trace_core.h:133-170); the regular polygon is the
 // half-plane intersection max_k(h . n_k) <= apothem (same set as the
 // reference's r <= apothem / cos(offset)).
@@ -256,6 +294,7 @@ GLD bool check_aperture(F3 hit, const DevSurf GLC& s, const float GLC* poly) {
     float hy = hit.y;
     float h2 = hx * hx + hy * hy;
     if (!(h2 <= s.sa * s.sa)) return false;
+    if (s.flags & SF_DEFORMED) return blade_inside(s, hx, hy, h2);
     if (s.flags & SF_POLYGON) {
         const float GLC* pc = poly + s.poly_off;
         const float GLC* ps = pc + GL_MAX_BLADES;
@@ -559,12 +598,15 @@ struct DevMask {
     float aspect;
     int32_t nb;
     float apothem;
+    int32_t deformed;                // the stop's blade shape is authored: test r <= r(theta)
     float c[GL_MAX_BLADES], s[GL_MAX_BLADES];
+    DevBlade blade;
 };
 GLD bool mask_contains(const DevMask& m, float u, float v) {
     const float ux = u / m.aspect;
     const float r2 = ux * ux + v * v;
     if (r2 > 1.0f) return false;
+    if (m.deformed) return sqrtf(r2) <= blade_radius_at(m.blade, atan2f(v, ux));
     if (m.nb >= 3) {
         float mx = ux * m.c[0] + v * m.s[0];
         for (int k = 1; k < m.nb; ++k) { float dd = ux * m.c[k] + v * m.s[k]; mx = dd > mx ? dd : mx; }

@@ -238,22 +238,42 @@ void poly_normals(int nb, float rot_rad, float scale, float* c, float* s, float*
     *apothem = (float)((double)scale * std::cos(M_PI / (double)nb));
 }
 
+// The device form of a surface's blade profile; false when its polygon is
+// plain (or it is not a bladed polygon).
+static bool deformed_blade(const gl_surface& g, DevBlade* out) {
+    if (g.aperture_shape != GL_APERTURE_POLYGON || g.aperture_blades < 3) return false;
+    gl_blade_profile b;
+    gl_blade_profile_build(g.aperture_blades, g.aperture_rotation_rad, g.aperture_curvature, g.aperture_twist,
+                           g.aperture_notch_rad, g.aperture_notch_angle_rad, &b);
+    if (b.blades < 3 || b.plain) return false;
+    DevBlade d{};
+    d.rotation = b.rotation; d.sigma = b.sigma; d.half = b.half; d.phi_w = b.phi_w;
+    d.u_w = b.u_w; d.u_rise = b.u_rise; d.qR = b.qR; d.qL = b.qL;
+    d.p = b.p; d.beta = b.beta; d.eA = b.eA; d.eB = b.eB; d.facets = b.facets;
+    *out = d;
+    return true;
+}
+
 DevMask build_mask(const gl_surface* s, int n, int bo, float rdeg) {
     int nb = 0; float rot = 0.f, aspect = 1.f;
+    const gl_surface* stop = nullptr;
     for (int i = 0; i < n; ++i)
         if (s[i].is_stop) {
             aspect = s[i].aperture_aspect;
             if (s[i].aperture_shape == GL_APERTURE_POLYGON && s[i].aperture_blades >= 3) {
                 nb = s[i].aperture_blades; rot = s[i].aperture_rotation_rad;
+                stop = &s[i];
             }
             break;
         }
-    if (bo > 0) { nb = bo; rot = rdeg * ((float)M_PI / 180.0f); }
+    // An override describes a plain n-gon: it drops any authored blade shape.
+    if (bo > 0) { nb = bo; rot = rdeg * ((float)M_PI / 180.0f); stop = nullptr; }
     DevMask m{};
     m.aspect = aspect;
     m.nb = (nb >= 3 && nb <= GL_MAX_BLADES) ? nb : 0;
     m.apothem = 1.0f;
     if (m.nb) poly_normals(m.nb, rot, 1.0f, m.c, m.s, &m.apothem);
+    m.deformed = (m.nb && stop && deformed_blade(*stop, &m.blade)) ? 1 : 0;
     return m;
 }
 
@@ -261,6 +281,7 @@ bool mask_contains_host(const DevMask& m, float u, float v) {
     const float ux = u / m.aspect;
     const float r2 = ux * ux + v * v;
     if (r2 > 1.0f) return false;
+    if (m.deformed) return std::sqrt(r2) <= blade_radius_at(m.blade, std::atan2(v, ux));
     if (m.nb >= 3) {
         float mx = ux * m.c[0] + v * m.s[0];
         for (int k = 1; k < m.nb; ++k) { float d = ux * m.c[k] + v * m.s[k]; mx = d > mx ? d : mx; }
@@ -342,6 +363,7 @@ void build_devsurfs(const gl_surface* s, int n, float fp64_rmin, const float* ar
         if (g.form == GL_FORM_SPHERE && fp64_rmin > 0.0f && std::fabs(g.radius) >= fp64_rmin) f |= SF_FP64;
         d.form = g.form;
         d.nblades = 0; d.poly_off = 0; d.apothem = 0.0f;
+        if (deformed_blade(g, &d.blade)) f |= SF_DEFORMED;
         if (g.aperture_shape == GL_APERTURE_POLYGON && g.aperture_blades >= 3 && g.aperture_blades <= GL_MAX_BLADES) {
             f |= SF_POLYGON;
             d.nblades = g.aperture_blades;

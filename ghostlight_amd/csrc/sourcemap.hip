@@ -378,34 +378,6 @@ const gl_surface* bladed_stop(const gl_lens* L) {
     return (s && s->aperture_shape == GL_APERTURE_POLYGON && s->aperture_blades >= 3) ? s : nullptr;
 }
 
-// Open area of a plain n-gon stop as a fraction of its circumscribed disk, by
-// Simpson's rule over one sector of the conic-edge form the aperture profile
-// stores (aperture_profile.h:124-131, 258-282, 285-316 with no curvature,
-// twist or notch): u(phi) = u_w - (u_w - 1) sin^2(phi/2) / sin^2(half/2).
-float plain_polygon_area_frac(int blades) {
-    if (blades < 3) return 1.0f;
-    const float sigma = 2.0f * (float)M_PI / (float)blades;
-    const float half = 0.5f * sigma;
-    const float a = std::cos(half);
-    const float r_w = a + 0.0f * (1.0f - a);
-    const float u_w = 1.0f / r_w;
-    const float u_rise = u_w - 1.0f;
-    const float sq = std::sin(0.5f * half);
-    const float q = 1.0f / (sq * sq);
-    const int n = 512;
-    const double h = (double)sigma / n;
-    double total = 0.0;
-    for (int i = 0; i <= n; ++i) {
-        const float phi = (float)(-(double)half + i * h);
-        const float sh = std::sin(0.5f * phi);
-        const float u = u_w - u_rise * (sh * sh) * q;
-        const double r = 1.0 / (double)u;
-        const double w = (i == 0 || i == n) ? 1.0 : ((i % 2) ? 4.0 : 2.0);
-        total += w * r * r;
-    }
-    return (float)(blades * (total * h / 3.0) / (2.0 * M_PI));
-}
-
 const float kLadderDeg[] = {0.0f, 4.0f, 8.0f, 14.0f, 22.0f, 32.0f, 45.0f, 60.0f};
 constexpr int kNLadder = 8;
 constexpr float kRefineTolDeg = 0.15f;
@@ -584,7 +556,10 @@ int calibrate(Prober& P, float d_line_nm, float file_focal_length, gl_calibratio
         }
         cal.entrance_pupil_semi_x = bx;
         cal.entrance_pupil_semi_y = by;
-        cal.pupil_area_frac = plain_polygon_area_frac(stop->aperture_blades);
+        gl_blade_profile prof;
+        gl_blade_profile_build(stop->aperture_blades, stop->aperture_rotation_rad, stop->aperture_curvature,
+                               stop->aperture_twist, stop->aperture_notch_rad, stop->aperture_notch_angle_rad, &prof);
+        cal.pupil_area_frac = prof.area_frac;
     } else {
         std::vector<float> h;
         if ((st = P.marginal({make_float2(1.0f, 0.0f), make_float2(0.0f, 1.0f)}, h))) return st;

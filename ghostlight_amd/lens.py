@@ -261,16 +261,14 @@ class OpticalSystem:
         arr = (_abi.gl_surface * max(n, 1))()
         arena: List[float] = []
         for i, s in enumerate(self.surfaces):
-            if (s.aperture_shape == ApertureShape.POLYGON and
-                    (s.aperture_curvature or s.aperture_twist or s.aperture_notch_rad)):
-                raise NotImplementedError(f"surface {i}: deformed aperture blades (curvature / twist / notch) "
-                                          "are not part of this build; only plain polygon stops render")
             g = arr[i]
             g.radius, g.thickness, g.ior, g.abbe_v = s.radius, s.thickness, s.ior, s.abbe_v
             g.semi_aperture, g.z = s.semi_aperture, s.z
             g.is_stop, g.is_active = int(bool(s.is_stop)), int(bool(s.is_active))
             g.aperture_shape, g.aperture_blades = int(s.aperture_shape), int(s.aperture_blades)
             g.aperture_rotation_rad, g.aperture_aspect = s.aperture_rotation_rad, s.aperture_aspect
+            g.aperture_curvature, g.aperture_twist = s.aperture_curvature, s.aperture_twist
+            g.aperture_notch_rad, g.aperture_notch_angle_rad = s.aperture_notch_rad, s.aperture_notch_angle_rad
             g.decenter_x, g.decenter_y = s.decenter_x, s.decenter_y
             for k in range(9):
                 g.rot[k] = s.rot[k]
@@ -330,6 +328,9 @@ class OpticalSystem:
                                is_active=bool(g.is_active), z=g.z, aperture_shape=ApertureShape(g.aperture_shape),
                                aperture_blades=int(g.aperture_blades), aperture_rotation_rad=g.aperture_rotation_rad,
                                aperture_aspect=g.aperture_aspect, decenter_x=g.decenter_x, decenter_y=g.decenter_y,
+                               aperture_curvature=g.aperture_curvature, aperture_twist=g.aperture_twist,
+                               aperture_notch_rad=g.aperture_notch_rad,
+                               aperture_notch_angle_rad=g.aperture_notch_angle_rad,
                                rot=tuple(g.rot[k] for k in range(9)), form=SurfaceForm(g.form), conic_k=g.conic_k,
                                asphere_terms=tuple(g.asphere_terms[k] for k in range(g.n_asphere_terms)),
                                cyl_axis=CylinderAxis(g.cyl_axis), disp_model=DispersionModel(g.disp_model),

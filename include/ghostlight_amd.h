@@ -90,7 +90,32 @@ typedef struct gl_surface {
     int32_t sa_offset;        /* SPECTRAL_ANGULAR: wl[nw] | ang[na] | r[nw*na]    */
     int32_t sa_n_wavelengths;
     int32_t sa_n_angles;
+    /* blade shape of a POLYGON stop (optical_system.h:189-192): all zero is the
+     * plain regular polygon */
+    float   aperture_curvature;       /* [-1, +1] waist depth (-1 scalloped, +1 circle) */
+    float   aperture_twist;           /* [-1, +1] waist position along the edge         */
+    float   aperture_notch_rad;       /* [-pi/4, +pi/4] tip facet                       */
+    float   aperture_notch_angle_rad; /* [0, pi/4] facet skew                           */
 } gl_surface;
+
+/* Derived blade silhouette of a bladed stop (ApertureProfile,
+ * aperture_profile.h:71-239): boundary radius r(theta) in the frame normalised
+ * to the blade-tip radius, each half blade edge a conic arc linear in 1/r,
+ * optional straight tip facets.  Built on the host by gl_blade_profile_build;
+ * the tracer, the pupil mask and the diffraction pupil all test against it. */
+typedef struct gl_blade_profile {
+    int32_t blades;       /* < 3: unused                                       */
+    int32_t plain;        /* 1: no curvature / twist / notch (regular polygon) */
+    int32_t facets;       /* 1: a tip facet is active                          */
+    float   rotation;     /* radians                                           */
+    float   sigma, half;  /* sector angle 2 pi / blades and its half           */
+    float   r_w, phi_w;   /* waist radius and its angle within the sector      */
+    float   u_w, u_rise;  /* 1 / r_w and u_w - 1                               */
+    float   qR, qL;       /* 1 / sin^2 of half the waist-to-tip spans          */
+    float   p, beta;      /* facet chord distance and normal direction         */
+    float   eA, eB;       /* facet angular span either side of the tip         */
+    float   area_frac;    /* open area / circumscribed disk (Simpson, 512)     */
+} gl_blade_profile;
 
 /* Pupil sample in normalised entrance-pupil coordinates
  * (ApertureSample, aperture_sampler.h:21). */
@@ -275,6 +300,18 @@ int  gl_bake_tmm_table(int32_t n_layers, const float* layer_n_re,
                        const float* layer_n_im, const float* thickness_nm,
                        const float* n_ambient31, const float* n_substrate31,
                        float* out);
+
+/* make_aperture_profile (aperture_profile.h:285-363, with ap_area_fraction
+ * :271-282): the blade silhouette of a stop with `blades` >= 3 blades at
+ * `rotation_rad` and the four blade-shape controls (clamped to their
+ * authored ranges).  blades < 3 gives the unused profile (area_frac 1). */
+int  gl_blade_profile_build(int32_t blades, float rotation_rad, float curvature,
+                            float twist, float notch_rad, float notch_angle_rad,
+                            gl_blade_profile* out);
+
+/* ApertureProfile::radius_at (aperture_profile.h:176-181): the normalised
+ * boundary radius at absolute angle theta (1 when blades < 3). */
+float gl_blade_radius_at(const gl_blade_profile* profile, float theta);
 
 /* ---- ghost path -------------------------------------------------------- */
 

@@ -44,17 +44,21 @@ struct PMask {
 
 PMask resolve_mask(const gl_surface* s, int n, int blades_override, float rotation_deg) {
     int nb = 0; float rot = 0.f, aspect = 1.f;
+    Blade blade;
     for (int i = 0; i < n; ++i)
         if (s[i].is_stop) {
             aspect = s[i].aperture_aspect;
             if (s[i].aperture_shape == GL_APERTURE_POLYGON && s[i].aperture_blades >= 3) {
                 nb = s[i].aperture_blades; rot = s[i].aperture_rotation_rad;
+                blade = orc_blade_of(s[i]);
             }
             break;
         }
-    if (blades_override > 0) { nb = blades_override; rot = rotation_deg * ((float)M_PI / 180.0f); }
+    // An override is a plain n-gon (aperture_sampler.h:112-120).
+    if (blades_override > 0) { nb = blades_override; rot = rotation_deg * ((float)M_PI / 180.0f); blade = Blade{}; }
     PMask m; m.aspect = aspect;
     poly_precompute(nb, rot, 1.0f, m.poly);
+    m.poly.blade = blade;
     return m;
 }
 
@@ -62,6 +66,8 @@ inline bool mask_contains(const PMask& m, float u, float v) {
     const float ux = u / m.aspect;
     const float r2 = ux * ux + v * v;
     if (r2 > 1.0f) return false;
+    if (m.poly.n >= 3 && m.poly.blade.deformed)
+        return std::sqrt(r2) <= blade_radius<float>(m.poly.blade, std::atan2(v, ux));
     if (m.poly.n >= 3) {
         float mx = ux * m.poly.c[0] + v * m.poly.s[0];
         for (int k = 1; k < m.poly.n; ++k) { float d = ux * m.poly.c[k] + v * m.poly.s[k]; mx = d > mx ? d : mx; }
@@ -292,6 +298,25 @@ int threads_of(int t) {
 }  // namespace
 
 extern "C" {
+
+// The oracle's blade silhouette as gl_blade_profile, and its r(theta).
+int orc_blade_profile(int blades, float rot, float curvature, float twist, float notch, float rake,
+                      gl_blade_profile* out) {
+    const Blade b = orc_blade_build(blades, rot, curvature, twist, notch, rake);
+    *out = gl_blade_profile{};
+    out->blades = b.blades; out->plain = (b.blades >= 3 && !b.deformed) || b.blades < 3 ? 1 : 0;
+    out->facets = b.facets; out->rotation = b.rotation; out->sigma = b.sigma; out->half = b.half;
+    out->r_w = b.r_w; out->phi_w = b.phi_w; out->u_w = b.u_w; out->u_rise = b.u_rise; out->qR = b.qR;
+    out->qL = b.qL; out->p = b.p; out->beta = b.beta; out->eA = b.eA; out->eB = b.eB;
+    out->area_frac = b.area_frac;
+    return 0;
+}
+int orc_blade_radius(int blades, float rot, float curvature, float twist, float notch, float rake,
+                     const float* theta, int n, float* out) {
+    const Blade b = orc_blade_build(blades, rot, curvature, twist, notch, rake);
+    for (int i = 0; i < n; ++i) out[i] = blade_radius<float>(b, theta[i]);
+    return 0;
+}
 
 // Single ghost / primary ray trace.  dbl selects the scalar type.
 int orc_trace_ghost(int dbl, const gl_surface* s, int n, const float* arena, int a, int b,

@@ -1,7 +1,7 @@
 // ORACLE — test infrastructure only (bench.py's roofline leg and tests).
 // Algorithmic FP32 operation count of the ghost tracer: the oracle's tracer
 // template instantiated with a counting scalar.  Every +, -, *, /, sqrt and
-// transcendental (cos, asin, exp) counts one operation; comparisons, fabs,
+// transcendental (sin, cos, asin, atan2, fmod, exp) counts one operation; comparisons, fabs,
 // fmin/fmax and conversions count none.  The count is what the reference's
 // per-segment arithmetic costs (trace_core.h intersect / aperture / refract /
 // coating), independent of how a compiler lowers it.
@@ -19,6 +19,9 @@ orc::CF fmax(orc::CF, orc::CF);
 orc::CF cos(orc::CF);
 orc::CF asin(orc::CF);
 orc::CF exp(orc::CF);
+orc::CF sin(orc::CF);
+orc::CF atan2(orc::CF, orc::CF);
+orc::CF fmod(orc::CF, orc::CF);
 }  // namespace std
 
 namespace orc {
@@ -56,6 +59,9 @@ orc::CF fmax(orc::CF a, orc::CF b) { return orc::CF(std::fmax(a.v, b.v)); }
 orc::CF cos(orc::CF a) { ++orc::g_ops; return orc::CF(std::cos(a.v)); }
 orc::CF asin(orc::CF a) { ++orc::g_ops; return orc::CF(std::asin(a.v)); }
 orc::CF exp(orc::CF a) { ++orc::g_ops; return orc::CF(std::exp(a.v)); }
+orc::CF sin(orc::CF a) { ++orc::g_ops; return orc::CF(std::sin(a.v)); }
+orc::CF atan2(orc::CF a, orc::CF b) { ++orc::g_ops; return orc::CF(std::atan2(a.v, b.v)); }
+orc::CF fmod(orc::CF a, orc::CF b) { ++orc::g_ops; return orc::CF(std::fmod(a.v, b.v)); }
 }  // namespace std
 
 #include "gl_oracle_trace.h"

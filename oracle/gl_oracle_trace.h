@@ -23,6 +23,7 @@
 #pragma once
 
 #include "../include/ghostlight_amd.h"
+#include "gl_oracle_blade.h"
 
 #include <cmath>
 #include <cstdint>
@@ -54,6 +55,7 @@ struct PolyPre {
     float apothem = 0.f;
     double cd[GL_MAX_BLADES], sd[GL_MAX_BLADES];
     double apothem_d = 0.0;
+    Blade blade;                 // authored blade shape (deformed polygon)
 };
 
 inline void poly_precompute(int n_blades, float rot_rad, float scale, PolyPre& p) {
@@ -97,8 +99,10 @@ inline void lens_init(Lens& L, const gl_surface* s, int n, const float* arena, f
     for (int i = 0; i < n && i < GL_MAX_SURFACES; ++i) {
         L.transformed[i] = (s[i].decenter_x != 0.0f || s[i].decenter_y != 0.0f || !rot_is_identity(s[i].rot));
         L.poly[i] = PolyPre{};
-        if (s[i].aperture_shape == GL_APERTURE_POLYGON)
+        if (s[i].aperture_shape == GL_APERTURE_POLYGON) {
             poly_precompute(s[i].aperture_blades, s[i].aperture_rotation_rad, s[i].semi_aperture, L.poly[i]);
+            L.poly[i].blade = orc_blade_of(s[i]);
+        }
     }
 }
 
@@ -292,6 +296,12 @@ template <class S> inline bool check_aperture(const V3<S>& hit, const gl_surface
     S h2 = hx * hx + hy * hy;
     S sa = S(s.semi_aperture);
     if (!(h2 <= sa * sa)) return false;
+    if (s.aperture_shape == GL_APERTURE_POLYGON && pp.blade.deformed) {
+        // authored blade shape: r <= sa * r(theta), as the reference
+        const S r = std::sqrt(h2);
+        if (r < S(1e-9f)) return true;
+        return r <= sa * blade_radius<S>(pp.blade, std::atan2(hy, hx));
+    }
     if (s.aperture_shape == GL_APERTURE_POLYGON && pp.n >= 3) {
         S m = hx * poly_get_c<S>(pp, 0) + hy * poly_get_s<S>(pp, 0);
         for (int k = 1; k < pp.n; ++k) {

@@ -35,6 +35,8 @@ OpticalSystem make_lens(const gl_surface* s, int n, const float* arena) {
         d.is_stop = g.is_stop != 0; d.is_active = g.is_active != 0;
         d.aperture_shape = g.aperture_shape; d.aperture_blades = g.aperture_blades;
         d.aperture_rotation_rad = g.aperture_rotation_rad; d.aperture_aspect = g.aperture_aspect;
+        d.aperture_curvature = g.aperture_curvature; d.aperture_twist = g.aperture_twist;
+        d.aperture_notch_rad = g.aperture_notch_rad; d.aperture_notch_angle_rad = g.aperture_notch_angle_rad;
         d.decenter_x = g.decenter_x; d.decenter_y = g.decenter_y;
         std::memcpy(d.rot, g.rot, sizeof d.rot);
         d.form = g.form; d.conic_k = g.conic_k;
@@ -83,6 +85,8 @@ int export_lens(const OpticalSystem& L, gl_surface* out, int cap, float* arena, 
         g.is_stop = d.is_stop ? 1 : 0; g.is_active = d.is_active ? 1 : 0;
         g.aperture_shape = d.aperture_shape; g.aperture_blades = d.aperture_blades;
         g.aperture_rotation_rad = d.aperture_rotation_rad; g.aperture_aspect = d.aperture_aspect;
+        g.aperture_curvature = d.aperture_curvature; g.aperture_twist = d.aperture_twist;
+        g.aperture_notch_rad = d.aperture_notch_rad; g.aperture_notch_angle_rad = d.aperture_notch_angle_rad;
         g.decenter_x = d.decenter_x; g.decenter_y = d.decenter_y;
         std::memcpy(g.rot, d.rot, sizeof g.rot);
         g.form = d.form; g.conic_k = d.conic_k;
@@ -134,6 +138,25 @@ void put(const TraceResult& r, float* pos, float* w, int* st) {
 }  // namespace
 
 extern "C" {
+
+// make_aperture_profile (aperture_profile.h:285) as gl_blade_profile.
+int ref_blade_profile(int blades, float rot, float curvature, float twist, float notch, float rake,
+                      gl_blade_profile* out) {
+    const ApertureProfile p = make_aperture_profile(blades, rot, ApertureShapeParams{curvature, twist, notch, rake});
+    out->blades = p.blades; out->plain = p.plain; out->facets = p.facets; out->rotation = p.rotation;
+    out->sigma = p.sigma; out->half = p.half; out->r_w = p.r_w; out->phi_w = p.phi_w;
+    out->u_w = p.u_w; out->u_rise = p.u_rise; out->qR = p.qR; out->qL = p.qL;
+    out->p = p.p; out->beta = p.beta; out->eA = p.eA; out->eB = p.eB; out->area_frac = p.area_frac;
+    return 0;
+}
+
+// ApertureProfile::radius_at for n angles.
+int ref_blade_radius(int blades, float rot, float curvature, float twist, float notch, float rake,
+                     const float* theta, int n, float* out) {
+    const ApertureProfile p = make_aperture_profile(blades, rot, ApertureShapeParams{curvature, twist, notch, rake});
+    for (int i = 0; i < n; ++i) out[i] = p.radius_at(theta[i]);
+    return 0;
+}
 
 // trace_ghost_ray (trace.cpp:171) for n rays (7 floats each).
 int ref_trace_ghost_rays(const gl_surface* s, int n, const float* arena, int a, int b,

@@ -197,7 +197,8 @@ SM_LENSES = [("double_gauss", "lenses/DoubleGauss.lens"), ("helios44", "lenses/H
              ("cooke_pancro", "lenses/spherical/Cooke_SpeedPancro_50mm_F2_US2959102A.lens"),
              ("summicron_100", "lenses/spherical/Leitz_Summicron_100mm.lens"),
              ("hex_stop", "validation/goldens/aperture_baseline/hex_stop.lens"),
-             ("hex_rot30", "validation/goldens/aperture_baseline/hex_stop_rot30.lens")]
+             ("hex_rot30", "validation/goldens/aperture_baseline/hex_stop_rot30.lens"),
+             ("blade_shape", "validation/goldens/aperture_baseline/blade_shape.lens")]
 # Screen NDC requests; None in the last two slots = the calibrated half extents.
 SM_REQUESTS = [(0.0, 0.0, None, None), (0.5, 0.0, None, None), (0.0, -0.5, None, None), (0.7, -0.6, None, None),
                (-0.95, 0.9, None, None), (1.6, 0.2, None, None), (-2.5, -1.5, None, None), (0.3, 0.3, 14.0, 14.0)]
@@ -265,7 +266,7 @@ def source_map_goldens():
               f"{[int(r[2]) for r in solves]}", flush=True)
     man = json.load(open(os.path.join(REF_ROOT, "validation", "goldens", "aperture_baseline", "manifest.json")))
     fields = [f for f in _abi.gl_calibration._fields_]
-    for key in ("double_gauss", "sirui", "hex_stop", "hex_rot30"):
+    for key in ("double_gauss", "sirui", "hex_stop", "hex_rot30", "blade_shape"):
         out[f"manifest_{key}"] = np.array([man[key][f[0]] for f in fields], np.float32)
     path = os.path.join(HERE, "source_map_goldens.npz")
     np.savez_compressed(path, **out)
@@ -300,22 +301,23 @@ def lens_file_goldens():
     print(f"wrote {path} ({os.path.getsize(path)} bytes, {len(out)} arrays)")
 
 
-AB_CASES = ("double_gauss", "sirui", "hex_stop", "hex_rot30")
+AB_CASES = ("double_gauss", "sirui", "hex_stop", "hex_rot30", "blade_shape")
 AB_LAYERS = ("ghost_axial", "ghost_offaxis", "ghost_corner", "starburst_sprite", "starburst_mdft", "psf",
              "source_chunk", "gate")
 AB_LENSES = {"double_gauss": "lenses/DoubleGauss.lens",
              "sirui": "lenses/anamorphic/Sirui_75mm_F1.8_1.33x_EP3936919NWA1.lens",
              "hex_stop": "validation/goldens/aperture_baseline/hex_stop.lens",
-             "hex_rot30": "validation/goldens/aperture_baseline/hex_stop_rot30.lens"}
-# Source position of the gate layer per lens (validation/aperture_baseline.py:49-55).
-AB_GATE_SX = {"double_gauss": 1.03, "sirui": 1.05, "hex_stop": 1.03, "hex_rot30": 1.03}
+             "hex_rot30": "validation/goldens/aperture_baseline/hex_stop_rot30.lens",
+             "blade_shape": "validation/goldens/aperture_baseline/blade_shape.lens"}
+# Source position of the gate layer per lens (validation/aperture_baseline.py:49-58).
+AB_GATE_SX = {"double_gauss": 1.03, "sirui": 1.05, "hex_stop": 1.03, "hex_rot30": 1.03, "blade_shape": 1.03}
 
 
 def aperture_baseline_goldens():
     """The reference's aperture-baseline goldens (validation/goldens/
     aperture_baseline/<lens>__<layer>.npy: 192x192x3 renders, 144x144x3 PSF
-    grid) for the four lenses without deformed blades, reduced to 4x4 box
-    sums, per-channel totals and the peak."""
+    grid) for the five lenses, reduced to 4x4 box sums, per-channel totals and
+    the peak."""
     gold = os.path.join(REF_ROOT, "validation", "goldens", "aperture_baseline")
     out = {}
     for case in AB_CASES:
@@ -344,6 +346,84 @@ def aperture_baseline_goldens():
     print(f"wrote {path} ({os.path.getsize(path)} bytes, {len(out)} arrays)")
 
 
+# (blades, rotation, curvature, twist, notch, notch angle): plain polygons,
+# each control alone and together, both clamps, blade_shape's own stop.
+BLADE_CASES = [
+    (6, 0.0, 0.0, 0.0, 0.0, 0.0), (5, 0.3, 0.0, 0.0, 0.0, 0.4), (9, 0.1, 0.35, 0.0, 0.0, 0.0),
+    (9, 0.0, -0.6, 0.0, 0.0, 0.0), (7, 0.2, 0.0, 0.7, 0.0, 0.0), (7, 0.0, 0.0, -1.0, 0.0, 0.0),
+    (6, 0.0, 0.0, 0.0, 0.5, 0.0), (6, 0.4, 0.0, 0.0, -0.3, 0.6), (11, 0.0, 0.5, -0.4, 0.3, 0.2),
+    (3, 1.0, -1.0, 0.9, 0.78, 0.78), (8, -0.5, 1.0, 0.0, 0.2, 0.0), (4, 0.0, 2.0, -3.0, 1.5, 2.0),
+    (13, 2.5, -0.2, 0.3, -0.6, 0.1), (32, 0.0, 0.3, 0.2, 0.1, 0.05),
+]
+
+
+def blade_goldens():
+    """The reference's blade silhouette (make_aperture_profile and radius_at,
+    aperture_profile.h:116-363) for BLADE_CASES and the blade_shape stop, the
+    reference's trace statuses of primary rays through the blade_shape lens
+    aimed at its stop edge, and its pupil sample grid for that lens."""
+    ref = pyoracle.ref()
+    out = {}
+    prof_fields = [f for f, _ in _abi.gl_blade_profile._fields_]
+    arr, n, arena = load_ref_lens(ref, AB_LENSES["blade_shape"])
+    stop = next(i for i in range(n) if arr[i].is_stop)
+    g = arr[stop]
+    cases = BLADE_CASES + [(g.aperture_blades, g.aperture_rotation_rad, g.aperture_curvature, g.aperture_twist,
+                            g.aperture_notch_rad, g.aperture_notch_angle_rad)]
+    theta = np.concatenate([np.linspace(-7.0, 7.0, 4001), np.random.default_rng(5).uniform(-40, 40, 999)])
+    theta = theta.astype(np.float32)
+    params, profs, radii = [], [], []
+    for c in cases:
+        pr = _abi.gl_blade_profile()
+        ref.ref_blade_profile(C.c_int(int(c[0])), *[C.c_float(float(x)) for x in c[1:]], C.byref(pr))
+        r = np.zeros(len(theta), np.float32)
+        ref.ref_blade_radius(C.c_int(int(c[0])), *[C.c_float(float(x)) for x in c[1:]],
+                             theta.ctypes.data_as(C.POINTER(C.c_float)), len(theta),
+                             r.ctypes.data_as(C.POINTER(C.c_float)))
+        params.append(np.array(c, np.float32))
+        profs.append(np.frombuffer(bytes(pr), np.uint8).copy())
+        radii.append(r)
+    out["bp_params"] = np.stack(params)
+    out["bp_profile_bytes"] = np.stack(profs)
+    out["bp_theta"] = theta
+    out["bp_radius"] = np.stack(radii)
+    out["bp_fields"] = np.array(prof_fields)
+    # Primary rays at the stop edge: trace a fan at 0, 3 and 6 degrees whose
+    # pupil positions straddle the deformed silhouette.
+    rng = np.random.default_rng(11)
+    m = 6000
+    fR = float(arr[0].semi_aperture)
+    z0 = float(arr[0].z) - 20.0
+    rays = np.zeros((m, 7), np.float32)
+    ang = np.radians(rng.choice([0.0, 3.0, 6.0], m))
+    phi = rng.uniform(-np.pi, np.pi, m)
+    rho = rng.uniform(0.3, 0.75, m)
+    rays[:, 0] = rho * np.cos(phi) * fR - 20.0 * np.tan(ang)
+    rays[:, 1] = rho * np.sin(phi) * fR
+    rays[:, 2] = z0
+    d = np.stack([np.tan(ang), np.zeros(m), np.ones(m)], 1)
+    rays[:, 3:6] = d / np.linalg.norm(d, axis=1, keepdims=True)
+    rays[:, 6] = 587.56
+    pos = np.zeros((m, 3), np.float32)
+    w = np.zeros(m, np.float32)
+    st = np.zeros(m, np.int32)
+    ref.ref_trace_primary_rays(arr, n, pyoracle._f(arena), pyoracle._f(rays), m, pyoracle._f(pos), pyoracle._f(w),
+                               st.ctypes.data_as(C.POINTER(C.c_int)))
+    out["bs_rays"] = rays
+    out["bs_status"] = st
+    out["bs_pos"] = pos
+    # The reference's pupil sample grid for the deformed stop (no override).
+    for N, jit in ((64, 0), (96, 1), (80, 2)):
+        cap = N * N
+        buf = (_abi.gl_sample * cap)()
+        cnt = C.c_int()
+        ref.ref_build_aperture_samples(arr, n, N, jit, 12345, 0, C.c_float(0.0), buf, cap, C.byref(cnt))
+        out[f"bs_grid_{N}_{jit}"] = np.frombuffer(bytes(buf), np.float32)[: 2 * cnt.value].reshape(-1, 2).copy()
+    path = os.path.join(HERE, "blade_goldens.npz")
+    np.savez_compressed(path, **out)
+    print(f"wrote {path} ({os.path.getsize(path)} bytes, {len(out)} arrays)")
+
+
 if __name__ == "__main__":
     if len(sys.argv) > 1 and sys.argv[1] == "point_flare":
         point_flare_goldens()
@@ -353,9 +433,12 @@ if __name__ == "__main__":
         lens_file_goldens()
     elif len(sys.argv) > 1 and sys.argv[1] == "aperture":
         aperture_baseline_goldens()
+    elif len(sys.argv) > 1 and sys.argv[1] == "blades":
+        blade_goldens()
     else:
         main()
         point_flare_goldens()
         source_map_goldens()
         lens_file_goldens()
         aperture_baseline_goldens()
+        blade_goldens()

@@ -2,7 +2,7 @@
 written by tests/golden/make_golden.py from the reference's CPU code:
 calibrate_lens, lens_calibration.cpp:344; solve_source_angle,
 source_map.cpp:182; source_map_screen, :456; source_map_landing, :487) for
-17 of the reference's lens files, plus the calibration manifest the reference
+18 of the reference's lens files, plus the calibration manifest the reference
 keeps beside its aperture goldens."""
 from __future__ import annotations
 
@@ -15,11 +15,11 @@ from ghostlight_amd import LensCalibration, OpticalSystem
 PATH = os.path.join(os.path.dirname(__file__), "golden", "source_map_goldens.npz")
 LENSES = ("double_gauss", "helios44", "atlas_40_f3", "atlas_40_f35", "cooke_50_2x", "sirui_24", "sirui",
           "sirui_venus", "exp_helios_2x", "exp_panatar", "artisans_50", "canon_100", "canon_24", "cooke_pancro",
-          "summicron_100", "hex_stop", "hex_rot30")
+          "summicron_100", "hex_stop", "hex_rot30", "blade_shape")
 # Polygonal stops: the product tests the pupil mask in half-plane form, the
 # reference through atan2f/fmodf/cosf, so a sample on a blade edge can fall on
 # the other side and the probe sums move by a sample's weight.
-POLYGON_STOPS = ("sirui_venus", "hex_stop", "hex_rot30")
+POLYGON_STOPS = ("sirui_venus", "hex_stop", "hex_rot30", "blade_shape")
 SOLVE_FIELDS = ("angle_x", "angle_y", "status", "residual_mm", "anchor_ax", "anchor_ay", "anchor_x", "anchor_y",
                 "J0", "J1", "J2", "J3", "eff_half_w", "eff_half_h", "inv_tan_w", "inv_tan_h", "probe_evals")
 

@@ -9,8 +9,9 @@ on the device, source positions solved on the device, then
   * an extended-source chunk (5 angular offsets),
   * the film-gate scatter layer,
 
-for the four lens files without deformed blades (a spherical control, the
-1.33x anamorphic and the two plain hexagonal stops).  Compared in 4 x 4 box
+for the five lens files (a spherical control, the 1.33x anamorphic, the two
+plain hexagonal stops and the 9-blade stop with every blade-shape control
+authored).  Compared in 4 x 4 box
 sums, per-channel totals and the peak; the reference renders with
 approximate division / square root and float atomics, so these are the
 bars, per layer."""
@@ -27,7 +28,7 @@ pytestmark = pytest.mark.gpu
 HERE = os.path.dirname(__file__)
 Z = np.load(os.path.join(HERE, "golden", "aperture_goldens.npz"))
 LENS_FILES = {"double_gauss": "DoubleGauss.lens", "sirui": "Sirui_75mm_F1.8_1.33x_EP3936919NWA1.lens",
-              "hex_stop": "hex_stop.lens", "hex_rot30": "hex_stop_rot30.lens"}
+              "hex_stop": "hex_stop.lens", "hex_rot30": "hex_stop_rot30.lens", "blade_shape": "blade_shape.lens"}
 W = H = 192
 # (per-channel total, 4x4-box rel-L2, peak) bars by layer.
 # Measured on MI355X: ghosts <= 1.7e-5 / 8.2e-4 / 1.7e-3 (the far-corner
@@ -95,7 +96,7 @@ def _render(name, layer):
         out = lens.render_psf(src, cfg)
         return np.stack([np.asarray(out[c]) for c in "rgb"], -1)
     if layer == "gate":
-        gate_sx = {"double_gauss": 1.03, "sirui": 1.05, "hex_stop": 1.03, "hex_rot30": 1.03}[name]
+        gate_sx = {"double_gauss": 1.03, "sirui": 1.05, "hex_stop": 1.03, "hex_rot30": 1.03, "blade_shape": 1.03}[name]
         cfg = _cfg(calib, gate_sx, 0.5)
         cfg.sensor_half_w = 0.70 * calib.sensor_half_w
         cfg.sensor_half_h = 0.70 * calib.sensor_half_h

@@ -17,7 +17,6 @@ HERE = os.path.dirname(__file__)
 LENS_DIR = os.path.join(HERE, "golden", "lenses")
 GOLD = np.load(os.path.join(HERE, "golden", "lens_file_goldens.npz"))
 FILES = sorted(f[:-5] for f in os.listdir(LENS_DIR) if f.endswith(".lens"))
-DEFORMED = {"blade_shape"}     # blade curvature / twist / notch: not rendered by this build
 
 
 @pytest.mark.parametrize("name", FILES)
@@ -30,10 +29,6 @@ def test_load_matches_reference_loader(name):
         return
     lens = OpticalSystem.load(path)
     assert lens.num_surfaces() == n
-    if name in DEFORMED:
-        with pytest.raises(NotImplementedError):
-            lens.pack()
-        return
     arr, arena = lens.pack()
     got = np.frombuffer(bytes(arr)[: n * C.sizeof(_abi.gl_surface)], np.uint8)
     assert np.array_equal(got, GOLD[f"lf_{name}_surfaces"]), "surface bytes differ"

@@ -15,7 +15,7 @@ def test_reference_calibration_matches_its_manifest():
     reference recorded with its aperture goldens (manifest.json)."""
     z = smf.fixture()
     for key, name in (("double_gauss", "double_gauss"), ("sirui", "sirui"), ("hex_stop", "hex_stop"),
-                      ("hex_rot30", "hex_rot30")):
+                      ("hex_rot30", "hex_rot30"), ("blade_shape", "blade_shape")):
         np.testing.assert_array_equal(z[f"sm_{name}_calib"], z[f"manifest_{key}"], err_msg=name)
 
 
