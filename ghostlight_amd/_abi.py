@@ -78,6 +78,14 @@ class gl_source(C.Structure):
                 ("r", C.c_float), ("g", C.c_float), ("b", C.c_float)]
 
 
+GL_MAX_BAFFLES = 8
+
+
+class gl_baffle(C.Structure):
+    _fields_ = [("shape", C.c_int32)] + [(n, C.c_float) for n in ("dz", "top", "bottom", "left", "right", "cx", "cy",
+                                                                   "radius", "aspect")]
+
+
 class gl_flare_params(C.Structure):
     _fields_ = [
         ("ray_grid", C.c_int32), ("spectral_samples", C.c_int32),
@@ -100,6 +108,8 @@ class gl_flare_params(C.Structure):
         ("batch_mode", C.c_int32),
         ("schedule", C.c_int32),
         ("frame_per_source", C.c_int32),
+        ("hurb", C.c_int32), ("hurb_kick", C.c_int32), ("hurb_min_sigma_rad", C.c_float),
+        ("hurb_max_kick_rad", C.c_float), ("n_baffles", C.c_int32), ("baffles", gl_baffle * GL_MAX_BAFFLES),
     ]
 
 
@@ -161,7 +171,7 @@ class gl_diffraction_params(C.Structure):
         ("veil_gain", C.c_float), ("veil_spread", C.c_float), ("veil_falloff", C.c_float),
         ("render_spectral_samples", C.c_int32), ("sensor_model", C.c_int32), ("output_cs", C.c_int32),
         ("custom_xyz_to_output", C.c_float * 9), ("aperture_blades", C.c_int32), ("aperture_rotation", C.c_float),
-        ("mdft_precision", C.c_int32),
+        ("mdft_precision", C.c_int32), ("n_baffles", C.c_int32), ("baffles", gl_baffle * GL_MAX_BAFFLES),
     ]
 
 
@@ -212,6 +222,8 @@ SIGNATURES = {
     "gl_blade_profile_build": (C.c_int, [C.c_int32, C.c_float, C.c_float, C.c_float, C.c_float, C.c_float,
                                          C.POINTER(gl_blade_profile)]),
     "gl_blade_radius_at": (C.c_float, [C.POINTER(gl_blade_profile), C.c_float]),
+    "gl_hurb_sample": (C.c_int, [C.c_float, F32P, C.c_int32, C.c_int32, C.c_float, C.c_uint32, F32P, F32P]),
+    "gl_aperture_edge_distance": (C.c_int, [C.POINTER(gl_surface), F32P, F32P, F32P]),
     "gl_enumerate_ghost_pairs": (C.c_int, [VP, C.POINTER(gl_pair), C.c_int32, I32P]),
     "gl_filter_ghost_pairs": (C.c_int, [VP, VP, C.c_float, C.c_float, C.POINTER(gl_flare_params),
                                         C.POINTER(gl_pair), C.c_int32, I32P]),

@@ -306,7 +306,36 @@ def to_params(cfg: FlareConfig):
     p.batch_mode = int(getattr(cfg, "batch_mode", 2))
     p.schedule = int(getattr(cfg, "schedule", -1))
     p.frame_per_source = int(bool(getattr(cfg, "frame_per_source", False)))
+    d = cfg.diffraction
+    p.hurb = int(bool(d.hurb))
+    p.hurb_kick = int(d.hurb_kick)
+    p.hurb_min_sigma_rad = float(d.hurb_min_sigma_rad)
+    p.hurb_max_kick_rad = float(d.hurb_max_kick_rad)
+    p.n_baffles = fill_baffles(d, p.baffles)
     return p, buf
+
+
+SPAWN_OFFSET_MM = 20.0     # the spawn plane sits this far ahead of the front vertex (spawn_plane.h)
+
+
+def fill_baffles(d: "DiffractionConfig", out) -> int:
+    """build_gpu_baffles (baffle.h:124-153): the matte box as one RECT baffle,
+    then the configured baffles, at most GL_MAX_BAFFLES; each plane's dz is
+    measured from the spawn plane.  Fills the gl_baffle array `out`."""
+    rows = []
+    m = d.matte_box
+    if m.enabled:
+        rows.append((0, SPAWN_OFFSET_MM - m.z_front_mm, m.top, m.bottom, m.left, m.right, 0.0, 0.0,
+                     MATTE_BOX_OPEN, 1.0))
+    for b in d.baffles:
+        rows.append((int(b.shape), SPAWN_OFFSET_MM - b.z_front_mm, b.top, b.bottom, b.left, b.right, b.cx, b.cy,
+                     b.radius, b.aspect if b.aspect > 0.0 else 1.0))
+    rows = rows[:_abi.GL_MAX_BAFFLES]
+    for k, r in enumerate(rows):
+        g = out[k]
+        g.shape = r[0]
+        g.dz, g.top, g.bottom, g.left, g.right, g.cx, g.cy, g.radius, g.aspect = (float(x) for x in r[1:])
+    return len(rows)
 
 
 def gate_params(gc: "GateConfig"):

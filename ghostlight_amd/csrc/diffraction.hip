@@ -273,10 +273,23 @@ __device__ __forceinline__ float sample_env(const float* __restrict__ env, int M
     return a * (1.0f - ty) + b * ty;
 }
 
+// What modifies the stop silhouette (PupilExtras, starburst_render.cu:104-133):
+// the cat's-eye envelope and the front-of-lens baffle stack, the latter met by
+// the pupil ray at (u ep_x, v ep_y) heading (tan_ax, tan_ay).  All zero: the
+// plain stop.
+struct PupilExtra {
+    const float* env;
+    int ME;
+    float ep_x, ep_y, tan_ax, tan_ay;
+    DevBaffles bf;
+};
+
 // Effective pupil A(u,v): 4x4-supersampled stop silhouette x cat's-eye
-// envelope (build_pupil_kernel, starburst_render.cu:155-191).
+// envelope x baffle clip (build_pupil_kernel, starburst_render.cu:155-191).
 __global__ __launch_bounds__(256) void pupil_kernel(float2* __restrict__ out, int N, float fill, DevMask mask,
-                                                    const float* __restrict__ env, int ME) {
+                                                    PupilExtra ex) {
+    const float* __restrict__ env = ex.env;
+    const int ME = ex.ME;
     const int idx = blockIdx.x * blockDim.x + threadIdx.x;
     if (idx >= N * N) return;
     const int i = idx % N, j = idx / N;
@@ -290,6 +303,7 @@ __global__ __launch_bounds__(256) void pupil_kernel(float2* __restrict__ out, in
             if (!mask_contains(mask, u, v)) continue;
             float a = 1.0f;
             if (env != nullptr) a *= sample_env(env, ME, u, v);
+            if (ex.bf.n > 0 && baffles_block(ex.bf, u * ex.ep_x, v * ex.ep_y, ex.tan_ax, ex.tan_ay)) continue;
             re += a;
         }
     out[idx] = make_float2(re * (1.0f / 16.0f), 0.0f);
@@ -858,12 +872,16 @@ int gl_render_starburst(gl_ctx* ctx, const gl_lens* lens, const gl_calibration* 
     const unsigned blocks = (unsigned)((NN + 255) / 256);
     double T = 1.0;
     double* dsum = bsum.as<double>();
-    if (d_env) {
-        hipLaunchKernelGGL(pupil_kernel, dim3(blocks), dim3(256), 0, c->stream, bp.as<float2>(), N, fill, mask,
-                           (const float*)nullptr, 0);
+    PupilExtra plain{}, eff{};
+    eff.env = d_env; eff.ME = env_N;
+    eff.ep_x = ep_x; eff.ep_y = ep_y;
+    eff.tan_ax = std::tan(field_ax); eff.tan_ay = std::tan(field_ay);
+    eff.bf.n = std::max(0, std::min((int)p->n_baffles, GL_MAX_BAFFLES));
+    for (int k = 0; k < eff.bf.n; ++k) eff.bf.b[k] = p->baffles[k];
+    if (d_env || eff.bf.n > 0) {
+        hipLaunchKernelGGL(pupil_kernel, dim3(blocks), dim3(256), 0, c->stream, bp.as<float2>(), N, fill, mask, plain);
         if ((st = reduce_to(c, bp.as<float>(), NN, 1, dsum))) return st;
-        hipLaunchKernelGGL(pupil_kernel, dim3(blocks), dim3(256), 0, c->stream, bp.as<float2>(), N, fill, mask, d_env,
-                           env_N);
+        hipLaunchKernelGGL(pupil_kernel, dim3(blocks), dim3(256), 0, c->stream, bp.as<float2>(), N, fill, mask, eff);
         if ((st = reduce_to(c, bp.as<float>(), NN, 1, dsum + 1))) return st;
         double s2[2];
         GL_HIP(hipMemcpyAsync(s2, dsum, 2 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
@@ -871,8 +889,7 @@ int gl_render_starburst(gl_ctx* ctx, const gl_lens* lens, const gl_calibration* 
         T = s2[0] > 0.0 ? s2[1] / s2[0] : 1.0;
         T = T < 0.0 ? 0.0 : (T > 1.0 ? 1.0 : T);
     } else {
-        hipLaunchKernelGGL(pupil_kernel, dim3(blocks), dim3(256), 0, c->stream, bp.as<float2>(), N, fill, mask,
-                           (const float*)nullptr, 0);
+        hipLaunchKernelGGL(pupil_kernel, dim3(blocks), dim3(256), 0, c->stream, bp.as<float2>(), N, fill, mask, plain);
     }
     GL_HIP(hipGetLastError());
     if (dbg && dbg->pupil && dbg->grid_cap >= N) {

@@ -34,7 +34,9 @@ struct Sched { int lam_div, blocks_per_cu, longest_first; };
 static constexpr Sched kSched[] = {{1, 4, 1}, {1, 8, 1}, {2, 8, 1}, {4, 8, 1}, {1, 8, 0}, {2, 4, 1}};
 static constexpr int kNumSched = (int)(sizeof(kSched) / sizeof(kSched[0]));
 
-struct SrcDev { float bx, by, dx, dy, dz, r, g, b; };
+// hseed: the source-angle part of the per-ray HURB seed, float bits of
+// angle_x * 0x9E3779B9 + angle_y * 0x85EBCA6B (ghost_render.cu:362-368).
+struct SrcDev { float bx, by, dx, dy, dz, r, g, b; uint32_t hseed; };
 
 struct ConcDev {
     const float4* rects;      // nullptr = off
@@ -81,6 +83,9 @@ struct GhostArgs {
     float4* __restrict__ rec;                   // SPLAT: deposit records
     unsigned long long rec_cap;
     unsigned long long* __restrict__ rec_count;
+    HurbDev hurb;                               // edge diffraction (HURB launches only)
+    uint32_t hurb_seed_base;                    // jitter_seed * 2654435761 + 1
+    DevBaffles baffles;                         // front-of-lens clip (n = 0: none)
 };
 
 __device__ __forceinline__ int enc_pupil(float x) { return __float_as_int(x + 2.0f); }
@@ -124,7 +129,9 @@ __device__ __forceinline__ int tile_order(int idx, int nr, int lim) {
 }
 
 // Sample position and estimator weight (ghost_render.cu:280-335).
-__device__ __forceinline__ bool sample_at(const GhostArgs& A, int ps, int gi, float& u, float& v, float& w_ray) {
+// gi comes back as the index that names the sample (the lattice index once a
+// concentrated lattice is walked in tiles); HURB seeds with it.
+__device__ __forceinline__ bool sample_at(const GhostArgs& A, int ps, int& gi, float& u, float& v, float& w_ray) {
     w_ray = A.ray_weight;
     if (A.conc.rects != nullptr) {
         const float4 rc = A.conc.rects[ps];
@@ -178,6 +185,28 @@ __device__ __forceinline__ void flush_stage(const GhostArgs& A, float4* stage, i
 // wavelengths [si0, si1) and stage a deposit record for every landing
 // (ghost_render.cu:372-462).  The wave must be converged: the lambda loop is
 // uniform and the staging uses wave ballots.
+//
+// Front-of-lens baffles drop the ray before any wavelength (achromatic).
+// H = HURB: each wavelength gets its own kick stream, seeded from the source
+// angle, sample, pair and jitter seed; a baffle edge kicks the entering ray on
+// a stream of its own (ghost_render.cu:351-392).
+__device__ __forceinline__ uint32_t hurb_ray_seed(const GhostArgs& A, const SrcDev& sr, int gi, int pair_idx) {
+    return sr.hseed + (uint32_t)gi * 0xC2B2AE35u + (uint32_t)pair_idx * 0x27D4EB2Fu + A.hurb_seed_base;
+}
+template <bool H>
+__device__ __forceinline__ F3 hurb_entry(const GhostArgs& A, const SrcDev& sr, F3 o0, F3 d0, float lam, uint32_t ray,
+                                         int si, uint32_t& hs) {
+    hs = hurb_hash(ray + (uint32_t)si * 0x9E3779B9u);
+    if (A.baffles.n > 0) {
+        uint32_t bs = hurb_hash(hs ^ 0xBAFF1E5Du);
+        F3 bn;
+        const float bd = baffles_edge(A.baffles, o0.x, o0.y, sr.bx, sr.by, bn);
+        hurb_apply(A.hurb, d0, bn, bd, lam, bs);
+    }
+    return d0;
+}
+
+template <bool H>
 __device__ __forceinline__ void deposit_range(const GhostArgs& A, int ps, int gi, int si0, int si1, float4* stage,
                                               int& staged, int lane) {
     const int pair_idx = ps / A.n_src, src_idx = ps % A.n_src;
@@ -188,16 +217,21 @@ __device__ __forceinline__ void deposit_range(const GhostArgs& A, int ps, int gi
     const SrcDev sr = A.src[src_idx];
     const F3 o0 = f3(u * A.front_R + (-20.0f * sr.bx), v * A.front_R + (-20.0f * sr.by), A.start_z);
     const F3 d0 = f3(sr.dx, sr.dy, sr.dz);
+    if (A.baffles.n > 0 && live && baffles_block(A.baffles, o0.x, o0.y, sr.bx, sr.by)) live = false;
+    const uint32_t hray = H ? hurb_ray_seed(A, sr, gi, pair_idx) : 0u;
     for (int si = si0; si < si1; ++si) {
         bool dep = false;
         float4 rec = make_float4(0.f, 0.f, 0.f, 0.f);
         if (live) do {
             const float GLC* spf = as_const(reinterpret_cast<const float*>(A.spec)) + 4 * si;
             const float4 sp = make_float4(spf[0], spf[1], spf[2], spf[3]);
-            TraceRes res = trace_ghost<true>(as_const(A.S), A.n, as_const(A.poly), A.arena, pr.surf_a, pr.surf_b, o0,
-                                             d0, sp.x, as_const(A.iat) + (size_t)si * A.n,
-                                             as_const(A.ibf) + (size_t)si * A.n, false,
-                                             A.lrow ? as_const(A.lrow) + (size_t)si * A.n : nullptr);
+            uint32_t hs = 0u;
+            const F3 dl = H ? hurb_entry<H>(A, sr, o0, d0, sp.x, hray, si, hs) : d0;
+            TraceRes res = trace_ghost<true, H>(as_const(A.S), A.n, as_const(A.poly), A.arena, pr.surf_a, pr.surf_b,
+                                                o0, dl, sp.x, as_const(A.iat) + (size_t)si * A.n,
+                                                as_const(A.ibf) + (size_t)si * A.n, false,
+                                                A.lrow ? as_const(A.lrow) + (size_t)si * A.n : nullptr, &A.hurb,
+                                                &hs);
             if (res.status != GL_TRACE_OK) break;
             if (!isfinite(res.px) || !isfinite(res.py)) break;
             float base = res.w * w_ray * A.gain * pr.area_boost;
@@ -255,7 +289,7 @@ void ghost_persist_kernel(GhostArgs A, PersistQ Q) {
         __syncthreads();
         if (item >= n_items) break;
         const int4 w = Q.work[item];
-        deposit_range(A, w.x, w.y * kBlock + (int)threadIdx.x, w.z, w.w, stage, staged, lane);
+        deposit_range<false>(A, w.x, w.y * kBlock + (int)threadIdx.x, w.z, w.w, stage, staged, lane);
     }
     if (staged) flush_stage(A, stage, staged, lane);
     __syncthreads();
@@ -270,7 +304,7 @@ __global__ void persist_reset_kernel(PersistQ Q) {
 
 // STATS = probe / diagnostic pass (counters, survivor bbox); otherwise the
 // render (SPLAT) and/or the segment count (COUNT).
-template <bool STATS, bool SPLAT, bool COUNT>
+template <bool STATS, bool SPLAT, bool COUNT, bool H>
 __global__ __launch_bounds__(kBlock) void ghost_kernel(GhostArgs A) {
     const int ps = blockIdx.x;
     const int pair_idx = ps / A.n_src, src_idx = ps % A.n_src;
@@ -298,7 +332,7 @@ __global__ __launch_bounds__(kBlock) void ghost_kernel(GhostArgs A) {
         __shared__ float4 stage_all[kBlock / 64][kStage];
         float4* stage = stage_all[threadIdx.x >> 6];
         int staged = 0;                              // wave-uniform
-        deposit_range(A, ps, gi, 0, A.n_spec, stage, staged, lane);
+        deposit_range<H>(A, ps, gi, 0, A.n_spec, stage, staged, lane);
         if (staged) flush_stage(A, stage, staged, lane);
         return;
     }
@@ -310,13 +344,17 @@ __global__ __launch_bounds__(kBlock) void ghost_kernel(GhostArgs A) {
     const SrcDev sr = A.src[src_idx];
     const F3 o0 = f3(u * A.front_R + (-20.0f * sr.bx), v * A.front_R + (-20.0f * sr.by), A.start_z);
     const F3 d0 = f3(sr.dx, sr.dy, sr.dz);
+    if (A.baffles.n > 0 && live && baffles_block(A.baffles, o0.x, o0.y, sr.bx, sr.by)) live = false;
+    const uint32_t hray = H ? hurb_ray_seed(A, sr, gi, pair_idx) : 0u;
     if (live)
         for (int si = si0; si < si1; ++si) {
             const float GLC* spf = as_const(reinterpret_cast<const float*>(A.spec)) + 4 * si;
             if (STATS) ++n_tr;
-            TraceRes res = trace_ghost<false>(as_const(A.S), A.n, as_const(A.poly), A.arena, pr.surf_a, pr.surf_b, o0,
-                                              d0, spf[0], as_const(A.iat) + (size_t)si * A.n,
-                                              as_const(A.ibf) + (size_t)si * A.n, false);
+            uint32_t hs = 0u;
+            const F3 dl = H ? hurb_entry<H>(A, sr, o0, d0, spf[0], hray, si, hs) : d0;
+            TraceRes res = trace_ghost<false, H>(as_const(A.S), A.n, as_const(A.poly), A.arena, pr.surf_a, pr.surf_b,
+                                                 o0, dl, spf[0], as_const(A.iat) + (size_t)si * A.n,
+                                                 as_const(A.ibf) + (size_t)si * A.n, false, nullptr, &A.hurb, &hs);
             n_seg += (unsigned long long)res.segs;
             if (!STATS) continue;
             if (res.status != GL_TRACE_OK) continue;
@@ -459,6 +497,10 @@ static SrcDev make_src(const gl_source& s) {
     float inv = (l > 1e-12f) ? 1.0f / l : 0.0f;
     d.dx = d.bx * inv; d.dy = d.by * inv; d.dz = 1.0f * inv;
     d.r = s.r; d.g = s.g; d.b = s.b;
+    uint32_t ax, ay;
+    std::memcpy(&ax, &s.angle_x, 4);
+    std::memcpy(&ay, &s.angle_y, 4);
+    d.hseed = ax * 0x9E3779B9u + ay * 0x85EBCA6Bu;
     return d;
 }
 
@@ -478,7 +520,8 @@ static int launch(const GhostArgs& A_in, int n_ps, int n_samples, hipStream_t s)
     A.lam_group = STATS ? lam_group(A.n_spec) : 0;
     const int per_block = A.lam_group > 0 ? kBlock / A.lam_group : kBlock;
     dim3 grid((unsigned)n_ps, (unsigned)((n_samples + per_block - 1) / per_block), 1);
-    hipLaunchKernelGGL((ghost_kernel<STATS, SPLAT, COUNT>), grid, dim3(kBlock), 0, s, A);
+    if (A.hurb.on) hipLaunchKernelGGL((ghost_kernel<STATS, SPLAT, COUNT, true>), grid, dim3(kBlock), 0, s, A);
+    else hipLaunchKernelGGL((ghost_kernel<STATS, SPLAT, COUNT, false>), grid, dim3(kBlock), 0, s, A);
     GL_HIP(hipGetLastError());
     return GL_OK;
 }
@@ -606,6 +649,13 @@ static int do_launch(Launch& Lc, float* out_r, float* out_g, float* out_b, float
     A.iat = static_cast<const float*>(d_iat); A.ibf = static_cast<const float*>(d_ibf);
     A.lrow = static_cast<const int2*>(d_lrow);
     A.conc.rects = nullptr;
+    A.hurb.on = p->hurb ? 1 : 0;
+    A.hurb.dist = p->hurb_kick;
+    A.hurb.min_sigma = p->hurb_min_sigma_rad;
+    A.hurb.max_kick = p->hurb_max_kick_rad;
+    A.hurb_seed_base = (uint32_t)p->jitter_seed * 2654435761u + 1u;
+    A.baffles.n = std::max(0, std::min((int)p->n_baffles, GL_MAX_BAFFLES));
+    for (int k = 0; k < A.baffles.n; ++k) A.baffles.b[k] = p->baffles[k];
 
     // ---- probe / cull / concentrate (ghost_render.cu:969-1238)
     const bool want_conc = p->concentrate_samples && n_pairs > 1 && ns != 3;
@@ -798,7 +848,8 @@ static int do_launch(Launch& Lc, float* out_r, float* out_g, float* out_b, float
     want = std::max<unsigned long long>(1, std::min(want, max_rec));
     // ---- launch form: grid, or persistent over a cost-ordered work list
     // (schedule from the autotuner), optionally replayed from a hipGraph.
-    const bool persistent = p->batch_mode != GL_BATCH_GRID;
+    // HURB traces run one grid launch: the persistent kernel has no kick path.
+    const bool persistent = p->batch_mode != GL_BATCH_GRID && !p->hurb;
     const bool use_graph = p->batch_mode == GL_BATCH_GRAPH;
     int cand = -1;
     bool trial = false;

@@ -41,8 +41,20 @@ enum : int32_t {
 // Blade silhouette r(theta) of a deformed polygon (gl_blade_profile as the
 // device evaluates it, aperture_profile.h:116-181).
 struct DevBlade {
-    float rotation, sigma, half, phi_w, u_w, u_rise, qR, qL, p, beta, eA, eB;
+    float rotation, sigma, half, phi_w, u_w, u_rise, qR, qL, p, beta, eA, eB, r_w;
     int32_t facets;
+};
+
+// HURB edge diffraction switches (GpuHurb, hurb.h:103-108).
+struct HurbDev {
+    int32_t on, dist;
+    float min_sigma, max_kick;
+};
+
+// Front-of-lens baffle stack (GpuBaffleStack, baffle.h:43-46).
+struct DevBaffles {
+    int32_t n;
+    gl_baffle b[GL_MAX_BAFFLES];
 };
 
 // One surface as the tracer reads it (built on the host from gl_surface).
@@ -60,6 +72,8 @@ struct DevSurf {
     float g_sigma, g_bg, g_peak, g_dx, g_dy;
     float asph[GL_MAX_ASPHERE_TERMS];
     DevBlade blade;                  // SF_DEFORMED only
+    float ap_rot;                    // polygon rotation (radians): HURB edge distance
+    int32_t ap_blades;               // polygon blade count (0: round, -1: image aperture)
 };
 
 struct TraceRes {
@@ -515,6 +529,194 @@ GLD bool step(const DevSurf GLC& s, const float GLC* poly, const float* __restri
     return true;
 }
 
+// ---- front-of-lens baffles (baffle_stack_blocks / baffle_stack_edge_distance,
+// baffle.h:49-113).  A pupil ray at spawn (px, py) heading (tx, ty) meets
+// baffle k at (px + tx dz_k, py + ty dz_k). -----------------------------------
+template <class BS> __host__ __device__ inline bool baffles_block(const BS& st, float px, float py, float tx,
+                                                                  float ty) {
+    for (int k = 0; k < st.n; ++k) {
+        const gl_baffle& b = st.b[k];
+        const float mx = px + tx * b.dz, my = py + ty * b.dz;
+        if (b.shape == GL_BAFFLE_RECT) {
+            if (mx > b.right || mx < -b.left || my > b.top || my < -b.bottom) return true;
+        } else {
+            const float ex = (mx - b.cx) / b.radius, ey = (my - b.cy) / (b.radius * b.aspect);
+            if (ex * ex + ey * ey > 1.0f) return true;
+        }
+    }
+    return false;
+}
+// Clearance to the nearest baffle edge inside the shapes, and its normal.
+template <class BS> __host__ __device__ inline float baffles_edge(const BS& st, float px, float py, float tx,
+                                                                 float ty, F3& nrm) {
+    float best = 1e30f;
+    F3 bn; bn.x = bn.y = bn.z = 0.0f;
+    for (int k = 0; k < st.n; ++k) {
+        const gl_baffle& b = st.b[k];
+        const float mx = px + tx * b.dz, my = py + ty * b.dz;
+        float d; F3 n; n.z = 0.0f;
+        if (b.shape == GL_BAFFLE_RECT) {
+            d = b.right - mx; n.x = 1.0f; n.y = 0.0f;
+            const float dl = mx + b.left, dt = b.top - my, db = my + b.bottom;
+            if (dl < d) { d = dl; n.x = -1.0f; n.y = 0.0f; }
+            if (dt < d) { d = dt; n.x = 0.0f; n.y = 1.0f; }
+            if (db < d) { d = db; n.x = 0.0f; n.y = -1.0f; }
+        } else {
+            const float rr = (b.radius > 1e-12f) ? b.radius : 1e-12f;
+            const float ra = rr * ((b.aspect > 0.0f) ? b.aspect : 1.0f);
+            const float ex = mx - b.cx, ey = my - b.cy;
+            const float g = sqrtf((ex / rr) * (ex / rr) + (ey / ra) * (ey / ra));
+            const float gx = ex / (rr * rr), gy = ey / (ra * ra);
+            const float gl = sqrtf(gx * gx + gy * gy);
+            if (gl < 1e-20f) { d = rr; n.x = 1.0f; n.y = 0.0f; }
+            else { d = (1.0f - g) / gl; n.x = gx / gl; n.y = gy / gl; }
+        }
+        if (d < best) { best = d; bn = n; }
+    }
+    if (best < 0.0f) best = 0.0f;
+    nrm = bn;
+    return best;
+}
+
+// ---- HURB (hurb.h:40-150): a ray passing an edge at distance d gets a random
+// transverse kick of scale lambda K / d, Cauchy or Gaussian, clamped. ---------
+__host__ __device__ inline float hurb_sigma(float lam_nm, float d_mm, int dist) {
+    if (d_mm <= 0.0f) return 0.0f;
+    const float lam_mm = lam_nm * 1.0e-6f;
+    return lam_mm * (dist == GL_HURB_LORENTZIAN ? 0.159154943f : 0.112539539f) / d_mm;
+}
+__host__ __device__ inline uint32_t hurb_hash(uint32_t s) {
+    s = (s ^ 61u) ^ (s >> 16); s *= 9u; s = s ^ (s >> 4); s *= 0x27d4eb2du; s = s ^ (s >> 15); return s;
+}
+__host__ __device__ inline float hurb_uniform(uint32_t& st) {
+    st ^= st << 13; st ^= st >> 17; st ^= st << 5;
+    return (float)(st & 0x00FFFFFFu) * (1.0f / 16777216.0f);
+}
+__host__ __device__ inline float hurb_draw(int dist, float sigma, float max_kick, uint32_t& st) {
+    float k;
+    if (dist == GL_HURB_LORENTZIAN) {
+        const float u = hurb_uniform(st);
+        k = sigma * tanf(3.14159265358979323846f * (u - 0.5f));
+    } else {
+        const float u1 = fmaxf(hurb_uniform(st), 1.0e-7f);
+        const float u2 = hurb_uniform(st);
+        k = sigma * sqrtf(-2.0f * logf(u1)) * cosf(2.0f * 3.14159265358979323846f * u2);
+    }
+    if (k > max_kick) k = max_kick;
+    if (k < -max_kick) k = -max_kick;
+    return k;
+}
+// Rotate `dir` by the kick inside the plane of dir and the edge normal (|dir|
+// stays 1); far edges (sigma <= min_sigma) draw nothing.
+template <class H> __host__ __device__ inline void hurb_apply(const H& h, F3& dir, F3 n_edge, float d_mm,
+                                                              float lam_nm, uint32_t& st) {
+    const float sigma = hurb_sigma(lam_nm, d_mm, h.dist);
+    if (sigma <= h.min_sigma) return;
+    const float theta = hurb_draw(h.dist, sigma, h.max_kick, st);
+    const float dn = n_edge.x * dir.x + n_edge.y * dir.y + n_edge.z * dir.z;
+    F3 t; t.x = n_edge.x - dir.x * dn; t.y = n_edge.y - dir.y * dn; t.z = n_edge.z - dir.z * dn;
+    const float tl = sqrtf(t.x * t.x + t.y * t.y + t.z * t.z);
+    if (tl < 1.0e-8f) return;
+    const float it = 1.0f / tl;
+    t.x *= it; t.y *= it; t.z *= it;
+    const float tk = tanf(theta);
+    F3 nd; nd.x = dir.x + t.x * tk; nd.y = dir.y + t.y * tk; nd.z = dir.z + t.z * tk;
+    const float nl = sqrtf(nd.x * nd.x + nd.y * nd.y + nd.z * nd.z);
+    if (nl < 1.0e-12f) return;
+    const float in = 1.0f / nl;
+    dir.x = nd.x * in; dir.y = nd.y * in; dir.z = nd.z * in;
+}
+
+// Distance from a deformed blade edge (ApertureProfile::edge_gap,
+// aperture_profile.h:205-230): first order to the conic edge, exact to a tip
+// facet, the nearer wins; r in the normalised frame.
+template <class B> __host__ __device__ inline float blade_edge_gap(const B& b, float r, float theta, float& nx,
+                                                                   float& ny) {
+    float phi = fmodf(theta - b.rotation, b.sigma);
+    if (phi < 0.0f) phi += b.sigma;
+    phi -= b.half;
+    const float ct = cosf(theta), st = sinf(theta);
+    const float psi = phi - b.phi_w;
+    const float sh = sinf(0.5f * psi);
+    const float q = (psi >= 0.0f) ? b.qR : b.qL;
+    const float u = b.u_w - b.u_rise * (sh * sh) * q;
+    const float du = -0.5f * b.u_rise * q * sinf(psi);
+    const float k = (-du / (u * u)) / r;
+    const float inv = 1.0f / sqrtf(1.0f + k * k);
+    float best = (1.0f / u - r) * inv;
+    nx = (ct + k * st) * inv;
+    ny = (st - k * ct) * inv;
+    if (b.facets) {
+        float w = fmodf(phi - b.half + b.half, b.sigma);
+        if (w < 0.0f) w += b.sigma;
+        w -= b.half;
+        const float ang = w + b.half - b.beta;
+        const float d = b.p - r * cosf(ang);
+        if (d < best) { best = d; nx = cosf(theta - ang); ny = sinf(theta - ang); }
+    }
+    return best;
+}
+
+// aperture_edge_distance (trace_core.h:206-290): distance from a world-space
+// hit on surface s to the clear-aperture edge and the world-space edge normal,
+// in the same stretched frame and polygon geometry the aperture test uses.
+template <class SV> __host__ __device__ inline float surface_edge(const SV& s, F3 hit, F3& n_out) {
+    if (s.ap_blades < 0) {                     // image aperture: no analytic edge, never kicks
+        n_out.x = n_out.y = n_out.z = 0.0f;
+        return 1e30f;
+    }
+    F3 h = hit;
+    const bool tr = (s.flags & SF_TRANSFORMED) != 0;
+    if (tr) {
+        F3 q; q.x = hit.x - s.dx; q.y = hit.y - s.dy; q.z = hit.z - s.z;
+        h.x = s.rot[0] * q.x + s.rot[3] * q.y + s.rot[6] * q.z;
+        h.y = s.rot[1] * q.x + s.rot[4] * q.y + s.rot[7] * q.z;
+        h.z = s.rot[2] * q.x + s.rot[5] * q.y + s.rot[8] * q.z + s.z;
+    }
+    const float sa = s.sa;
+    const float hx = h.x / s.aspect, hy = h.y;
+    const float r_s = sqrtf(hx * hx + hy * hy);
+    float d;
+    F3 n; n.z = 0.0f;
+    if (s.ap_blades >= 3) {
+        const bool deformed = (s.flags & SF_DEFORMED) != 0;
+        const float sector = 2.0f * 3.14159265358979323846f / (float)s.ap_blades;
+        const float apothem = sa * cosf(0.5f * sector);
+        if (r_s < 1e-9f) {
+            d = deformed ? sa * s.blade.r_w : apothem;
+            n.x = 1.0f; n.y = 0.0f;
+        } else if (deformed) {
+            const float theta = atan2f(hy, hx);
+            float nx, ny;
+            d = sa * blade_edge_gap(s.blade, r_s / sa, theta, nx, ny);
+            n.x = nx; n.y = ny;
+        } else {
+            const float theta = atan2f(hy, hx);
+            float off = fmodf(theta - s.ap_rot, sector);
+            if (off < 0.0f) off += sector;
+            off -= 0.5f * sector;
+            d = apothem - r_s * cosf(off);
+            const float phi = theta - off;
+            n.x = cosf(phi); n.y = sinf(phi);
+        }
+    } else {
+        const float asp2 = s.aspect * s.aspect;
+        const float gx = h.x / asp2, gy = h.y;
+        const float glen = sqrtf(gx * gx + gy * gy);
+        if (glen < 1e-12f) { d = sa; n.x = 1.0f; n.y = 0.0f; }
+        else { d = (sa - r_s) * r_s / glen; n.x = gx / glen; n.y = gy / glen; }
+    }
+    if (d < 0.0f) d = 0.0f;
+    if (tr) {
+        n_out.x = s.rot[0] * n.x + s.rot[1] * n.y + s.rot[2] * n.z;
+        n_out.y = s.rot[3] * n.x + s.rot[4] * n.y + s.rot[5] * n.z;
+        n_out.z = s.rot[6] * n.x + s.rot[7] * n.y + s.rot[8] * n.z;
+    } else {
+        n_out = n;
+    }
+    return d;
+}
+
 GLD void to_sensor(F3 o, F3 d, bool t_ge0, TraceRes& res) {   // trace_cuda.h:366-380
     if (fabsf(d.z) < 1e-12f) { res.status = GL_TRACE_MISSED_SURFACE; return; }
     float ts = -o.z / d.z;
@@ -532,10 +734,15 @@ GLD void to_sensor(F3 o, F3 d, bool t_ge0, TraceRes& res) {   // trace_cuda.h:36
 // instantiated once (a third of the code of three inlined legs, which keeps
 // the kernel inside the instruction cache).  The surface record is read
 // through the constant address space (scalar loads, uniform index).
-template <bool W = true>
+//
+// H = HURB: after every surface interaction the new direction takes an edge
+// kick (hurb_apply) drawn from the per-trace stream `hs`
+// (d_trace_ghost_ray<HURB>, trace_cuda.h:248-361).
+template <bool W = true, bool H = false>
 GLD TraceRes trace_ghost(const DevSurf GLC* S, int n, const float GLC* poly, const float* __restrict__ arena,
                          int a, int b, F3 o, F3 d, float lam, const float GLC* iat, const float GLC* ibf,
-                         bool t_ge0, const int2 GLC* lrow = nullptr) {
+                         bool t_ge0, const int2 GLC* lrow = nullptr, const HurbDev* hb = nullptr,
+                         uint32_t* hs = nullptr) {
     TraceRes res; res.status = GL_TRACE_VIGNETTED; res.w = 1.0f; res.segs = 0; res.px = res.py = res.pz = 0.f;
     float cur = 1.0f;
     const float refl_a = iat[a];
@@ -554,6 +761,11 @@ GLD TraceRes trace_ghost(const DevSurf GLC* S, int n, const float GLC* poly, con
         const DevSurf GLC& s = S[k];
         if (!(s.flags & SF_ACTIVE)) continue;
         if (!step<W>(s, poly, arena, o, d, lam, n2, refl, rior, cur, res, lrow ? lrow + k : nullptr)) return res;
+        if (H) {
+            F3 en;
+            const float ed = surface_edge(s, o, en);
+            hurb_apply(*hb, d, en, ed, lam, *hs);
+        }
     }
     to_sensor(o, d, t_ge0, res);
     return res;

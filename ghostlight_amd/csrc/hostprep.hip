@@ -249,7 +249,7 @@ static bool deformed_blade(const gl_surface& g, DevBlade* out) {
     DevBlade d{};
     d.rotation = b.rotation; d.sigma = b.sigma; d.half = b.half; d.phi_w = b.phi_w;
     d.u_w = b.u_w; d.u_rise = b.u_rise; d.qR = b.qR; d.qL = b.qL;
-    d.p = b.p; d.beta = b.beta; d.eA = b.eA; d.eB = b.eB; d.facets = b.facets;
+    d.p = b.p; d.beta = b.beta; d.eA = b.eA; d.eB = b.eB; d.facets = b.facets; d.r_w = b.r_w;
     *out = d;
     return true;
 }
@@ -364,6 +364,9 @@ void build_devsurfs(const gl_surface* s, int n, float fp64_rmin, const float* ar
         d.form = g.form;
         d.nblades = 0; d.poly_off = 0; d.apothem = 0.0f;
         if (deformed_blade(g, &d.blade)) f |= SF_DEFORMED;
+        d.ap_rot = g.aperture_rotation_rad;
+        d.ap_blades = (g.aperture_shape == GL_APERTURE_IMAGE) ? -1
+                      : ((g.aperture_shape == GL_APERTURE_POLYGON && g.aperture_blades >= 3) ? g.aperture_blades : 0);
         if (g.aperture_shape == GL_APERTURE_POLYGON && g.aperture_blades >= 3 && g.aperture_blades <= GL_MAX_BLADES) {
             f |= SF_POLYGON;
             d.nblades = g.aperture_blades;
@@ -473,6 +476,9 @@ void gl_flare_params_init(gl_flare_params* p) {
     p->fp64_radius_min = 1000.0f;
     p->batch_mode = GL_BATCH_GRAPH;
     p->schedule = -1;
+    p->hurb = 0; p->hurb_kick = GL_HURB_LORENTZIAN;
+    p->hurb_min_sigma_rad = 1.0e-5f; p->hurb_max_kick_rad = 0.35f;
+    p->n_baffles = 0;
 }
 
 int gl_build_spectral_table(int32_t n, int32_t output_cs, const float* custom9, gl_spec* out) {
@@ -492,6 +498,36 @@ int gl_spectral_correction(const gl_spec* table, int32_t n, int32_t output_cs, c
 int gl_color_matrices(int32_t in_cs, const float* cin, int32_t out_cs, const float* cout, float* Min, float* Mout) {
     gl::input_matrix(in_cs, cin, Min);
     gl::output_matrix(out_cs, cout, Mout);
+    return GL_OK;
+}
+
+int gl_hurb_sample(float lambda_nm, const float* d_mm, int32_t n, int32_t dist, float max_kick_rad, uint32_t seed,
+                   float* kicks, float* sigma) {
+    if (n < 0 || (n && (!d_mm || !kicks || !sigma))) return gl::set_error(GL_ERR_INVALID, "null argument");
+    for (int i = 0; i < n; ++i) {
+        const float sg = gl::hurb_sigma(lambda_nm, d_mm[i], dist);
+        uint32_t st = gl::hurb_hash(seed ^ (2654435761u * (uint32_t)i + 1u));
+        if (st == 0u) st = 1u;
+        kicks[i] = (sg > 0.0f) ? gl::hurb_draw(dist, sg, max_kick_rad, st) : 0.0f;
+        sigma[i] = sg;
+    }
+    return GL_OK;
+}
+
+int gl_aperture_edge_distance(const gl_surface* s, const float hit[3], float* d_mm, float normal[3]) {
+    if (!s || !hit || !d_mm || !normal) return gl::set_error(GL_ERR_INVALID, "null argument");
+    if (s->aperture_shape == GL_APERTURE_IMAGE) {
+        *d_mm = 1e30f;
+        normal[0] = normal[1] = normal[2] = 0.0f;
+        return GL_OK;
+    }
+    std::vector<gl::DevSurf> ds;
+    std::vector<float> poly;
+    gl::build_devsurfs(s, 1, 0.0f, nullptr, 0, ds, poly);
+    gl::F3 h, nrm;
+    h.x = hit[0]; h.y = hit[1]; h.z = hit[2];
+    *d_mm = gl::surface_edge(ds[0], h, nrm);
+    normal[0] = nrm.x; normal[1] = nrm.y; normal[2] = nrm.z;
     return GL_OK;
 }
 

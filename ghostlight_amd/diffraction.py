@@ -10,7 +10,7 @@ import math
 import numpy as np
 
 from . import _abi
-from .config import FlareConfig, PointFlareConfig
+from .config import FlareConfig, PointFlareConfig, fill_baffles
 from .lens import LensCalibration, OpticalSystem
 from .runtime import ctx_handle, lib
 
@@ -45,15 +45,8 @@ def diffraction_params(cfg: FlareConfig, mdft_precision: int = 0):
     p.aperture_blades = int(cfg.aperture_blades)
     p.aperture_rotation = float(cfg.aperture_rotation)
     p.mdft_precision = int(mdft_precision)
+    p.n_baffles = fill_baffles(d, p.baffles)
     return p
-
-
-def _check_unsupported(cfg: FlareConfig):
-    d = cfg.diffraction
-    if d.matte_box.enabled or d.baffles:
-        raise NotImplementedError("matte box / baffles are not part of this build")
-    if d.hurb:
-        raise NotImplementedError("HURB edge diffraction is not part of this build")
 
 
 def source_pixels(lens: OpticalSystem, calib: LensCalibration, sources, width, height, half_w, half_h):
@@ -79,7 +72,6 @@ def _sources(sources):
 def render_starburst(lens: OpticalSystem, calibration: LensCalibration, sources, src_px, src_py, width: int,
                      height: int, sensor_half_w: float, sensor_half_h: float, config: FlareConfig, fmt=None, ctx=None,
                      debug: bool = False, mdft_precision: int = 0):
-    _check_unsupported(config)
     fw, fh = fmt if fmt is not None else (width, height)
     sarr, n = _sources(sources)
     px = np.ascontiguousarray(src_px, np.float32)

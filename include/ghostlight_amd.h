@@ -130,6 +130,23 @@ typedef struct gl_pair { int32_t surf_a, surf_b; float area_boost; } gl_pair;
 /* Flare source by field angle (FlareSource, ghost.h:21). */
 typedef struct gl_source { float angle_x, angle_y, r, g, b; } gl_source;
 
+/* Front-of-lens occluder (GpuBaffle, baffle.h:33-41): a clipping plane dz mm
+ * downstream of the spawn plane (20 mm ahead of the front vertex), so a
+ * Baffle at z_front_mm ahead of the vertex has dz = 20 - z_front_mm.  A pupil
+ * ray at spawn (px, py) heading (tan_ax, tan_ay) meets it at
+ * (px + tan_ax dz, py + tan_ay dz) and is blocked outside the shape. */
+#define GL_MAX_BAFFLES 8
+enum { GL_BAFFLE_RECT = 0, GL_BAFFLE_CIRCLE = 1 };
+typedef struct gl_baffle {
+    int32_t shape;                  /* GL_BAFFLE_*                                 */
+    float   dz;
+    float   top, bottom, left, right;   /* RECT half-extents (mm)                  */
+    float   cx, cy, radius, aspect;     /* CIRCLE / ellipse (y radius = r*aspect)  */
+} gl_baffle;
+
+/* HURB kick distributions (HurbKickDistribution, render_config.h:118). */
+enum { GL_HURB_GAUSSIAN = 0, GL_HURB_LORENTZIAN = 1 };
+
 /* The FlareConfig / RenderConfig fields the ghost path reads
  * (render_config.h:14-46, 273-337).  Defaults are set by gl_flare_params_init. */
 typedef struct gl_flare_params {
@@ -181,6 +198,15 @@ typedef struct gl_flare_params {
      * output planes then hold n_sources x height x width floats each (the
      * reference needs one call per source for this). */
     int32_t frame_per_source;
+    /* DiffractionConfig parts the ghost trace reads (render_config.h:156-188):
+     * HURB edge diffraction (hurb.h) and the baffle stack, matte box first
+     * (build_gpu_baffles, baffle.h:124-153). */
+    int32_t hurb;                   /* 1: kick each ray at every edge it passes    */
+    int32_t hurb_kick;              /* GL_HURB_*                                   */
+    float   hurb_min_sigma_rad;
+    float   hurb_max_kick_rad;
+    int32_t n_baffles;
+    gl_baffle baffles[GL_MAX_BAFFLES];
 } gl_flare_params;
 
 #define GL_BATCH_GRID        0
@@ -313,6 +339,18 @@ int  gl_blade_profile_build(int32_t blades, float rotation_rad, float curvature,
  * boundary radius at absolute angle theta (1 when blades < 3). */
 float gl_blade_radius_at(const gl_blade_profile* profile, float theta);
 
+/* _hurb_sample_debug (bind_renderers.cpp:457-481): for each edge distance
+ * d[i] the HURB kick scale sigma = lambda K / d and one kick drawn from an
+ * independent stream seeded wang(seed ^ (2654435761 i + 1)) (hurb.h:49-96). */
+int  gl_hurb_sample(float lambda_nm, const float* d_mm, int32_t n, int32_t dist,
+                    float max_kick_rad, uint32_t seed, float* kicks, float* sigma);
+
+/* _aperture_edge_distance_debug (bind_trace.cpp:93): perpendicular distance
+ * from a world-space point on surface s to its clear-aperture edge and the
+ * world-space edge normal (aperture_edge_distance, trace_core.h:206-290). */
+int  gl_aperture_edge_distance(const gl_surface* s, const float hit[3], float* d_mm,
+                               float normal[3]);
+
 /* ---- ghost path -------------------------------------------------------- */
 
 /* enumerate_ghost_pairs (ghost.cpp:17): all active a < b.  Writes at most
@@ -409,6 +447,8 @@ typedef struct gl_diffraction_params {
     int32_t aperture_blades;
     float   aperture_rotation;      /* degrees                                     */
     int32_t mdft_precision;         /* GL_MDFT_*: build-specific (3xBF16 MFMA split) */
+    int32_t n_baffles;              /* hard clip of the pupil (build_pupil_kernel) */
+    gl_baffle baffles[GL_MAX_BAFFLES];
 } gl_diffraction_params;
 
 /* StarburstDebug (diffraction.h:22-39).  Arrays are caller-owned with room

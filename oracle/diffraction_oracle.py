@@ -1,6 +1,7 @@
 """ORACLE — test infrastructure only (tests/, __graft_entry__.smoke(), bench.py).
 numpy FP64 restatement of the reference's diffraction passes:
   build_pupil_kernel        ghostlight/src/starburst_render.cu:155-191
+  baffle_stack_blocks       ghostlight/src/baffle.h:49-73
   psf_from_fft + unit sum   ghostlight/src/starburst_render.cu:202-232, 976-996
   bake_sprite_kernel        ghostlight/src/starburst_render.cu:255-278
   downsample_kernel         ghostlight/src/starburst_render.cu:283-299
@@ -57,8 +58,30 @@ def sample_env(env, u, v):
     return a * (1 - ty) + b * ty
 
 
-def build_pupil(N, fill, normals, env=None):
-    """4x4-supersampled stop x envelope, float32 sample positions as the kernel."""
+def baffles_block(baffles, px, py, tan_ax, tan_ay):
+    """baffle_stack_blocks (baffle.h:62-73) in float32: rows of (shape, dz, top,
+    bottom, left, right, cx, cy, radius, aspect); True where any plane blocks."""
+    f = np.float32
+    px = np.asarray(px, np.float32)
+    py = np.asarray(py, np.float32)
+    out = np.zeros(np.broadcast(px, py).shape, bool)
+    for b in baffles:
+        shape, dz, top, bottom, left, right, cx, cy, radius, aspect = (f(x) for x in b)
+        mx = (px + f(tan_ax) * dz).astype(np.float32)
+        my = (py + f(tan_ay) * dz).astype(np.float32)
+        if int(shape) == 0:
+            out |= (mx > right) | (mx < -left) | (my > top) | (my < -bottom)
+        else:
+            dx = ((mx - cx) / radius).astype(np.float32)
+            dy = ((my - cy) / (radius * aspect)).astype(np.float32)
+            out |= (dx * dx + dy * dy) > f(1.0)
+    return out
+
+
+def build_pupil(N, fill, normals, env=None, baffles=None, ep=(1.0, 1.0), tan_a=(0.0, 0.0)):
+    """4x4-supersampled stop x envelope x baffle clip, float32 sample positions
+    as the kernel; a baffle sees the pupil ray at (u ep_x, v ep_y) heading
+    tan_a."""
     i = np.arange(N, dtype=np.float32)
     acc = np.zeros((N, N), np.float64)
     f = np.float32
@@ -70,6 +93,8 @@ def build_pupil(N, fill, normals, env=None):
             m = mask_contains(U, V, 1.0, normals).astype(np.float64)
             if env is not None:
                 m *= sample_env(env, U.astype(np.float64), V.astype(np.float64))
+            if baffles:
+                m[baffles_block(baffles, U * np.float32(ep[0]), V * np.float32(ep[1]), tan_a[0], tan_a[1])] = 0.0
             acc += m
     return acc / 16.0
 

@@ -145,7 +145,22 @@ struct Job {
     const float* ior_at; const float* ior_before;
     float front_R, start_z;
     float gain, ray_weight;
+    Hurb hurb;                     // edge diffraction (ghost_render.cu:358-392)
+    uint32_t hurb_seed_base = 0;   // jitter_seed * 2654435761 + 1
+    const gl_baffle* baffles = nullptr;
+    int n_baffles = 0;
 };
+
+// The flare parameters' HURB switches and baffle stack into the job.
+inline void job_diffraction(Job& J, const gl_flare_params* p) {
+    J.hurb.on = p->hurb ? 1 : 0;
+    J.hurb.dist = p->hurb_kick;
+    J.hurb.min_sigma = p->hurb_min_sigma_rad;
+    J.hurb.max_kick = p->hurb_max_kick_rad;
+    J.hurb_seed_base = (uint32_t)p->jitter_seed * 2654435761u + 1u;
+    J.baffles = p->baffles;
+    J.n_baffles = std::max(0, std::min((int)p->n_baffles, GL_MAX_BAFFLES));
+}
 
 // Sample position + weight; returns false when the sample contributes nothing.
 inline bool sample_pos(const Job& J, const Conc* conc, int ps, int gi, const gl_sample* grid, int n_grid,
@@ -218,12 +233,35 @@ bool do_sample(const Job& J, const Conc* conc, const Frame& F, int ps, int gi, c
     base.o = V3<S>(S(u * J.front_R + sdx), S(v * J.front_R + sdy), S(J.start_z));
     if (sizeof(S) == 4) base.d = V3<S>(S(bdf.x), S(bdf.y), S(bdf.z));
     else base.d = normalized(V3<S>(S(bx), S(by), S(1)));
+    // Front-of-lens baffles drop the ray before any wavelength (ghost_render.cu:351-356).
+    const float ox = u * J.front_R + sdx, oy = v * J.front_R + sdy;
+    if (J.n_baffles > 0 && baffles_block(J.baffles, J.n_baffles, ox, oy, bx, by)) return false;
+    uint32_t hray = 0u;
+    if (J.hurb.on) {
+        uint32_t axb, ayb;
+        std::memcpy(&axb, &sr.angle_x, 4);
+        std::memcpy(&ayb, &sr.angle_y, 4);
+        hray = axb * 0x9E3779B9u + ayb * 0x85EBCA6Bu + (uint32_t)gi * 0xC2B2AE35u +
+               (uint32_t)pair_idx * 0x27D4EB2Fu + J.hurb_seed_base;
+    }
     bool onsens_any = false;
     for (int si = 0; si < n_spec; ++si) {
         if (STATS) ++C.traces;
         const gl_spec& sp = spec[si];
-        TraceOut<S> res = trace_ghost<S>(J.L, pr.surf_a, pr.surf_b, base, S(sp.lambda),
-                                          iat + (size_t)si * J.L.n, ibf + (size_t)si * J.L.n, false);
+        Ray<S> r = base;
+        uint32_t hs = 0u;
+        if (J.hurb.on) {
+            hs = hurb_wang(hray + (uint32_t)si * 0x9E3779B9u);
+            if (J.n_baffles > 0) {
+                uint32_t bs = hurb_wang(hs ^ 0xBAFF1E5Du);
+                V3<float> bn;
+                const float bd = baffles_edge(J.baffles, J.n_baffles, ox, oy, bx, by, bn);
+                hurb_apply<S>(J.hurb, r.d, V3<S>(S(bn.x), S(bn.y), S(bn.z)), S(bd), S(sp.lambda), bs);
+            }
+        }
+        TraceOut<S> res = trace_ghost<S>(J.L, pr.surf_a, pr.surf_b, r, S(sp.lambda),
+                                          iat + (size_t)si * J.L.n, ibf + (size_t)si * J.L.n, false,
+                                          J.hurb.on ? &J.hurb : nullptr, &hs);
         C.segs += (uint64_t)res.segments;
         if (res.status != GL_TRACE_OK) continue;
         if (!std::isfinite(res.pos.x) || !std::isfinite(res.pos.y)) continue;
@@ -298,6 +336,29 @@ int threads_of(int t) {
 }  // namespace
 
 extern "C" {
+
+// aperture_edge_distance (trace_core.h:206-290) of the oracle, float.
+int orc_edge_distance(const gl_surface* s, const float* hit, float* d, float* nrm) {
+    Lens L;
+    lens_init(L, s, 1, nullptr, 0.0f);
+    V3<float> n;
+    *d = edge_distance<float>(V3<float>(hit[0], hit[1], hit[2]), s[0], L.poly[0], L.transformed[0], n);
+    nrm[0] = n.x; nrm[1] = n.y; nrm[2] = n.z;
+    return 0;
+}
+
+// HURB kicks of the oracle for the _hurb_sample_debug streams.
+int orc_hurb_sample(float lam, const float* dd, int n, int dist, float max_kick, uint32_t seed, float* kicks,
+                    float* sigma) {
+    for (int i = 0; i < n; ++i) {
+        const float sg = hurb_sigma<float>(lam, dd[i], dist);
+        uint32_t st = hurb_wang(seed ^ (2654435761u * (uint32_t)i + 1u));
+        if (st == 0u) st = 1u;
+        kicks[i] = sg > 0.0f ? hurb_kick<float>(dist, sg, max_kick, st) : 0.0f;
+        sigma[i] = sg;
+    }
+    return 0;
+}
 
 // The oracle's blade silhouette as gl_blade_profile, and its r(theta).
 int orc_blade_profile(int blades, float rot, float curvature, float twist, float notch, float rake,
@@ -448,6 +509,7 @@ int orc_render_ghosts(int dbl, const gl_surface* s, int n, const float* arena,
     lens_init(J.L, s, n, arena, p->fp64_radius_min);
     J.pairs = pairs; J.n_pairs = n_pairs; J.src = src; J.n_src = n_src; J.bxy = bxy;
     J.spec = spec; J.n_spec = n_spec;
+    job_diffraction(J, p);
     std::vector<float> at((size_t)n_spec * n), bf((size_t)n_spec * n);
     ior_tables(s, n, spec, n_spec, at.data(), bf.data());
     J.ior_at = at.data(); J.ior_before = bf.data();
@@ -642,6 +704,7 @@ int orc_ghost_stats(const gl_surface* s, int n, const float* arena, const gl_pai
     lens_init(J.L, s, n, arena, p->fp64_radius_min);
     J.pairs = pairs; J.n_pairs = n_pairs; J.src = src; J.n_src = n_src; J.bxy = bxy;
     J.spec = spec; J.n_spec = n_spec;
+    job_diffraction(J, p);
     std::vector<float> at((size_t)n_spec * n), bf((size_t)n_spec * n);
     ior_tables(s, n, spec, n_spec, at.data(), bf.data());
     J.ior_at = at.data(); J.ior_before = bf.data();

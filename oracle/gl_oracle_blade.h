@@ -58,6 +58,35 @@ template <class S> inline S blade_radius(const Blade& b, S theta) {
     return S(1) / blade_u<S>(b, phi - S(b.half));
 }
 
+// ApertureProfile::edge_gap (aperture_profile.h:205-230): first-order
+// distance from normalised (r, theta) to the conic edge, exact to a tip facet,
+// the nearer wins, with the boundary normal.
+template <class S> inline S blade_edge_gap(const Blade& b, S r, S theta, S& nx, S& ny) {
+    S phi = std::fmod(theta - S(b.rotation), S(b.sigma));
+    if (phi < S(0)) phi += S(b.sigma);
+    phi -= S(b.half);
+    const S ct = std::cos(theta), st = std::sin(theta);
+    const S psi = phi - S(b.phi_w);
+    const S sh = std::sin(S(0.5f) * psi);
+    const S q = (psi >= S(0)) ? S(b.qR) : S(b.qL);
+    const S u = S(b.u_w) - S(b.u_rise) * (sh * sh) * q;
+    const S du = S(-0.5f) * S(b.u_rise) * q * std::sin(psi);
+    const S k = (-du / (u * u)) / r;
+    const S inv = S(1) / std::sqrt(S(1) + k * k);
+    S best = (S(1) / u - r) * inv;
+    nx = (ct + k * st) * inv;
+    ny = (st - k * ct) * inv;
+    if (b.facets) {
+        S w = std::fmod((phi - S(b.half)) + S(b.half), S(b.sigma));
+        if (w < S(0)) w += S(b.sigma);
+        w -= S(b.half);
+        const S ang = w + S(b.half) - S(b.beta);
+        const S d = S(b.p) - r * std::cos(ang);
+        if (d < best) { best = d; nx = std::cos(theta - ang); ny = std::sin(theta - ang); }
+    }
+    return best;
+}
+
 inline Blade orc_blade_build(int blades, float rot, float curvature_in, float twist_in, float notch_in,
                              float rake_in) {
     Blade b;

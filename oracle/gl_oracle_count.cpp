@@ -22,6 +22,8 @@ orc::CF exp(orc::CF);
 orc::CF sin(orc::CF);
 orc::CF atan2(orc::CF, orc::CF);
 orc::CF fmod(orc::CF, orc::CF);
+orc::CF tan(orc::CF);
+orc::CF log(orc::CF);
 }  // namespace std
 
 namespace orc {
@@ -62,6 +64,8 @@ orc::CF exp(orc::CF a) { ++orc::g_ops; return orc::CF(std::exp(a.v)); }
 orc::CF sin(orc::CF a) { ++orc::g_ops; return orc::CF(std::sin(a.v)); }
 orc::CF atan2(orc::CF a, orc::CF b) { ++orc::g_ops; return orc::CF(std::atan2(a.v, b.v)); }
 orc::CF fmod(orc::CF a, orc::CF b) { ++orc::g_ops; return orc::CF(std::fmod(a.v, b.v)); }
+orc::CF tan(orc::CF a) { ++orc::g_ops; return orc::CF(std::tan(a.v)); }
+orc::CF log(orc::CF a) { ++orc::g_ops; return orc::CF(std::log(a.v)); }
 }  // namespace std
 
 #include "gl_oracle_trace.h"

@@ -511,6 +511,134 @@ template <class S> inline V3<S> reflect_ray(const V3<S>& dir, const V3<S>& nrm) 
 
 template <class S> struct TraceOut { V3<S> pos; S weight; int status; int segments; };
 
+// ---- HURB edge diffraction (hurb.h:40-150) ---------------------------------
+struct Hurb { int on = 0, dist = GL_HURB_LORENTZIAN; float min_sigma = 1.0e-5f, max_kick = 0.35f; };
+inline uint32_t hurb_wang(uint32_t s) {
+    s = (s ^ 61u) ^ (s >> 16); s *= 9u; s ^= s >> 4; s *= 0x27d4eb2du; s ^= s >> 15; return s;
+}
+inline float hurb_u(uint32_t& st) {
+    st ^= st << 13; st ^= st >> 17; st ^= st << 5;
+    return (float)(st & 0x00FFFFFFu) * (1.0f / 16777216.0f);
+}
+template <class S> inline S hurb_sigma(S lam_nm, S d, int dist) {
+    if (d <= S(0)) return S(0);
+    const S lam_mm = lam_nm * S(1.0e-6f);
+    return lam_mm * S(dist == GL_HURB_LORENTZIAN ? 0.159154943f : 0.112539539f) / d;
+}
+template <class S> inline S hurb_kick(int dist, S sigma, S max_kick, uint32_t& st) {
+    S k;
+    const S pi = S(3.14159265358979323846f);
+    if (dist == GL_HURB_LORENTZIAN) {
+        const S u = S(hurb_u(st));
+        k = sigma * std::tan(pi * (u - S(0.5f)));
+    } else {
+        const S u1 = S(std::fmax(hurb_u(st), 1.0e-7f));
+        const S u2 = S(hurb_u(st));
+        k = sigma * std::sqrt(S(-2.0f) * std::log(u1)) * std::cos(S(2.0f) * pi * u2);
+    }
+    if (k > max_kick) k = max_kick;
+    if (k < -max_kick) k = -max_kick;
+    return k;
+}
+template <class S> inline void hurb_apply(const Hurb& h, V3<S>& dir, const V3<S>& n, S d, S lam, uint32_t& st) {
+    const S sigma = hurb_sigma<S>(lam, d, h.dist);
+    if (sigma <= S(h.min_sigma)) return;
+    const S theta = hurb_kick<S>(h.dist, sigma, S(h.max_kick), st);
+    V3<S> t = n - dir * dot(n, dir);
+    const S tl = std::sqrt(dot(t, t));
+    if (tl < S(1.0e-8f)) return;
+    t = t * (S(1) / tl);
+    V3<S> nd = dir + t * std::tan(theta);
+    const S nl = std::sqrt(dot(nd, nd));
+    if (nl < S(1.0e-12f)) return;
+    dir = nd * (S(1) / nl);
+}
+
+// aperture_edge_distance (trace_core.h:206-290).
+template <class S> inline S edge_distance(const V3<S>& hit, const gl_surface& s, const PolyPre& pp, bool transformed,
+                                          V3<S>& n_out) {
+    if (s.aperture_shape == GL_APERTURE_IMAGE) { n_out = V3<S>(0, 0, 0); return S(1e30f); }
+    V3<S> h = hit;
+    if (transformed) h = m_tmul<S>(s.rot, hit - V3<S>(S(s.decenter_x), S(s.decenter_y), S(s.z))) + V3<S>(0, 0, S(s.z));
+    const S sa = S(s.semi_aperture);
+    const S hx = h.x / S(s.aperture_aspect), hy = h.y;
+    const S r_s = std::sqrt(hx * hx + hy * hy);
+    S d;
+    V3<S> n(0, 0, 0);
+    if (s.aperture_shape == GL_APERTURE_POLYGON && s.aperture_blades >= 3) {
+        const S sector = S(2.0f * (float)M_PI / (float)s.aperture_blades);
+        const S apothem = sa * std::cos(S(0.5f) * sector);
+        if (r_s < S(1e-9f)) {
+            d = pp.blade.deformed ? sa * S(pp.blade.r_w) : apothem;
+            n = V3<S>(1, 0, 0);
+        } else if (pp.blade.deformed) {
+            const S theta = std::atan2(hy, hx);
+            S nx, ny;
+            d = sa * blade_edge_gap<S>(pp.blade, r_s / sa, theta, nx, ny);
+            n = V3<S>(nx, ny, 0);
+        } else {
+            const S theta = std::atan2(hy, hx);
+            S off = std::fmod(theta - S(s.aperture_rotation_rad), sector);
+            if (off < S(0)) off += sector;
+            off -= S(0.5f) * sector;
+            d = apothem - r_s * std::cos(off);
+            const S phi = theta - off;
+            n = V3<S>(std::cos(phi), std::sin(phi), 0);
+        }
+    } else {
+        const S asp2 = S(s.aperture_aspect) * S(s.aperture_aspect);
+        const S gx = h.x / asp2, gy = h.y;
+        const S glen = std::sqrt(gx * gx + gy * gy);
+        if (glen < S(1e-12f)) { d = sa; n = V3<S>(1, 0, 0); }
+        else { d = (sa - r_s) * r_s / glen; n = V3<S>(gx / glen, gy / glen, 0); }
+    }
+    if (d < S(0)) d = S(0);
+    n_out = transformed ? m_mul<S>(s.rot, n) : n;
+    return d;
+}
+
+// ---- front-of-lens baffles (baffle.h:49-113) -------------------------------
+inline bool baffles_block(const gl_baffle* b, int nb, float px, float py, float tx, float ty) {
+    for (int k = 0; k < nb; ++k) {
+        const float mx = px + tx * b[k].dz, my = py + ty * b[k].dz;
+        if (b[k].shape == GL_BAFFLE_RECT) {
+            if (mx > b[k].right || mx < -b[k].left || my > b[k].top || my < -b[k].bottom) return true;
+        } else {
+            const float dx = (mx - b[k].cx) / b[k].radius, dy = (my - b[k].cy) / (b[k].radius * b[k].aspect);
+            if (dx * dx + dy * dy > 1.0f) return true;
+        }
+    }
+    return false;
+}
+inline float baffles_edge(const gl_baffle* b, int nb, float px, float py, float tx, float ty, V3<float>& n_out) {
+    float best = 1e30f;
+    V3<float> bn(0, 0, 0);
+    for (int k = 0; k < nb; ++k) {
+        const float mx = px + tx * b[k].dz, my = py + ty * b[k].dz;
+        float d; V3<float> n;
+        if (b[k].shape == GL_BAFFLE_RECT) {
+            d = b[k].right - mx; n = V3<float>(1, 0, 0);
+            const float dl = mx + b[k].left, dt = b[k].top - my, db = my + b[k].bottom;
+            if (dl < d) { d = dl; n = V3<float>(-1, 0, 0); }
+            if (dt < d) { d = dt; n = V3<float>(0, 1, 0); }
+            if (db < d) { d = db; n = V3<float>(0, -1, 0); }
+        } else {
+            const float rr = b[k].radius > 1e-12f ? b[k].radius : 1e-12f;
+            const float ra = rr * (b[k].aspect > 0.0f ? b[k].aspect : 1.0f);
+            const float ex = mx - b[k].cx, ey = my - b[k].cy;
+            const float g = std::sqrt((ex / rr) * (ex / rr) + (ey / ra) * (ey / ra));
+            const float gx = ex / (rr * rr), gy = ey / (ra * ra);
+            const float gl = std::sqrt(gx * gx + gy * gy);
+            if (gl < 1e-20f) { d = rr; n = V3<float>(1, 0, 0); }
+            else { d = (1.0f - g) / gl; n = V3<float>(gx / gl, gy / gl, 0); }
+        }
+        if (d < best) { best = d; bn = n; }
+    }
+    if (best < 0.0f) best = 0.0f;
+    n_out = bn;
+    return best;
+}
+
 // One surface interaction; returns false when the ray died (status set).
 template <class S>
 inline bool step(const Lens& L, int k, Ray<S>& ray, S lambda, S n2, bool reflecting, S reflect_ior,
@@ -548,23 +676,35 @@ template <class S> inline void to_sensor(const Ray<S>& ray, bool t_ge0, TraceOut
 }
 
 // Three-leg ghost trace.  ior_at / ior_before: this lambda's slices.
+// hurb (may be null): kick the new direction after every surface on the
+// stream *hs (d_trace_ghost_ray<HURB>, trace_cuda.h:248-361).
 template <class S>
 inline TraceOut<S> trace_ghost(const Lens& L, int a, int b, Ray<S> ray, S lambda,
-                               const float* ior_at, const float* ior_before, bool t_ge0) {
+                               const float* ior_at, const float* ior_before, bool t_ge0,
+                               const Hurb* hurb = nullptr, uint32_t* hs = nullptr) {
     TraceOut<S> res; res.status = GL_TRACE_VIGNETTED; res.weight = S(1); res.segments = 0;
     if (a < 0 || b <= a || b >= L.n) { res.status = GL_TRACE_INVALID_INPUT; res.weight = S(0); return res; }
+    auto kick = [&](int k) {
+        if (!hurb || !hurb->on) return;
+        V3<S> en;
+        const S ed = edge_distance<S>(ray.o, L.s[k], L.poly[k], L.transformed[k], en);
+        hurb_apply<S>(*hurb, ray.d, en, ed, lambda, *hs);
+    };
     S cur = S(1);
     for (int k = 0; k <= b; ++k) {
         if (!L.s[k].is_active) continue;
         if (!step<S>(L, k, ray, lambda, S(ior_at[k]), k == b, cur, cur, res)) return res;
+        kick(k);
     }
     for (int k = b - 1; k >= a; --k) {
         if (!L.s[k].is_active) continue;
         if (!step<S>(L, k, ray, lambda, S(ior_before[k]), k == a, S(ior_at[a]), cur, res)) return res;
+        kick(k);
     }
     for (int k = a + 1; k < L.n; ++k) {
         if (!L.s[k].is_active) continue;
         if (!step<S>(L, k, ray, lambda, S(ior_at[k]), false, S(0), cur, res)) return res;
+        kick(k);
     }
     to_sensor(ray, t_ge0, res);
     return res;
