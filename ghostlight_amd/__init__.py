@@ -7,7 +7,7 @@ behind a C ABI (include/ghostlight_amd.h), with the reference's Python API
 from .config import (BaffleShape, Baffle, DiffractionConfig, FlareConfig, GateConfig, GateLobe, GhostAovMode,
                      GhostFilter, HurbKickDistribution, InputColorSpace, MatteBox, OutputColorSpace,
                      PointFlareConfig, PSFCenterMode, PSFConfig, RenderConfig, SensorModel, StarburstEngine)
-from .lens import (ApertureShape, Coating, CoatingModel, CylinderAxis, DispersionModel, LensCalibration,
+from .lens import (ApertureImage, ApertureShape, Coating, CoatingModel, CylinderAxis, DispersionModel, LensCalibration,
                    OpticalSystem, Surface, SurfaceForm, TraceStatus)
 from .runtime import Context, device_available
 from .render import (GhostPair, Ray, TraceResult, enumerate_ghost_pairs, filter_ghost_pairs, render_ghosts,

@@ -52,6 +52,8 @@ class gl_surface(C.Structure):
         ("sa_n_angles", C.c_int32),
         ("aperture_curvature", C.c_float), ("aperture_twist", C.c_float),
         ("aperture_notch_rad", C.c_float), ("aperture_notch_angle_rad", C.c_float),
+        ("aperture_semi_diameter", C.c_float), ("image_offset", C.c_int32), ("image_width", C.c_int32),
+        ("image_height", C.c_int32),
     ]
 
 
@@ -172,6 +174,7 @@ class gl_diffraction_params(C.Structure):
         ("render_spectral_samples", C.c_int32), ("sensor_model", C.c_int32), ("output_cs", C.c_int32),
         ("custom_xyz_to_output", C.c_float * 9), ("aperture_blades", C.c_int32), ("aperture_rotation", C.c_float),
         ("mdft_precision", C.c_int32), ("n_baffles", C.c_int32), ("baffles", gl_baffle * GL_MAX_BAFFLES),
+        ("use_surface_textures", C.c_int32),
     ]
 
 

@@ -282,6 +282,11 @@ struct PupilExtra {
     int ME;
     float ep_x, ep_y, tan_ax, tan_ay;
     DevBaffles bf;
+    // front-glass IMAGE aperture as graded transmission: the pupil ray crosses
+    // that surface tex_d0 downstream of the spawn plane
+    const float* tex;
+    int img_w, img_h;
+    float tex_aspect, tex_semi, tex_d0;
 };
 
 // Effective pupil A(u,v): 4x4-supersampled stop silhouette x cat's-eye
@@ -304,6 +309,13 @@ __global__ __launch_bounds__(256) void pupil_kernel(float2* __restrict__ out, in
             float a = 1.0f;
             if (env != nullptr) a *= sample_env(env, ME, u, v);
             if (ex.bf.n > 0 && baffles_block(ex.bf, u * ex.ep_x, v * ex.ep_y, ex.tan_ax, ex.tan_ay)) continue;
+            if (ex.tex != nullptr) {
+                const float hx = (u * ex.ep_x + ex.tan_ax * ex.tex_d0) / ex.tex_aspect;
+                const float hy = v * ex.ep_y + ex.tan_ay * ex.tex_d0;
+                const float tu = 0.5f + hx / (2.0f * ex.tex_semi);
+                const float tv = 0.5f + hy / (2.0f * ex.tex_semi);
+                a *= image_sample(ex, ex.tex, tu, tv);
+            }
             re += a;
         }
     out[idx] = make_float2(re * (1.0f / 16.0f), 0.0f);
@@ -878,7 +890,27 @@ int gl_render_starburst(gl_ctx* ctx, const gl_lens* lens, const gl_calibration* 
     eff.tan_ax = std::tan(field_ax); eff.tan_ay = std::tan(field_ay);
     eff.bf.n = std::max(0, std::min((int)p->n_baffles, GL_MAX_BAFFLES));
     for (int k = 0; k < eff.bf.n; ++k) eff.bf.b[k] = p->baffles[k];
-    if (d_env || eff.bf.n > 0) {
+    if (p->use_surface_textures) {
+        // the first surface carrying a loaded bitmap (starburst_render.cu:807-846)
+        const std::vector<gl_surface>& hs = L->host;
+        for (size_t k = 0; k < hs.size(); ++k) {
+            const gl_surface& g = hs[k];
+            if (g.aperture_shape != GL_APERTURE_IMAGE || g.image_width <= 0 || g.image_height <= 0) continue;
+            if ((long long)g.image_offset + (long long)g.image_width * g.image_height > (long long)L->arena.size())
+                break;
+            if (g.aperture_semi_diameter > 0.0f) {
+                LensDev D;
+                if ((st = lens_sync(L, 0.0f, c->stream, &D))) return st;
+                eff.tex = D.arena + g.image_offset;
+                eff.img_w = g.image_width; eff.img_h = g.image_height;
+                eff.tex_aspect = g.aperture_aspect > 0.0f ? g.aperture_aspect : 1.0f;
+                eff.tex_semi = g.aperture_semi_diameter;
+                eff.tex_d0 = g.z - (hs[0].z - 20.0f);
+            }
+            break;
+        }
+    }
+    if (d_env || eff.bf.n > 0 || eff.tex) {
         hipLaunchKernelGGL(pupil_kernel, dim3(blocks), dim3(256), 0, c->stream, bp.as<float2>(), N, fill, mask, plain);
         if ((st = reduce_to(c, bp.as<float>(), NN, 1, dsum))) return st;
         hipLaunchKernelGGL(pupil_kernel, dim3(blocks), dim3(256), 0, c->stream, bp.as<float2>(), N, fill, mask, eff);

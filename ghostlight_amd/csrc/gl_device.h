@@ -36,6 +36,7 @@ enum : int32_t {
     SF_TAB_SORTED = 128,      // 1-D coating table keys non-decreasing (binary search allowed)
     SF_SA_SORTED = 256,       // spectral-angular wavelength and angle axes non-decreasing
     SF_DEFORMED = 512,        // polygon with authored blade shape: test r <= sa * r(theta)
+    SF_IMAGE = 1024,          // IMAGE aperture with a loaded bitmap (img_* below)
 };
 
 // Blade silhouette r(theta) of a deformed polygon (gl_blade_profile as the
@@ -74,6 +75,8 @@ struct DevSurf {
     DevBlade blade;                  // SF_DEFORMED only
     float ap_rot;                    // polygon rotation (radians): HURB edge distance
     int32_t ap_blades;               // polygon blade count (0: round, -1: image aperture)
+    int32_t img_off, img_w, img_h;   // SF_IMAGE: bitmap at arena[img_off], img_w x img_h
+    float img_semi;                  // SF_IMAGE: world radius of the bitmap's edge (mm)
 };
 
 struct TraceRes {
@@ -300,6 +303,37 @@ __device__ __noinline__ bool blade_inside(const DevSurf GLC& s, float hx, float 
     return r <= s.sa * blade_radius_at(s.blade, atan2f(hy, hx));
 }
 
+// Bitmap transmission at normalised (u, v): bilinear between texel centres
+// with clamped edges (tex2D with linear filtering, normalised coordinates and
+// clamp addressing, as the reference samples its aperture textures).
+template <class SV> __host__ __device__ inline float image_sample(const SV& s, const float* img, float u, float v) {
+    const int W = s.img_w, H = s.img_h;
+    const float x = u * (float)W - 0.5f, y = v * (float)H - 0.5f;
+    const float fx = floorf(x), fy = floorf(y);
+    const float a = x - fx, b = y - fy;
+    int i0 = (int)fx, j0 = (int)fy;
+    int i1 = i0 + 1, j1 = j0 + 1;
+    i0 = i0 < 0 ? 0 : (i0 > W - 1 ? W - 1 : i0);
+    i1 = i1 < 0 ? 0 : (i1 > W - 1 ? W - 1 : i1);
+    j0 = j0 < 0 ? 0 : (j0 > H - 1 ? H - 1 : j0);
+    j1 = j1 < 0 ? 0 : (j1 > H - 1 ? H - 1 : j1);
+    const float top = (1.0f - a) * img[j0 * W + i0] + a * img[j0 * W + i1];
+    const float bot = (1.0f - a) * img[j1 * W + i0] + a * img[j1 * W + i1];
+    return (1.0f - b) * top + b * bot;
+}
+
+// d_image_aperture_passes (trace_cuda.h:37-54): the bitmap at the surface-
+// frame hit must exceed 0.5.
+GLD bool image_passes(const DevSurf GLC& s, const float* __restrict__ arena, F3 hit) {
+    if (!(s.aspect > 0.0f) || !(s.img_semi > 0.0f)) return false;
+    F3 local = hit;
+    if (s.flags & SF_TRANSFORMED) local = add(m_tmul(s.rot, sub(hit, f3(s.dx, s.dy, s.z))), f3(0.f, 0.f, s.z));
+    const float hx = local.x / s.aspect;
+    const float u = 0.5f + hx / (2.0f * s.img_semi);
+    const float v = 0.5f + local.y / (2.0f * s.img_semi);
+    return image_sample(s, arena + s.img_off, u, v) > 0.5f;
+}
+
 // check_aperture (trace_core.h:133-170); the regular polygon is the
 // half-plane intersection max_k(h . n_k) <= apothem (same set as the
 // reference's r <= apothem / cos(offset)).
@@ -508,6 +542,7 @@ GLD bool step(const DevSurf GLC& s, const float GLC* poly, const float* __restri
     if (!intersect(o, d, s, poly, t, nrm)) return false;
     F3 hit = add(o, mul(d, t));
     o = hit;
+    if ((s.flags & SF_IMAGE) && !image_passes(s, arena, hit)) return false;
     const float n1 = cur;
     if (reflecting) {
         if (W) {

@@ -365,6 +365,12 @@ void build_devsurfs(const gl_surface* s, int n, float fp64_rmin, const float* ar
         d.nblades = 0; d.poly_off = 0; d.apothem = 0.0f;
         if (deformed_blade(g, &d.blade)) f |= SF_DEFORMED;
         d.ap_rot = g.aperture_rotation_rad;
+        if (g.aperture_shape == GL_APERTURE_IMAGE && g.image_width > 0 && g.image_height > 0 && g.image_offset >= 0 &&
+            arena != nullptr && (long long)g.image_offset + (long long)g.image_width * g.image_height <= arena_len) {
+            f |= SF_IMAGE;
+            d.img_off = g.image_offset; d.img_w = g.image_width; d.img_h = g.image_height;
+            d.img_semi = g.aperture_semi_diameter;
+        }
         d.ap_blades = (g.aperture_shape == GL_APERTURE_IMAGE) ? -1
                       : ((g.aperture_shape == GL_APERTURE_POLYGON && g.aperture_blades >= 3) ? g.aperture_blades : 0);
         if (g.aperture_shape == GL_APERTURE_POLYGON && g.aperture_blades >= 3 && g.aperture_blades <= GL_MAX_BLADES) {

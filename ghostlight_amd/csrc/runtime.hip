@@ -249,8 +249,10 @@ int gl_lens_create(const gl_surface* s, int32_t n, const float* arena, int32_t a
             (!arena || g.sa_offset < 0 ||
              g.sa_offset + g.sa_n_wavelengths + g.sa_n_angles + g.sa_n_wavelengths * g.sa_n_angles > arena_len))
             return gl::set_error(GL_ERR_INVALID, "surface %d: spectral-angular table outside the arena", i);
-        if (g.aperture_shape == GL_APERTURE_IMAGE)
-            return gl::set_error(GL_ERR_UNSUPPORTED, "surface %d: image apertures are not part of this build", i);
+        if (g.aperture_shape == GL_APERTURE_IMAGE && g.image_width > 0 &&
+            (!arena || g.image_offset < 0 || g.image_height <= 0 ||
+             (long long)g.image_offset + (long long)g.image_width * g.image_height > (long long)arena_len))
+            return gl::set_error(GL_ERR_INVALID, "surface %d: aperture bitmap outside the arena", i);
     }
     gl_lens* L = new gl_lens();
     L->host.assign(s, s + n);

@@ -46,6 +46,7 @@ def diffraction_params(cfg: FlareConfig, mdft_precision: int = 0):
     p.aperture_rotation = float(cfg.aperture_rotation)
     p.mdft_precision = int(mdft_precision)
     p.n_baffles = fill_baffles(d, p.baffles)
+    p.use_surface_textures = int(bool(d.use_surface_textures))
     return p
 
 

@@ -176,6 +176,26 @@ class LensCalibration:
         return cls(**{k: float(v) for k, v in zip(cls.FIELDS, a)})
 
 
+@dataclass
+class ApertureImage:
+    """Bitmap behind an IMAGE aperture (ApertureImage, optical_system.h:262-269):
+    row-major [0, 1] luminance, height x width; empty pixels = not loaded (the
+    surface then passes every ray).  The loader fills source_path and
+    semi_diameter; the caller decodes the file and sets the pixels."""
+    width: int = 0
+    height: int = 0
+    pixels: Optional[np.ndarray] = None
+    semi_diameter: float = 0.0
+    source_path: str = ""
+
+    def set_pixels(self, img) -> None:
+        a = np.ascontiguousarray(img, np.float32)
+        if a.ndim != 2:
+            raise ValueError("aperture image must be a 2-D (height, width) array")
+        self.height, self.width = a.shape
+        self.pixels = a.ravel().copy()
+
+
 class OpticalSystem:
     """Flat surface sequence with the sensor at z = 0 (optical_system.h:305-416)."""
 
@@ -183,6 +203,8 @@ class OpticalSystem:
         self.name = name
         self.focal_length = 0.0
         self.surfaces: List[Surface] = list(surfaces or [])
+        # one per surface, parallel to `surfaces` (OpticalSystem::aperture_images)
+        self.aperture_images: List[ApertureImage] = [ApertureImage() for _ in self.surfaces]
         self._dev = None        # cached device handle (gl_lens*) keyed by pack digest
         self._dev_key = None
 
@@ -297,6 +319,16 @@ class OpticalSystem:
                 r = np.asarray(c.sa_r, np.float32).reshape(len(wl), len(an))
                 g.sa_offset, g.sa_n_wavelengths, g.sa_n_angles = len(arena), len(wl), len(an)
                 arena.extend(wl.tolist()); arena.extend(an.tolist()); arena.extend(r.ravel().tolist())
+            g.aperture_semi_diameter = float(s.aperture_semi_diameter)
+            g.image_offset = g.image_width = g.image_height = 0
+            img = self.aperture_images[i] if i < len(self.aperture_images) else None
+            if (s.aperture_shape == ApertureShape.IMAGE and img is not None and img.pixels is not None
+                    and len(img.pixels)):
+                px = np.asarray(img.pixels, np.float32).ravel()
+                if img.width <= 0 or img.height <= 0 or len(px) != img.width * img.height:
+                    raise ValueError(f"surface {i}: aperture image dimensions do not match its pixel data")
+                g.image_offset, g.image_width, g.image_height = len(arena), int(img.width), int(img.height)
+                arena.extend(px.tolist())
         return arr, np.asarray(arena if arena else [0.0], np.float32)
 
     @classmethod
@@ -331,12 +363,19 @@ class OpticalSystem:
                                aperture_curvature=g.aperture_curvature, aperture_twist=g.aperture_twist,
                                aperture_notch_rad=g.aperture_notch_rad,
                                aperture_notch_angle_rad=g.aperture_notch_angle_rad,
+                               aperture_semi_diameter=g.aperture_semi_diameter,
                                rot=tuple(g.rot[k] for k in range(9)), form=SurfaceForm(g.form), conic_k=g.conic_k,
                                asphere_terms=tuple(g.asphere_terms[k] for k in range(g.n_asphere_terms)),
                                cyl_axis=CylinderAxis(g.cyl_axis), disp_model=DispersionModel(g.disp_model),
                                sellmeier_B=tuple(g.sellmeier_B[k] for k in range(3)),
                                sellmeier_C=tuple(g.sellmeier_C[k] for k in range(3))))
         lens = cls(out, name=name)
+        for i, g in enumerate(surfaces):
+            if g.image_width > 0 and g.image_height > 0:
+                im = lens.aperture_images[i]
+                im.semi_diameter = g.aperture_semi_diameter
+                im.set_pixels(ar[g.image_offset:g.image_offset + g.image_width * g.image_height]
+                              .reshape(g.image_height, g.image_width))
         return lens
 
     def pack_key(self) -> bytes:

@@ -35,8 +35,8 @@ from typing import Dict, List, Optional, Tuple
 
 import numpy as np
 
-from .lens import (ApertureShape, Coating, CoatingModel, CylinderAxis, DispersionModel, OpticalSystem, Surface,
-                   SurfaceForm)
+from .lens import (ApertureImage, ApertureShape, Coating, CoatingModel, CylinderAxis, DispersionModel, OpticalSystem,
+                   Surface, SurfaceForm)
 
 f32 = np.float32
 LENS_FORMAT_MAJOR = 1
@@ -582,7 +582,8 @@ def load_lens(path) -> OpticalSystem:
     lens = OpticalSystem(surfaces, name=name)
     lens.focal_length = focal
     lens.surface_ids = ids
-    lens.aperture_images = images
+    lens.aperture_images = [ApertureImage(semi_diameter=im["semi_diameter"], source_path=im["source_path"])
+                            if im else ApertureImage() for im in images]
     # Post-flatten: bake physical layer stacks now that every medium is final.
     for i, specs in enumerate(layer_specs):
         if specs:

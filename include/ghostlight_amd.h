@@ -96,6 +96,13 @@ typedef struct gl_surface {
     float   aperture_twist;           /* [-1, +1] waist position along the edge         */
     float   aperture_notch_rad;       /* [-pi/4, +pi/4] tip facet                       */
     float   aperture_notch_angle_rad; /* [0, pi/4] facet skew                           */
+    /* IMAGE aperture (Surface::aperture_semi_diameter + ApertureImage,
+     * optical_system.h:183, 262-269): row-major [0,1] luminance bitmap of
+     * image_width x image_height floats at arena[image_offset]; width 0 = not
+     * loaded (the surface passes every ray) */
+    float   aperture_semi_diameter;   /* world radius at the bitmap's edge (mm)          */
+    int32_t image_offset;
+    int32_t image_width, image_height;
 } gl_surface;
 
 /* Derived blade silhouette of a bladed stop (ApertureProfile,
@@ -449,6 +456,8 @@ typedef struct gl_diffraction_params {
     int32_t mdft_precision;         /* GL_MDFT_*: build-specific (3xBF16 MFMA split) */
     int32_t n_baffles;              /* hard clip of the pupil (build_pupil_kernel) */
     gl_baffle baffles[GL_MAX_BAFFLES];
+    int32_t use_surface_textures;   /* fold the first loaded IMAGE aperture's bitmap
+                                     * into the pupil as graded transmission      */
 } gl_diffraction_params;
 
 /* StarburstDebug (diffraction.h:22-39).  Arrays are caller-owned with room

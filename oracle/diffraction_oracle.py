@@ -78,10 +78,28 @@ def baffles_block(baffles, px, py, tan_ax, tan_ay):
     return out
 
 
-def build_pupil(N, fill, normals, env=None, baffles=None, ep=(1.0, 1.0), tan_a=(0.0, 0.0)):
-    """4x4-supersampled stop x envelope x baffle clip, float32 sample positions
-    as the kernel; a baffle sees the pupil ray at (u ep_x, v ep_y) heading
-    tan_a."""
+def image_bilinear(img, u, v):
+    """tex2D with linear filtering, normalised coordinates and clamp addressing
+    (texel centres at (i + 1/2) / W), float32 operations in the device's order."""
+    f = np.float32
+    H, W = img.shape
+    x = (np.asarray(u, np.float32) * f(W) - f(0.5)).astype(np.float32)
+    y = (np.asarray(v, np.float32) * f(H) - f(0.5)).astype(np.float32)
+    fx, fy = np.floor(x), np.floor(y)
+    a, b = (x - fx).astype(np.float32), (y - fy).astype(np.float32)
+    i0 = np.clip(fx.astype(np.int64), 0, W - 1); i1 = np.clip(fx.astype(np.int64) + 1, 0, W - 1)
+    j0 = np.clip(fy.astype(np.int64), 0, H - 1); j1 = np.clip(fy.astype(np.int64) + 1, 0, H - 1)
+    img = np.asarray(img, np.float32)
+    top = ((f(1) - a) * img[j0, i0] + a * img[j0, i1]).astype(np.float32)
+    bot = ((f(1) - a) * img[j1, i0] + a * img[j1, i1]).astype(np.float32)
+    return ((f(1) - b) * top + b * bot).astype(np.float32)
+
+
+def build_pupil(N, fill, normals, env=None, baffles=None, ep=(1.0, 1.0), tan_a=(0.0, 0.0), tex=None):
+    """4x4-supersampled stop x envelope x baffle clip x front bitmap, float32
+    sample positions as the kernel; a baffle sees the pupil ray at
+    (u ep_x, v ep_y) heading tan_a; tex = (image, aspect, semi, d0) is sampled
+    where that ray crosses the bitmap's surface d0 downstream."""
     i = np.arange(N, dtype=np.float32)
     acc = np.zeros((N, N), np.float64)
     f = np.float32
@@ -95,6 +113,14 @@ def build_pupil(N, fill, normals, env=None, baffles=None, ep=(1.0, 1.0), tan_a=(
                 m *= sample_env(env, U.astype(np.float64), V.astype(np.float64))
             if baffles:
                 m[baffles_block(baffles, U * np.float32(ep[0]), V * np.float32(ep[1]), tan_a[0], tan_a[1])] = 0.0
+            if tex is not None:
+                img, t_asp, t_semi, t_d0 = tex
+                g = np.float32
+                hx = ((U * g(ep[0]) + g(tan_a[0]) * g(t_d0)) / g(t_asp)).astype(np.float32)
+                hy = (V * g(ep[1]) + g(tan_a[1]) * g(t_d0)).astype(np.float32)
+                tu = (g(0.5) + hx / (g(2.0) * g(t_semi))).astype(np.float32)
+                tv = (g(0.5) + hy / (g(2.0) * g(t_semi))).astype(np.float32)
+                m *= image_bilinear(img, tu, tv).astype(np.float64)
             acc += m
     return acc / 16.0
 

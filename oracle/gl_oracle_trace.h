@@ -639,6 +639,33 @@ inline float baffles_edge(const gl_baffle* b, int nb, float px, float py, float 
     return best;
 }
 
+// d_image_aperture_passes (trace_cuda.h:37-54): bitmap transmission at the
+// surface-frame hit, bilinear between texel centres with clamped edges (the
+// reference's tex2D, linear filter, normalised coordinates, clamp), > 0.5.
+// No bitmap loaded: every ray passes.
+inline float image_bilinear(const float* img, int W, int H, float u, float v) {
+    const float x = u * (float)W - 0.5f, y = v * (float)H - 0.5f;
+    const float fx = std::floor(x), fy = std::floor(y);
+    const float a = x - fx, b = y - fy;
+    auto cl = [](int i, int n) { return i < 0 ? 0 : (i > n - 1 ? n - 1 : i); };
+    const int i0 = cl((int)fx, W), i1 = cl((int)fx + 1, W), j0 = cl((int)fy, H), j1 = cl((int)fy + 1, H);
+    const float top = (1.0f - a) * img[j0 * W + i0] + a * img[j0 * W + i1];
+    const float bot = (1.0f - a) * img[j1 * W + i0] + a * img[j1 * W + i1];
+    return (1.0f - b) * top + b * bot;
+}
+template <class S> inline bool image_aperture_passes(const Lens& L, int k, const V3<S>& hit) {
+    const gl_surface& s = L.s[k];
+    if (s.aperture_shape != GL_APERTURE_IMAGE || s.image_width <= 0 || s.image_height <= 0 || !L.arena) return true;
+    if (!(s.aperture_aspect > 0.0f) || !(s.aperture_semi_diameter > 0.0f)) return false;
+    V3<float> local((float)hit.x, (float)hit.y, (float)hit.z);
+    if (L.transformed[k])
+        local = m_tmul<float>(s.rot, local - V3<float>(s.decenter_x, s.decenter_y, s.z)) + V3<float>(0, 0, s.z);
+    const float hx = local.x / s.aperture_aspect;
+    const float u = 0.5f + hx / (2.0f * s.aperture_semi_diameter);
+    const float v = 0.5f + local.y / (2.0f * s.aperture_semi_diameter);
+    return image_bilinear(L.arena + s.image_offset, s.image_width, s.image_height, u, v) > 0.5f;
+}
+
 // One surface interaction; returns false when the ray died (status set).
 template <class S>
 inline bool step(const Lens& L, int k, Ray<S>& ray, S lambda, S n2, bool reflecting, S reflect_ior,
@@ -649,6 +676,7 @@ inline bool step(const Lens& L, int k, Ray<S>& ray, S lambda, S n2, bool reflect
     if (!intersect_surface<S>(ray, s, L.poly[k], L.transformed[k], L.fp64_rmin, t, nrm)) return false;
     V3<S> hit = ray.o + ray.d * t;
     ray.o = hit;
+    if (!image_aperture_passes<S>(L, k, hit)) return false;
     const S n1 = cur_ior;
     if (reflecting) {
         res.weight *= surface_weight<S>(ray.d, nrm, n1, n2, s, L.arena, lambda, true);

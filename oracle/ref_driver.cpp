@@ -37,6 +37,7 @@ OpticalSystem make_lens(const gl_surface* s, int n, const float* arena) {
         d.aperture_rotation_rad = g.aperture_rotation_rad; d.aperture_aspect = g.aperture_aspect;
         d.aperture_curvature = g.aperture_curvature; d.aperture_twist = g.aperture_twist;
         d.aperture_notch_rad = g.aperture_notch_rad; d.aperture_notch_angle_rad = g.aperture_notch_angle_rad;
+        d.aperture_semi_diameter = g.aperture_semi_diameter;
         d.decenter_x = g.decenter_x; d.decenter_y = g.decenter_y;
         std::memcpy(d.rot, g.rot, sizeof d.rot);
         d.form = g.form; d.conic_k = g.conic_k;
@@ -87,6 +88,7 @@ int export_lens(const OpticalSystem& L, gl_surface* out, int cap, float* arena, 
         g.aperture_rotation_rad = d.aperture_rotation_rad; g.aperture_aspect = d.aperture_aspect;
         g.aperture_curvature = d.aperture_curvature; g.aperture_twist = d.aperture_twist;
         g.aperture_notch_rad = d.aperture_notch_rad; g.aperture_notch_angle_rad = d.aperture_notch_angle_rad;
+        g.aperture_semi_diameter = d.aperture_semi_diameter;
         g.decenter_x = d.decenter_x; g.decenter_y = d.decenter_y;
         std::memcpy(g.rot, d.rot, sizeof g.rot);
         g.form = d.form; g.conic_k = d.conic_k;

@@ -1,0 +1,93 @@
+"""CPU: IMAGE apertures (a bitmap mask on a surface, trace_cuda.h:37-54) in the
+oracle and the lens packing.
+
+* A bitmap not loaded passes every ray: the surface acts as its clear
+  semi-aperture (the reference's pass-through state).
+* A disc bitmap of radius rho semi_diameter vignettes as a circular stop of
+  that radius, up to the rays whose stop hit falls in the one-texel band
+  where the bilinear sample crosses 0.5.
+* The bitmap travels through pack / from_packed unchanged."""
+import math
+import os
+
+import numpy as np
+
+from ghostlight_amd import ApertureShape, OpticalSystem
+from oracle import pyoracle
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LENS = os.path.join(HERE, "golden", "lenses", "DoubleGauss.lens")
+
+
+def _disc(n, rho):
+    c = (np.arange(n) + 0.5) / n * 2 - 1
+    X, Y = np.meshgrid(c, c)
+    return (np.hypot(X, Y) < rho).astype(np.float32)
+
+
+def _rays(lens, n, seed):
+    rng = np.random.default_rng(seed)
+    fR = lens.surfaces[0].semi_aperture
+    z0 = lens.surfaces[0].z - 20.0
+    ang = np.radians(rng.uniform(-4, 4, (n, 2)))
+    d = np.stack([np.tan(ang[:, 0]), np.tan(ang[:, 1]), np.ones(n)], 1)
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    r = np.zeros((n, 7), np.float32)
+    r[:, :2] = rng.uniform(-1, 1, (n, 2)) * fR - 20.0 * d[:, :2] / d[:, 2:3]
+    r[:, 2] = z0
+    r[:, 3:6] = d
+    r[:, 6] = 587.56
+    return r
+
+
+def _image_stop(lens, pixels=None):
+    k = next(i for i, s in enumerate(lens.surfaces) if s.is_stop)
+    s = lens.surfaces[k]
+    s.aperture_shape = ApertureShape.IMAGE
+    s.aperture_semi_diameter = s.semi_aperture
+    if pixels is not None:
+        lens.aperture_images[k].set_pixels(pixels)
+        lens.aperture_images[k].semi_diameter = s.semi_aperture
+    return k
+
+
+def test_unloaded_bitmap_passes_like_the_clear_aperture():
+    ref = OpticalSystem.load(LENS)
+    img = OpticalSystem.load(LENS)
+    _image_stop(img)
+    rays = _rays(ref, 3000, 1)
+    _p0, _w0, s0, g0 = pyoracle.trace_ghost(ref, -1, -1, rays, t_ge0=True)
+    _p1, _w1, s1, g1 = pyoracle.trace_ghost(img, -1, -1, rays, t_ge0=True)
+    assert np.array_equal(s0, s1) and np.array_equal(g0, g1)
+
+
+def test_disc_bitmap_is_a_circular_stop():
+    rho = 0.6
+    circ = OpticalSystem.load(LENS)
+    k = next(i for i, s in enumerate(circ.surfaces) if s.is_stop)
+    semi = circ.surfaces[k].semi_aperture
+    circ.surfaces[k].semi_aperture = rho * semi
+    img = OpticalSystem.load(LENS)
+    _image_stop(img, _disc(256, rho))
+    rays = _rays(circ, 20000, 2)
+    _p0, _w0, s0, _g0 = pyoracle.trace_ghost(circ, -1, -1, rays, t_ge0=True)
+    _p1, _w1, s1, _g1 = pyoracle.trace_ghost(img, -1, -1, rays, t_ge0=True)
+    ok0, ok1 = s0 == 0, s1 == 0
+    assert ok0.sum() > 500
+    # the disagreements are the rays in the edge band: a texel-wide annulus
+    # holds about 2 / (256 rho) of the disc's rays
+    band = 2.5 * 2.0 / (256 * rho) * ok0.sum()
+    assert int((ok0 != ok1).sum()) <= max(3, int(band)), ((ok0 != ok1).sum(), band)
+
+
+def test_bitmap_survives_pack_round_trip():
+    lens = OpticalSystem.load(LENS)
+    pat = np.random.default_rng(0).random((40, 56)).astype(np.float32)
+    k = _image_stop(lens, pat)
+    arr, arena = lens.pack()
+    g = arr[k]
+    assert (g.image_width, g.image_height) == (56, 40) and g.aperture_semi_diameter > 0
+    back = OpticalSystem.from_packed(arr, arena)
+    im = back.aperture_images[k]
+    assert (im.width, im.height) == (56, 40)
+    assert np.array_equal(im.pixels.reshape(40, 56), pat)
