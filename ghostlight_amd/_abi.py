@@ -53,7 +53,7 @@ class gl_surface(C.Structure):
         ("aperture_curvature", C.c_float), ("aperture_twist", C.c_float),
         ("aperture_notch_rad", C.c_float), ("aperture_notch_angle_rad", C.c_float),
         ("aperture_semi_diameter", C.c_float), ("image_offset", C.c_int32), ("image_width", C.c_int32),
-        ("image_height", C.c_int32),
+        ("image_height", C.c_int32), ("image_sdf", C.c_int32), ("image_sdf_offset", C.c_int32),
     ]
 
 
@@ -227,6 +227,7 @@ SIGNATURES = {
     "gl_blade_radius_at": (C.c_float, [C.POINTER(gl_blade_profile), C.c_float]),
     "gl_hurb_sample": (C.c_int, [C.c_float, F32P, C.c_int32, C.c_int32, C.c_float, C.c_uint32, F32P, F32P]),
     "gl_aperture_edge_distance": (C.c_int, [C.POINTER(gl_surface), F32P, F32P, F32P]),
+    "gl_bake_aperture_sdf": (C.c_int, [F32P, C.c_int32, C.c_int32, C.c_float, C.c_float, F32P]),
     "gl_enumerate_ghost_pairs": (C.c_int, [VP, C.POINTER(gl_pair), C.c_int32, I32P]),
     "gl_filter_ghost_pairs": (C.c_int, [VP, VP, C.c_float, C.c_float, C.POINTER(gl_flare_params),
                                         C.POINTER(gl_pair), C.c_int32, I32P]),

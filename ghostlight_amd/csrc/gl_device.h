@@ -77,6 +77,7 @@ struct DevSurf {
     int32_t ap_blades;               // polygon blade count (0: round, -1: image aperture)
     int32_t img_off, img_w, img_h;   // SF_IMAGE: bitmap at arena[img_off], img_w x img_h
     float img_semi;                  // SF_IMAGE: world radius of the bitmap's edge (mm)
+    int32_t sdf_off;                 // image stop: [d, nx, ny, 0] texels at arena[sdf_off]; -1 none
 };
 
 struct TraceRes {
@@ -306,7 +307,8 @@ __device__ __noinline__ bool blade_inside(const DevSurf GLC& s, float hx, float 
 // Bitmap transmission at normalised (u, v): bilinear between texel centres
 // with clamped edges (tex2D with linear filtering, normalised coordinates and
 // clamp addressing, as the reference samples its aperture textures).
-template <class SV> __host__ __device__ inline float image_sample(const SV& s, const float* img, float u, float v) {
+template <class SV> __host__ __device__ inline float image_sample(const SV& s, const float* img, float u, float v,
+                                                                  int stride = 1) {
     const int W = s.img_w, H = s.img_h;
     const float x = u * (float)W - 0.5f, y = v * (float)H - 0.5f;
     const float fx = floorf(x), fy = floorf(y);
@@ -317,8 +319,8 @@ template <class SV> __host__ __device__ inline float image_sample(const SV& s, c
     i1 = i1 < 0 ? 0 : (i1 > W - 1 ? W - 1 : i1);
     j0 = j0 < 0 ? 0 : (j0 > H - 1 ? H - 1 : j0);
     j1 = j1 < 0 ? 0 : (j1 > H - 1 ? H - 1 : j1);
-    const float top = (1.0f - a) * img[j0 * W + i0] + a * img[j0 * W + i1];
-    const float bot = (1.0f - a) * img[j1 * W + i0] + a * img[j1 * W + i1];
+    const float top = (1.0f - a) * img[(j0 * W + i0) * stride] + a * img[(j0 * W + i1) * stride];
+    const float bot = (1.0f - a) * img[(j1 * W + i0) * stride] + a * img[(j1 * W + i1) * stride];
     return (1.0f - b) * top + b * bot;
 }
 
@@ -695,11 +697,12 @@ template <class B> __host__ __device__ inline float blade_edge_gap(const B& b, f
 // aperture_edge_distance (trace_core.h:206-290): distance from a world-space
 // hit on surface s to the clear-aperture edge and the world-space edge normal,
 // in the same stretched frame and polygon geometry the aperture test uses.
-template <class SV> __host__ __device__ inline float surface_edge(const SV& s, F3 hit, F3& n_out) {
-    if (s.ap_blades < 0) {                     // image aperture: no analytic edge, never kicks
-        n_out.x = n_out.y = n_out.z = 0.0f;
-        return 1e30f;
-    }
+//
+// An image aperture has no analytic edge: the stop's baked distance field is
+// sampled where the trace samples its bitmap (d_aperture_edge_distance,
+// trace_cuda.h:72-97); any other image surface never kicks.
+template <class SV> __host__ __device__ inline float surface_edge(const SV& s, F3 hit, F3& n_out,
+                                                                  const float* arena) {
     F3 h = hit;
     const bool tr = (s.flags & SF_TRANSFORMED) != 0;
     if (tr) {
@@ -707,6 +710,26 @@ template <class SV> __host__ __device__ inline float surface_edge(const SV& s, F
         h.x = s.rot[0] * q.x + s.rot[3] * q.y + s.rot[6] * q.z;
         h.y = s.rot[1] * q.x + s.rot[4] * q.y + s.rot[7] * q.z;
         h.z = s.rot[2] * q.x + s.rot[5] * q.y + s.rot[8] * q.z + s.z;
+    }
+    if (s.ap_blades < 0) {
+        if (s.sdf_off < 0 || arena == nullptr) {
+            n_out.x = n_out.y = n_out.z = 0.0f;
+            return 1e30f;
+        }
+        const float hxi = h.x / s.aspect;
+        const float ui = 0.5f + hxi / (2.0f * s.img_semi);
+        const float vi = 0.5f + h.y / (2.0f * s.img_semi);
+        const float* f = arena + s.sdf_off;
+        const float d = image_sample(s, f, ui, vi, 4);
+        F3 nl; nl.x = image_sample(s, f + 1, ui, vi, 4); nl.y = image_sample(s, f + 2, ui, vi, 4); nl.z = 0.0f;
+        if (tr) {
+            n_out.x = s.rot[0] * nl.x + s.rot[1] * nl.y + s.rot[2] * nl.z;
+            n_out.y = s.rot[3] * nl.x + s.rot[4] * nl.y + s.rot[5] * nl.z;
+            n_out.z = s.rot[6] * nl.x + s.rot[7] * nl.y + s.rot[8] * nl.z;
+        } else {
+            n_out = nl;
+        }
+        return d > 0.0f ? d : 0.0f;
     }
     const float sa = s.sa;
     const float hx = h.x / s.aspect, hy = h.y;
@@ -798,7 +821,7 @@ GLD TraceRes trace_ghost(const DevSurf GLC* S, int n, const float GLC* poly, con
         if (!step<W>(s, poly, arena, o, d, lam, n2, refl, rior, cur, res, lrow ? lrow + k : nullptr)) return res;
         if (H) {
             F3 en;
-            const float ed = surface_edge(s, o, en);
+            const float ed = surface_edge(s, o, en, arena);
             hurb_apply(*hb, d, en, ed, lam, *hs);
         }
     }

@@ -365,11 +365,15 @@ void build_devsurfs(const gl_surface* s, int n, float fp64_rmin, const float* ar
         d.nblades = 0; d.poly_off = 0; d.apothem = 0.0f;
         if (deformed_blade(g, &d.blade)) f |= SF_DEFORMED;
         d.ap_rot = g.aperture_rotation_rad;
+        d.sdf_off = -1;
         if (g.aperture_shape == GL_APERTURE_IMAGE && g.image_width > 0 && g.image_height > 0 && g.image_offset >= 0 &&
             arena != nullptr && (long long)g.image_offset + (long long)g.image_width * g.image_height <= arena_len) {
             f |= SF_IMAGE;
             d.img_off = g.image_offset; d.img_w = g.image_width; d.img_h = g.image_height;
             d.img_semi = g.aperture_semi_diameter;
+            if (g.image_sdf && g.image_sdf_offset >= 0 &&
+                (long long)g.image_sdf_offset + 4LL * g.image_width * g.image_height <= arena_len)
+                d.sdf_off = g.image_sdf_offset;
         }
         d.ap_blades = (g.aperture_shape == GL_APERTURE_IMAGE) ? -1
                       : ((g.aperture_shape == GL_APERTURE_POLYGON && g.aperture_blades >= 3) ? g.aperture_blades : 0);
@@ -532,7 +536,7 @@ int gl_aperture_edge_distance(const gl_surface* s, const float hit[3], float* d_
     gl::build_devsurfs(s, 1, 0.0f, nullptr, 0, ds, poly);
     gl::F3 h, nrm;
     h.x = hit[0]; h.y = hit[1]; h.z = hit[2];
-    *d_mm = gl::surface_edge(ds[0], h, nrm);
+    *d_mm = gl::surface_edge(ds[0], h, nrm, nullptr);
     normal[0] = nrm.x; normal[1] = nrm.y; normal[2] = nrm.z;
     return GL_OK;
 }

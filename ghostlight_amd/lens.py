@@ -282,6 +282,7 @@ class OpticalSystem:
             raise ValueError(f"lens has {n} surfaces; at most {_abi.GL_MAX_SURFACES} supported")
         arr = (_abi.gl_surface * max(n, 1))()
         arena: List[float] = []
+        stop_index = next((i for i, s in enumerate(self.surfaces) if s.is_stop), -1)
         for i, s in enumerate(self.surfaces):
             g = arr[i]
             g.radius, g.thickness, g.ior, g.abbe_v = s.radius, s.thickness, s.ior, s.abbe_v
@@ -320,7 +321,7 @@ class OpticalSystem:
                 g.sa_offset, g.sa_n_wavelengths, g.sa_n_angles = len(arena), len(wl), len(an)
                 arena.extend(wl.tolist()); arena.extend(an.tolist()); arena.extend(r.ravel().tolist())
             g.aperture_semi_diameter = float(s.aperture_semi_diameter)
-            g.image_offset = g.image_width = g.image_height = 0
+            g.image_offset = g.image_width = g.image_height = g.image_sdf = g.image_sdf_offset = 0
             img = self.aperture_images[i] if i < len(self.aperture_images) else None
             if (s.aperture_shape == ApertureShape.IMAGE and img is not None and img.pixels is not None
                     and len(img.pixels)):
@@ -329,6 +330,15 @@ class OpticalSystem:
                     raise ValueError(f"surface {i}: aperture image dimensions do not match its pixel data")
                 g.image_offset, g.image_width, g.image_height = len(arena), int(img.width), int(img.height)
                 arena.extend(px.tolist())
+                if s.is_stop and i == stop_index:
+                    # the stop's bitmap also carries its edge distance field for HURB
+                    # (find_sdf_target_surface / bake_aperture_sdf, aperture_sdf.cpp:94-189)
+                    sdf = np.zeros(4 * len(px), np.float32)
+                    if s.aperture_semi_diameter > 0 and _abi.load_library().gl_bake_aperture_sdf(
+                            _abi.fptr(px), int(img.width), int(img.height), float(s.aperture_semi_diameter),
+                            float(s.aperture_aspect), _abi.fptr(sdf)) == 0:
+                        g.image_sdf, g.image_sdf_offset = 1, len(arena)
+                        arena.extend(sdf.tolist())
         return arr, np.asarray(arena if arena else [0.0], np.float32)
 
     @classmethod

@@ -103,6 +103,11 @@ typedef struct gl_surface {
     float   aperture_semi_diameter;   /* world radius at the bitmap's edge (mm)          */
     int32_t image_offset;
     int32_t image_width, image_height;
+    /* the stop's bitmap only: 1 = arena[image_sdf_offset] holds its signed
+     * distance field, [d_mm, nx, ny, 0] per texel (gl_bake_aperture_sdf), the
+     * edge distance HURB kicks by */
+    int32_t image_sdf;
+    int32_t image_sdf_offset;
 } gl_surface;
 
 /* Derived blade silhouette of a bladed stop (ApertureProfile,
@@ -357,6 +362,13 @@ int  gl_hurb_sample(float lambda_nm, const float* d_mm, int32_t n, int32_t dist,
  * world-space edge normal (aperture_edge_distance, trace_core.h:206-290). */
 int  gl_aperture_edge_distance(const gl_surface* s, const float hit[3], float* d_mm,
                                float normal[3]);
+
+/* bake_aperture_sdf (aperture_sdf.cpp:110-189): signed distance (mm,
+ * positive where the bitmap passes) from every texel centre to the 0.5 matte
+ * edge and the unit edge normal, as W*H float4 [d_mm, nx, ny, 0], under the
+ * trace's UV map (x spans 2 aspect semi, y spans 2 semi). */
+int  gl_bake_aperture_sdf(const float* pixels, int32_t width, int32_t height,
+                          float semi_diameter_mm, float aspect, float* out4);
 
 /* ---- ghost path -------------------------------------------------------- */
 

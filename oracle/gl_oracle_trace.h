@@ -511,6 +511,20 @@ template <class S> inline V3<S> reflect_ray(const V3<S>& dir, const V3<S>& nrm) 
 
 template <class S> struct TraceOut { V3<S> pos; S weight; int status; int segments; };
 
+// Bilinear bitmap sample between texel centres, clamped edges (tex2D, linear
+// filter, normalised coordinates, clamp); stride > 1 reads one channel of
+// interleaved texels.
+inline float image_bilinear(const float* img, int W, int H, float u, float v, int stride = 1) {
+    const float x = u * (float)W - 0.5f, y = v * (float)H - 0.5f;
+    const float fx = std::floor(x), fy = std::floor(y);
+    const float a = x - fx, b = y - fy;
+    auto cl = [](int i, int n) { return i < 0 ? 0 : (i > n - 1 ? n - 1 : i); };
+    const int i0 = cl((int)fx, W), i1 = cl((int)fx + 1, W), j0 = cl((int)fy, H), j1 = cl((int)fy + 1, H);
+    const float top = (1.0f - a) * img[(j0 * W + i0) * stride] + a * img[(j0 * W + i1) * stride];
+    const float bot = (1.0f - a) * img[(j1 * W + i0) * stride] + a * img[(j1 * W + i1) * stride];
+    return (1.0f - b) * top + b * bot;
+}
+
 // ---- HURB edge diffraction (hurb.h:40-150) ---------------------------------
 struct Hurb { int on = 0, dist = GL_HURB_LORENTZIAN; float min_sigma = 1.0e-5f, max_kick = 0.35f; };
 inline uint32_t hurb_wang(uint32_t s) {
@@ -555,11 +569,29 @@ template <class S> inline void hurb_apply(const Hurb& h, V3<S>& dir, const V3<S>
 }
 
 // aperture_edge_distance (trace_core.h:206-290).
+//
+// An image stop samples its baked distance field (arena[image_sdf_offset])
+// where the trace samples the bitmap; other image surfaces never kick
+// (d_aperture_edge_distance, trace_cuda.h:72-97).
 template <class S> inline S edge_distance(const V3<S>& hit, const gl_surface& s, const PolyPre& pp, bool transformed,
-                                          V3<S>& n_out) {
-    if (s.aperture_shape == GL_APERTURE_IMAGE) { n_out = V3<S>(0, 0, 0); return S(1e30f); }
+                                          V3<S>& n_out, const float* arena = nullptr) {
     V3<S> h = hit;
     if (transformed) h = m_tmul<S>(s.rot, hit - V3<S>(S(s.decenter_x), S(s.decenter_y), S(s.z))) + V3<S>(0, 0, S(s.z));
+    if (s.aperture_shape == GL_APERTURE_IMAGE) {
+        if (!arena || !s.image_sdf || s.image_width <= 0 || s.image_height <= 0) {
+            n_out = V3<S>(0, 0, 0);
+            return S(1e30f);
+        }
+        const float hx = (float)h.x / s.aperture_aspect;
+        const float u = 0.5f + hx / (2.0f * s.aperture_semi_diameter);
+        const float v = 0.5f + (float)h.y / (2.0f * s.aperture_semi_diameter);
+        const float* f = arena + s.image_sdf_offset;
+        const float d = image_bilinear(f, s.image_width, s.image_height, u, v, 4);
+        const V3<S> nl(S(image_bilinear(f + 1, s.image_width, s.image_height, u, v, 4)),
+                       S(image_bilinear(f + 2, s.image_width, s.image_height, u, v, 4)), S(0));
+        n_out = transformed ? m_mul<S>(s.rot, nl) : nl;
+        return S(d > 0.0f ? d : 0.0f);
+    }
     const S sa = S(s.semi_aperture);
     const S hx = h.x / S(s.aperture_aspect), hy = h.y;
     const S r_s = std::sqrt(hx * hx + hy * hy);
@@ -643,16 +675,6 @@ inline float baffles_edge(const gl_baffle* b, int nb, float px, float py, float 
 // surface-frame hit, bilinear between texel centres with clamped edges (the
 // reference's tex2D, linear filter, normalised coordinates, clamp), > 0.5.
 // No bitmap loaded: every ray passes.
-inline float image_bilinear(const float* img, int W, int H, float u, float v) {
-    const float x = u * (float)W - 0.5f, y = v * (float)H - 0.5f;
-    const float fx = std::floor(x), fy = std::floor(y);
-    const float a = x - fx, b = y - fy;
-    auto cl = [](int i, int n) { return i < 0 ? 0 : (i > n - 1 ? n - 1 : i); };
-    const int i0 = cl((int)fx, W), i1 = cl((int)fx + 1, W), j0 = cl((int)fy, H), j1 = cl((int)fy + 1, H);
-    const float top = (1.0f - a) * img[j0 * W + i0] + a * img[j0 * W + i1];
-    const float bot = (1.0f - a) * img[j1 * W + i0] + a * img[j1 * W + i1];
-    return (1.0f - b) * top + b * bot;
-}
 template <class S> inline bool image_aperture_passes(const Lens& L, int k, const V3<S>& hit) {
     const gl_surface& s = L.s[k];
     if (s.aperture_shape != GL_APERTURE_IMAGE || s.image_width <= 0 || s.image_height <= 0 || !L.arena) return true;
@@ -715,7 +737,7 @@ inline TraceOut<S> trace_ghost(const Lens& L, int a, int b, Ray<S> ray, S lambda
     auto kick = [&](int k) {
         if (!hurb || !hurb->on) return;
         V3<S> en;
-        const S ed = edge_distance<S>(ray.o, L.s[k], L.poly[k], L.transformed[k], en);
+        const S ed = edge_distance<S>(ray.o, L.s[k], L.poly[k], L.transformed[k], en, L.arena);
         hurb_apply<S>(*hurb, ray.d, en, ed, lambda, *hs);
     };
     S cur = S(1);

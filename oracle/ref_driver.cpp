@@ -12,6 +12,7 @@
 #include "tmm.h"
 #include "lens_calibration.h"
 #include "source_map.h"
+#include "aperture_sdf.h"
 
 #include "../include/ghostlight_amd.h"
 
@@ -140,6 +141,17 @@ void put(const TraceResult& r, float* pos, float* w, int* st) {
 }  // namespace
 
 extern "C" {
+
+// bake_aperture_sdf (aperture_sdf.cpp:110) of a W x H bitmap: W*H float4.
+int ref_bake_aperture_sdf(const float* px, int W, int H, float semi, float aspect, float* out4) {
+    ApertureImage img;
+    img.width = W; img.height = H;
+    img.pixels.assign(px, px + (size_t)W * H);
+    ApertureSdfBake b;
+    if (!bake_aperture_sdf(img, semi, aspect, b)) return -1;
+    std::copy(b.texels.begin(), b.texels.end(), out4);
+    return 0;
+}
 
 // make_aperture_profile (aperture_profile.h:285) as gl_blade_profile.
 int ref_blade_profile(int blades, float rot, float curvature, float twist, float notch, float rake,

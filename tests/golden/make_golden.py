@@ -419,6 +419,23 @@ def blade_goldens():
         cnt = C.c_int()
         ref.ref_build_aperture_samples(arr, n, N, jit, 12345, 0, C.c_float(0.0), buf, cap, C.byref(cnt))
         out[f"bs_grid_{N}_{jit}"] = np.frombuffer(bytes(buf), np.float32)[: 2 * cnt.value].reshape(-1, 2).copy()
+    # The reference's signed distance field bake (aperture_sdf.cpp:110-189)
+    # of a few bitmaps: a disc, an anisotropic star, a soft random matte.
+    sdf_cases = []
+    c = (np.arange(48) + 0.5) / 48 * 2 - 1
+    X, Y = np.meshgrid(c, c)
+    sdf_cases.append(((np.hypot(X, Y) < 0.6).astype(np.float32), 10.0, 1.0))
+    th = np.arctan2(Y, X)
+    sdf_cases.append(((np.hypot(X, Y) < 0.55 + 0.2 * np.cos(5 * th)).astype(np.float32)[:, :40].copy(), 7.5, 1.33))
+    sdf_cases.append((np.random.default_rng(4).random((20, 28)).astype(np.float32), 3.0, 0.8))
+    for q, (img, semi, asp) in enumerate(sdf_cases):
+        H_, W_ = img.shape
+        o4 = np.zeros(4 * W_ * H_, np.float32)
+        ref.ref_bake_aperture_sdf(pyoracle._f(np.ascontiguousarray(img)), W_, H_, C.c_float(semi), C.c_float(asp),
+                                  pyoracle._f(o4))
+        out[f"sdf{q}_img"] = img
+        out[f"sdf{q}_par"] = np.array([semi, asp], np.float32)
+        out[f"sdf{q}_out"] = o4
     path = os.path.join(HERE, "blade_goldens.npz")
     np.savez_compressed(path, **out)
     print(f"wrote {path} ({os.path.getsize(path)} bytes, {len(out)} arrays)")

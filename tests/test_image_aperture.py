@@ -91,3 +91,55 @@ def test_bitmap_survives_pack_round_trip():
     im = back.aperture_images[k]
     assert (im.width, im.height) == (56, 40)
     assert np.array_equal(im.pixels.reshape(40, 56), pat)
+
+
+# ---- the stop bitmap's signed distance field (HURB edge distance) ----------
+import ctypes as C  # noqa: E402
+
+from ghostlight_amd import _abi  # noqa: E402
+
+GOLD = np.load(os.path.join(HERE, "golden", "blade_goldens.npz"))
+
+
+def _bake(img, semi, aspect):
+    img = np.ascontiguousarray(img, np.float32)
+    H, W = img.shape
+    out = np.zeros(4 * W * H, np.float32)
+    assert _abi.load_library().gl_bake_aperture_sdf(_abi.fptr(img), W, H, semi, aspect, _abi.fptr(out)) == 0
+    return out.reshape(H, W, 4)
+
+
+def test_sdf_bake_bitexact_with_reference():
+    for q in range(3):
+        semi, asp = (float(x) for x in GOLD[f"sdf{q}_par"])
+        got = _bake(GOLD[f"sdf{q}_img"], semi, asp)
+        assert np.array_equal(got.ravel(), GOLD[f"sdf{q}_out"]), q
+
+
+def test_sdf_disc_distance_and_radial_normal():
+    n, rho, semi = 128, 0.6, 10.0
+    f = _bake(_disc(n, rho), semi, 1.0)
+    c = (np.arange(n) + 0.5) / n * 2 - 1
+    X, Y = np.meshgrid(c, c)
+    r = np.hypot(X, Y)
+    exact = (rho - r) * semi                          # mm, positive inside
+    band = np.abs(exact) < 0.3 * semi
+    texel = 2 * semi / n
+    assert np.max(np.abs(f[..., 0][band] - exact[band])) < 0.75 * texel
+    ring = band & (np.abs(exact) > 2 * texel)
+    radial = np.stack([X, Y], -1)[ring] / r[ring][:, None]
+    cosang = np.abs(np.sum(f[..., 1:3][ring] * radial, axis=1))
+    assert np.median(cosang) > 0.995
+
+
+def test_sdf_sign_matches_the_matte_and_scales_anisotropically():
+    img = np.zeros((40, 64), np.float32)
+    img[:, 20:] = 1.0                                  # pass for columns >= 20: a vertical edge
+    f = _bake(img, 5.0, 2.0)
+    inside = img > 0.5
+    assert np.all(f[..., 0][inside] >= 0) and np.all(f[..., 0][~inside] <= 0)
+    sx = 2 * 2.0 * 5.0 / 64                            # mm per texel across x (aspect 2)
+    # half-texel corrected: the texel next to the edge sits half a texel from it
+    assert abs(f[20, 20, 0] - 0.5 * sx) < 1e-5
+    assert abs(f[20, 30, 0] - 10.5 * sx) < 1e-4
+    assert abs(abs(f[20, 30, 1]) - 1.0) < 1e-6 and abs(f[20, 30, 2]) < 1e-6
