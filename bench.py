@@ -28,9 +28,10 @@ sys.path.insert(0, ROOT)
 
 FP32_VECTOR_PEAK_TFLOPS = 157.3   # MI355X published FP32 vector peak
 # HBM bytes per launch of the trace kernel on this workload, from rocprofv3
-# FETCH_SIZE / WRITE_SIZE runs scaled by a 1 GiB copy (profiles/ghost_persist_kernel_c2.md):
-# 5.4 MiB read (lens, tables, pupil grid) + 849 MiB of deposit records written.
-TRACE_TRAFFIC_BYTES = 896e6
+# FETCH_SIZE / WRITE_SIZE runs scaled by a 1 GiB copy: 147 MiB read (lens,
+# tables, pupil grid, spill stack) + 961 MiB written (849 MiB of deposit
+# records, the rest the spill stack of eight-wave occupancy).
+TRACE_TRAFFIC_BYTES = 1.16e9
 HBM_PEAK_GBS = 8000.0
 
 
