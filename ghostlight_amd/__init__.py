@@ -7,8 +7,8 @@ behind a C ABI (include/ghostlight_amd.h), with the reference's Python API
 from .config import (BaffleShape, Baffle, DiffractionConfig, FlareConfig, GateConfig, GateLobe, GhostAovMode,
                      GhostFilter, HurbKickDistribution, InputColorSpace, MatteBox, OutputColorSpace,
                      PointFlareConfig, PSFCenterMode, PSFConfig, RenderConfig, SensorModel, StarburstEngine)
-from .lens import (ApertureImage, ApertureShape, Coating, CoatingModel, CylinderAxis, DispersionModel, LensCalibration,
-                   OpticalSystem, Surface, SurfaceForm, TraceStatus)
+from .lens import (ApertureImage, ApertureProfile, ApertureShape, Coating, CoatingModel, CylinderAxis, DispersionModel,
+                   LensCalibration, OpticalSystem, Surface, SurfaceForm, TraceStatus, Vec3f)
 from .runtime import Context, device_available
 from .render import (GhostPair, Ray, TraceResult, enumerate_ghost_pairs, filter_ghost_pairs, render_ghosts,
                      render_point_flare, render_source_flare, render_sources, solve_source_angle, trace_ghost_ray, trace_ghost_rays,
@@ -21,6 +21,9 @@ from .source_map import (SourceMapSolve, _solve_source_map, _source_map_landing,
                          solve_source_map)
 
 
+from .edge_debug import _aperture_edge_distance_debug, _aperture_sdf_debug, _hurb_sample_debug  # noqa: E402
+
+
 def _cuda_available() -> bool:
     """Reference name for the device check (bind_renderers.cpp:518)."""
     return device_available()
@@ -28,4 +31,6 @@ def _cuda_available() -> bool:
 
 __all__ = [n for n in dir() if not n.startswith("_")] + ["_cuda_available", "_render_starburst_debug",
                                                          "_render_veil_debug", "_solve_source_map",
-                                                         "_source_map_landing", "_source_map_screen"]
+                                                         "_source_map_landing", "_source_map_screen",
+                                                         "_hurb_sample_debug", "_aperture_edge_distance_debug",
+                                                         "_aperture_sdf_debug"]

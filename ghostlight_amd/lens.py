@@ -87,6 +87,46 @@ _IDENT = (1.0, 0.0, 0.0, 0.0, 1.0, 0.0, 0.0, 0.0, 1.0)
 
 
 @dataclass
+class Vec3f:
+    """World-space point or direction (Vec3f, bind_geometry.cpp:30-37)."""
+    x: float = 0.0
+    y: float = 0.0
+    z: float = 0.0
+
+    def __iter__(self):
+        return iter((self.x, self.y, self.z))
+
+
+class ApertureProfile:
+    """Read-only derived blade geometry of a surface's aperture
+    (ApertureProfile, bind_geometry.cpp:121-141), built by the host library's
+    gl_blade_profile_build from the surface's authored blade controls;
+    radius_at(theta) is the boundary radius as a fraction of the tip radius."""
+
+    def __init__(self, s: "Surface"):
+        from . import _abi
+        self._lib = _abi.load_library()
+        self._p = _abi.gl_blade_profile()
+        blades = int(s.aperture_blades) if int(s.aperture_shape) == int(ApertureShape.POLYGON) else 0
+        _abi.check(self._lib.gl_blade_profile_build(blades, float(s.aperture_rotation_rad),
+                                                    float(s.aperture_curvature), float(s.aperture_twist),
+                                                    float(s.aperture_notch_rad), float(s.aperture_notch_angle_rad),
+                                                    _abi.C.byref(self._p)), "aperture_profile")
+
+    def __getattr__(self, name):
+        if name.startswith("_"):
+            raise AttributeError(name)
+        return getattr(self._p, name)
+
+    def deformed(self) -> bool:
+        return self._p.blades >= 3 and not self._p.plain
+
+    def radius_at(self, theta: float) -> float:
+        from . import _abi
+        return float(self._lib.gl_blade_radius_at(_abi.C.byref(self._p), float(theta)))
+
+
+@dataclass
 class Surface:
     """One optical surface (optical_system.h:157-249)."""
     radius: float = 0.0
@@ -119,6 +159,12 @@ class Surface:
     disp_model: DispersionModel = DispersionModel.ABBE
     sellmeier_B: tuple = (0.0, 0.0, 0.0)
     sellmeier_C: tuple = (0.0, 0.0, 0.0)
+
+    @property
+    def aperture_profile(self) -> ApertureProfile:
+        """Blade geometry derived from the authored controls, rebuilt on each
+        read so it always follows the current fields."""
+        return ApertureProfile(self)
 
     @property
     def n_asphere_terms(self) -> int:
@@ -188,6 +234,14 @@ class ApertureImage:
     semi_diameter: float = 0.0
     source_path: str = ""
 
+    def __setattr__(self, name, value):
+        # assigning a 2-D array to `pixels` sets the size too, as the
+        # reference's pixels property does (bind_geometry.cpp:327-351)
+        if name == "pixels" and isinstance(value, np.ndarray) and value.ndim == 2:
+            self.set_pixels(value)
+            return
+        object.__setattr__(self, name, value)
+
     def set_pixels(self, img) -> None:
         a = np.ascontiguousarray(img, np.float32)
         if a.ndim != 2:
@@ -218,6 +272,8 @@ class OpticalSystem:
         thickness, float32 throughout."""
         if not self.surfaces:
             return False
+        while len(self.aperture_images) < len(self.surfaces):
+            self.aperture_images.append(ApertureImage())
         total = np.float32(0.0)
         for s in self.surfaces:
             total = np.float32(total + np.float32(s.thickness))
