@@ -11,13 +11,13 @@
 // arithmetic as the split-on-load kernel in mdft.hip, without re-splitting
 // every element once per output tile.
 //
-// Tile: 128 x 128 per workgroup, 4 waves each owning a 64 x 64 quadrant (2 x 2
-// MFMA blocks of 32 x 32, re and im accumulators: 128 accumulator registers),
+// Tile: 128 x 128 per workgroup, 8 waves each owning a 64 x 32 block (2 x 1
+// MFMA blocks of 32 x 32, re and im accumulators: 64 accumulator registers),
 // K step 32.  Two LDS stages of 80 KB (rows padded by 16 bytes against bank
 // conflicts): while a step's 96 MFMAs per wave run out of one stage, the next
 // step's operands go from registers into the other and the step after that is
-// fetched from HBM.  One workgroup per CU, one wave per SIMD with its ~300
-// registers.
+// fetched from HBM.  One workgroup per CU, two waves per SIMD, so one wave's
+// LDS reads, stash and barrier overlap with the other's MFMAs.
 // Epilogues: complex float2, |z|^2 float, or the result split again into bf16
 // planes (so a GEMM can feed the next one as its B operand).
 #include "gl_runtime.h"
@@ -65,7 +65,7 @@ __global__ __launch_bounds__(256) void split_planes_kernel(const float2* __restr
 // shared).  EPI 0: float2 C, 1: float |z|^2, 2: 4-plane bf16 split of C
 // (row-major M x N, batch stride sC elements of the plane set).
 template <int EPI>
-__global__ __launch_bounds__(256) void cgemm_nt3_kernel(const __bf16* __restrict__ A, long long sA,
+__global__ __launch_bounds__(512) void cgemm_nt3_kernel(const __bf16* __restrict__ A, long long sA,
                                                            const __bf16* __restrict__ Bt, long long sB,
                                                            void* __restrict__ Cv, int ldc, long long sC, int M, int N,
                                                            int K, float alpha) {
@@ -77,24 +77,24 @@ __global__ __launch_bounds__(256) void cgemm_nt3_kernel(const __bf16* __restrict
     const size_t pa = (size_t)M * K, pb = (size_t)N * K;
     const int m0 = blockIdx.y * kT, n0 = blockIdx.x * kT;
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
-    const int wm = (wave >> 1) * 64, wn = (wave & 1) * 64;
+    const int wm = (wave >> 2) * 64, wn = (wave & 3) * 32;
 
-    f32x16_t acc[2][2][2];
+    f32x16_t acc[2][1][2];
 #pragma unroll
     for (int i = 0; i < 2; ++i)
 #pragma unroll
-        for (int j = 0; j < 2; ++j)
+        for (int j = 0; j < 1; ++j)
 #pragma unroll
             for (int v = 0; v < 16; ++v) { acc[i][j][0][v] = 0.0f; acc[i][j][1][v] = 0.0f; }
 
     // Loader: per operand 4 planes x 128 rows x 32 k = 2048 vectors of 8;
-    // thread t moves vectors t + 256 j (j < 8): plane (v >> 9), row
+    // thread t moves vectors t + 512 j (j < 4): plane (v >> 9), row
     // ((v >> 2) & 127), k offset 8 (v & 3).
-    u32x4_t ra[8], rb[8];
+    u32x4_t ra[4], rb[4];
     auto fetch = [&](int k0) {
 #pragma unroll
-        for (int j = 0; j < 8; ++j) {
-            const int v = tid + 256 * j;
+        for (int j = 0; j < 4; ++j) {
+            const int v = tid + 512 * j;
             const int p = v >> 9, row = (v >> 2) & 127, kq = (v & 3) * 8;
             ra[j] = *reinterpret_cast<const u32x4_t*>(Ab + p * pa + (size_t)(m0 + row) * K + k0 + kq);
             rb[j] = *reinterpret_cast<const u32x4_t*>(Bb + p * pb + (size_t)(n0 + row) * K + k0 + kq);
@@ -104,8 +104,8 @@ __global__ __launch_bounds__(256) void cgemm_nt3_kernel(const __bf16* __restrict
         __bf16* sa = lds + stage * kStage;
         __bf16* sb = sa + 4 * kT * kRow;
 #pragma unroll
-        for (int j = 0; j < 8; ++j) {
-            const int v = tid + 256 * j;
+        for (int j = 0; j < 4; ++j) {
+            const int v = tid + 512 * j;
             const int p = v >> 9, row = (v >> 2) & 127, kq = (v & 3) * 8;
             *reinterpret_cast<u32x4_t*>(sa + (p * kT + row) * kRow + kq) = ra[j];
             *reinterpret_cast<u32x4_t*>(sb + (p * kT + row) * kRow + kq) = rb[j];
@@ -126,16 +126,16 @@ __global__ __launch_bounds__(256) void cgemm_nt3_kernel(const __bf16* __restrict
         const __bf16* sb = sa + 4 * kT * kRow;
 #pragma unroll
         for (int ks = 0; ks < kBK; ks += 16) {
-            bf16x8_t a[2][4], b[2][4];
+            bf16x8_t a[2][4], b[1][4];
 #pragma unroll
-            for (int i = 0; i < 2; ++i)
+            for (int p = 0; p < 4; ++p) {
 #pragma unroll
-                for (int p = 0; p < 4; ++p) {
+                for (int i = 0; i < 2; ++i)
                     a[i][p] = *reinterpret_cast<const bf16x8_t*>(sa + (p * kT + wm + 32 * i + r) * kRow + ks + 8 * kh);
-                    b[i][p] = *reinterpret_cast<const bf16x8_t*>(sb + (p * kT + wn + 32 * i + r) * kRow + ks + 8 * kh);
-                }
+                b[0][p] = *reinterpret_cast<const bf16x8_t*>(sb + (p * kT + wn + r) * kRow + ks + 8 * kh);
+            }
 #pragma unroll
-            for (int j = 0; j < 2; ++j) {
+            for (int j = 0; j < 1; ++j) {
                 const bf16x8_t nbih = neg_bf16x8(b[j][2]), nbil = neg_bf16x8(b[j][3]);
 #pragma unroll
                 for (int i = 0; i < 2; ++i) {
@@ -169,7 +169,7 @@ __global__ __launch_bounds__(256) void cgemm_nt3_kernel(const __bf16* __restrict
 #pragma unroll
     for (int i = 0; i < 2; ++i)
 #pragma unroll
-        for (int j = 0; j < 2; ++j) {
+        for (int j = 0; j < 1; ++j) {
             const int col = n0 + wn + 32 * j + r;
 #pragma unroll
             for (int v = 0; v < 16; ++v) {
@@ -213,11 +213,11 @@ int cgemm_nt3(hipStream_t s, int epi, const __bf16* A, long long sA, const __bf1
     const dim3 grid((unsigned)(N / kT), (unsigned)(M / kT), (unsigned)batch);
     const size_t lds = cgemm_nt3_lds_bytes();
     if (epi == 0)
-        hipLaunchKernelGGL(cgemm_nt3_kernel<0>, grid, dim3(256), lds, s, A, sA, Bt, sB, C, ldc, sC, M, N, K, alpha);
+        hipLaunchKernelGGL(cgemm_nt3_kernel<0>, grid, dim3(512), lds, s, A, sA, Bt, sB, C, ldc, sC, M, N, K, alpha);
     else if (epi == 1)
-        hipLaunchKernelGGL(cgemm_nt3_kernel<1>, grid, dim3(256), lds, s, A, sA, Bt, sB, C, ldc, sC, M, N, K, alpha);
+        hipLaunchKernelGGL(cgemm_nt3_kernel<1>, grid, dim3(512), lds, s, A, sA, Bt, sB, C, ldc, sC, M, N, K, alpha);
     else
-        hipLaunchKernelGGL(cgemm_nt3_kernel<2>, grid, dim3(256), lds, s, A, sA, Bt, sB, C, ldc, sC, M, N, K, alpha);
+        hipLaunchKernelGGL(cgemm_nt3_kernel<2>, grid, dim3(512), lds, s, A, sA, Bt, sB, C, ldc, sC, M, N, K, alpha);
     GL_HIP(hipGetLastError());
     return GL_OK;
 }
