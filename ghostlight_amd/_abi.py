@@ -225,6 +225,7 @@ SIGNATURES = {
     "gl_blade_profile_build": (C.c_int, [C.c_int32, C.c_float, C.c_float, C.c_float, C.c_float, C.c_float,
                                          C.POINTER(gl_blade_profile)]),
     "gl_blade_radius_at": (C.c_float, [C.POINTER(gl_blade_profile), C.c_float]),
+    "gl_blade_dr_dtheta": (C.c_float, [C.POINTER(gl_blade_profile), C.c_float]),
     "gl_hurb_sample": (C.c_int, [C.c_float, F32P, C.c_int32, C.c_int32, C.c_float, C.c_uint32, F32P, F32P]),
     "gl_aperture_edge_distance": (C.c_int, [C.POINTER(gl_surface), F32P, F32P, F32P]),
     "gl_bake_aperture_sdf": (C.c_int, [F32P, C.c_int32, C.c_int32, C.c_float, C.c_float, F32P]),

@@ -170,4 +170,28 @@ float gl_blade_radius_at(const gl_blade_profile* b, float theta) {
     return 1.0f / u_at(*b, sector_angle(*b, theta));
 }
 
+// dr/dtheta = -(du/dphi) / u^2 of whichever boundary u_at picked: the conic
+// edge (du/dphi = -u_rise q sin(psi) / 2 from u = u_w - u_rise q sin^2(psi/2))
+// or the tip facet (u = cos(w + half - beta) / p).
+float gl_blade_dr_dtheta(const gl_blade_profile* b, float theta) {
+    if (!b || b->blades < 3) return 0.0f;
+    const float phi = sector_angle(*b, theta);
+    const float psi = phi - b->phi_w;
+    const float q = (psi >= 0.0f) ? b->qR : b->qL;
+    float u = edge_u(*b, phi);
+    float du = -0.5f * b->u_rise * q * std::sin(psi);
+    if (b->facets) {
+        const float w = fold(*b, phi - b->half);
+        if (w >= -b->eA && w <= b->eB) {
+            const float ang = w + b->half - b->beta;
+            const float uf = std::cos(ang) / b->p;
+            if (uf > u) {
+                u = uf;
+                du = -std::sin(ang) / b->p;
+            }
+        }
+    }
+    return -du / (u * u);
+}
+
 }  // extern "C"

@@ -125,6 +125,10 @@ class ApertureProfile:
         from . import _abi
         return float(self._lib.gl_blade_radius_at(_abi.C.byref(self._p), float(theta)))
 
+    def dr_dtheta(self, theta: float) -> float:
+        from . import _abi
+        return float(self._lib.gl_blade_dr_dtheta(_abi.C.byref(self._p), float(theta)))
+
 
 @dataclass
 class Surface:

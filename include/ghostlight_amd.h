@@ -350,6 +350,9 @@ int  gl_blade_profile_build(int32_t blades, float rotation_rad, float curvature,
 /* ApertureProfile::radius_at (aperture_profile.h:176-181): the normalised
  * boundary radius at absolute angle theta (1 when blades < 3). */
 float gl_blade_radius_at(const gl_blade_profile* profile, float theta);
+/* Analytic slope d(radius)/d(theta) of the same boundary, as the HURB edge
+ * distance uses it (ApertureProfile::dr_dtheta, aperture_profile.h:235-255). */
+float gl_blade_dr_dtheta(const gl_blade_profile* profile, float theta);
 
 /* _hurb_sample_debug (bind_renderers.cpp:457-481): for each edge distance
  * d[i] the HURB kick scale sigma = lambda K / d and one kick drawn from an

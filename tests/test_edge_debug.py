@@ -153,3 +153,18 @@ def test_sdf_only_for_a_loaded_image_stop():
     empty = _image_stop_system(np.ones((8, 8), np.float32))
     empty.aperture_images[0].pixels = None
     assert _impl._aperture_sdf_debug(empty)["stop_index"] == -1
+
+
+@pytest.mark.parametrize("controls", [dict(aperture_curvature=-0.8), dict(aperture_curvature=0.6),
+                                      dict(aperture_curvature=-0.5, aperture_twist=0.9),
+                                      dict(aperture_curvature=-0.9, aperture_twist=-0.8,
+                                           aperture_notch_rad=math.radians(35.8),
+                                           aperture_notch_angle_rad=math.radians(45.0))])
+def test_profile_slope_matches_finite_differences(controls):
+    p = _stop(**controls).aperture_profile
+    for i in range(60):
+        th = 2 * math.pi * (i + 0.31) / 60
+        h = 1e-4
+        fd = (p.radius_at(th + h) - p.radius_at(th - h)) / (2 * h)
+        assert p.dr_dtheta(th) == pytest.approx(fd, abs=0.02, rel=0.05)
+    assert _stop(aperture_shape=ghostlight.ApertureShape.CIRCLE).aperture_profile.dr_dtheta(0.3) == 0.0
