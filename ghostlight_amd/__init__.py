@@ -8,14 +8,15 @@ from .config import (BaffleShape, Baffle, DiffractionConfig, FlareConfig, GateCo
                      GhostFilter, HurbKickDistribution, InputColorSpace, MatteBox, OutputColorSpace,
                      PointFlareConfig, PSFCenterMode, PSFConfig, RenderConfig, SensorModel, StarburstEngine)
 from .lens import (ApertureImage, ApertureProfile, ApertureShape, Coating, CoatingModel, CylinderAxis, DispersionModel,
-                   LensCalibration, OpticalSystem, Surface, SurfaceForm, TraceStatus, Vec3f)
+                   LensCalibration, OpticalSystem, Surface, SurfaceForm, TraceStatus, Vec3f, cross, dot)
 from .runtime import Context, device_available
 from .render import (GhostPair, Ray, TraceResult, enumerate_ghost_pairs, filter_ghost_pairs, render_ghosts,
                      render_point_flare, render_source_flare, render_sources, solve_source_angle, trace_ghost_ray, trace_ghost_rays,
                      trace_primary_ray)
 from .hostprep import build_spectral_lambdas
 from .diffraction import _render_starburst_debug, _render_veil_debug, fft2d
-from .psf import render_psf
+from .psf import PSFCellStatus, render_psf
+from .config import MATTE_BOX_OPEN  # noqa: E402
 from .lens_file import LensFileError, load_lens
 from .source_map import (SourceMapSolve, _solve_source_map, _source_map_landing, _source_map_screen, calibrate_lens,
                          solve_source_map)

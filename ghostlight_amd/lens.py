@@ -97,6 +97,20 @@ class Vec3f:
         return iter((self.x, self.y, self.z))
 
 
+def dot(a, b) -> float:
+    """Dot product of two 3-vectors (Vec3f or any 3-sequence), float32 arithmetic."""
+    ax, ay, az = (np.float32(v) for v in a)
+    bx, by, bz = (np.float32(v) for v in b)
+    return float(ax * bx + ay * by + az * bz)
+
+
+def cross(a, b) -> Vec3f:
+    """Cross product of two 3-vectors as a Vec3f, float32 arithmetic."""
+    ax, ay, az = (np.float32(v) for v in a)
+    bx, by, bz = (np.float32(v) for v in b)
+    return Vec3f(float(ay * bz - az * by), float(az * bx - ax * bz), float(ax * by - ay * bx))
+
+
 class ApertureProfile:
     """Read-only derived blade geometry of a surface's aperture
     (ApertureProfile, bind_geometry.cpp:121-141), built by the host library's

@@ -13,6 +13,7 @@ neighbours), with the per-source control flow of newton_aim
 """
 from __future__ import annotations
 
+import enum
 import ctypes as C
 import math
 from typing import Optional
@@ -28,6 +29,13 @@ from .runtime import ctx_handle, lib
 PSF_CELL_OK = 0
 PSF_CELL_CHIEF_VIGNETTED = 1
 PSF_CELL_DARK = 2
+
+
+class PSFCellStatus(enum.IntEnum):
+    """Per-cell status in the `status` array of render_psf (PSFRenderer.h:22-26)."""
+    OK = PSF_CELL_OK                            # the chief ray reached the cell centre
+    CHIEF_VIGNETTED = PSF_CELL_CHIEF_VIGNETTED  # chief blocked, part of the pupil transmits
+    DARK = PSF_CELL_DARK                        # no transmission near the cell
 
 CHIEF_LAMBDA_NM = 587.56
 

@@ -168,3 +168,15 @@ def test_profile_slope_matches_finite_differences(controls):
         fd = (p.radius_at(th + h) - p.radius_at(th - h)) / (2 * h)
         assert p.dr_dtheta(th) == pytest.approx(fd, abs=0.02, rel=0.05)
     assert _stop(aperture_shape=ghostlight.ApertureShape.CIRCLE).aperture_profile.dr_dtheta(0.3) == 0.0
+
+
+def test_psf_status_enum_constants_and_vector_helpers():
+    from ghostlight_amd import psf
+    assert [int(v) for v in ghostlight.PSFCellStatus] == [psf.PSF_CELL_OK, psf.PSF_CELL_CHIEF_VIGNETTED,
+                                                          psf.PSF_CELL_DARK] == [0, 1, 2]
+    assert ghostlight.MATTE_BOX_OPEN == ghostlight.MatteBox().top == 1.0e9
+    a, b = ghostlight.Vec3f(1.0, 2.0, 3.0), ghostlight.Vec3f(-2.0, 0.5, 4.0)
+    c = ghostlight.cross(a, b)
+    assert ghostlight.dot(a, b) == pytest.approx(11.0)
+    assert abs(ghostlight.dot(c, a)) < 1e-6 and abs(ghostlight.dot(c, b)) < 1e-6
+    assert (c.x, c.y, c.z) == pytest.approx((6.5, -10.0, 4.5))
