@@ -22,7 +22,8 @@ from .source_map import (SourceMapSolve, _solve_source_map, _source_map_landing,
                          solve_source_map)
 
 
-from .edge_debug import _aperture_edge_distance_debug, _aperture_sdf_debug, _hurb_sample_debug  # noqa: E402
+from .edge_debug import (_aperture_edge_distance_debug, _aperture_sdf_debug, _gate_params_debug,  # noqa: E402
+                         _gate_sample_debug, _hurb_sample_debug)
 
 
 def _cuda_available() -> bool:
@@ -34,4 +35,5 @@ __all__ = [n for n in dir() if not n.startswith("_")] + ["_cuda_available", "_re
                                                          "_render_veil_debug", "_solve_source_map",
                                                          "_source_map_landing", "_source_map_screen",
                                                          "_hurb_sample_debug", "_aperture_edge_distance_debug",
-                                                         "_aperture_sdf_debug"]
+                                                         "_aperture_sdf_debug", "_gate_params_debug",
+                                                         "_gate_sample_debug"]

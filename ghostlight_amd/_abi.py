@@ -158,6 +158,12 @@ class gl_gate_params(C.Structure):
                 ("gain", C.c_float), ("scatter_samples", C.c_int32), ("spectral_samples", C.c_int32)]
 
 
+class gl_gate_resolved(C.Structure):
+    _fields_ = [(n, C.c_float) for n in ("x_pos", "x_neg", "y_pos", "y_neg", "t_mm", "zs_mm", "sig_wide", "sig_tight",
+                                         "cos_g", "sin_g", "r0", "max_kick", "inv_scatter", "gain")] + [
+        ("n_scatter", C.c_int32), ("lobe", C.c_int32)]
+
+
 class gl_source_map(C.Structure):
     _fields_ = [("angle_x", C.c_float), ("angle_y", C.c_float), ("status", C.c_int32), ("residual_mm", C.c_float),
                 ("anchor_ax", C.c_float), ("anchor_ay", C.c_float), ("anchor_x", C.c_float), ("anchor_y", C.c_float),
@@ -226,6 +232,9 @@ SIGNATURES = {
                                          C.POINTER(gl_blade_profile)]),
     "gl_blade_radius_at": (C.c_float, [C.POINTER(gl_blade_profile), C.c_float]),
     "gl_blade_dr_dtheta": (C.c_float, [C.POINTER(gl_blade_profile), C.c_float]),
+    "gl_gate_resolve": (C.c_int, [C.POINTER(gl_gate_params), C.c_float, C.c_float, C.POINTER(gl_gate_resolved)]),
+    "gl_gate_sample": (C.c_int, [C.POINTER(gl_gate_params), C.c_int32, C.c_uint32, C.POINTER(C.c_float),
+                                 C.POINTER(C.c_float), C.POINTER(C.c_float)]),
     "gl_hurb_sample": (C.c_int, [C.c_float, F32P, C.c_int32, C.c_int32, C.c_float, C.c_uint32, F32P, F32P]),
     "gl_aperture_edge_distance": (C.c_int, [C.POINTER(gl_surface), F32P, F32P, F32P]),
     "gl_bake_aperture_sdf": (C.c_int, [F32P, C.c_int32, C.c_int32, C.c_float, C.c_float, F32P]),

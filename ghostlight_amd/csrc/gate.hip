@@ -56,7 +56,7 @@ struct GateArgs {
 };
 
 // 32-bit xorshift uniform in [0, 1) (gate.h:66-72).
-__device__ __forceinline__ float next_uniform(unsigned& s) {
+__host__ __device__ __forceinline__ float next_uniform(unsigned& s) {
     s ^= s << 13;
     s ^= s >> 17;
     s ^= s << 5;
@@ -64,12 +64,12 @@ __device__ __forceinline__ float next_uniform(unsigned& s) {
 }
 
 // Clamped Cauchy (inverse CDF) and clamped Box-Muller Gaussian (gate.h:77-98).
-__device__ __forceinline__ float draw_cauchy(float sigma, float max_kick, unsigned& s) {
+__host__ __device__ __forceinline__ float draw_cauchy(float sigma, float max_kick, unsigned& s) {
     const float u = next_uniform(s);
     const float k = sigma * tanf(3.14159265358979323846f * (u - 0.5f));
     return fminf(fmaxf(k, -max_kick), max_kick);
 }
-__device__ __forceinline__ float draw_gauss(float sigma, float max_kick, unsigned& s) {
+__host__ __device__ __forceinline__ float draw_gauss(float sigma, float max_kick, unsigned& s) {
     const float u1 = fmaxf(next_uniform(s), 1.0e-7f);
     const float u2 = next_uniform(s);
     const float k = sigma * sqrtf(-2.0f * logf(u1)) * cosf(2.0f * 3.14159265358979323846f * u2);
@@ -78,7 +78,7 @@ __device__ __forceinline__ float draw_gauss(float sigma, float max_kick, unsigne
 
 // One lobe draw in the wall's own frame: along the wall normal, along the
 // wall, down the axis (gate.h:116-131).
-__device__ __forceinline__ void draw_lobe(const GateDev& g, unsigned& s, float& d_axis, float& d_other, float& d_z) {
+__host__ __device__ __forceinline__ void draw_lobe(const GateDev& g, unsigned& s, float& d_axis, float& d_other, float& d_z) {
     if (g.lobe == GL_GATE_CAUCHY_ISO) {
         d_axis = draw_cauchy(g.sig_wide, g.max_kick, s);
         d_other = draw_cauchy(g.sig_wide, g.max_kick, s);
@@ -325,6 +325,25 @@ int gl_render_gate(gl_ctx* ctx, const gl_lens* lens, const gl_source* sources, i
     GL_HIP(hipMemcpyAsync(out_b, o + 2 * npx, npx * 4, hipMemcpyDeviceToHost, c->stream));
     if (ctr) GL_HIP(hipMemcpyAsync(counters_out, ctr, 16, hipMemcpyDeviceToHost, c->stream));
     GL_HIP(hipStreamSynchronize(c->stream));
+    return GL_OK;
+}
+
+int gl_gate_resolve(const gl_gate_params* gate, float half_w, float half_h, gl_gate_resolved* out) {
+    if (!gate || !out) return set_error(GL_ERR_INVALID, "null argument");
+    static_assert(sizeof(GateDev) == sizeof(gl_gate_resolved), "gl_gate_resolved mirrors GateDev");
+    const GateDev g = resolve_gate(*gate, half_w, half_h);
+    std::memcpy(out, &g, sizeof g);
+    return GL_OK;
+}
+
+int gl_gate_sample(const gl_gate_params* gate, int32_t n, uint32_t seed, float* axis, float* other, float* z) {
+    if (!gate || n < 0 || (n && (!axis || !other || !z))) return set_error(GL_ERR_INVALID, "null argument");
+    const GateDev g = resolve_gate(*gate, 1.0f, 1.0f);
+    for (int i = 0; i < n; ++i) {
+        unsigned st = wang_hash(seed ^ (2654435761u * (uint32_t)i + 1u));
+        if (st == 0u) st = 1u;
+        draw_lobe(g, st, axis[i], other[i], z[i]);
+    }
     return GL_OK;
 }
 

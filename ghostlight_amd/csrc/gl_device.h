@@ -846,7 +846,7 @@ GLD TraceRes trace_primary(const DevSurf GLC* S, int n, const float GLC* poly, c
 }
 
 // ---- pupil sampling on the device (aperture_sampler.h:136-170) ------------
-GLD uint32_t wang_hash(uint32_t s) {
+__host__ __device__ __forceinline__ uint32_t wang_hash(uint32_t s) {
     s = (s ^ 61u) ^ (s >> 16u); s *= 9u; s ^= s >> 4u; s *= 0x27d4eb2du; s ^= s >> 15u; return s;
 }
 GLD float halton2(uint32_t n) {

@@ -1,9 +1,10 @@
-"""Host checks of the edge-diffraction pieces, under the reference's binding
-names: `_hurb_sample_debug` (bind_renderers.cpp:457-481),
-`_aperture_edge_distance_debug` (bind_trace.cpp:93) and `_aperture_sdf_debug`
-(bind_renderers.cpp:483-520).  They call the same host functions the kernels'
+"""Host checks of the edge-diffraction and film-gate pieces, under the
+reference's binding names: `_hurb_sample_debug` (bind_renderers.cpp:457-481),
+`_aperture_edge_distance_debug` (bind_trace.cpp:93), `_aperture_sdf_debug`
+(bind_renderers.cpp:483-520), `_gate_params_debug` and `_gate_sample_debug`
+(bind_renderers.cpp:416-455).  They call the same host functions the kernels'
 math is built from (gl_hurb_sample, gl_aperture_edge_distance,
-gl_bake_aperture_sdf)."""
+gl_bake_aperture_sdf, gl_gate_resolve, gl_gate_sample)."""
 from __future__ import annotations
 
 import ctypes as C
@@ -65,3 +66,26 @@ def _aperture_sdf_debug(lens: OpticalSystem) -> dict:
     out.update(stop_index=k, width=img.width, height=img.height, sdf=f[..., 0].copy(), nx=f[..., 1].copy(),
                ny=f[..., 2].copy(), sx=2.0 * aspect * semi / img.width, sy=2.0 * semi / img.height)
     return out
+
+
+def _gate_params_debug(config, sensor_half_w: float, sensor_half_h: float) -> dict:
+    """The gate as the scatter pass resolves it from a GateConfig and the
+    effective sensor half extents (every clamp applied)."""
+    from .config import gate_params
+    g = gate_params(config)
+    r = _abi.gl_gate_resolved()
+    _abi.check(_abi.load_library().gl_gate_resolve(C.byref(g), float(sensor_half_w), float(sensor_half_h),
+                                                   C.byref(r)), "_gate_params_debug")
+    return {name: getattr(r, name) for name, _t in _abi.gl_gate_resolved._fields_}
+
+
+def _gate_sample_debug(config, n: int, seed: int) -> dict:
+    """n draws of the wall's scatter lobe (axis, other, z) plus the lobe widths."""
+    from .config import gate_params
+    g = gate_params(config)
+    n = max(int(n), 0)
+    ax, ot, az = (np.zeros(n, np.float32) for _ in range(3))
+    _abi.check(_abi.load_library().gl_gate_sample(C.byref(g), n, int(seed) & 0xFFFFFFFF, _abi.fptr(ax), _abi.fptr(ot),
+                                                  _abi.fptr(az)), "_gate_sample_debug")
+    p = _gate_params_debug(config, 1.0, 1.0)
+    return {"axis": ax, "other": ot, "z": az, "sig_wide": p["sig_wide"], "sig_tight": p["sig_tight"]}

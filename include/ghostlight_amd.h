@@ -645,6 +645,30 @@ int  gl_render_gate(gl_ctx* ctx, const gl_lens* lens, const gl_source* sources, 
                     const gl_flare_params* render, const gl_gate_params* gate,
                     float* out_r, float* out_g, float* out_b, uint64_t* counters_out);
 
+/* The gate as the scatter pass resolves it (build_gpu_gate, gate.h:168-211):
+ * the opening from the effective sensor half extents and the per-side
+ * offsets, the clamped wall depth, roughness, groove frame, R0, kick cap and
+ * scatter count.  Host only; backs the reference's _gate_params_debug
+ * (bind_renderers.cpp:416-432). */
+typedef struct gl_gate_resolved {
+    float   x_pos, x_neg, y_pos, y_neg;
+    float   t_mm, zs_mm;
+    float   sig_wide, sig_tight;
+    float   cos_g, sin_g;
+    float   r0, max_kick, inv_scatter, gain;
+    int32_t n_scatter, lobe;
+} gl_gate_resolved;
+
+int  gl_gate_resolve(const gl_gate_params* gate, float sensor_half_w, float sensor_half_h,
+                     gl_gate_resolved* out);
+
+/* n draws of the wall's scatter lobe in its own frame (along the wall
+ * normal, along the wall, down the axis), draw i from the stream seeded
+ * wang(seed ^ (2654435761 i + 1)) with the device's xorshift, Cauchy and
+ * Box-Muller draws (gate_sample_lobe, gate.h:116-131).  Host only; backs
+ * the reference's _gate_sample_debug (bind_renderers.cpp:435-455). */
+int  gl_gate_sample(const gl_gate_params* gate, int32_t n, uint32_t seed, float* axis, float* other, float* z);
+
 /* Veil applied to a whole image by FFT convolution (the extended-source form
  * of render_veil, veil_render.h:54-70): out_c = veil_gain * (in_c (*) g) with
  * the unit-sum glare-spread kernel g(r) = (a^2/(r^2+a^2))^p of

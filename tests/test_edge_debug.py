@@ -180,3 +180,39 @@ def test_psf_status_enum_constants_and_vector_helpers():
     assert ghostlight.dot(a, b) == pytest.approx(11.0)
     assert abs(ghostlight.dot(c, a)) < 1e-6 and abs(ghostlight.dot(c, b)) < 1e-6
     assert (c.x, c.y, c.z) == pytest.approx((6.5, -10.0, 4.5))
+
+
+def test_gate_params_opening_offsets_and_clamps():
+    g = _impl._gate_params_debug(ghostlight.GateConfig(), 12.4, 7.0)
+    assert (g["x_pos"], g["x_neg"], g["y_pos"], g["y_neg"]) == pytest.approx((12.4, -12.4, 7.0, -7.0))
+    c = ghostlight.GateConfig()
+    c.offset_right_mm, c.offset_left_mm, c.offset_top_mm, c.offset_bottom_mm = 2.0, -1.0, 0.5, -0.5
+    g = _impl._gate_params_debug(c, 12.4, 7.0)
+    assert (g["x_pos"], g["x_neg"], g["y_pos"], g["y_neg"]) == pytest.approx((14.4, -11.4, 7.5, -6.5))
+    c = ghostlight.GateConfig()
+    c.offset_left_mm = -50.0
+    assert _impl._gate_params_debug(c, 12.4, 7.0)["x_neg"] == pytest.approx(-0.1)   # the opening cannot invert
+    c = ghostlight.GateConfig()
+    c.thickness_mm, c.standoff_mm, c.roughness_rad, c.groove_aniso = -1.0, -3.0, 99.0, 5.0
+    c.reflectance_r0, c.scatter_samples, c.max_kick_rad = 7.0, 1000, 9.0
+    g = _impl._gate_params_debug(c, 10.0, 10.0)
+    assert g["t_mm"] > 0 and g["zs_mm"] == 0.0 and g["sig_wide"] == pytest.approx(0.5) and g["r0"] == 1.0
+    assert g["n_scatter"] == 64 and g["inv_scatter"] == pytest.approx(1 / 64) and g["max_kick"] == pytest.approx(1.5)
+    c = ghostlight.GateConfig()
+    c.lobe = ghostlight.GateLobe.CAUCHY_ISO
+    g = _impl._gate_params_debug(c, 10.0, 10.0)
+    assert g["sig_tight"] == pytest.approx(g["sig_wide"])
+
+
+def test_gate_lobe_anisotropy_and_streams():
+    c = ghostlight.GateConfig()
+    c.roughness_rad, c.groove_aniso, c.groove_angle_deg = 0.1, 0.1, 0.0
+    c.lobe = ghostlight.GateLobe.GROOVED
+    d = _impl._gate_sample_debug(c, 20000, 12345)
+    across = float(np.percentile(np.abs(d["axis"]), 68))
+    along = float(np.percentile(np.abs(d["other"]), 68))
+    assert across > 3 * along                 # wide Cauchy across the marks, tight Gaussian along them
+    assert np.allclose(d["z"], 0.5 * d["axis"], rtol=1e-6, atol=1e-9)   # groove angle 0: z is half the wide draw
+    again = _impl._gate_sample_debug(c, 100, 12345)
+    assert np.array_equal(again["axis"], d["axis"][:100])
+    assert np.all(np.abs(d["axis"]) <= _impl._gate_params_debug(c, 1, 1)["max_kick"] + 1e-6)
