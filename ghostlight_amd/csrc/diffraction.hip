@@ -1,12 +1,13 @@
 // diffraction.hip — the aperture starburst (sprite engine), the veiling-glare
 // pass and the FFT seam on gfx950.
 //
-//   FFT   (fft_backend.h:25)       2-D C2C as LDS radix-2 row transforms: one
-//                                  workgroup owns one whole row in LDS (N <= 16384,
-//                                  128 KiB), twiddles from an FP64-built table;
-//                                  columns via LDS-tiled transposes.  The last
-//                                  transpose of the PSF path fuses |F|^2 and the
-//                                  fftshift, so the spectrum is never re-read.
+//   FFT   (fft_backend.h:25)       2-D C2C in place: register-resident Stockham
+//                                  row transforms of radix 16 staged through LDS
+//                                  (gl_fft.h), columns by an in-place tile-pair
+//                                  transpose, so the transform needs no scratch
+//                                  array.  The last transpose of the PSF path
+//                                  fuses |F|^2 and the fftshift, so the spectrum
+//                                  is never re-read.
 //   reductions                     two-level, FP64 partials, result left on the
 //                                  device and consumed by the next kernel (no
 //                                  host round trip, unlike starburst_render.cu:979-996).
@@ -15,6 +16,7 @@
 //                                  output pixel loops over the sources, so the
 //                                  deposit is deterministic and atomic-free.
 #include "gl_runtime.h"
+#include "gl_fft.h"
 
 #include <algorithm>
 #include <cmath>
@@ -37,33 +39,6 @@ __device__ __forceinline__ float2 cmul(float2 a, float2 b) {
     return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
 }
 
-__device__ __forceinline__ unsigned bitrev(unsigned x, int bits) { return __brev(x) >> (32 - bits); }
-
-// One row per workgroup, in-place radix-2 DIT in LDS.  tw[k] = exp(-2 pi i k / N).
-__global__ __launch_bounds__(1024) void fft_rows_kernel(float2* __restrict__ data, int N, int logN,
-                                                        const float2* __restrict__ tw) {
-    extern __shared__ float2 sm[];
-    float2* row = data + (size_t)blockIdx.x * N;
-    for (int i = threadIdx.x; i < N; i += blockDim.x) sm[bitrev((unsigned)i, logN)] = row[i];
-    __syncthreads();
-    for (int s = 1; s <= logN; ++s) {
-        const int half = 1 << (s - 1);
-        const int tstride = logN - s;
-        for (int k = threadIdx.x; k < (N >> 1); k += blockDim.x) {
-            const int j = k & (half - 1);
-            const int i0 = ((k >> (s - 1)) << s) + j;
-            const int i1 = i0 + half;
-            const float2 w = tw[j << tstride];
-            const float2 a = sm[i0];
-            const float2 b = cmul(sm[i1], w);
-            sm[i0] = make_float2(a.x + b.x, a.y + b.y);
-            sm[i1] = make_float2(a.x - b.x, a.y - b.y);
-        }
-        __syncthreads();
-    }
-    for (int i = threadIdx.x; i < N; i += blockDim.x) row[i] = sm[i];
-}
-
 // Radix-2 Stockham stage in global memory for rows longer than one
 // workgroup's LDS: half-size `ns` (1, 2, 4, ..., N/2), out of place, all rows.
 __global__ __launch_bounds__(256) void fft_stage_global_kernel(const float2* __restrict__ x, float2* __restrict__ y,
@@ -83,21 +58,6 @@ __global__ __launch_bounds__(256) void fft_stage_global_kernel(const float2* __r
     const int base = ((j / ns) * 2 * ns) + k;
     yr[base] = make_float2(a.x + b.x, a.y + b.y);
     yr[base + ns] = make_float2(a.x - b.x, a.y - b.y);
-}
-
-// 32x32 LDS-tiled transpose (padded rows avoid bank conflicts).
-__global__ __launch_bounds__(256) void transpose_kernel(const float2* __restrict__ in, float2* __restrict__ out, int N) {
-    __shared__ float2 tile[32][33];
-    const int bx = blockIdx.x * 32, by = blockIdx.y * 32;
-    for (int r = threadIdx.y; r < 32; r += 8) {
-        const int y = by + r, x = bx + threadIdx.x;
-        if (x < N && y < N) tile[r][threadIdx.x] = in[(size_t)y * N + x];
-    }
-    __syncthreads();
-    for (int r = threadIdx.y; r < 32; r += 8) {
-        const int y = bx + r, x = by + threadIdx.x;
-        if (x < N && y < N) out[(size_t)y * N + x] = tile[threadIdx.x][r];
-    }
 }
 
 // Final transpose of the PSF path fused with |F|^2 and the fftshift
@@ -152,13 +112,17 @@ static int max_lds_bytes(gl_ctx* c) {
     return c->max_lds;
 }
 
+// True when the LDS row transform of length N fits one workgroup.
+bool fft_rows_fit(gl_ctx* c, int N) {
+    return (long long)fft::rows_lds_bytes(ilog2(N)) <= (long long)max_lds_bytes(c);
+}
+
 static int fft_rows(gl_ctx* c, float2* d, int N) {
-    const float2* tw;
-    int st = fft_twiddles(c, N, &tw);
-    if (st) return st;
-    const size_t lds = (size_t)N * sizeof(float2);
-    if ((long long)lds > (long long)max_lds_bytes(c)) {
+    if (!fft_rows_fit(c, N)) {
         // Row longer than a workgroup's LDS: ping-pong Stockham stages in HBM.
+        const float2* tw;
+        int st = fft_twiddles(c, N, &tw);
+        if (st) return st;
         DevBuf& pp = c->buf("fft_pingpong");
         if ((st = pp.ensure((size_t)N * N * sizeof(float2)))) return st;
         float2 *x = d, *y = pp.as<float2>();
@@ -174,28 +138,30 @@ static int fft_rows(gl_ctx* c, float2* d, int N) {
         GL_HIP(hipGetLastError());
         return GL_OK;
     }
-    const int threads = std::min(1024, std::max(64, N / 2));
-    hipLaunchKernelGGL(fft_rows_kernel, dim3((unsigned)N), dim3(threads), lds, c->stream, d, N, ilog2(N), tw);
-    GL_HIP(hipGetLastError());
+    GL_HIP(fft::rows(c->stream, ilog2(N), fft::RowsInPlace{d, N}, N));
     return GL_OK;
 }
 
-// 2-D forward FFT in place; `tmp` is an N*N scratch.  When psf != nullptr the
-// result is written as the shifted power spectrum instead (data is clobbered).
-static int fft2d(gl_ctx* c, float2* data, float2* tmp, int N, float* psf) {
+// 2-D forward FFT in place: rows, transpose, rows, transpose.  When psf !=
+// nullptr the result is written as the shifted power spectrum instead (data
+// is clobbered).
+static int fft2d(gl_ctx* c, float2* data, int N, float* psf) {
     if (N < 2 || (N & (N - 1)) || N > 16384) return set_error(GL_ERR_INVALID, "FFT size %d is not a power of two in [2, 16384]", N);
     int st;
     if ((st = fft_rows(c, data, N))) return st;
-    const dim3 tb(32, 8), tg((unsigned)((N + 31) / 32), (unsigned)((N + 31) / 32));
-    hipLaunchKernelGGL(transpose_kernel, tg, tb, 0, c->stream, data, tmp, N);
-    if ((st = fft_rows(c, tmp, N))) return st;
-    if (psf) hipLaunchKernelGGL(transpose_power_shift_kernel, tg, tb, 0, c->stream, tmp, psf, N);
-    else hipLaunchKernelGGL(transpose_kernel, tg, tb, 0, c->stream, tmp, data, N);
-    GL_HIP(hipGetLastError());
+    GL_HIP(fft::transpose_square_inplace(c->stream, data, N));
+    if ((st = fft_rows(c, data, N))) return st;
+    if (psf) {
+        const dim3 tb(32, 8), tg((unsigned)((N + 31) / 32), (unsigned)((N + 31) / 32));
+        hipLaunchKernelGGL(transpose_power_shift_kernel, tg, tb, 0, c->stream, data, psf, N);
+        GL_HIP(hipGetLastError());
+    } else {
+        GL_HIP(fft::transpose_square_inplace(c->stream, data, N));
+    }
     return GL_OK;
 }
 
-int fft2d_dev(gl_ctx* c, float2* data, float2* tmp, int N, float* psf) { return fft2d(c, data, tmp, N, psf); }
+int fft2d_dev(gl_ctx* c, float2* data, int N, float* psf) { return fft2d(c, data, N, psf); }
 
 // ---------------------------------------------------------------------------
 // Reductions: FP64 partials, result stays on the device.
@@ -709,9 +675,7 @@ int gl_fft2d_c2c_forward_inplace(gl_ctx* ctx, void* d, int32_t N) {
     gl_ctx* c = resolve_ctx(ctx, &st);
     if (!c) return st;
     if (!d) return set_error(GL_ERR_INVALID, "null data");
-    DevBuf& tmp = c->buf("fft_tmp");
-    if ((st = tmp.ensure((size_t)N * N * sizeof(float2)))) return st;
-    return fft2d(c, static_cast<float2*>(d), tmp.as<float2>(), N, nullptr);
+    return fft2d(c, static_cast<float2*>(d), N, nullptr);
 }
 
 int gl_fft2d_c2c_forward_host(gl_ctx* ctx, const float* in, float* out, int32_t N) {
@@ -721,10 +685,10 @@ int gl_fft2d_c2c_forward_host(gl_ctx* ctx, const float* in, float* out, int32_t 
     std::lock_guard<std::mutex> g(c->mu);
     if (N < 2 || (N & (N - 1)) || N > 16384) return set_error(GL_ERR_INVALID, "FFT size %d is not a power of two in [2, 16384]", N);
     const size_t bytes = (size_t)N * N * sizeof(float2);
-    DevBuf &d = c->buf("fft_host_data"), &tmp = c->buf("fft_tmp");
-    if ((st = d.ensure(bytes)) || (st = tmp.ensure(bytes))) return st;
+    DevBuf& d = c->buf("fft_host_data");
+    if ((st = d.ensure(bytes))) return st;
     GL_HIP(hipMemcpyAsync(d.p, in, bytes, hipMemcpyHostToDevice, c->stream));
-    if ((st = fft2d(c, d.as<float2>(), tmp.as<float2>(), N, nullptr))) return st;
+    if ((st = fft2d(c, d.as<float2>(), N, nullptr))) return st;
     GL_HIP(hipMemcpyAsync(out, d.p, bytes, hipMemcpyDeviceToHost, c->stream));
     GL_HIP(hipStreamSynchronize(c->stream));
     return GL_OK;
@@ -867,10 +831,10 @@ int gl_render_starburst(gl_ctx* ctx, const gl_lens* lens, const gl_calibration* 
     }
 
     const size_t NN = (size_t)N * N;
-    DevBuf &bp = c->buf("star_pupil"), &bt = c->buf("fft_tmp"), &bpsf = c->buf("star_psf");
+    DevBuf &bp = c->buf("star_pupil"), &bpsf = c->buf("star_psf");
     DevBuf &bsp = c->buf("star_sprite"), &bds = c->buf("star_sprite_ds"), &bspec = c->buf("star_spec");
     DevBuf &benv = c->buf("star_env"), &bsum = c->buf("star_sum");
-    if ((st = bp.ensure(NN * 8)) || (st = bt.ensure(NN * 8)) || (st = bpsf.ensure(NN * 4)) ||
+    if ((st = bp.ensure(NN * 8)) || (st = bpsf.ensure(NN * 4)) ||
         (st = bsp.ensure(NN * 12)) || (st = bds.ensure(NN * 12)) || (st = bspec.ensure(spec.size() * 16)) ||
         (st = bsum.ensure(4 * sizeof(double))))
         return st;
@@ -951,7 +915,7 @@ int gl_render_starburst(gl_ctx* ctx, const gl_lens* lens, const gl_calibration* 
     }
 
     // ---- FFT -> shifted power spectrum -> unit sum -> lambda bake
-    if ((st = fft2d(c, bp.as<float2>(), bt.as<float2>(), N, bpsf.as<float>()))) return st;
+    if ((st = fft2d(c, bp.as<float2>(), N, bpsf.as<float>()))) return st;
     if ((st = reduce_to(c, bpsf.as<float>(), NN, 0, dsum + 2))) return st;
     hipLaunchKernelGGL(scale_by_inv_sum_kernel, dim3(blocks), dim3(256), 0, c->stream, bpsf.as<float>(), NN, dsum + 2);
     hipLaunchKernelGGL(bake_sprite_kernel, dim3(blocks), dim3(256), 0, c->stream, bpsf.as<float>(), N,

@@ -280,7 +280,7 @@ __global__ __launch_bounds__(256) void mdft_combine_kernel(const float* __restri
     o_b[p] += sb * gain_b;
 }
 
-int fft2d_dev(gl_ctx* c, float2* data, float2* tmp, int N, float* psf);
+int fft2d_dev(gl_ctx* c, float2* data, int N, float* psf);
 
 int mdft_render(gl_ctx* c, const float2* d_pupil, int N, float fill_in, const std::vector<float4>& spec,
                 float lambda_ref, float dx_x, float dx_y, float px_mm_x, float px_mm_y,
@@ -294,12 +294,12 @@ int mdft_render(gl_ctx* c, const float2* d_pupil, int N, float fill_in, const st
     const int Mc = 2 * mc + 1;
     const size_t NN = (size_t)N * N;
     int st;
-    DevBuf &bw = c->buf("mdft_work"), &bt = c->buf("fft_tmp"), &bc = c->buf("mdft_C");
-    if ((st = bw.ensure(NN * 8)) || (st = bt.ensure(NN * 8)) || (st = bc.ensure((size_t)Mc * Mc * 8))) return st;
+    DevBuf &bw = c->buf("mdft_work"), &bc = c->buf("mdft_C");
+    if ((st = bw.ensure(NN * 8)) || (st = bc.ensure((size_t)Mc * Mc * 8))) return st;
     GL_HIP(hipMemcpyAsync(bw.p, d_pupil, NN * 8, hipMemcpyDeviceToDevice, c->stream));
-    if ((st = fft2d_dev(c, bw.as<float2>(), bt.as<float2>(), N, nullptr))) return st;
+    if ((st = fft2d_dev(c, bw.as<float2>(), N, nullptr))) return st;
     hipLaunchKernelGGL(magsq_complex_kernel, dim3((unsigned)((NN + 255) / 256)), dim3(256), 0, c->stream, bw.as<float2>(), NN);
-    if ((st = fft2d_dev(c, bw.as<float2>(), bt.as<float2>(), N, nullptr))) return st;
+    if ((st = fft2d_dev(c, bw.as<float2>(), N, nullptr))) return st;
     hipLaunchKernelGGL(extract_c_kernel, dim3((unsigned)((Mc * Mc + 255) / 256)), dim3(256), 0, c->stream,
                        bc.as<float2>(), bw.as<float2>(), N, mc);
     float2 c00;

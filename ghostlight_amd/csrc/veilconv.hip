@@ -6,15 +6,27 @@
 // pixel offsets (r in mm from the per-axis pixel pitch; a = veil_spread *
 // sensor_half_h mm) and normalised to unit sum over its support.  The
 // convolution is linear, not circular: the image is zero-padded to a P x P
-// power of two with P >= width + half-support and height + half-support, and
-// the kernel support is the whole P x P grid centred on the origin.
+// power of two with P >= 2 max(width, height), and the kernel support is the
+// whole P x P grid centred on the origin.
 //
 // g is real and even, so its transform G is real: two colour planes ride in
-// one complex FFT (R + iG), are multiplied by G, and come back through the
-// same forward FFT by the conjugation identity ifft(X) = conj(fft(conj(X))) / P^2.
-// Per call: 2 forward + 2 inverse P x P transforms (LDS radix rows +
-// LDS-tiled transposes) for three planes; G is cached per (P, a, p, pitch).
+// one complex transform (R + iG), are multiplied by G, and come back through
+// the same forward transform by the conjugation identity
+// ifft(X) = conj(fft(conj(X))) / P^2.  One plane pair costs six passes:
+//
+//   1. row FFT of the H image rows, the zero-padded pack fused in  -> A (H x P)
+//   2. transpose A -> B (P x P); rows >= H of the padded image are zero
+//   3. row FFT of B (the column transform), epilogue conj(X * G^T)
+//   4. row FFT of B (first half of the inverse, along y)
+//   5. transpose the first H columns of B -> A
+//   6. row FFT of A's H rows (along x), epilogue writes the W output
+//      pixels of each row, scaled, into the two output planes
+//
+// Every pass streams its array once, and the two row passes over the image
+// rows touch H / P of the grid.  G^T (the transform left transposed, as
+// pass 3 wants it) is cached per (P, a, p, pitch) on the context.
 #include "gl_runtime.h"
+#include "gl_fft.h"
 
 #include <algorithm>
 #include <cmath>
@@ -22,7 +34,10 @@
 
 namespace gl {
 
-int fft2d_dev(gl_ctx* c, float2* data, float2* tmp, int N, float* psf);
+int fft2d_dev(gl_ctx* c, float2* data, int N, float* psf);
+bool fft_rows_fit(gl_ctx* c, int N);
+
+namespace {
 
 __global__ __launch_bounds__(256) void gsf_kernel(float2* __restrict__ k, int P, float px_mm_x, float px_mm_y,
                                                   float a_mm, float p) {
@@ -35,7 +50,7 @@ __global__ __launch_bounds__(256) void gsf_kernel(float2* __restrict__ k, int P,
     k[idx] = make_float2(powf(a2 / (rx * rx + ry * ry + a2), p), 0.0f);
 }
 
-// Sum of the real parts (FP64 partials).
+// Sum of the real parts (FP64 partials, one atomic per workgroup).
 __global__ __launch_bounds__(256) void sum_re_kernel(const float2* __restrict__ k, size_t n, double* __restrict__ out) {
     __shared__ double sm[256];
     double acc = 0.0;
@@ -62,7 +77,55 @@ __global__ __launch_bounds__(256) void gsf_finish_kernel(const float2* __restric
 
 __global__ void zero_d_kernel(double* d) { *d = 0.0; }
 
-// Pack two planes (b may be null) into the zero-padded complex buffer.
+// ---- fused row policies of the six-pass pipeline ----------------------------
+
+// Pass 1: reads image row y (x < W; b may be null), zero beyond W.
+struct IOPackRows {
+    const float* a;
+    const float* b;
+    int w;
+    float2* out;   // pitch p
+    int p;
+    __device__ float2 load(int row, int i) const {
+        if (i >= w) return make_float2(0.0f, 0.0f);
+        const size_t o = (size_t)row * w + i;
+        return make_float2(a[o], b ? b[o] : 0.0f);
+    }
+    __device__ void store(int row, int i, float2 v) const { out[(size_t)row * p + i] = v; }
+};
+
+// Pass 3: in place, epilogue X <- conj(X * Gt) (the multiply and the first
+// conjugation of the inverse).
+struct IOMulConj {
+    float2* d;
+    const float* g;
+    int p;
+    __device__ float2 load(int row, int i) const { return d[(size_t)row * p + i]; }
+    __device__ void store(int row, int i, float2 v) const {
+        const float s = g[(size_t)row * p + i];
+        d[(size_t)row * p + i] = make_float2(v.x * s, -(v.y * s));
+    }
+};
+
+// Pass 6: out_a = s Re(conj(Z)) = s Re(Z), out_b = s Im(conj(Z)) = -s Im(Z)
+// for the W pixels of each image row.
+struct IOUnpackRows {
+    const float2* d;
+    int p;
+    float s;
+    float* a;
+    float* b;
+    int w;
+    __device__ float2 load(int row, int i) const { return d[(size_t)row * p + i]; }
+    __device__ void store(int row, int i, float2 v) const {
+        if (i >= w) return;
+        const size_t o = (size_t)row * w + i;
+        a[o] = s * v.x;
+        if (b) b[o] = -(s * v.y);
+    }
+};
+
+// Fallback element-wise forms for grids whose rows exceed one workgroup's LDS.
 __global__ __launch_bounds__(256) void pack2_kernel(float2* __restrict__ z, int P, const float* __restrict__ a,
                                                     const float* __restrict__ b, int W, int H) {
     const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -77,16 +140,16 @@ __global__ __launch_bounds__(256) void pack2_kernel(float2* __restrict__ z, int 
     z[idx] = make_float2(re, im);
 }
 
-// X <- conj(X * G): the multiply and the first conjugation of the inverse.
-__global__ __launch_bounds__(256) void mul_conj_kernel(float2* __restrict__ z, const float* __restrict__ G, size_t n) {
+// Natural-layout multiply: z[ky][kx] <- conj(z * Gt[kx][ky]).
+__global__ __launch_bounds__(256) void mul_conj_t_kernel(float2* __restrict__ z, const float* __restrict__ Gt, int P) {
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const float g = G[i];
+    if (i >= (size_t)P * P) return;
+    const int x = (int)(i % P), y = (int)(i / P);
+    const float g = Gt[(size_t)x * P + y];
     const float2 v = z[i];
     z[i] = make_float2(v.x * g, -(v.y * g));
 }
 
-// out_a = s * Re(conj(Z)) = s * Re(Z), out_b = s * Im(conj(Z)) = -s * Im(Z).
 __global__ __launch_bounds__(256) void unpack2_kernel(const float2* __restrict__ z, int P, float s,
                                                       float* __restrict__ a, float* __restrict__ b, int W, int H) {
     const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -97,48 +160,120 @@ __global__ __launch_bounds__(256) void unpack2_kernel(const float2* __restrict__
     if (b) b[idx] = -(s * v.y);
 }
 
-static int veil_conv_dev(gl_ctx* c, const float* in_r, const float* in_g, const float* in_b, int W, int H,
-                         float px_mm_x, float px_mm_y, float a_mm, float fp, float gain, float* o_r, float* o_g,
-                         float* o_b, int* P_out) {
-    // Half-support large enough that the kernel tail is resolved across the
-    // whole frame: P >= 2 * max(W, H).
+}  // namespace
+
+// Work arrays of one convolution: A >= H x P complex, B = P x P complex,
+// Gt = P x P float, sum = one double.
+struct VeilWork {
+    float2* A;
+    float2* B;
+    float* Gt;
+    double* sum;
+};
+
+static int ilog2i(int n) { int l = 0; while ((1 << l) < n) ++l; return l; }
+
+static int veil_pad(int W, int H) {
     int P = 2;
     while (P < 2 * std::max(W, H)) P <<= 1;
+    return P;
+}
+
+// G^T into w.Gt: g on the grid (B), unit-sum scale, rows / transpose / rows,
+// i.e. the 2-D transform left transposed.
+static int veil_kernel_transform(gl_ctx* c, hipStream_t s, int P, float px_mm_x, float px_mm_y, float a_mm, float fp,
+                                 const VeilWork& w) {
+    const size_t PP = (size_t)P * P;
+    const unsigned blocks = (unsigned)((PP + 255) / 256);
+    hipLaunchKernelGGL(gsf_kernel, dim3(blocks), dim3(256), 0, s, w.B, P, px_mm_x, px_mm_y, a_mm, fp);
+    hipLaunchKernelGGL(zero_d_kernel, dim3(1), dim3(1), 0, s, w.sum);
+    hipLaunchKernelGGL(sum_re_kernel, dim3(1024), dim3(256), 0, s, w.B, PP, w.sum);
+    GL_HIP(hipGetLastError());
+    if (c && !fft_rows_fit(c, P)) {
+        int st = fft2d_dev(c, w.B, P, nullptr);
+        if (st) return st;
+        GL_HIP(fft::transpose_square_inplace(s, w.B, P));
+    } else {
+        const int L = ilog2i(P);
+        GL_HIP(fft::rows(s, L, fft::RowsInPlace{w.B, P}, P));
+        GL_HIP(fft::transpose_square_inplace(s, w.B, P));
+        GL_HIP(fft::rows(s, L, fft::RowsInPlace{w.B, P}, P));
+    }
+    hipLaunchKernelGGL(gsf_finish_kernel, dim3(blocks), dim3(256), 0, s, w.B, w.Gt, PP, w.sum);
+    GL_HIP(hipGetLastError());
+    return GL_OK;
+}
+
+// The six passes for one plane pair (b / ob may be null).
+static int veil_pair(hipStream_t s, int P, const float* a, const float* b, int W, int H, float scale, float* oa,
+                     float* ob, const VeilWork& w) {
+    const int L = ilog2i(P);
+    GL_HIP(fft::rows(s, L, IOPackRows{a, b, W, w.A, P}, H));
+    GL_HIP(fft::transpose_limited(s, w.A, H, w.B, P, P));
+    GL_HIP(fft::rows(s, L, IOMulConj{w.B, w.Gt, P}, P));
+    GL_HIP(fft::rows(s, L, fft::RowsInPlace{w.B, P}, P));
+    GL_HIP(fft::transpose_limited(s, w.B, P, w.A, H, P));
+    GL_HIP(fft::rows(s, L, IOUnpackRows{w.A, P, scale, oa, ob, W}, H));
+    return GL_OK;
+}
+
+// Grids too long for the LDS row transform: generic 2-D transforms on B.
+static int veil_pair_generic(gl_ctx* c, int P, const float* a, const float* b, int W, int H, float scale, float* oa,
+                             float* ob, const VeilWork& w) {
+    const size_t PP = (size_t)P * P;
+    const unsigned blocks = (unsigned)((PP + 255) / 256);
+    hipLaunchKernelGGL(pack2_kernel, dim3(blocks), dim3(256), 0, c->stream, w.B, P, a, b, W, H);
+    int st;
+    if ((st = fft2d_dev(c, w.B, P, nullptr))) return st;
+    hipLaunchKernelGGL(mul_conj_t_kernel, dim3(blocks), dim3(256), 0, c->stream, w.B, w.Gt, P);
+    if ((st = fft2d_dev(c, w.B, P, nullptr))) return st;
+    hipLaunchKernelGGL(unpack2_kernel, dim3((unsigned)(((size_t)W * H + 255) / 256)), dim3(256), 0, c->stream, w.B, P,
+                       scale, oa, ob, W, H);
+    GL_HIP(hipGetLastError());
+    return GL_OK;
+}
+
+struct VeilGeom {
+    float a_mm, fp, px_mm_x, px_mm_y;
+};
+
+static VeilGeom veil_geom(const gl_diffraction_params* p, int W, int H, float shw, float shh) {
+    float spread = p->veil_spread;
+    if (!(spread > 0.0f)) spread = 0.12f;
+    spread = spread < 1e-3f ? 1e-3f : (spread > 4.0f ? 4.0f : spread);
+    float fp = p->veil_falloff;
+    if (!(fp >= 1.0f)) fp = 1.0f;
+    if (fp > 3.0f) fp = 3.0f;
+    return {spread * shh, fp, 2.0f * shw / (float)W, 2.0f * shh / (float)H};
+}
+
+static int veil_conv_dev(gl_ctx* c, const float* in_r, const float* in_g, const float* in_b, int W, int H,
+                         const VeilGeom& g, float gain, float* o_r, float* o_g, float* o_b) {
+    const int P = veil_pad(W, H);
     if (P > 16384) return set_error(GL_ERR_INVALID, "image %dx%d too large for the FFT veil (P = %d)", W, H, P);
-    *P_out = P;
     const size_t PP = (size_t)P * P;
     int st;
-    DevBuf &bk = c->buf("vconv_G"), &bz = c->buf("vconv_z"), &bt = c->buf("fft_tmp"), &bs = c->buf("vconv_sum");
-    if ((st = bk.ensure(PP * 4)) || (st = bz.ensure(PP * 8)) || (st = bt.ensure(PP * 8)) || (st = bs.ensure(8))) return st;
-    const unsigned blocks = (unsigned)((PP + 255) / 256);
-    // Kernel transform, cached per geometry.
+    DevBuf &bk = c->buf("vconv_G"), &ba = c->buf("vconv_a"), &bb = c->buf("vconv_b"), &bs = c->buf("vconv_sum");
+    if ((st = bk.ensure(PP * 4)) || (st = ba.ensure((size_t)H * P * 8)) || (st = bb.ensure(PP * 8)) ||
+        (st = bs.ensure(8)))
+        return st;
+    const VeilWork w{ba.as<float2>(), bb.as<float2>(), bk.as<float>(), bs.as<double>()};
     std::vector<uint8_t> key(sizeof(int) + 4 * sizeof(float));
     std::memcpy(key.data(), &P, sizeof P);
-    const float kv[4] = {px_mm_x, px_mm_y, a_mm, fp};
+    const float kv[4] = {g.px_mm_x, g.px_mm_y, g.a_mm, g.fp};
     std::memcpy(key.data() + sizeof P, kv, sizeof kv);
     if (c->vconv_key != key) {
-        hipLaunchKernelGGL(gsf_kernel, dim3(blocks), dim3(256), 0, c->stream, bz.as<float2>(), P, px_mm_x, px_mm_y, a_mm,
-                           fp);
-        hipLaunchKernelGGL(zero_d_kernel, dim3(1), dim3(1), 0, c->stream, bs.as<double>());
-        hipLaunchKernelGGL(sum_re_kernel, dim3(1024), dim3(256), 0, c->stream, bz.as<float2>(), PP, bs.as<double>());
-        if ((st = fft2d_dev(c, bz.as<float2>(), bt.as<float2>(), P, nullptr))) return st;
-        hipLaunchKernelGGL(gsf_finish_kernel, dim3(blocks), dim3(256), 0, c->stream, bz.as<float2>(), bk.as<float>(), PP,
-                           bs.as<double>());
-        GL_HIP(hipGetLastError());
+        if ((st = veil_kernel_transform(c, c->stream, P, g.px_mm_x, g.px_mm_y, g.a_mm, g.fp, w))) return st;
         c->vconv_key = key;
     }
     const float s = gain / (float)PP;
+    const bool fused = fft_rows_fit(c, P);
     const float* ins[2][2] = {{in_r, in_g}, {in_b, nullptr}};
     float* outs[2][2] = {{o_r, o_g}, {o_b, nullptr}};
     for (int k = 0; k < 2; ++k) {
-        hipLaunchKernelGGL(pack2_kernel, dim3(blocks), dim3(256), 0, c->stream, bz.as<float2>(), P, ins[k][0], ins[k][1],
-                           W, H);
-        if ((st = fft2d_dev(c, bz.as<float2>(), bt.as<float2>(), P, nullptr))) return st;
-        hipLaunchKernelGGL(mul_conj_kernel, dim3(blocks), dim3(256), 0, c->stream, bz.as<float2>(), bk.as<float>(), PP);
-        if ((st = fft2d_dev(c, bz.as<float2>(), bt.as<float2>(), P, nullptr))) return st;
-        hipLaunchKernelGGL(unpack2_kernel, dim3((unsigned)(((size_t)W * H + 255) / 256)), dim3(256), 0, c->stream,
-                           bz.as<float2>(), P, s, outs[k][0], outs[k][1], W, H);
-        GL_HIP(hipGetLastError());
+        st = fused ? veil_pair(c->stream, P, ins[k][0], ins[k][1], W, H, s, outs[k][0], outs[k][1], w)
+                   : veil_pair_generic(c, P, ins[k][0], ins[k][1], W, H, s, outs[k][0], outs[k][1], w);
+        if (st) return st;
     }
     return GL_OK;
 }
@@ -159,14 +294,7 @@ int gl_veil_convolve(gl_ctx* ctx, const float* in_r, const float* in_g, const fl
     gl_ctx* c = resolve_ctx(ctx, &st);
     if (!c) return st;
     std::lock_guard<std::mutex> g(c->mu);
-    float spread = p->veil_spread;
-    if (!(spread > 0.0f)) spread = 0.12f;
-    spread = spread < 1e-3f ? 1e-3f : (spread > 4.0f ? 4.0f : spread);
-    const float a_mm = spread * sensor_half_h;
-    float fp = p->veil_falloff;
-    if (!(fp >= 1.0f)) fp = 1.0f;
-    if (fp > 3.0f) fp = 3.0f;
-    const float px_mm_x = 2.0f * sensor_half_w / (float)width, px_mm_y = 2.0f * sensor_half_h / (float)height;
+    const VeilGeom geom = veil_geom(p, width, height, sensor_half_w, sensor_half_h);
     const size_t npx = (size_t)width * height;
     const float *ir = in_r, *ig = in_g, *ib = in_b;
     float *o_r = out_r, *o_g = out_g, *o_b = out_b;
@@ -180,10 +308,8 @@ int gl_veil_convolve(gl_ctx* ctx, const float* in_r, const float* in_g, const fl
         ir = di; ig = di + npx; ib = di + 2 * npx;
         o_r = bo.as<float>(); o_g = o_r + npx; o_b = o_r + 2 * npx;
     }
-    int P = 0;
     GL_HIP(hipEventRecord(c->ev0, c->stream));
-    if ((st = veil_conv_dev(c, ir, ig, ib, width, height, px_mm_x, px_mm_y, a_mm, fp, p->veil_gain, o_r, o_g, o_b, &P)))
-        return st;
+    if ((st = veil_conv_dev(c, ir, ig, ib, width, height, geom, p->veil_gain, o_r, o_g, o_b))) return st;
     GL_HIP(hipEventRecord(c->ev1, c->stream));
     if (!device_ptrs) {
         GL_HIP(hipMemcpyAsync(out_r, o_r, npx * 4, hipMemcpyDeviceToHost, c->stream));

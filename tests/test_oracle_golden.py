@@ -131,9 +131,12 @@ def test_tmm_bake_matches_reference():
         assert got == np.float32(want), (nl, lam, aoi, got, want)
 
 
-@pytest.mark.skipif(pyoracle.ref() is None, reason="oracle/_ref not built (reference sources absent)")
 def test_oracle_trace_vs_reference_library_random():
     """Beyond the fixtures: 20k fresh random rays through the C1 lens."""
+    # Looked up inside the test, never at collection: the compiled reference
+    # exists only where its sources are, and is not shipped with the tree.
+    if pyoracle.ref() is None:
+        pytest.skip("oracle/_ref not built (reference sources absent)")
     from ghostlight_amd import synthetic
     lens = synthetic.config1_lens()
     surfs, arena = lens.pack()
