@@ -275,6 +275,8 @@ SIGNATURES = {
                                       C.POINTER(gl_psf_params), F32P, F32P, F32P, F32P]),
     "gl_veil_convolve": (C.c_int, [VP, VP, VP, VP, C.c_int32, C.c_int32, C.c_float, C.c_float,
                                    C.POINTER(gl_diffraction_params), C.c_int32, VP, VP, VP, F32P]),
+    "gl_veil_convolve_noalloc": (C.c_int, [VP, VP, VP, VP, C.c_int32, C.c_int32, C.c_float, C.c_float,
+                                           C.POINTER(gl_diffraction_params), VP, VP, VP]),
     "gl_cgemm_host": (C.c_int, [VP, C.c_int32, C.c_int32, C.c_int32, F32P, F32P, F32P, C.c_int32]),
     "gl_mft_psf": (C.c_int, [VP, F32P, C.c_int32, C.c_int32, C.c_float, C.c_float, F32P, C.c_int32, C.c_int32, F32P,
                              C.POINTER(F32P), F32P]),

@@ -31,6 +31,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 
 namespace gl {
 
@@ -278,6 +279,46 @@ static int veil_conv_dev(gl_ctx* c, const float* in_r, const float* in_g, const 
     return GL_OK;
 }
 
+// ---- the entry point that allocates nothing ---------------------------------
+//
+// Work arrays in one file-scope device array, sized for grids up to
+// kNoAllocMaxP (images up to kNoAllocMaxP / 2 on their longer side):
+//   B  kNoAllocMaxP^2 complex            32 MiB
+//   A  kNoAllocMaxP^2 / 2 complex        16 MiB   (H <= P / 2 rows)
+//   Gt kNoAllocMaxP^2 float              16 MiB
+//   sum                                  256 B
+// Each device the library runs on holds its own copy until the process ends.
+// Its device address is looked up once per device (hipGetSymbolAddress, with
+// that device current) and only that address is handed to kernels.
+static constexpr int kNoAllocMaxP = 2048;
+static constexpr size_t kNoAllocB = (size_t)kNoAllocMaxP * kNoAllocMaxP;   // float2 units
+static constexpr size_t kNoAllocA = kNoAllocB / 2;
+static constexpr size_t kNoAllocG = kNoAllocB / 2;                          // P^2 floats
+static constexpr size_t kNoAllocF2 = kNoAllocB + kNoAllocA + kNoAllocG + 32;
+static __device__ __align__(256) float2 scratch[kNoAllocF2];
+
+static constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static float2* g_scratch_ptr[kMaxDevices] = {};
+
+static int scratch_ptr(int device, float2** out) {
+    if (device < 0 || device >= kMaxDevices) return set_error(GL_ERR_INVALID, "device %d out of range", device);
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[device]) {
+        int prev = 0;
+        GL_HIP(hipGetDevice(&prev));
+        if (prev != device) GL_HIP(hipSetDevice(device));
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (prev != device) GL_HIP(hipSetDevice(prev));
+        if (e != hipSuccess || !p)
+            return set_error(GL_ERR_DEVICE, "hipGetSymbolAddress(scratch) -- %s", hipGetErrorString(e));
+        g_scratch_ptr[device] = static_cast<float2*>(p);
+    }
+    *out = g_scratch_ptr[device];
+    return GL_OK;
+}
+
 }  // namespace gl
 
 using namespace gl;
@@ -319,6 +360,38 @@ int gl_veil_convolve(gl_ctx* ctx, const float* in_r, const float* in_g, const fl
     GL_HIP(hipStreamSynchronize(c->stream));
     if (ms_out) GL_HIP(hipEventElapsedTime(ms_out, c->ev0, c->ev1));
     return GL_OK;
+}
+
+int gl_veil_convolve_noalloc(gl_ctx* ctx, const float* in_r, const float* in_g, const float* in_b, int32_t width,
+                             int32_t height, float sensor_half_w, float sensor_half_h,
+                             const gl_diffraction_params* p, float* out_r, float* out_g, float* out_b) {
+    if (!ctx) return set_error(GL_ERR_INVALID, "gl_veil_convolve_noalloc needs a context created beforehand");
+    if (!in_r || !in_g || !in_b || !out_r || !out_g || !out_b || !p) return set_error(GL_ERR_INVALID, "null argument");
+    if (width <= 0 || height <= 0) return set_error(GL_ERR_INVALID, "degenerate image");
+    if (!(sensor_half_w > 0.0f) || !(sensor_half_h > 0.0f)) return set_error(GL_ERR_INVALID, "sensor half extents must be positive");
+    const int P = veil_pad(width, height);
+    if (P > kNoAllocMaxP)
+        return set_error(GL_ERR_INVALID, "image %dx%d needs a %d^2 grid; gl_veil_convolve_noalloc holds at most %d^2 "
+                         "(images up to %d on the longer side)", width, height, P, kNoAllocMaxP, kNoAllocMaxP, kNoAllocMaxP / 2);
+    std::lock_guard<std::mutex> g(ctx->mu);
+    float2* base = nullptr;
+    int st = scratch_ptr(ctx->device, &base);
+    if (st) return st;
+    // Carve the arrays at this call's pitch P (A holds height <= P / 2 rows).
+    const size_t PP = (size_t)P * P;
+    VeilWork w;
+    w.B = base;
+    w.A = base + PP;
+    w.Gt = reinterpret_cast<float*>(base + PP + PP / 2);
+    w.sum = reinterpret_cast<double*>(base + PP + PP / 2 + PP / 2);
+    const VeilGeom geom = veil_geom(p, width, height, sensor_half_w, sensor_half_h);
+    hipStream_t s = ctx->stream;
+    // The kernel transform is rebuilt on every call: the array keeps whatever
+    // the last call on this device left in it.
+    if ((st = veil_kernel_transform(nullptr, s, P, geom.px_mm_x, geom.px_mm_y, geom.a_mm, geom.fp, w))) return st;
+    const float scale = p->veil_gain / (float)PP;
+    if ((st = veil_pair(s, P, in_r, in_g, width, height, scale, out_r, out_g, w))) return st;
+    return veil_pair(s, P, in_b, nullptr, width, height, scale, out_b, nullptr, w);
 }
 
 }  // extern "C"

@@ -267,3 +267,23 @@ def veil_convolve(planes, sensor_half_w: float, sensor_half_h: float, config: Fl
                                       float(sensor_half_w), float(sensor_half_h), C.byref(p), 0, vp(outs[0]),
                                       vp(outs[1]), vp(outs[2]), C.byref(ms)), "veil_convolve")
     return (outs, ms.value) if timing else outs
+
+
+def veil_convolve_device(d_in, d_out, width: int, height: int, sensor_half_w: float, sensor_half_h: float,
+                         config: FlareConfig, ctx, noalloc: bool = False) -> None:
+    """Device-plane form of veil_convolve: d_in / d_out are three device
+    addresses (ints) of width*height float32 planes on ctx's device.  With
+    noalloc the call goes to gl_veil_convolve_noalloc (no device allocation,
+    images up to 1024 on the longer side, queued on ctx's stream without
+    waiting); call ctx.synchronize() before reading d_out."""
+    p = diffraction_params(config)
+    h = ctx_handle(ctx)
+    vp = C.c_void_p
+    if noalloc:
+        _abi.check(lib().gl_veil_convolve_noalloc(h, vp(d_in[0]), vp(d_in[1]), vp(d_in[2]), int(width), int(height),
+                                                  float(sensor_half_w), float(sensor_half_h), C.byref(p),
+                                                  vp(d_out[0]), vp(d_out[1]), vp(d_out[2])), "veil_convolve_noalloc")
+    else:
+        _abi.check(lib().gl_veil_convolve(h, vp(d_in[0]), vp(d_in[1]), vp(d_in[2]), int(width), int(height),
+                                          float(sensor_half_w), float(sensor_half_h), C.byref(p), 1, vp(d_out[0]),
+                                          vp(d_out[1]), vp(d_out[2]), None), "veil_convolve")

@@ -683,6 +683,22 @@ int  gl_veil_convolve(gl_ctx* ctx, const float* in_r, const float* in_g, const f
                       const gl_diffraction_params* p, int32_t device_ptrs,
                       float* out_r, float* out_g, float* out_b, float* ms_out);
 
+/* gl_veil_convolve with device pointers, for callers that cannot allocate
+ * (code under HIP graph capture, tight serving loops): no counterpart in the
+ * reference.  It calls no hipMalloc / hipFree and no allocating library
+ * routine; its work arrays live in one file-scope device array of 64 MiB
+ * (per device, held until the process ends), so the padded grid is limited
+ * to 2048^2: width and height <= 1024, larger images are rejected with
+ * GL_ERR_INVALID.  ctx must have been created beforehand.  in_* / out_* are
+ * device planes of width*height floats on ctx's device.  The work is queued
+ * on ctx's stream and the call returns without waiting (gl_ctx_synchronize
+ * before reading out_*).  Every call on one device shares that array and
+ * keeps the last call's contents, so calls that overlap on one device (other
+ * contexts, other threads) race. */
+int  gl_veil_convolve_noalloc(gl_ctx* ctx, const float* in_r, const float* in_g, const float* in_b,
+                              int32_t width, int32_t height, float sensor_half_w, float sensor_half_h,
+                              const gl_diffraction_params* p, float* out_r, float* out_g, float* out_b);
+
 /* Survival of primary rays launched from (x, y) on the spawn plane in one
  * direction (the CPU tracer's t >= 0 rule, trace.cpp:243): the kernel behind
  * the starburst's survivor centroid and cat's-eye envelope
