@@ -36,6 +36,13 @@ using namespace orc;
 
 namespace {
 
+// Planted faults for the tests that show the statistical gates of the
+// concentrated estimator catch real mistakes (0 = none):
+//   2  survivor rects shrunk by 10% of their extent on every side
+//   3  adaptive budgets trace one 256-sample chunk fewer than they weight for
+//   4  concentrated weights drop the rect-area factor
+int g_plant_fault = 0;
+
 // ---- pupil sampler (aperture_sampler.h) -----------------------------------
 struct PMask {
     float aspect = 1.f;
@@ -173,6 +180,7 @@ inline bool sample_pos(const Job& J, const Conc* conc, int ps, int gi, const gl_
             int n_r_ps = conc->n_r, nr_lat = conc->nr_lattice;
             if (conc->budgets) { n_r_ps = conc->budgets[2 * ps]; nr_lat = conc->budgets[2 * ps + 1]; }
             if (gi >= n_r_ps) return false;
+            if (g_plant_fault == 3 && conc->budgets && gi >= n_r_ps - 256) return false;
             float fu, fv;
             if (conc->jitter == 2) { fu = halton2((uint32_t)gi); fv = halton3((uint32_t)gi); }
             else {
@@ -187,6 +195,7 @@ inline bool sample_pos(const Job& J, const Conc* conc, int ps, int gi, const gl_
             if (!mask_contains(conc->mask, u, v)) return false;
             w_ray = (rc[2] - rc[0]) * (rc[3] - rc[1]) *
                     (conc->budgets ? conc->inv_aref / (float)n_r_ps : conc->area_norm);
+            if (g_plant_fault == 4) w_ray = 4.0f * (conc->budgets ? conc->inv_aref / (float)n_r_ps : conc->area_norm);
             return true;
         }
     }
@@ -336,6 +345,8 @@ int threads_of(int t) {
 }  // namespace
 
 extern "C" {
+
+void orc_plant_fault(int mode) { g_plant_fault = mode; }
 
 // aperture_edge_distance (trace_core.h:206-290) of the oracle, float.
 int orc_edge_distance(const gl_surface* s, const float* hit, float* d, float* nrm) {
@@ -589,6 +600,10 @@ int orc_render_ghosts(int dbl, const gl_surface* s, int n, const float* arena,
                     if (!(u0 <= u1 && v0 <= v1)) continue;
                     r[0] = std::max(-1.0f, u0 - dil); r[1] = std::max(-1.0f, v0 - dil);
                     r[2] = std::min(1.0f, u1 + dil); r[3] = std::min(1.0f, v1 + dil);
+                    if (g_plant_fault == 2) {
+                        const float du = 0.1f * (r[2] - r[0]), dv = 0.1f * (r[3] - r[1]);
+                        r[0] += du; r[2] -= du; r[1] += dv; r[3] -= dv;
+                    }
                 }
                 rect_ready = true;
             }

@@ -104,6 +104,22 @@ def source_bxy(sources):
                     np.float32).reshape(-1, 2)
 
 
+class planted_fault:
+    """Context manager that plants one of the oracle's test faults in the
+    concentrated estimator (see g_plant_fault in gl_oracle.cpp)."""
+
+    def __init__(self, mode: int):
+        self.mode = int(mode)
+
+    def __enter__(self):
+        lib().orc_plant_fault(self.mode)
+        return self
+
+    def __exit__(self, *exc):
+        lib().orc_plant_fault(0)
+        return False
+
+
 def render_ghosts(lens, pairs, boosts, sources, spec, params, half_w, half_h, width, height, fmt=None,
                   correction=None, dbl=False, threads=0, want_masks=False):
     """launch_ghost_render semantics on the CPU.  params: gl_flare_params.
