@@ -478,22 +478,24 @@ static int survivor_center(gl_ctx* c, gl_lens* L, float ax, float ay, float lam,
     const float hx = span_x / (float)(S - 1), hy = span_y / (float)(S - 1);
     const int K = kSurvivorCenterSS;
     const float inv_k2 = 1.0f / (float)(K * K);
-    std::vector<int> boundary;
+    // Pass 1 marks the cells whose 8-neighbourhood changes state and lists
+    // their K x K supersamples; one launch traces them all.
+    std::vector<unsigned char> bnd((size_t)S * S, 0);
     std::vector<float2> ss;
-    double sxs = 0.0, sys = 0.0, ws = 0.0;
     for (int j = 0; j < S; ++j)
         for (int i = 0; i < S; ++i) {
             const unsigned char s = sv[(size_t)j * S + i];
-            bool bnd = false;
-            for (int dj = -1; dj <= 1 && !bnd; ++dj)
+            bool b = false;
+            for (int dj = -1; dj <= 1 && !b; ++dj)
                 for (int di = -1; di <= 1; ++di) {
                     if (!di && !dj) continue;
                     const int ni = i + di, nj = j + dj;
                     if (ni < 0 || ni >= S || nj < 0 || nj >= S) continue;
-                    if (sv[(size_t)nj * S + ni] != s) { bnd = true; break; }
+                    if (sv[(size_t)nj * S + ni] != s) { b = true; break; }
                 }
+            if (!b) continue;
+            bnd[(size_t)j * S + i] = 1;
             const float2 p = pts[(size_t)j * S + i];
-            if (!bnd) { if (s) { sxs += p.x; sys += p.y; ws += 1.0; } continue; }
             for (int sy = 0; sy < K; ++sy) {
                 const float lly = p.y + hy * ((2.0f * ((float)sy + 0.5f) / (float)K) - 1.0f);
                 for (int sx = 0; sx < K; ++sx)
@@ -502,8 +504,20 @@ static int survivor_center(gl_ctx* c, gl_lens* L, float ax, float ay, float lam,
         }
     std::vector<unsigned char> sv2;
     if ((st = survive(c, L, ss, ax, ay, lam, sv2))) return st;
-    for (size_t k = 0; k < ss.size(); ++k)
-        if (sv2[k]) { sxs += (double)ss[k].x * inv_k2; sys += (double)ss[k].y * inv_k2; ws += inv_k2; }
+    // Pass 2 sums in raster order, each boundary cell's supersamples in place
+    // of its centre, so the FP64 sums round as the reference's do.
+    double sxs = 0.0, sys = 0.0, ws = 0.0;
+    size_t k = 0;
+    for (int j = 0; j < S; ++j)
+        for (int i = 0; i < S; ++i) {
+            const size_t idx = (size_t)j * S + i;
+            if (!bnd[idx]) {
+                if (sv[idx]) { sxs += pts[idx].x; sys += pts[idx].y; ws += 1.0; }
+                continue;
+            }
+            for (int q = 0; q < K * K; ++q, ++k)
+                if (sv2[k]) { sxs += (double)ss[k].x * inv_k2; sys += (double)ss[k].y * inv_k2; ws += inv_k2; }
+        }
     if (ws > 0.0) { cx = (float)(sxs / ws); cy = (float)(sys / ws); any = true; }
     return GL_OK;
 }

@@ -47,15 +47,26 @@ def mask_contains(u, v, aspect, normals):
 
 
 def sample_env(env, u, v):
+    """Bilinear lookup of the ME x ME envelope at pupil (u, v), clamped to the
+    cell centres, in float32 with the device's operation order."""
+    f = np.float32
+    env = np.asarray(env, np.float32)
     ME = env.shape[0]
-    fx = np.clip((u * 0.5 + 0.5) * ME - 0.5, 0, ME - 1)
-    fy = np.clip((v * 0.5 + 0.5) * ME - 0.5, 0, ME - 1)
-    x0 = np.floor(fx).astype(int); y0 = np.floor(fy).astype(int)
-    x1 = np.minimum(x0 + 1, ME - 1); y1 = np.minimum(y0 + 1, ME - 1)
-    tx, ty = fx - x0, fy - y0
-    a = env[y0, x0] * (1 - tx) + env[y0, x1] * tx
-    b = env[y1, x0] * (1 - tx) + env[y1, x1] * tx
-    return a * (1 - ty) + b * ty
+    u = np.asarray(u, np.float32)
+    v = np.asarray(v, np.float32)
+    fx = ((u * f(0.5) + f(0.5)) * f(ME) - f(0.5)).astype(np.float32)
+    fy = ((v * f(0.5) + f(0.5)) * f(ME) - f(0.5)).astype(np.float32)
+    fx = np.minimum(np.maximum(fx, f(0.0)), f(ME - 1)).astype(np.float32)
+    fy = np.minimum(np.maximum(fy, f(0.0)), f(ME - 1)).astype(np.float32)
+    x0 = np.floor(fx).astype(int)
+    y0 = np.floor(fy).astype(int)
+    x1 = np.minimum(x0 + 1, ME - 1)
+    y1 = np.minimum(y0 + 1, ME - 1)
+    tx = (fx - x0.astype(np.float32)).astype(np.float32)
+    ty = (fy - y0.astype(np.float32)).astype(np.float32)
+    a = (env[y0, x0] * (f(1) - tx) + env[y0, x1] * tx).astype(np.float32)
+    b = (env[y1, x0] * (f(1) - tx) + env[y1, x1] * tx).astype(np.float32)
+    return (a * (f(1) - ty) + b * ty).astype(np.float32)
 
 
 def baffles_block(baffles, px, py, tan_ax, tan_ay):
@@ -110,7 +121,7 @@ def build_pupil(N, fill, normals, env=None, baffles=None, ep=(1.0, 1.0), tan_a=(
             U, V = np.meshgrid(u, v)
             m = mask_contains(U, V, 1.0, normals).astype(np.float64)
             if env is not None:
-                m *= sample_env(env, U.astype(np.float64), V.astype(np.float64))
+                m *= sample_env(env, U, V).astype(np.float64)
             if baffles:
                 m[baffles_block(baffles, U * np.float32(ep[0]), V * np.float32(ep[1]), tan_a[0], tan_a[1])] = 0.0
             if tex is not None:

@@ -427,6 +427,31 @@ int orc_trace_primary(int dbl, const gl_surface* s, int n, const float* arena, c
     return 0;
 }
 
+// FP32 primary survival of n launches (x, y) on the plane z0, all heading
+// normalize(tan ax, tan ay, 1) (float tangents and Vec3f::normalized, as the
+// reference's host code forms them) at one wavelength, with the CPU tracer's
+// t >= 0 rule: 1 when the ray reaches the sensor with a finite landing.  The
+// survivor centroid and the cat's-eye envelope of the starburst pupil
+// (starburst_render.cu:393-582) are built from these answers.
+int orc_primary_survival(const gl_surface* s, int n, const float* arena, const float* launch, int n_launch,
+                         float ax, float ay, float z0, float lambda_nm, const float* ior_at,
+                         unsigned char* out, int threads) {
+    Lens L; lens_init(L, s, n, arena, 0.0f);
+    const float bx = std::tan(ax), by = std::tan(ay);
+    const float len = std::sqrt(bx * bx + by * by + 1.0f * 1.0f);
+    const float inv = len > 1e-12f ? 1.0f / len : 0.0f;
+    const float dx = bx * inv, dy = by * inv, dz = 1.0f * inv;
+#pragma omp parallel for schedule(static) num_threads(threads > 0 ? threads : omp_get_max_threads())
+    for (int i = 0; i < n_launch; ++i) {
+        Ray<float> r;
+        r.o = V3<float>(launch[2 * i], launch[2 * i + 1], z0);
+        r.d = V3<float>(dx, dy, dz);
+        TraceOut<float> o = trace_primary<float>(L, r, lambda_nm, ior_at, true);
+        out[i] = (o.status == GL_TRACE_OK && std::isfinite(o.pos.x) && std::isfinite(o.pos.y)) ? 1 : 0;
+    }
+    return 0;
+}
+
 int orc_build_samples(const gl_surface* s, int n, int N, int jitter, int seed, int bo, float rdeg,
                       gl_sample* out, int cap, int* n_out) {
     std::vector<gl_sample> g = build_samples(s, n, N, jitter, seed, bo, rdeg);
