@@ -79,7 +79,7 @@ struct GhostArgs {
     unsigned long long* __restrict__ segs;      // optional segment counter
     const int2* __restrict__ lrow;              // SPLAT: per-(lambda, surface) coating lookups
     int lam_group;                              // STATS: lanes per sample (one lambda each), 0 = loop
-    int frame_mode;                             // SPLAT: records carry their source as the frame
+    int frame_mode;                             // SPLAT: records carry a frame: 1 = source, 2 = pair
     float4* __restrict__ rec;                   // SPLAT: deposit records
     unsigned long long rec_cap;
     unsigned long long* __restrict__ rec_count;
@@ -244,7 +244,8 @@ __device__ __forceinline__ void deposit_range(const GhostArgs& A, int ps, int gi
             const int x0 = (int)floorf(px - 0.5f), y0 = (int)floorf(py - 0.5f);
             if (x0 + 1 < 0 || x0 >= A.width || y0 + 1 < 0 || y0 >= A.height) break;   // footprint off the image
             const float P = sr.r * sp.y + sr.g * sp.z + sr.b * sp.w;
-            rec = make_float4(px, py, P * base, __int_as_float(si | (A.frame_mode ? (src_idx << 8) : 0)));
+            const int frame = A.frame_mode == 2 ? pair_idx : (A.frame_mode ? src_idx : 0);
+            rec = make_float4(px, py, P * base, __int_as_float(si | (frame << 8)));
             dep = true;
         } while (false);
         const unsigned long long bal = __ballot(dep);
@@ -534,6 +535,10 @@ struct Launch {
     std::vector<SrcDev> src;
     int n_src, width, height, fmt_w, fmt_h, fx0, fy0;
     float sw, sh;
+    // PER_PAIR AOV: one output frame per pair, and the pairs are rendered as
+    // the reference's single-pair launches are (no probe, cull or
+    // concentration, ghost_render.cu:1540-1553 with n_pairs = 1).
+    bool frame_by_pair = false;
 };
 
 // Host caches: pupil grid (GridKey) and spectral table (SpecKey).
@@ -584,8 +589,8 @@ static int do_launch(Launch& Lc, float* out_r, float* out_g, float* out_b, float
     ior_tables(L->host.data(), n, c->spec.data(), ns, iat.data(), ibf.data());
     const double t_prep1 = now_ms();
 
-    // Frame mode: one output frame per source, frame-major planes.
-    const int frames = p->frame_per_source ? std::max(1, n_src) : 1;
+    // Frame mode: one output frame per source (or per pair), frame-major planes.
+    const int frames = Lc.frame_by_pair ? std::max(1, n_pairs) : (p->frame_per_source ? std::max(1, n_src) : 1);
     const size_t npx = (size_t)Lc.width * Lc.height * (size_t)frames;
     DevBuf &bo_r = c->buf("out_r"), &bo_g = c->buf("out_g"), &bo_b = c->buf("out_b");
     if ((st = bo_r.ensure(npx * 4)) || (st = bo_g.ensure(npx * 4)) || (st = bo_b.ensure(npx * 4))) return st;
@@ -658,12 +663,13 @@ static int do_launch(Launch& Lc, float* out_r, float* out_g, float* out_b, float
     for (int k = 0; k < A.baffles.n; ++k) A.baffles.b[k] = p->baffles[k];
 
     // ---- probe / cull / concentrate (ghost_render.cu:969-1238)
-    const bool want_conc = p->concentrate_samples && n_pairs > 1 && ns != 3;
+    const bool probes_allowed = n_pairs > 1 && ns != 3 && !Lc.frame_by_pair;
+    const bool want_conc = p->concentrate_samples && probes_allowed;
     const int n_ps = n_pairs * n_src;
     bool cull_ready = false, rect_ready = false;
     std::vector<float4> rects;
     std::vector<unsigned char> active(n_pairs, 1);
-    if ((p->cull_dead_pairs || want_conc) && n_pairs > 1 && ns != 3) {
+    if ((p->cull_dead_pairs || want_conc) && probes_allowed) {
         DevBuf &bc = c->buf("probe_cnt"), &bb = c->buf("probe_bbox"), &bg = c->buf("probe_grid");
         DevBuf &bm = c->buf("pair_mask"), &brf = c->buf("refine_mask");
         if ((st = bc.ensure(3 * n_pairs * 8)) || (st = bb.ensure((size_t)n_ps * 5 * 4)) ||
@@ -843,7 +849,7 @@ static int do_launch(Launch& Lc, float* out_r, float* out_g, float* out_b, float
     }
     max_rec *= (unsigned long long)ns;
     const SplatGeom sg = splat_geom(Lc.width, Lc.height, frames);
-    A.frame_mode = frames > 1 ? 1 : 0;
+    A.frame_mode = frames > 1 ? (Lc.frame_by_pair ? 2 : 1) : 0;
     unsigned long long want = std::max<unsigned long long>(c->splat_hint + c->splat_hint / 4, 1ull << 22);
     want = std::max<unsigned long long>(1, std::min(want, max_rec));
     // ---- launch form: grid, or persistent over a cost-ordered work list
@@ -1084,6 +1090,22 @@ static int do_launch(Launch& Lc, float* out_r, float* out_g, float* out_b, float
     return GL_OK;
 }
 
+// Combined image of a frame-per-pair render: out_c[k] = sum over frames f, in
+// frame order, of in_c[f * npx + k] (a fixed order, so the sum is the same on
+// every run).
+__global__ __launch_bounds__(256) void sum_frames_kernel(const float* __restrict__ r, const float* __restrict__ g,
+                                                         const float* __restrict__ b, int frames, size_t npx,
+                                                         float* __restrict__ out) {
+    const size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= npx) return;
+    float sr = 0.0f, sg = 0.0f, sb = 0.0f;
+    for (int f = 0; f < frames; ++f) {
+        const size_t o = (size_t)f * npx + k;
+        sr += r[o]; sg += g[o]; sb += b[o];
+    }
+    out[k] = sr; out[npx + k] = sg; out[2 * npx + k] = sb;
+}
+
 static int validate(const gl_lens* L, const gl_flare_params* p, int width, int height) {
     if (!L || !p) return set_error(GL_ERR_INVALID, "null lens or params");
     if (width <= 0 || height <= 0) return set_error(GL_ERR_INVALID, "width and height must be positive");
@@ -1229,18 +1251,41 @@ int gl_render_ghost_pipeline(gl_ctx* ctx, const gl_lens* lens, const gl_source* 
     }
     if (pairs.empty()) return GL_OK;
     if (p->aov_mode == GL_AOV_PER_PAIR && aov_out) {
+        // All layers from one launch: each deposit record carries its pair as
+        // the frame, the resolve writes frame-major layers, and the combined
+        // image is their sum (ghost_render.cu:1526-1567 renders the pairs one
+        // launch each and sums on the host).  Pairs past aov_max_pairs are
+        // left out of both, as there.
         const int n_layers = p->aov_max_pairs < 0 ? (int)pairs.size() : std::min((int)pairs.size(), p->aov_max_pairs);
         if (n_layers > aov_cap) return set_error(GL_ERR_INVALID, "AOV capacity %d < %d layers", aov_cap, n_layers);
-        std::vector<float> tr(npx), tg(npx), tb(npx);
-        for (int i = 0; i < n_layers; ++i) {
-            float* lr = aov_out + (size_t)i * 3 * npx;
-            if ((st = gl_launch_ghost_render(ctx, lens, &pairs[i], 1, sources, n_sources, hw, hh, width, height, fmt_w,
-                                             fmt_h, fx0, fy0, p, lr, lr + npx, lr + 2 * npx, nullptr, nullptr)))
-                return st;
-            if (aov_pairs) { aov_pairs[2 * i] = pairs[i].surf_a; aov_pairs[2 * i + 1] = pairs[i].surf_b; }
-            if (out_r)
-                for (size_t k = 0; k < npx; ++k) { out_r[k] += lr[k]; out_g[k] += lr[npx + k]; out_b[k] += lr[2 * npx + k]; }
+        if (n_layers <= 0) return GL_OK;
+        pairs.resize(n_layers);
+        gl_ctx* c = resolve_ctx(ctx, &st);
+        if (!c) return st;
+        std::lock_guard<std::mutex> g(c->mu);
+        Launch Lc{c, const_cast<gl_lens*>(lens), p, pairs, {}, n_sources, width, height, fmt_w, fmt_h, fx0, fy0, hw, hh};
+        Lc.frame_by_pair = true;
+        Lc.src.reserve(n_sources);
+        for (int i = 0; i < n_sources; ++i) Lc.src.push_back(make_src(sources[i]));
+        float* dev[3] = {nullptr, nullptr, nullptr};
+        if ((st = do_launch(Lc, nullptr, nullptr, nullptr, dev, nullptr))) return st;
+        if (out_r) {
+            DevBuf& bs = c->buf("aov_sum");
+            if ((st = bs.ensure(3 * npx * sizeof(float)))) return st;
+            hipLaunchKernelGGL(sum_frames_kernel, dim3((unsigned)((npx + 255) / 256)), dim3(256), 0, c->stream, dev[0],
+                               dev[1], dev[2], n_layers, npx, bs.as<float>());
+            GL_HIP(hipGetLastError());
+            GL_HIP(hipMemcpyAsync(out_r, bs.as<float>(), npx * 4, hipMemcpyDeviceToHost, c->stream));
+            GL_HIP(hipMemcpyAsync(out_g, bs.as<float>() + npx, npx * 4, hipMemcpyDeviceToHost, c->stream));
+            GL_HIP(hipMemcpyAsync(out_b, bs.as<float>() + 2 * npx, npx * 4, hipMemcpyDeviceToHost, c->stream));
         }
+        // layer i, channel ch -> aov_out[(i * 3 + ch) * npx ...]
+        for (int ch = 0; ch < 3; ++ch)
+            GL_HIP(hipMemcpy2DAsync(aov_out + (size_t)ch * npx, 3 * npx * sizeof(float), dev[ch], npx * sizeof(float),
+                                    npx * sizeof(float), (size_t)n_layers, hipMemcpyDeviceToHost, c->stream));
+        GL_HIP(hipStreamSynchronize(c->stream));
+        if (aov_pairs)
+            for (int i = 0; i < n_layers; ++i) { aov_pairs[2 * i] = pairs[i].surf_a; aov_pairs[2 * i + 1] = pairs[i].surf_b; }
         if (aov_n) *aov_n = n_layers;
         return GL_OK;
     }

@@ -299,14 +299,24 @@ def render_sources(sources, width: int, height: int, lens: OpticalSystem, calibr
         st = _Stats(max_pairs, max_pairs * n_src)
     aov_buf = aov_pairs = None
     aov_n = C.c_int32(0)
+    aov_cap = 0
     if int(config.aov_mode) == 1:
-        aov_buf = np.zeros((max_pairs, 3, height, width), np.float32)
-        aov_pairs = np.zeros(2 * max_pairs, np.int32)
+        # One layer per pair that survives the pre-filter and the GhostFilter,
+        # capped by aov_max_pairs: the pipeline's own selection, so the host
+        # buffer holds exactly the layers it writes.
+        fp, _ = filter_ghost_pairs(lens, hw, hh, config, ctx=ctx)
+        gf = config.ghost_filter
+        if int(gf.mode) != 0 and gf.pairs:
+            want = {(int(a), int(b)) for a, b in gf.pairs}
+            fp = [q for q in fp if (q in want) == (int(gf.mode) == 1)]
+        aov_cap = len(fp) if config.aov_max_pairs < 0 else min(len(fp), int(config.aov_max_pairs))
+        aov_buf = np.zeros((max(1, aov_cap), 3, height, width), np.float32)
+        aov_pairs = np.zeros(2 * max(1, aov_cap), np.int32)
     _abi.check(lib().gl_render_ghost_pipeline(
         ctx_handle(ctx), dl.handle, sarr, n_src, float(calibration.sensor_half_w), float(calibration.sensor_half_h),
         width, height, width, height, 0, 0, C.byref(p), _abi.fptr(planes[0]), _abi.fptr(planes[1]),
         _abi.fptr(planes[2]), _abi.fptr(aov_buf) if aov_buf is not None else None,
-        _abi.iptr(aov_pairs) if aov_pairs is not None else None, max_pairs if aov_buf is not None else 0,
+        _abi.iptr(aov_pairs) if aov_pairs is not None else None, aov_cap,
         C.byref(aov_n), C.byref(st.s) if st else None), "render_source_flare")
     out["ghost_r"], out["ghost_g"], out["ghost_b"] = planes
     if config.diffraction.starburst:
