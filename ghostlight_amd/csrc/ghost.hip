@@ -78,6 +78,7 @@ struct GhostArgs {
     unsigned long long* __restrict__ cnt;       // STATS: traces | survived | on-sensor (per pair)
     unsigned long long* __restrict__ segs;      // optional segment counter
     const int2* __restrict__ lrow;              // SPLAT: per-(lambda, surface) coating lookups
+    const float2* __restrict__ rat;             // per-(lambda, surface) refraction ratios (trace_ghost)
     int lam_group;                              // STATS: lanes per sample (one lambda each), 0 = loop
     int frame_mode;                             // SPLAT: records carry a frame: 1 = source, 2 = pair
     float4* __restrict__ rec;                   // SPLAT: deposit records
@@ -206,7 +207,7 @@ __device__ __forceinline__ F3 hurb_entry(const GhostArgs& A, const SrcDev& sr, F
     return d0;
 }
 
-template <bool H>
+template <bool H, bool LF = false>
 __device__ __forceinline__ void deposit_range(const GhostArgs& A, int ps, int gi, int si0, int si1, float4* stage,
                                               int& staged, int lane) {
     const int pair_idx = ps / A.n_src, src_idx = ps % A.n_src;
@@ -227,11 +228,11 @@ __device__ __forceinline__ void deposit_range(const GhostArgs& A, int ps, int gi
             const float4 sp = make_float4(spf[0], spf[1], spf[2], spf[3]);
             uint32_t hs = 0u;
             const F3 dl = H ? hurb_entry<H>(A, sr, o0, d0, sp.x, hray, si, hs) : d0;
-            TraceRes res = trace_ghost<true, H>(as_const(A.S), A.n, as_const(A.poly), A.arena, pr.surf_a, pr.surf_b,
-                                                o0, dl, sp.x, as_const(A.iat) + (size_t)si * A.n,
-                                                as_const(A.ibf) + (size_t)si * A.n, false,
-                                                A.lrow ? as_const(A.lrow) + (size_t)si * A.n : nullptr, &A.hurb,
-                                                &hs);
+            TraceRes res = trace_ghost<true, H, LF>(as_const(A.S), A.n, as_const(A.poly), A.arena, pr.surf_a,
+                                                    pr.surf_b, o0, dl, sp.x, as_const(A.iat) + (size_t)si * A.n,
+                                                    as_const(A.ibf) + (size_t)si * A.n, false,
+                                                    A.lrow ? as_const(A.lrow) + (size_t)si * A.n : nullptr, &A.hurb,
+                                                    &hs, A.rat ? as_const(A.rat) + (size_t)si * A.n : nullptr);
             if (res.status != GL_TRACE_OK) break;
             if (!isfinite(res.px) || !isfinite(res.py)) break;
             float base = res.w * w_ray * A.gain * pr.area_boost;
@@ -274,6 +275,8 @@ struct PersistQ {
 
 // Eight waves per SIMD: the trace is VALU-bound with long dependent chains,
 // and the extra waves hide more latency than the few spilled registers cost.
+// LF: the lens fits the tracer's lean path (lean_lens).
+template <bool LF>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8, 8)))
 void ghost_persist_kernel(GhostArgs A, PersistQ Q) {
     __shared__ float4 stage_all[kBlock / 64][kStage];
@@ -290,7 +293,7 @@ void ghost_persist_kernel(GhostArgs A, PersistQ Q) {
         __syncthreads();
         if (item >= n_items) break;
         const int4 w = Q.work[item];
-        deposit_range<false>(A, w.x, w.y * kBlock + (int)threadIdx.x, w.z, w.w, stage, staged, lane);
+        deposit_range<false, LF>(A, w.x, w.y * kBlock + (int)threadIdx.x, w.z, w.w, stage, staged, lane);
     }
     if (staged) flush_stage(A, stage, staged, lane);
     __syncthreads();
@@ -629,6 +632,12 @@ static int do_launch(Launch& Lc, float* out_r, float* out_g, float* out_b, float
     lambda_rows(L->host.data(), n, L->arena.data(), c->spec.data(), ns, lrows);
     void* d_lrow;
     if ((st = upload(c, "lrow", lrows.data(), lrows.size() * sizeof(int2), &d_lrow))) return st;
+    // Refraction ratios per (lambda, surface): forward ibf / iat, backward
+    // iat / ibf, the same IEEE quotients the tracer would form.
+    std::vector<float2> rats((size_t)ns * n);
+    for (size_t q = 0; q < rats.size(); ++q) rats[q] = make_float2(ibf[q] / iat[q], iat[q] / ibf[q]);
+    void* d_rat;
+    if ((st = upload(c, "rat", rats.data(), rats.size() * sizeof(float2), &d_rat))) return st;
     if (!c->grid_uploaded || !d_grid || c->buf("grid").cap < n_grid * sizeof(gl_sample)) {
         if ((st = upload(c, "grid", c->grid.data(), n_grid * sizeof(gl_sample), &d_grid))) return st;
         c->grid_uploaded = true;
@@ -653,6 +662,7 @@ static int do_launch(Launch& Lc, float* out_r, float* out_g, float* out_b, float
     A.spec = static_cast<const float4*>(d_spec); A.n_spec = ns;
     A.iat = static_cast<const float*>(d_iat); A.ibf = static_cast<const float*>(d_ibf);
     A.lrow = static_cast<const int2*>(d_lrow);
+    A.rat = static_cast<const float2*>(d_rat);
     A.conc.rects = nullptr;
     A.hurb.on = p->hurb ? 1 : 0;
     A.hurb.dist = p->hurb_kick;
@@ -953,7 +963,12 @@ static int do_launch(Launch& Lc, float* out_r, float* out_g, float* out_b, float
             if (persistent) hipLaunchKernelGGL(persist_reset_kernel, dim3(1), dim3(1), 0, c->stream, Q);
             if (events) GL_HIP(hipEventRecord(c->ev0, c->stream));
             if (persistent) {
-                hipLaunchKernelGGL(ghost_persist_kernel, dim3((unsigned)workers), dim3(kBlock), 0, c->stream, A, Q);
+                if (D.lean)
+                    hipLaunchKernelGGL(ghost_persist_kernel<true>, dim3((unsigned)workers), dim3(kBlock), 0, c->stream,
+                                       A, Q);
+                else
+                    hipLaunchKernelGGL(ghost_persist_kernel<false>, dim3((unsigned)workers), dim3(kBlock), 0,
+                                       c->stream, A, Q);
                 GL_HIP(hipGetLastError());
             } else if ((e = launch<false, true, false>(A, n_ps, n_grid, c->stream))) {
                 return e;

@@ -37,6 +37,7 @@ enum : int32_t {
     SF_SA_SORTED = 256,       // spectral-angular wavelength and angle axes non-decreasing
     SF_DEFORMED = 512,        // polygon with authored blade shape: test r <= sa * r(theta)
     SF_IMAGE = 1024,          // IMAGE aperture with a loaded bitmap (img_* below)
+    SF_SA_UNIFORM = 2048,     // spectral-angular angle axis on an exact uniform grid (sa_a0, sa_inv_da)
 };
 
 // Blade silhouette r(theta) of a deformed polygon (gl_blade_profile as the
@@ -68,7 +69,8 @@ struct DevSurf {
     int32_t coat_model, ar_layers, discard;
     float angle_ref_ior;
     int32_t tab_off, tab_n, sa_off, sa_nw;
-    int32_t sa_na, n_asph, pad0, pad1;
+    int32_t sa_na, n_asph;
+    float sa_a0, sa_inv_da;          // SF_SA_UNIFORM: first angle, 1 / spacing
     float tint_r, tint_g, tint_b, tint_strength;
     float g_sigma, g_bg, g_peak, g_dx, g_dy;
     float asph[GL_MAX_ASPHERE_TERMS];
@@ -144,9 +146,15 @@ GLD float ref_angle_deg(float cos_i, float n1, float aref) {   // trace_core.h:6
     const float sin_i = __builtin_amdgcn_sqrtf(fmaxf(0.0f, 1.0f - cos_i * cos_i));
     const float s = (aref > 0.0f) ? (n1 * sin_i * __builtin_amdgcn_rcpf(aref)) : sin_i;
     if (s >= 1.0f) return 90.0f;
-    const float q = 1.5707963050f + s * (-0.2145988016f + s * (0.0889789874f + s * (-0.0501743046f +
-                    s * (0.0308918810f + s * (-0.0170881256f + s * (0.0066700901f + s * -0.0012624911f))))));
-    return (1.57079632679489662f - __builtin_amdgcn_sqrtf(1.0f - s) * q) * (180.0f / 3.14159265358979323846f);
+    float q = __builtin_fmaf(s, -0.0012624911f, 0.0066700901f);
+    q = __builtin_fmaf(s, q, -0.0170881256f);
+    q = __builtin_fmaf(s, q, 0.0308918810f);
+    q = __builtin_fmaf(s, q, -0.0501743046f);
+    q = __builtin_fmaf(s, q, 0.0889789874f);
+    q = __builtin_fmaf(s, q, -0.2145988016f);
+    q = __builtin_fmaf(s, q, 1.5707963050f);
+    return __builtin_fmaf(-__builtin_amdgcn_sqrtf(1.0f - s), q, 1.57079632679489662f) *
+           (180.0f / 3.14159265358979323846f);
 }
 GLD int2 ld_i2(const int2 GLC* p) {
     const int GLC* q = reinterpret_cast<const int GLC*>(p);
@@ -246,7 +254,17 @@ GLD float surface_weight(F3 dir, F3 nrm, float n1, float n2, const DevSurf GLC& 
             int ia = 0; float fa = 0.0f;
             if (ang <= an[0]) { if (oob && ang < an[0]) disc = true; }
             else if (ang >= an[na - 1]) { ia = na > 1 ? na - 2 : 0; fa = na > 1 ? 1.0f : 0.0f; if (oob && ang > an[na - 1]) disc = true; }
-            else {
+            else if (s.flags & SF_SA_UNIFORM) {
+                // First j in [1, na-1] with ang <= an[j]: an arithmetic guess,
+                // then exact steps against the table (at most one or two).
+                int j = 1 + (int)((ang - s.sa_a0) * s.sa_inv_da);
+                j = j < 1 ? 1 : (j > na - 1 ? na - 1 : j);
+                while (j > 1 && ang <= an[j - 1]) --j;
+                while (j < na - 1 && ang > an[j]) ++j;
+                ia = j - 1;
+                const float span = an[j] - an[j - 1];
+                fa = span > 0.0f ? (ang - an[j - 1]) * __builtin_amdgcn_rcpf(span) : 0.0f;
+            } else {
                 const int i = sorted ? lower_cell(an, 1, 1, na - 1, ang) : 1;
                 for (int j = i; j < na; ++j) if (ang <= an[j]) {
                     ia = j - 1; float span = an[j] - an[j - 1];
@@ -255,7 +273,9 @@ GLD float surface_weight(F3 dir, F3 nrm, float n1, float n2, const DevSurf GLC& 
             if (disc) return 0.0f;
             const int iw1 = iw + 1 < nw ? iw + 1 : iw, ia1 = ia + 1 < na ? ia + 1 : ia;
             float r00 = rr[iw * na + ia], r01 = rr[iw * na + ia1], r10 = rr[iw1 * na + ia], r11 = rr[iw1 * na + ia1];
-            R = (1.0f - fw) * ((1.0f - fa) * r00 + fa * r01) + fw * ((1.0f - fa) * r10 + fa * r11);
+            // bilinear blend (weight only: fused multiply-adds)
+            const float lo = __builtin_fmaf(fa, r01 - r00, r00), hi = __builtin_fmaf(fa, r11 - r10, r10);
+            R = __builtin_fmaf(fw, hi - lo, lo);
         }
     } else {
         R = fresnel_R(cos_i, n1, n2);
@@ -339,12 +359,15 @@ GLD bool image_passes(const DevSurf GLC& s, const float* __restrict__ arena, F3 
 // check_aperture (trace_core.h:133-170); the regular polygon is the
 // half-plane intersection max_k(h . n_k) <= apothem (same set as the
 // reference's r <= apothem / cos(offset)).
+// LF ("lean features"): the lens has no stretched and no deformed apertures
+// (see kLeanFlags), so those branches are compiled out.
+template <bool LF = false>
 GLD bool check_aperture(F3 hit, const DevSurf GLC& s, const float GLC* poly) {
-    float hx = (s.flags & SF_ASPECT) ? hit.x / s.aspect : hit.x;
+    float hx = (!LF && (s.flags & SF_ASPECT)) ? hit.x / s.aspect : hit.x;
     float hy = hit.y;
     float h2 = hx * hx + hy * hy;
     if (!(h2 <= s.sa * s.sa)) return false;
-    if (s.flags & SF_DEFORMED) return blade_inside(s, hx, hy, h2);
+    if (!LF && (s.flags & SF_DEFORMED)) return blade_inside(s, hx, hy, h2);
     if (s.flags & SF_POLYGON) {
         const float GLC* pc = poly + s.poly_off;
         const float GLC* ps = pc + GL_MAX_BLADES;
@@ -388,9 +411,11 @@ GLD float asph_dsag(float r, float inv_R, float K, const DevSurf GLC& s) {    //
 }
 
 // intersect_surface (trace_core.h:299-525).  Returns false on miss / clip.
+// LF: every surface is flat or spherical, untransformed and solved in FP32.
+template <bool LF = false>
 GLD bool intersect(F3 o, F3 d, const DevSurf GLC& s, const float GLC* poly, float& t_out, F3& n_out) {
     const float zv = s.z;
-    const bool tr = (s.flags & SF_TRANSFORMED) != 0;
+    const bool tr = !LF && (s.flags & SF_TRANSFORMED) != 0;
     if (tr) {
         o = add(m_tmul(s.rot, sub(o, f3(s.dx, s.dy, zv))), f3(0.f, 0.f, zv));
         d = m_tmul(s.rot, d);
@@ -400,7 +425,7 @@ GLD bool intersect(F3 o, F3 d, const DevSurf GLC& s, const float GLC* poly, floa
         float t = (zv - o.z) / d.z;
         if (!(t > 1e-6f)) return false;
         F3 hit = add(o, mul(d, t));
-        if (!check_aperture(hit, s, poly)) return false;
+        if (!check_aperture<LF>(hit, s, poly)) return false;
         t_out = t;
         n_out = f3(0.f, 0.f, (d.z > 0.0f) ? -1.0f : 1.0f);
         if (tr) n_out = m_mul(s.rot, n_out);
@@ -410,7 +435,7 @@ GLD bool intersect(F3 o, F3 d, const DevSurf GLC& s, const float GLC* poly, floa
         const float R = s.R;
         F3 ctr = f3(0.f, 0.f, zv + R);
         float t;
-        if (s.flags & SF_FP64) {
+        if (!LF && (s.flags & SF_FP64)) {
             // FP64 quadratic for large |R| (sag cancellation in |oc|^2 - R^2).
             const double Rd = (double)R, zd = (double)zv;
             const double ox = o.x, oy = o.y, oz = (double)o.z - (zd + Rd);
@@ -439,13 +464,14 @@ GLD bool intersect(F3 o, F3 d, const DevSurf GLC& s, const float GLC* poly, floa
             if (!pick_root((-b - sd) * inv_2a, (-b + sd) * inv_2a, o.z, d.z, zv, t)) return false;
         }
         F3 hit = add(o, mul(d, t));
-        if (!check_aperture(hit, s, poly)) return false;
+        if (!check_aperture<LF>(hit, s, poly)) return false;
         F3 nrm = mul(sub(hit, ctr), s.inv_absR);
         if (dot3(nrm, d) > 0.0f) nrm = neg(nrm);
         if (tr) nrm = m_mul(s.rot, nrm);
         t_out = t; n_out = nrm;
         return true;
     }
+    if (LF) return false;
     if (s.form == GL_FORM_CYLINDRICAL) {
         const float vx = (s.flags & SF_CYL_Y) ? 1.0f : 0.0f, vy = (s.flags & SF_CYL_Y) ? 0.0f : 1.0f;
         const float R = s.R;
@@ -460,7 +486,7 @@ GLD bool intersect(F3 o, F3 d, const DevSurf GLC& s, const float GLC* poly, floa
         float sd = sqrtf(disc), inv_2a = 0.5f / qa, t;
         if (!pick_root((-qb - sd) * inv_2a, (-qb + sd) * inv_2a, o.z, d.z, zv, t)) return false;
         F3 hit = add(o, mul(d, t));
-        if (!check_aperture(hit, s, poly)) return false;
+        if (!check_aperture<LF>(hit, s, poly)) return false;
         float hv = hit.x * vx + hit.y * vy;
         F3 nrm = f3(hv * vx, hv * vy, hit.z - zv - R);
         float nlen = sqrtf(nrm.x * nrm.x + nrm.y * nrm.y + nrm.z * nrm.z);
@@ -503,7 +529,7 @@ GLD bool intersect(F3 o, F3 d, const DevSurf GLC& s, const float GLC* poly, floa
         if (!(t > 1e-6f)) return false;
         F3 hit = add(o, mul(d, t));
         float rad = sqrtf(hit.x * hit.x + hit.y * hit.y);
-        if (!check_aperture(hit, s, poly)) return false;
+        if (!check_aperture<LF>(hit, s, poly)) return false;
         float r_safe = (rad > 1e-9f) ? rad : 1e-9f;
         float dsag = asph_dsag(rad, inv_R, K, s);
         F3 nrm = f3(-dsag * hit.x / r_safe, -dsag * hit.y / r_safe, 1.0f);
@@ -516,8 +542,8 @@ GLD bool intersect(F3 o, F3 d, const DevSurf GLC& s, const float GLC* poly, floa
     }
 }
 
-GLD bool refract(F3 dir, F3 nrm, float n1, float n2, F3& out) {   // trace_core.h:533-552
-    float n_ratio = n1 / n2;
+// n_ratio = n1 / n2, IEEE-rounded (by the caller, or from the lambda row).
+GLD bool refract_ratio(F3 dir, F3 nrm, float n_ratio, F3& out) {   // trace_core.h:533-552
     float cos_i = -dot3(nrm, dir);
     float sin2_t = n_ratio * n_ratio * (1.0f - cos_i * cos_i);
     if (sin2_t >= 1.0f) return false;
@@ -529,36 +555,39 @@ GLD bool refract(F3 dir, F3 nrm, float n1, float n2, F3& out) {   // trace_core.
     out = mul(tmp, inv);
     return true;
 }
+GLD bool refract(F3 dir, F3 nrm, float n1, float n2, F3& out) { return refract_ratio(dir, nrm, n1 / n2, out); }
 GLD F3 reflect(F3 dir, F3 nrm) { return normalized(sub(dir, mul(nrm, 2.0f * dot3(dir, nrm)))); }
 
 // One surface interaction (trace_cuda.h:226-361 loop bodies).  Returns false
 // when the ray died; res.status is set to TIR for total internal reflection.
 // W = false skips the Fresnel / coating / attenuator weights (callers that
 // need only landings, statuses and segment counts).
-template <bool W>
+// ratio: n1 / n2 for a refraction when the caller has it from a table
+// (negative: divide here).  LF: no IMAGE apertures and no attenuators.
+template <bool W, bool LF = false>
 GLD bool step(const DevSurf GLC& s, const float GLC* poly, const float* __restrict__ arena, F3& o, F3& d,
               float lam, float n2, bool reflecting, float reflect_ior, float& cur, TraceRes& res,
-              const int2 GLC* lr) {
+              const int2 GLC* lr, float ratio = -1.0f) {
     float t; F3 nrm;
     ++res.segs;
-    if (!intersect(o, d, s, poly, t, nrm)) return false;
+    if (!intersect<LF>(o, d, s, poly, t, nrm)) return false;
     F3 hit = add(o, mul(d, t));
     o = hit;
-    if ((s.flags & SF_IMAGE) && !image_passes(s, arena, hit)) return false;
+    if (!LF && (s.flags & SF_IMAGE) && !image_passes(s, arena, hit)) return false;
     const float n1 = cur;
     if (reflecting) {
         if (W) {
             res.w *= surface_weight(d, nrm, n1, n2, s, arena, lam, true, lr);
-            res.w *= attenuator(hit, s);
+            if (!LF) res.w *= attenuator(hit, s);
         }
         d = reflect(d, nrm);
         cur = reflect_ior;
     } else {
         F3 nd;
-        if (!refract(d, nrm, n1, n2, nd)) { res.status = GL_TRACE_TIR; return false; }
+        if (!refract_ratio(d, nrm, ratio >= 0.0f ? ratio : n1 / n2, nd)) { res.status = GL_TRACE_TIR; return false; }
         if (W) {
             res.w *= surface_weight(d, nrm, n1, n2, s, arena, lam, false, lr);
-            res.w *= attenuator(hit, s);
+            if (!LF) res.w *= attenuator(hit, s);
         }
         d = nd;
         cur = n2;
@@ -796,11 +825,18 @@ GLD void to_sensor(F3 o, F3 d, bool t_ge0, TraceRes& res) {   // trace_cuda.h:36
 // H = HURB: after every surface interaction the new direction takes an edge
 // kick (hurb_apply) drawn from the per-trace stream `hs`
 // (d_trace_ghost_ray<HURB>, trace_cuda.h:248-361).
-template <bool W = true, bool H = false>
+//
+// rat (may be null): this wavelength's refraction ratios, {ibf[k] / iat[k],
+// iat[k] / ibf[k]} per surface, divided on the host in IEEE float.  The index
+// on the incident side of surface k is ibf[k] on the two forward legs and
+// iat[k] on the backward one (muted surfaces are skipped, so the medium
+// carried between surfaces is exactly that), so the ratio the refraction
+// needs is rat[k].x forward and rat[k].y backward.
+template <bool W = true, bool H = false, bool LF = false>
 GLD TraceRes trace_ghost(const DevSurf GLC* S, int n, const float GLC* poly, const float* __restrict__ arena,
                          int a, int b, F3 o, F3 d, float lam, const float GLC* iat, const float GLC* ibf,
                          bool t_ge0, const int2 GLC* lrow = nullptr, const HurbDev* hb = nullptr,
-                         uint32_t* hs = nullptr) {
+                         uint32_t* hs = nullptr, const float2 GLC* rat = nullptr) {
     TraceRes res; res.status = GL_TRACE_VIGNETTED; res.w = 1.0f; res.segs = 0; res.px = res.py = res.pz = 0.f;
     float cur = 1.0f;
     const float refl_a = iat[a];
@@ -818,7 +854,13 @@ GLD TraceRes trace_ghost(const DevSurf GLC* S, int n, const float GLC* poly, con
         }
         const DevSurf GLC& s = S[k];
         if (!(s.flags & SF_ACTIVE)) continue;
-        if (!step<W>(s, poly, arena, o, d, lam, n2, refl, rior, cur, res, lrow ? lrow + k : nullptr)) return res;
+        float ratio = -1.0f;
+        if (rat) {
+            const float GLC* rk = reinterpret_cast<const float GLC*>(rat + k);
+            ratio = (i >= end1 && i < end2) ? rk[1] : rk[0];
+        }
+        if (!step<W, LF>(s, poly, arena, o, d, lam, n2, refl, rior, cur, res, lrow ? lrow + k : nullptr, ratio))
+            return res;
         if (H) {
             F3 en;
             const float ed = surface_edge(s, o, en, arena);

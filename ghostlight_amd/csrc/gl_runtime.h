@@ -92,6 +92,7 @@ struct LensDev {
     const DevSurf* surf = nullptr;
     const float* poly = nullptr;             // per polygon surface: c[32] | s[32]
     const float* arena = nullptr;
+    bool lean = false;                       // every active surface fits the tracer's lean path
 };
 }  // namespace gl
 
@@ -108,6 +109,7 @@ struct gl_lens {
         float fp64_rmin;
         gl::DevSurf* d_surf;
         float* d_poly;
+        bool lean;
     };
     std::vector<Variant> variants;
     std::mutex mu;
@@ -118,6 +120,10 @@ gl_ctx* resolve_ctx(gl_ctx* ctx, int* status);
 int lens_sync(gl_lens* lens, float fp64_rmin, hipStream_t stream, LensDev* out);
 void build_devsurfs(const gl_surface* s, int n, float fp64_rmin, const float* arena, int arena_len,
                     std::vector<DevSurf>& out, std::vector<float>& poly);
+// True when every active surface is flat or spherical, untransformed, solved
+// in FP32, with a round or regular-polygon aperture of aspect 1 and no
+// bitmap or attenuator: the tracer's lean instantiation covers the lens.
+bool lean_lens(const std::vector<DevSurf>& ds);
 // Per-lambda coating lookups of the lambda-only part of surface_weight, one
 // int2 per (lambda, surface): see LambdaRow in gl_device.h.
 void lambda_rows(const gl_surface* s, int n, const float* arena, const gl_spec* spec, int ns, std::vector<int2>& out);

@@ -391,8 +391,23 @@ void build_devsurfs(const gl_surface* s, int n, float fp64_rmin, const float* ar
         if (g.sa_n_wavelengths > 0 && g.sa_n_angles > 0 && g.sa_offset >= 0 &&
             g.sa_offset + g.sa_n_wavelengths + g.sa_n_angles <= arena_len &&
             nondecreasing(arena + g.sa_offset, g.sa_n_wavelengths, 1) &&
-            nondecreasing(arena + g.sa_offset + g.sa_n_wavelengths, g.sa_n_angles, 1))
+            nondecreasing(arena + g.sa_offset + g.sa_n_wavelengths, g.sa_n_angles, 1)) {
             f |= SF_SA_SORTED;
+            // An angle axis on an exact uniform grid (the TMM bake's 0, 5, ...,
+            // 85 deg) lets the lookup start from an arithmetic guess.
+            const float* an = arena + g.sa_offset + g.sa_n_wavelengths;
+            const int na_ = g.sa_n_angles;
+            if (na_ >= 2 && an[1] > an[0]) {
+                const float da = an[1] - an[0];
+                bool uniform = true;
+                for (int j = 2; j < na_ && uniform; ++j) uniform = an[j] == an[0] + (float)j * da;
+                if (uniform) {
+                    f |= SF_SA_UNIFORM;
+                    d.sa_a0 = an[0];
+                    d.sa_inv_da = 1.0f / da;
+                }
+            }
+        }
         d.flags = f;
         d.coat_model = g.coating_model; d.ar_layers = g.ar_layers; d.discard = g.out_of_range_discard;
         d.angle_ref_ior = g.angle_ref_ior;

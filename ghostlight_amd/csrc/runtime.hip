@@ -107,7 +107,7 @@ int lens_sync(gl_lens* L, float fp64_rmin, hipStream_t stream, LensDev* out) {
         std::vector<DevSurf> ds;
         std::vector<float> poly;
         build_devsurfs(L->host.data(), (int)L->host.size(), fp64_rmin, L->arena.data(), (int)L->arena.size(), ds, poly);
-        gl_lens::Variant nv{fp64_rmin, nullptr, nullptr};
+        gl_lens::Variant nv{fp64_rmin, nullptr, nullptr, lean_lens(ds)};
         {
             std::lock_guard<std::mutex> ga(device_mem_mutex());
             GL_HIP(hipMalloc(&nv.d_surf, std::max<size_t>(1, ds.size()) * sizeof(DevSurf)));
@@ -122,7 +122,19 @@ int lens_sync(gl_lens* L, float fp64_rmin, hipStream_t stream, LensDev* out) {
     out->surf = v->d_surf;
     out->poly = v->d_poly;
     out->arena = L->d_arena;
+    out->lean = v->lean;
     return GL_OK;
+}
+
+bool lean_lens(const std::vector<DevSurf>& ds) {
+    const int32_t heavy = SF_TRANSFORMED | SF_FP64 | SF_ASPECT | SF_DEFORMED | SF_IMAGE;
+    for (const DevSurf& d : ds) {
+        if (!(d.flags & SF_ACTIVE)) continue;
+        if (d.flags & heavy) return false;
+        if (!(d.flags & SF_FLAT) && d.form != GL_FORM_SPHERE) return false;
+        if (d.coat_model == GL_COAT_ATTENUATOR_GAUSS) return false;
+    }
+    return true;
 }
 
 }  // namespace gl
