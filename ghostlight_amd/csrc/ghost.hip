@@ -207,7 +207,7 @@ __device__ __forceinline__ F3 hurb_entry(const GhostArgs& A, const SrcDev& sr, F
     return d0;
 }
 
-template <bool H, bool LF = false>
+template <bool H, bool LF = false, int CM = CC_ALL>
 __device__ __forceinline__ void deposit_range(const GhostArgs& A, int ps, int gi, int si0, int si1, float4* stage,
                                               int& staged, int lane) {
     const int pair_idx = ps / A.n_src, src_idx = ps % A.n_src;
@@ -228,7 +228,7 @@ __device__ __forceinline__ void deposit_range(const GhostArgs& A, int ps, int gi
             const float4 sp = make_float4(spf[0], spf[1], spf[2], spf[3]);
             uint32_t hs = 0u;
             const F3 dl = H ? hurb_entry<H>(A, sr, o0, d0, sp.x, hray, si, hs) : d0;
-            TraceRes res = trace_ghost<true, H, LF>(as_const(A.S), A.n, as_const(A.poly), A.arena, pr.surf_a,
+            TraceRes res = trace_ghost<true, H, LF, CM>(as_const(A.S), A.n, as_const(A.poly), A.arena, pr.surf_a,
                                                     pr.surf_b, o0, dl, sp.x, as_const(A.iat) + (size_t)si * A.n,
                                                     as_const(A.ibf) + (size_t)si * A.n, false,
                                                     A.lrow ? as_const(A.lrow) + (size_t)si * A.n : nullptr, &A.hurb,
@@ -275,8 +275,9 @@ struct PersistQ {
 
 // Eight waves per SIMD: the trace is VALU-bound with long dependent chains,
 // and the extra waves hide more latency than the few spilled registers cost.
-// LF: the lens fits the tracer's lean path (lean_lens).
-template <bool LF>
+// LF: the lens fits the tracer's lean path (lean_lens); CM: the coating
+// classes its surfaces use (coating_classes).
+template <bool LF, int CM>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8, 8)))
 void ghost_persist_kernel(GhostArgs A, PersistQ Q) {
     __shared__ float4 stage_all[kBlock / 64][kStage];
@@ -293,7 +294,7 @@ void ghost_persist_kernel(GhostArgs A, PersistQ Q) {
         __syncthreads();
         if (item >= n_items) break;
         const int4 w = Q.work[item];
-        deposit_range<false, LF>(A, w.x, w.y * kBlock + (int)threadIdx.x, w.z, w.w, stage, staged, lane);
+        deposit_range<false, LF, CM>(A, w.x, w.y * kBlock + (int)threadIdx.x, w.z, w.w, stage, staged, lane);
     }
     if (staged) flush_stage(A, stage, staged, lane);
     __syncthreads();
@@ -963,12 +964,15 @@ static int do_launch(Launch& Lc, float* out_r, float* out_g, float* out_b, float
             if (persistent) hipLaunchKernelGGL(persist_reset_kernel, dim3(1), dim3(1), 0, c->stream, Q);
             if (events) GL_HIP(hipEventRecord(c->ev0, c->stream));
             if (persistent) {
-                if (D.lean)
-                    hipLaunchKernelGGL(ghost_persist_kernel<true>, dim3((unsigned)workers), dim3(kBlock), 0, c->stream,
-                                       A, Q);
+                const dim3 gw((unsigned)workers), bw(kBlock);
+                if (D.lean && D.coat_classes == (CC_SA | CC_FRESNEL))
+                    hipLaunchKernelGGL((ghost_persist_kernel<true, CC_SA | CC_FRESNEL>), gw, bw, 0, c->stream, A, Q);
+                else if (D.lean && D.coat_classes == (CC_AIRY | CC_FRESNEL))
+                    hipLaunchKernelGGL((ghost_persist_kernel<true, CC_AIRY | CC_FRESNEL>), gw, bw, 0, c->stream, A, Q);
+                else if (D.lean)
+                    hipLaunchKernelGGL((ghost_persist_kernel<true, CC_ALL>), gw, bw, 0, c->stream, A, Q);
                 else
-                    hipLaunchKernelGGL(ghost_persist_kernel<false>, dim3((unsigned)workers), dim3(kBlock), 0,
-                                       c->stream, A, Q);
+                    hipLaunchKernelGGL((ghost_persist_kernel<false, CC_ALL>), gw, bw, 0, c->stream, A, Q);
                 GL_HIP(hipGetLastError());
             } else if ((e = launch<false, true, false>(A, n_ps, n_grid, c->stream))) {
                 return e;

@@ -189,16 +189,40 @@ GLD float lerp1d(const float* t, int n, float key, bool disc, bool* out_disc, bo
     return t[2 * (n - 1) + 1];
 }
 
+// Coating classes a lens can use (coating_classes); a kernel instantiated
+// for a subset compiles only those branches of surface_weight.
+enum CoatClass : int {
+    CC_FRESNEL = 1,   // bare interface (also every table-less model)
+    CC_AIRY = 2,      // SIMPLE with MgF2 layers
+    CC_ROW = 4,       // lambda-only value from the lambda row (SPECTRAL, ARTIST)
+    CC_TABLE = 8,     // ANGULAR 1-D table on the angle
+    CC_SA = 16,       // SPECTRAL_ANGULAR
+    CC_ALL = 31,
+};
+
 // surface_weight (trace_core.h:635-783).
 // lr: this surface's entry of the per-lambda lookup row (the lambda-only part
 // of the weight, precomputed on the host with the same float operations), or
-// nullptr to evaluate it here.
+// nullptr to evaluate it here.  CM: the coating classes the lens uses.
+template <int CM = CC_ALL>
 GLD float surface_weight(F3 dir, F3 nrm, float n1, float n2, const DevSurf GLC& s, const float* __restrict__ arena,
                          float lam, bool reflecting, const int2 GLC* lr) {
     float cos_i = fabsf(dot3(nrm, dir));
     float R;
     const int m = s.coat_model;
-    if (m == GL_COAT_SIMPLE) {
+    if (CM == CC_FRESNEL) {
+        R = fresnel_R(cos_i, n1, n2);
+    } else if (CM == (CC_SA | CC_FRESNEL) && m != GL_COAT_SPECTRAL_ANGULAR) {
+        R = fresnel_R(cos_i, n1, n2);
+    } else if (CM == (CC_AIRY | CC_FRESNEL) && m == GL_COAT_SIMPLE && s.ar_layers <= 0) {
+        R = fresnel_R(cos_i, n1, n2);
+    } else if (CM == (CC_AIRY | CC_FRESNEL)) {
+        const float mg = 1.38f;
+        const float qw = 550.0f / (4.0f * mg);
+        R = coating_R(cos_i, n1, n2, mg, qw, lam);
+        for (int i = 1; i < s.ar_layers; ++i) R *= 0.25f;
+        R = clamp01(R);
+    } else if (m == GL_COAT_SIMPLE) {
         if (s.ar_layers <= 0) R = fresnel_R(cos_i, n1, n2);
         else {
             const float mg = 1.38f;
@@ -564,7 +588,7 @@ GLD F3 reflect(F3 dir, F3 nrm) { return normalized(sub(dir, mul(nrm, 2.0f * dot3
 // need only landings, statuses and segment counts).
 // ratio: n1 / n2 for a refraction when the caller has it from a table
 // (negative: divide here).  LF: no IMAGE apertures and no attenuators.
-template <bool W, bool LF = false>
+template <bool W, bool LF = false, int CM = CC_ALL>
 GLD bool step(const DevSurf GLC& s, const float GLC* poly, const float* __restrict__ arena, F3& o, F3& d,
               float lam, float n2, bool reflecting, float reflect_ior, float& cur, TraceRes& res,
               const int2 GLC* lr, float ratio = -1.0f) {
@@ -577,7 +601,7 @@ GLD bool step(const DevSurf GLC& s, const float GLC* poly, const float* __restri
     const float n1 = cur;
     if (reflecting) {
         if (W) {
-            res.w *= surface_weight(d, nrm, n1, n2, s, arena, lam, true, lr);
+            res.w *= surface_weight<CM>(d, nrm, n1, n2, s, arena, lam, true, lr);
             if (!LF) res.w *= attenuator(hit, s);
         }
         d = reflect(d, nrm);
@@ -586,7 +610,7 @@ GLD bool step(const DevSurf GLC& s, const float GLC* poly, const float* __restri
         F3 nd;
         if (!refract_ratio(d, nrm, ratio >= 0.0f ? ratio : n1 / n2, nd)) { res.status = GL_TRACE_TIR; return false; }
         if (W) {
-            res.w *= surface_weight(d, nrm, n1, n2, s, arena, lam, false, lr);
+            res.w *= surface_weight<CM>(d, nrm, n1, n2, s, arena, lam, false, lr);
             if (!LF) res.w *= attenuator(hit, s);
         }
         d = nd;
@@ -832,7 +856,7 @@ GLD void to_sensor(F3 o, F3 d, bool t_ge0, TraceRes& res) {   // trace_cuda.h:36
 // iat[k] on the backward one (muted surfaces are skipped, so the medium
 // carried between surfaces is exactly that), so the ratio the refraction
 // needs is rat[k].x forward and rat[k].y backward.
-template <bool W = true, bool H = false, bool LF = false>
+template <bool W = true, bool H = false, bool LF = false, int CM = CC_ALL>
 GLD TraceRes trace_ghost(const DevSurf GLC* S, int n, const float GLC* poly, const float* __restrict__ arena,
                          int a, int b, F3 o, F3 d, float lam, const float GLC* iat, const float GLC* ibf,
                          bool t_ge0, const int2 GLC* lrow = nullptr, const HurbDev* hb = nullptr,
@@ -859,7 +883,7 @@ GLD TraceRes trace_ghost(const DevSurf GLC* S, int n, const float GLC* poly, con
             const float GLC* rk = reinterpret_cast<const float GLC*>(rat + k);
             ratio = (i >= end1 && i < end2) ? rk[1] : rk[0];
         }
-        if (!step<W, LF>(s, poly, arena, o, d, lam, n2, refl, rior, cur, res, lrow ? lrow + k : nullptr, ratio))
+        if (!step<W, LF, CM>(s, poly, arena, o, d, lam, n2, refl, rior, cur, res, lrow ? lrow + k : nullptr, ratio))
             return res;
         if (H) {
             F3 en;

@@ -93,6 +93,7 @@ struct LensDev {
     const float* poly = nullptr;             // per polygon surface: c[32] | s[32]
     const float* arena = nullptr;
     bool lean = false;                       // every active surface fits the tracer's lean path
+    int coat_classes = CC_ALL;               // coating-weight specialisation (coating_classes)
 };
 }  // namespace gl
 
@@ -110,6 +111,7 @@ struct gl_lens {
         gl::DevSurf* d_surf;
         float* d_poly;
         bool lean;
+        int coat_classes;
     };
     std::vector<Variant> variants;
     std::mutex mu;
@@ -124,6 +126,10 @@ void build_devsurfs(const gl_surface* s, int n, float fp64_rmin, const float* ar
 // in FP32, with a round or regular-polygon aperture of aspect 1 and no
 // bitmap or attenuator: the tracer's lean instantiation covers the lens.
 bool lean_lens(const std::vector<DevSurf>& ds);
+// The coating-weight specialisation a lens can use: CC_SA | CC_FRESNEL when
+// every active surface is spectral-angular or a bare interface,
+// CC_AIRY | CC_FRESNEL when every active surface is SIMPLE, else CC_ALL.
+int coating_classes(const std::vector<DevSurf>& ds);
 // Per-lambda coating lookups of the lambda-only part of surface_weight, one
 // int2 per (lambda, surface): see LambdaRow in gl_device.h.
 void lambda_rows(const gl_surface* s, int n, const float* arena, const gl_spec* spec, int ns, std::vector<int2>& out);

@@ -107,7 +107,7 @@ int lens_sync(gl_lens* L, float fp64_rmin, hipStream_t stream, LensDev* out) {
         std::vector<DevSurf> ds;
         std::vector<float> poly;
         build_devsurfs(L->host.data(), (int)L->host.size(), fp64_rmin, L->arena.data(), (int)L->arena.size(), ds, poly);
-        gl_lens::Variant nv{fp64_rmin, nullptr, nullptr, lean_lens(ds)};
+        gl_lens::Variant nv{fp64_rmin, nullptr, nullptr, lean_lens(ds), coating_classes(ds)};
         {
             std::lock_guard<std::mutex> ga(device_mem_mutex());
             GL_HIP(hipMalloc(&nv.d_surf, std::max<size_t>(1, ds.size()) * sizeof(DevSurf)));
@@ -123,7 +123,27 @@ int lens_sync(gl_lens* L, float fp64_rmin, hipStream_t stream, LensDev* out) {
     out->poly = v->d_poly;
     out->arena = L->d_arena;
     out->lean = v->lean;
+    out->coat_classes = v->coat_classes;
     return GL_OK;
+}
+
+int coating_classes(const std::vector<DevSurf>& ds) {
+    int cls = 0;
+    bool all_simple = true;
+    for (const DevSurf& d : ds) {
+        if (!(d.flags & SF_ACTIVE)) continue;
+        const int m = d.coat_model;
+        if (m != GL_COAT_SIMPLE) all_simple = false;
+        if (m == GL_COAT_SIMPLE) cls |= d.ar_layers <= 0 ? CC_FRESNEL : CC_AIRY;
+        else if (m == GL_COAT_SPECTRAL_ANGULAR) cls |= (d.sa_nw > 0 && d.sa_na > 0) ? CC_SA : CC_FRESNEL;
+        else if (m == GL_COAT_SPECTRAL) cls |= d.tab_n <= 0 ? CC_FRESNEL : CC_ROW;
+        else if (m == GL_COAT_ANGULAR) cls |= d.tab_n <= 0 ? CC_FRESNEL : CC_TABLE;
+        else if (m == GL_COAT_ARTIST || m == GL_COAT_ATTENUATOR_GAUSS) cls |= CC_ROW;
+        else cls |= CC_FRESNEL;
+    }
+    if ((cls & ~(CC_SA | CC_FRESNEL)) == 0) return CC_SA | CC_FRESNEL;
+    if (all_simple) return CC_AIRY | CC_FRESNEL;
+    return CC_ALL;
 }
 
 bool lean_lens(const std::vector<DevSurf>& ds) {
