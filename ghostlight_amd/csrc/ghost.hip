@@ -224,12 +224,15 @@ __device__ __forceinline__ void deposit_range(const GhostArgs& A, int ps, int gi
         bool dep = false;
         float4 rec = make_float4(0.f, 0.f, 0.f, 0.f);
         if (live) do {
+            // Only the wavelength is needed during the trace; the colour
+            // weights are read after it, so they hold no registers across
+            // the surface loop.
             const float GLC* spf = as_const(reinterpret_cast<const float*>(A.spec)) + 4 * si;
-            const float4 sp = make_float4(spf[0], spf[1], spf[2], spf[3]);
+            const float lam = spf[0];
             uint32_t hs = 0u;
-            const F3 dl = H ? hurb_entry<H>(A, sr, o0, d0, sp.x, hray, si, hs) : d0;
+            const F3 dl = H ? hurb_entry<H>(A, sr, o0, d0, lam, hray, si, hs) : d0;
             TraceRes res = trace_ghost<true, H, LF, CM>(as_const(A.S), A.n, as_const(A.poly), A.arena, pr.surf_a,
-                                                    pr.surf_b, o0, dl, sp.x, as_const(A.iat) + (size_t)si * A.n,
+                                                    pr.surf_b, o0, dl, lam, as_const(A.iat) + (size_t)si * A.n,
                                                     as_const(A.ibf) + (size_t)si * A.n, false,
                                                     A.lrow ? as_const(A.lrow) + (size_t)si * A.n : nullptr, &A.hurb,
                                                     &hs, A.rat ? as_const(A.rat) + (size_t)si * A.n : nullptr);
@@ -244,7 +247,8 @@ __device__ __forceinline__ void deposit_range(const GhostArgs& A, int ps, int gi
             py = fmaxf(-2.0e9f, fminf(2.0e9f, py));
             const int x0 = (int)floorf(px - 0.5f), y0 = (int)floorf(py - 0.5f);
             if (x0 + 1 < 0 || x0 >= A.width || y0 + 1 < 0 || y0 >= A.height) break;   // footprint off the image
-            const float P = sr.r * sp.y + sr.g * sp.z + sr.b * sp.w;
+            const float cr = spf[1], cg = spf[2], cb = spf[3];
+            const float P = sr.r * cr + sr.g * cg + sr.b * cb;
             const int frame = A.frame_mode == 2 ? pair_idx : (A.frame_mode ? src_idx : 0);
             rec = make_float4(px, py, P * base, __int_as_float(si | (frame << 8)));
             dep = true;
