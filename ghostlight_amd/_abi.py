@@ -212,6 +212,7 @@ SIGNATURES = {
     "gl_ctx_synchronize": (C.c_int, [VP]),
     "gl_ctx_last_kernel_ms": (C.c_float, [VP]),
     "gl_ctx_last_splat_ms": (C.c_float, [VP]),
+    "gl_ctx_last_splat_counts": (C.c_int, [VP, C.POINTER(C.c_uint64)]),
     "gl_device_memory": (C.c_int, [C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
     "gl_ctx_memory_bytes": (C.c_uint64, [VP]),
     "gl_ctx_trim": (C.c_int, [VP]),

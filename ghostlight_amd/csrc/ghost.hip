@@ -1023,8 +1023,11 @@ static int do_launch(Launch& Lc, float* out_r, float* out_g, float* out_b, float
             return st;
         }
         unsigned long long counts[4] = {0, 0, 0, 0};
+        unsigned chunks[2] = {0u, 0u};
         GL_HIP(hipMemcpyAsync(&counts[0], sb.rec_count, 8, hipMemcpyDeviceToHost, c->stream));
         GL_HIP(hipMemcpyAsync(&counts[1], sb.bin_total, 8, hipMemcpyDeviceToHost, c->stream));
+        GL_HIP(hipMemcpyAsync(&chunks[0], sb.chunk_start + sg.n_tiles, 4, hipMemcpyDeviceToHost, c->stream));
+        GL_HIP(hipMemcpyAsync(&chunks[1], sb.mc_count, 4, hipMemcpyDeviceToHost, c->stream));
         if (persistent) GL_HIP(hipMemcpyAsync(&counts[2], Q.clk, 16, hipMemcpyDeviceToHost, c->stream));
         if (out_r) {
             GL_HIP(hipMemcpyAsync(out_r, A.out_r, npx * 4, hipMemcpyDeviceToHost, c->stream));
@@ -1033,6 +1036,8 @@ static int do_launch(Launch& Lc, float* out_r, float* out_g, float* out_b, float
         }
         GL_HIP(hipStreamSynchronize(c->stream));
         c->splat_hint = counts[0];
+        c->splat_counts[0] = counts[0]; c->splat_counts[1] = counts[1];
+        c->splat_counts[2] = chunks[0]; c->splat_counts[3] = chunks[1];
         // Trace-kernel time: HIP events around the launch when it is issued
         // eagerly; inside a replayed graph, the device wall clock its first
         // and last workgroups read.

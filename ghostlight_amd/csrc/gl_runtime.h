@@ -49,6 +49,7 @@ struct gl_ctx {
     float last_kernel_ms = 0.0f;         // main trace kernel, event-timed
     float last_splat_ms = 0.0f;          // tiled splat resolve after it, event-timed
     unsigned long long splat_hint = 0;   // deposit records the last render produced
+    unsigned long long splat_counts[4] = {0, 0, 0, 0};  // records, binned records, chunks, staged tiles
     // Persistent-launch autotuner: per workload shape, the measured kernel
     // time of each schedule candidate (-1 = not tried yet).
     struct TuneState { std::vector<float> ms; };

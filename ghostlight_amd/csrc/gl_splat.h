@@ -29,6 +29,12 @@ struct SplatBufs {
     unsigned long long* tile_cursor;
     unsigned* chunk_start;                // n_tiles + 1
     unsigned* queue;                      // resolve work-queue head
+    unsigned* tile_mag;                   // per tile: bits of the largest |power x colour weight|
+    unsigned* mc_slot;                    // per tile with several chunks: its staging slot
+    unsigned* mc_tile;                    // per staging slot: the tile
+    unsigned* mc_count;                   // tiles with several chunks
+    unsigned long long* stage;            // int64 fixed-point planes of those tiles (kept zero between renders)
+    unsigned stage_tiles;                 // staging slots allocated
 };
 
 SplatGeom splat_geom(int width, int height, int frames = 1);

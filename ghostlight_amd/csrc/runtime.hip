@@ -262,6 +262,15 @@ float gl_ctx_last_splat_ms(gl_ctx* ctx) {
     return c ? c->last_splat_ms : -1.0f;
 }
 
+int gl_ctx_last_splat_counts(gl_ctx* ctx, uint64_t out[4]) {
+    if (!out) return gl::set_error(GL_ERR_INVALID, "null out");
+    int st;
+    gl_ctx* c = gl::resolve_ctx(ctx, &st);
+    if (!c) return st;
+    for (int i = 0; i < 4; ++i) out[i] = c->splat_counts[i];
+    return GL_OK;
+}
+
 float gl_ctx_last_kernel_ms(gl_ctx* ctx) {
     int st;
     gl_ctx* c = gl::resolve_ctx(ctx, &st);

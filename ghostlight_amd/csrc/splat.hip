@@ -1,4 +1,5 @@
-// splat.hip — the ghost splat without global float atomics in the hot loop.
+// splat.hip — the ghost splat without global float atomics in the hot loop,
+// and with an image that does not depend on the order records arrive in.
 //
 // The trace kernel does not add into the image.  Each surviving (ray, lambda)
 // becomes one 16-byte deposit record {px, py, power, lambda index}, staged per
@@ -7,15 +8,27 @@
 //
 //   bin_count    per-workgroup LDS histogram of records over screen tiles
 //                (a bilinear footprint touches 1, 2 or 4 tiles), merged into
-//                per-tile counts
-//   bin_scan     one workgroup: exclusive scan -> tile starts / cursors and the
-//                work-chunk list (a tile with many records is split into chunks)
+//                per-tile counts; also the largest |power x colour weight|
+//                a tile receives
+//   bin_scan     one workgroup: exclusive scan -> tile starts / cursors, the
+//                work-chunk list (a tile with many records is split into
+//                chunks) and a staging slot per tile with several chunks
 //   bin_scatter  copies each record into its tiles' ranges
-//   resolve      persistent workgroups pull chunks from a queue, accumulate the
-//                chunk's bilinear contributions in an LDS tile (LDS atomics),
-//                then write the tile out: plain read-add-write when the tile has
-//                one chunk (its pixels belong to that workgroup alone), one
-//                atomic per pixel otherwise.
+//   resolve      persistent workgroups pull chunks from a queue and sum the
+//                chunk's bilinear contributions into an LDS tile; a tile with
+//                one chunk is written out by that workgroup (read-add-write),
+//                a tile with several adds its chunk sums into its staging slot
+//   finish       one workgroup per staged tile writes it out and zeroes the slot
+//
+// Every output pixel therefore has one writer, and the sums are kept in fixed
+// point: each contribution is rounded to a multiple of 2^s, with s chosen per
+// tile from its record count and largest contribution so that no int64 sum
+// can overflow (count x largest < 2^62).  Integer addition is associative, so
+// the LDS and staging atomics give the same bits in any order: a render is
+// bit-identical from run to run and across launch schedules, however the
+// trace's workgroups interleave their record flushes.  The quantum is 2^-38
+// or finer relative to the tile's largest contribution (2^-(62 - log2 count)),
+// far below float32 resolution for any pixel the tile's bright records reach.
 //
 // Frame mode: a record's lambda word also carries its frame (one frame per
 // source, rec.w bits 8..), tiles are numbered frame-major and the resolve
@@ -39,7 +52,8 @@ static constexpr int kBinBlocks = 1024;
 static constexpr int kBinThreads = 256;
 static constexpr int kChunk = 8192;          // records per resolve work item
 static constexpr int kResolveBlocks = 1024;
-static constexpr int kMaxTiles = 8192;       // LDS: 12 B a tile in bin_scatter
+static constexpr int kMaxTiles = 8192;       // LDS: 8 B a tile in bin_count, 12 B in bin_scatter
+static constexpr int kFinishBlocks = 512;
 
 // Bilinear footprint of a record: pixels x0..x0+1, y0..y0+1 clipped to the
 // image (the splat's cell arithmetic, ghost_render.cu:430-441).
@@ -80,61 +94,91 @@ __device__ __forceinline__ unsigned long long n_records(const SplatBufs& B) {
 
 __global__ __launch_bounds__(256) void splat_clear_kernel(SplatBufs B, int n_tiles) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n_tiles) { B.tile_count[i] = 0u; }
-    if (i == 0) { *B.rec_count = 0ull; *B.queue = 0u; *B.bin_total = 0ull; }
+    if (i < n_tiles) { B.tile_count[i] = 0u; B.tile_mag[i] = 0u; }
+    if (i == 0) { *B.rec_count = 0ull; *B.queue = 0u; *B.bin_total = 0ull; *B.mc_count = 0u; }
 }
 
-__global__ __launch_bounds__(kBinThreads) void bin_count_kernel(SplatBufs B, SplatGeom g) {
-    extern __shared__ unsigned hist[];
+// Upper bound on every contribution a record makes: |power| x the largest
+// |colour weight| of its lambda, rounded like the products the resolve forms
+// (rounding is monotonic, and the bilinear weights are at most 1).  Returned
+// as float bits, which order like the non-negative floats they encode.
+__device__ __forceinline__ unsigned record_mag(const float4& r, const float4* __restrict__ spec) {
+    const float4 s = spec[__float_as_int(r.w) & 0xFF];
+    const float m = fmaxf(fabsf(s.y), fmaxf(fabsf(s.z), fabsf(s.w)));
+    return __float_as_uint(fabsf(r.z) * m);
+}
+
+__global__ __launch_bounds__(kBinThreads) void bin_count_kernel(SplatBufs B, SplatGeom g,
+                                                                const float4* __restrict__ spec) {
+    extern __shared__ unsigned hist[];            // counts, then magnitudes (LDS form)
+    unsigned* lmag = hist + g.n_tiles;
     if (g.lds_hist) {
-        for (int t = threadIdx.x; t < g.n_tiles; t += blockDim.x) hist[t] = 0u;
+        for (int t = threadIdx.x; t < 2 * g.n_tiles; t += blockDim.x) hist[t] = 0u;
         __syncthreads();
     }
     unsigned* h = g.lds_hist ? hist : B.tile_count;
+    unsigned* hm = g.lds_hist ? lmag : B.tile_mag;
     const unsigned long long n = n_records(B);
     const unsigned long long per = (n + gridDim.x - 1) / gridDim.x;
     const unsigned long long r0 = per * blockIdx.x, r1 = min(n, r0 + per);
     for (unsigned long long i = r0 + threadIdx.x; i < r1; i += blockDim.x) {
+        const float4 r = B.rec[i];
         int tx0, tx1, ty0, ty1, base;
-        if (!tiles_of(B.rec[i], g, tx0, tx1, ty0, ty1, base)) continue;
+        if (!tiles_of(r, g, tx0, tx1, ty0, ty1, base)) continue;
+        const unsigned m = record_mag(r, spec);
         for (int ty = ty0; ty <= ty1; ++ty)
-            for (int tx = tx0; tx <= tx1; ++tx) atomicAdd(&h[base + ty * g.tiles_x + tx], 1u);
+            for (int tx = tx0; tx <= tx1; ++tx) {
+                const int t = base + ty * g.tiles_x + tx;
+                atomicAdd(&h[t], 1u);
+                atomicMax(&hm[t], m);
+            }
     }
     if (g.lds_hist) {
         __syncthreads();
         for (int t = threadIdx.x; t < g.n_tiles; t += blockDim.x)
-            if (hist[t]) atomicAdd(&B.tile_count[t], hist[t]);
+            if (hist[t]) {
+                atomicAdd(&B.tile_count[t], hist[t]);
+                atomicMax(&B.tile_mag[t], lmag[t]);
+            }
     }
 }
 
 // Exclusive scan of tile counts (one workgroup of 1024 lanes, a contiguous
-// span of tiles each): tile_start, tile_cursor, chunk_start, and the totals.
+// span of tiles each): tile_start, tile_cursor, chunk_start, the staging slots
+// of tiles with several chunks, and the totals.
 __global__ __launch_bounds__(1024) void bin_scan_kernel(SplatBufs B, SplatGeom g) {
     __shared__ unsigned long long s_rec[1024];
     __shared__ unsigned s_chk[1024];
+    __shared__ unsigned s_mc[1024];
     const int per = (g.n_tiles + 1023) / 1024;
     const int t0 = threadIdx.x * per, t1 = min(g.n_tiles, t0 + per);
     unsigned long long rs = 0;
-    unsigned cs = 0;
+    unsigned cs = 0, ms = 0;
     for (int t = t0; t < t1; ++t) {
         const unsigned c = B.tile_count[t];
         rs += c;
         cs += (c + kChunk - 1) / kChunk;
+        ms += c > (unsigned)kChunk ? 1u : 0u;
     }
     s_rec[threadIdx.x] = rs;
     s_chk[threadIdx.x] = cs;
+    s_mc[threadIdx.x] = ms;
     __syncthreads();
     // Hillis-Steele inclusive scan over the 1024 lane totals.
     for (int off = 1; off < 1024; off <<= 1) {
-        const unsigned long long a = threadIdx.x >= (unsigned)off ? s_rec[threadIdx.x - off] : 0ull;
-        const unsigned b = threadIdx.x >= (unsigned)off ? s_chk[threadIdx.x - off] : 0u;
+        const bool in = threadIdx.x >= (unsigned)off;
+        const unsigned long long a = in ? s_rec[threadIdx.x - off] : 0ull;
+        const unsigned b = in ? s_chk[threadIdx.x - off] : 0u;
+        const unsigned m = in ? s_mc[threadIdx.x - off] : 0u;
         __syncthreads();
         s_rec[threadIdx.x] += a;
         s_chk[threadIdx.x] += b;
+        s_mc[threadIdx.x] += m;
         __syncthreads();
     }
     unsigned long long run = threadIdx.x ? s_rec[threadIdx.x - 1] : 0ull;
     unsigned crun = threadIdx.x ? s_chk[threadIdx.x - 1] : 0u;
+    unsigned mrun = threadIdx.x ? s_mc[threadIdx.x - 1] : 0u;
     for (int t = t0; t < t1; ++t) {
         const unsigned c = B.tile_count[t];
         B.tile_start[t] = run;
@@ -142,10 +186,16 @@ __global__ __launch_bounds__(1024) void bin_scan_kernel(SplatBufs B, SplatGeom g
         B.chunk_start[t] = crun;
         run += c;
         crun += (c + kChunk - 1) / kChunk;
+        if (c > (unsigned)kChunk) {
+            B.mc_slot[t] = mrun;
+            if (mrun < B.stage_tiles) B.mc_tile[mrun] = (unsigned)t;
+            ++mrun;
+        }
     }
     if (threadIdx.x == 1023) {
         *B.bin_total = s_rec[1023];
         B.chunk_start[g.n_tiles] = s_chk[1023];
+        *B.mc_count = s_mc[1023];
     }
 }
 
@@ -198,11 +248,27 @@ __global__ __launch_bounds__(kBinThreads) void bin_scatter_kernel(SplatBufs B, S
     }
 }
 
+// Fixed-point exponent of a tile: every contribution is below 2^e (e from the
+// tile's largest one) and a pixel receives at most `count` of them, so with
+// quantum 2^s, s = e + bitlength(count) - 62, every int64 sum stays below 2^62.
+__device__ __forceinline__ int tile_quantum(unsigned count, unsigned mag_bits) {
+    int e;
+    (void)frexpf(__uint_as_float(mag_bits), &e);
+    return e + (32 - __clz(count)) - 62;
+}
+
+// A contribution rounded to the nearest multiple of 2^s, in units of 2^s.
+// ldexpf is exact here (the scaled value is at most 2^62), so the rounding
+// happens once, in the conversion.
+__device__ __forceinline__ long long to_fixed(float x, int s) { return __float2ll_rn(ldexpf(x, -s)); }
+
+__device__ __forceinline__ float from_fixed(long long v, int s) { return (float)ldexp((double)v, s); }
+
 // One work item = up to kChunk records of one tile.
 __global__ __launch_bounds__(256) void resolve_kernel(SplatBufs B, SplatGeom g, const float4* __restrict__ spec,
                                                       float* __restrict__ out_r, float* __restrict__ out_g,
                                                       float* __restrict__ out_b) {
-    extern __shared__ float tile[];               // 3 planes of TS x TS
+    extern __shared__ unsigned long long acc[];   // 3 fixed-point planes of TS x TS
     __shared__ unsigned s_item;
     const int TS = 1 << g.tile_shift, TT = TS * TS;
     const unsigned n_items = B.chunk_start[g.n_tiles];
@@ -224,7 +290,8 @@ __global__ __launch_bounds__(256) void resolve_kernel(SplatBufs B, SplatGeom g, 
         const unsigned long long rb = B.tile_start[t] + (unsigned long long)k * kChunk;
         const unsigned long long re = min(min(B.tile_start[t] + B.tile_count[t], rb + kChunk), bin_n);
         const int nchunks = (int)(B.chunk_start[t + 1] - B.chunk_start[t]);
-        for (int i = threadIdx.x; i < 3 * TT; i += blockDim.x) tile[i] = 0.0f;
+        const int sq = tile_quantum(B.tile_count[t], B.tile_mag[t]);
+        for (int i = threadIdx.x; i < 3 * TT; i += blockDim.x) acc[i] = 0ull;
         __syncthreads();
         const int frame = t / g.tiles_per_frame, tf = t % g.tiles_per_frame;
         const int ox = (tf % g.tiles_x) << g.tile_shift, oy = (tf / g.tiles_x) << g.tile_shift;
@@ -238,34 +305,68 @@ __global__ __launch_bounds__(256) void resolve_kernel(SplatBufs B, SplatGeom g, 
             const float w01 = (1.0f - f.fx) * f.fy, w11 = f.fx * f.fy;
             const int xs[4] = {f.x0, f.x0 + 1, f.x0, f.x0 + 1}, ys[4] = {f.y0, f.y0, f.y0 + 1, f.y0 + 1};
             const float ws[4] = {w00, w10, w01, w11};
+            const bool ur = fabsf(cr) > 1e-14f, ug = fabsf(cg) > 1e-14f, ub = fabsf(cb) > 1e-14f;
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
                 const int lx = xs[q] - ox, ly = ys[q] - oy;
                 if (xs[q] < 0 || xs[q] >= g.width || ys[q] < 0 || ys[q] >= g.height) continue;
                 if (lx < 0 || lx >= TS || ly < 0 || ly >= TS) continue;
                 const int p = ly * TS + lx;
-                if (fabsf(cr) > 1e-14f) atomicAdd(&tile[p], cr * ws[q]);
-                if (fabsf(cg) > 1e-14f) atomicAdd(&tile[TT + p], cg * ws[q]);
-                if (fabsf(cb) > 1e-14f) atomicAdd(&tile[2 * TT + p], cb * ws[q]);
+                const long long vr = ur ? to_fixed(cr * ws[q], sq) : 0ll;
+                const long long vg = ug ? to_fixed(cg * ws[q], sq) : 0ll;
+                const long long vb = ub ? to_fixed(cb * ws[q], sq) : 0ll;
+                if (vr) atomicAdd(&acc[p], (unsigned long long)vr);
+                if (vg) atomicAdd(&acc[TT + p], (unsigned long long)vg);
+                if (vb) atomicAdd(&acc[2 * TT + p], (unsigned long long)vb);
             }
         }
         __syncthreads();
+        if (nchunks == 1) {
+            for (int p = threadIdx.x; p < TT; p += blockDim.x) {
+                const int x = ox + (p & (TS - 1)), y = oy + (p >> g.tile_shift);
+                if (x >= g.width || y >= g.height) continue;
+                const size_t o = fo + (size_t)y * g.width + x;
+                const long long vr = (long long)acc[p], vg = (long long)acc[TT + p], vb = (long long)acc[2 * TT + p];
+                if (vr) out_r[o] += from_fixed(vr, sq);
+                if (vg) out_g[o] += from_fixed(vg, sq);
+                if (vb) out_b[o] += from_fixed(vb, sq);
+            }
+        } else {
+            const unsigned slot = B.mc_slot[t];
+            if (slot < B.stage_tiles) {
+                unsigned long long* st = B.stage + (size_t)slot * 3 * TT;
+                for (int i = threadIdx.x; i < 3 * TT; i += blockDim.x)
+                    if (acc[i]) atomicAdd(st + i, acc[i]);
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// Tiles with several chunks: one workgroup each converts the staged sums,
+// adds them to the image and leaves the slot zero for the next render.
+__global__ __launch_bounds__(256) void resolve_finish_kernel(SplatBufs B, SplatGeom g, float* __restrict__ out_r,
+                                                             float* __restrict__ out_g, float* __restrict__ out_b) {
+    const int TS = 1 << g.tile_shift, TT = TS * TS;
+    const unsigned n = min(*B.mc_count, B.stage_tiles);
+    for (unsigned i = blockIdx.x; i < n; i += gridDim.x) {
+        const int t = (int)B.mc_tile[i];
+        const int sq = tile_quantum(B.tile_count[t], B.tile_mag[t]);
+        const int frame = t / g.tiles_per_frame, tf = t % g.tiles_per_frame;
+        const int ox = (tf % g.tiles_x) << g.tile_shift, oy = (tf / g.tiles_x) << g.tile_shift;
+        const size_t fo = (size_t)frame * g.width * g.height;
+        unsigned long long* st = B.stage + (size_t)i * 3 * TT;
         for (int p = threadIdx.x; p < TT; p += blockDim.x) {
+            const long long vr = (long long)st[p], vg = (long long)st[TT + p], vb = (long long)st[2 * TT + p];
+            if (!(vr | vg | vb)) continue;
+            st[p] = 0ull; st[TT + p] = 0ull; st[2 * TT + p] = 0ull;
             const int x = ox + (p & (TS - 1)), y = oy + (p >> g.tile_shift);
             if (x >= g.width || y >= g.height) continue;
             const size_t o = fo + (size_t)y * g.width + x;
-            const float vr = tile[p], vg = tile[TT + p], vb = tile[2 * TT + p];
-            if (nchunks == 1) {
-                if (vr != 0.0f) out_r[o] += vr;
-                if (vg != 0.0f) out_g[o] += vg;
-                if (vb != 0.0f) out_b[o] += vb;
-            } else {
-                if (vr != 0.0f) atomicAdd(out_r + o, vr);
-                if (vg != 0.0f) atomicAdd(out_g + o, vg);
-                if (vb != 0.0f) atomicAdd(out_b + o, vb);
-            }
+            if (vr) out_r[o] += from_fixed(vr, sq);
+            if (vg) out_g[o] += from_fixed(vg, sq);
+            if (vb) out_b[o] += from_fixed(vb, sq);
         }
-        __syncthreads();
     }
 }
 
@@ -294,13 +395,15 @@ int splat_prepare(gl_ctx* c, const SplatGeom& g, unsigned long long rec_cap, Spl
     if (g.tiles_per_frame > kMaxTiles)
         return set_error(GL_ERR_INVALID, "image too large for the tiled splat (%d tiles a frame)", g.tiles_per_frame);
     DevBuf &brec = c->buf("splat_rec"), &bbin = c->buf("splat_bin"), &bt = c->buf("splat_tiles");
+    DevBuf& bst = c->buf("splat_stage");
     int st;
     const unsigned long long bin_cap = rec_cap + rec_cap / 2;
     if ((st = brec.ensure((size_t)rec_cap * sizeof(float4))) || (st = bbin.ensure((size_t)bin_cap * sizeof(float4))))
         return st;
-    // tile arrays: count (u32), start/cursor (u64), chunk_start (u32, n+1), scalars
+    // tile arrays: start/cursor (u64), scalars, count (u32), chunk_start
+    // (u32, n+1), magnitude, staging slot, staged tile (u32 each)
     const size_t nt = (size_t)g.n_tiles;
-    const size_t bytes = nt * 4 + nt * 8 * 2 + (nt + 1) * 4 + 64;
+    const size_t bytes = nt * 8 * 2 + 64 + nt * 4 + (nt + 1) * 4 + nt * 4 * 3;
     if ((st = bt.ensure(bytes + 256))) return st;
     char* p = bt.as<char>();
     B.rec = brec.as<float4>();
@@ -314,6 +417,24 @@ int splat_prepare(gl_ctx* c, const SplatGeom& g, unsigned long long rec_cap, Spl
     B.tile_count = reinterpret_cast<unsigned*>(B.bin_total + 1);
     B.chunk_start = B.tile_count + nt;
     B.queue = B.chunk_start + nt + 1;
+    B.mc_count = B.queue + 1;
+    B.tile_mag = B.mc_count + 1;
+    B.mc_slot = B.tile_mag + nt;
+    B.mc_tile = B.mc_slot + nt;
+    // Staging for tiles with several chunks: each holds more than kChunk of
+    // the binned records, so there are at most bin_cap / kChunk of them (a
+    // render whose binning overflows is redone with larger buffers; its
+    // excess tiles are skipped, never written out of bounds).
+    const int TS = 1 << g.tile_shift;
+    const unsigned long long mc_max = std::min<unsigned long long>(nt, B.bin_cap / kChunk + 1);
+    const size_t stage_bytes = (size_t)mc_max * 3 * TS * TS * sizeof(unsigned long long);
+    const bool fresh = !(bst.p && stage_bytes <= bst.cap);
+    if ((st = bst.ensure(stage_bytes))) return st;
+    // The finish pass zeroes every slot it drains, so only a new allocation
+    // needs clearing (here, outside any captured sequence).
+    if (fresh) GL_HIP(hipMemsetAsync(bst.p, 0, bst.cap, c->stream));
+    B.stage = bst.as<unsigned long long>();
+    B.stage_tiles = (unsigned)(bst.cap / ((size_t)3 * TS * TS * sizeof(unsigned long long)));
     return GL_OK;
 }
 
@@ -325,14 +446,15 @@ int splat_clear(hipStream_t s, const SplatGeom& g, const SplatBufs& B) {
 
 int splat_resolve(hipStream_t s, const SplatGeom& g, const SplatBufs& B, const float4* spec, float* out_r, float* out_g,
                   float* out_b) {
-    const size_t hist_bytes = g.lds_hist ? (size_t)g.n_tiles * 4 : 0;
+    const size_t hist_bytes = g.lds_hist ? (size_t)g.n_tiles * 8 : 0;
     const size_t scat_bytes = g.lds_hist ? (size_t)((g.n_tiles + 1) & ~1) * 4 + (size_t)g.n_tiles * 8 : 0;
-    hipLaunchKernelGGL(bin_count_kernel, dim3(kBinBlocks), dim3(kBinThreads), hist_bytes, s, B, g);
+    hipLaunchKernelGGL(bin_count_kernel, dim3(kBinBlocks), dim3(kBinThreads), hist_bytes, s, B, g, spec);
     hipLaunchKernelGGL(bin_scan_kernel, dim3(1), dim3(1024), 0, s, B, g);
     hipLaunchKernelGGL(bin_scatter_kernel, dim3(kBinBlocks), dim3(kBinThreads), scat_bytes, s, B, g);
     const int TS = 1 << g.tile_shift;
-    hipLaunchKernelGGL(resolve_kernel, dim3(kResolveBlocks), dim3(256), (size_t)3 * TS * TS * 4, s, B, g, spec, out_r,
-                       out_g, out_b);
+    hipLaunchKernelGGL(resolve_kernel, dim3(kResolveBlocks), dim3(256), (size_t)3 * TS * TS * sizeof(unsigned long long),
+                       s, B, g, spec, out_r, out_g, out_b);
+    hipLaunchKernelGGL(resolve_finish_kernel, dim3(kFinishBlocks), dim3(256), 0, s, B, g, out_r, out_g, out_b);
     GL_HIP(hipGetLastError());
     return GL_OK;
 }

@@ -48,6 +48,12 @@ class Context:
     def last_splat_ms(self) -> float:
         return float(lib().gl_ctx_last_splat_ms(self._h))
 
+    def last_splat_counts(self) -> dict:
+        """Deposit-record counts of the last ghost render on this context."""
+        a = (C.c_uint64 * 4)()
+        _abi.check(lib().gl_ctx_last_splat_counts(self._h, a), "gl_ctx_last_splat_counts")
+        return {"records": int(a[0]), "binned": int(a[1]), "chunks": int(a[2]), "staged_tiles": int(a[3])}
+
     def last_schedule(self) -> dict:
         """Schedule the last ghost render on this context used."""
         a = (C.c_int32 * 8)()

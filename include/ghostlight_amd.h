@@ -273,6 +273,13 @@ int  gl_ctx_synchronize(gl_ctx* ctx);
 float gl_ctx_last_kernel_ms(gl_ctx* ctx);
 float gl_ctx_last_splat_ms(gl_ctx* ctx);
 
+/* Deposit-record counts of the last ghost render on ctx (build-specific, no
+ * reference counterpart): out[0] records the trace wrote, out[1] records
+ * binned into tiles (a record touching k tiles counts k times), out[2]
+ * resolve work chunks, out[3] tiles with several chunks (summed through the
+ * fixed-point staging pass). */
+int   gl_ctx_last_splat_counts(gl_ctx* ctx, uint64_t out[4]);
+
 /* HBM accounting for admission control (build-specific; the reference has
  * no serving layer).  gl_device_memory: free / total bytes of the current
  * device.  gl_ctx_memory_bytes: device bytes a context's buffer cache holds.
