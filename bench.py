@@ -27,6 +27,11 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 FP32_VECTOR_PEAK_TFLOPS = 157.3   # MI355X published FP32 vector peak
+# Algorithmic FP32 ops per traced segment on this workload (adds, multiplies,
+# divides, square roots and compares of the tracer's arithmetic), counted by
+# instrumenting the FP32 restatement of the trace on a sample of its rays;
+# tests/test_bench_constants.py recounts it.
+OPS_PER_SEGMENT = 116.08
 # HBM bytes per launch of the trace kernel on this workload, from rocprofv3
 # FETCH_SIZE / WRITE_SIZE runs scaled by a 1 GiB copy: 147 MiB read (lens,
 # tables, pupil grid, spill stack) + 961 MiB written (849 MiB of deposit
@@ -45,9 +50,25 @@ def workload():
     return lens, cfg, src, W, H, hw, hh
 
 
+def cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor() or platform.machine()
+
+
+def host_threads() -> int:
+    return int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+
+
 def ops_per_segment(lens, pairs, cfg, src, n_pairs_sample=12, n_rays=256):
     """Algorithmic FP32 ops per traced segment, counted by the oracle's
-    counting scalar on a sample of this workload's rays."""
+    counting scalar on a sample of this workload's rays (the count behind
+    OPS_PER_SEGMENT; called by the test that keeps the constant honest)."""
     import ctypes as C
     from oracle import pyoracle
     from ghostlight_amd import hostprep
@@ -90,7 +111,7 @@ def cpu_baseline(lens, cfg, src, pairs, boosts, W, H, hw, hh, target_s=15.0):
     spec = hostprep.spectral_table(cfg.spectral_samples)
     C = hostprep.spectral_correction(spec)
     p, _keep = to_params(cfg)
-    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    threads = host_threads()
     k = 2
     t0 = time.perf_counter()
     _, info = pyoracle.render_ghosts(lens, pairs[:k], boosts[:k], src, spec, p, hw, hh, W, H, correction=C,
@@ -103,7 +124,7 @@ def cpu_baseline(lens, cfg, src, pairs, boosts, W, H, hw, hh, target_s=15.0):
     dt = time.perf_counter() - t0
     return {"value": info["segments"] / dt, "unit": "segments/s", "cores": threads, "kind": "port",
             "sample": f"FP32 oracle render of the first {k2} of {len(pairs)} ghost pairs of the same workload "
-                      f"({info['segments']} segments in {dt:.2f} s, {platform.processor() or 'x86_64'})"}
+                      f"({info['segments']} segments in {dt:.2f} s, {cpu_model()})"}
 
 
 def precision_sample(lens, cfg, src, pairs, boosts, W, H, hw, hh, k, ctx):
@@ -126,12 +147,88 @@ def precision_sample(lens, cfg, src, pairs, boosts, W, H, hw, hh, k, ctx):
             "sample": f"ghost image of the first {k} of {len(pairs)} pairs of the same workload"}
 
 
-def precision_starburst_leg(ghost_planes, W, H, hw, hh, check=(400.0, 550.0, 700.0)):
+def veil_pad(W, H):
+    """Side of the zero-padded FFT grid of the veil convolution: the least
+    power of two >= 2 max(W, H) (linear, not circular, convolution)."""
+    P = 2
+    while P < 2 * max(W, H):
+        P *= 2
+    return P
+
+
+def veil_pass_bytes(W, H):
+    """HBM bytes the six-pass veil pipeline streams for a W x H, 3-plane
+    image: per plane pair, pass 1 reads the planes and writes the H x P row
+    transforms (A), pass 2 reads A and writes the P x P grid (B), passes 3
+    and 4 read and write B (pass 3 also reads the P x P float kernel), pass 5
+    reads H columns of B and writes A, pass 6 reads A and writes the planes."""
+    P = veil_pad(W, H)
+    A, B, G, plane = H * P * 8, P * P * 8, P * P * 4, W * H * 4
+    return sum(5 * A + 5 * B + G + 2 * k * plane for k in (2, 1)), P
+
+
+def fft_rooflines(sizes=(1024, 2048, 4096), reps=20):
+    """In-place N x N forward transforms (rows, in-place transpose, rows,
+    in-place transpose), HIP-event timed; each pass reads and writes the
+    N^2 complex64 array once, 4 passes = 64 N^2 bytes."""
+    from ghostlight_amd import diffraction as dfr
+    out = {}
+    for N in sizes:
+        ms = dfr.fft2d_time(N, reps)
+        b = 64 * N * N
+        gbs = b / (ms * 1e-3) / 1e9
+        out[str(N)] = {"ms": round(ms, 4), "bytes": b, "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS,
+                       "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 4)}
+    return out
+
+
+def cpu_baseline_starburst(P, M, q, lams, ghost_planes, hw, hh):
+    """NumPy on the host cores, the forms the reference's validation uses:
+    the matrix DFT as two complex64 matmuls per lambda (threaded BLAS), and
+    the veil convolution as complex64 fft2 / ifft2 of each zero-padded plane
+    (numpy's FFT is single-threaded)."""
+    from ghostlight_amd import FlareConfig
+    N = P.shape[0]
+    n = np.arange(N) - N / 2
+    k = np.arange(M) - M / 2
+    P64 = P.astype(np.complex64)
+    t0 = time.perf_counter()
+    for lam in lams:
+        A = np.exp(-2j * np.pi * np.outer(k, n) / (q * float(lam) / 550.0 * N)).astype(np.complex64)
+        E = A @ P64 @ A.T
+        _ = (E.real * E.real + E.imag * E.imag)
+    mft_s = time.perf_counter() - t0
+    H, W = ghost_planes[0].shape
+    Pp = veil_pad(W, H)
+    d = FlareConfig().diffraction
+    px, py, a = 2 * hw / W, 2 * hh / H, d.veil_spread * hh
+    t0 = time.perf_counter()
+    dd = np.arange(Pp)
+    dd = np.where(dd < Pp // 2, dd, dd - Pp).astype(np.float32)
+    g = (a * a / ((dd[None, :] * px) ** 2 + (dd[:, None] * py) ** 2 + a * a)) ** d.veil_falloff
+    G = np.fft.fft2((g / g.sum()).astype(np.complex64))
+    for x in ghost_planes:
+        z = np.zeros((Pp, Pp), np.complex64)
+        z[:H, :W] = x
+        _ = d.veil_gain * np.real(np.fft.ifft2(np.fft.fft2(z) * G))[:H, :W]
+    veil_s = time.perf_counter() - t0
+    return {"mft": {"value": len(lams) / mft_s, "unit": "PSF/s", "ms_per_lambda": round(mft_s * 1e3 / len(lams), 2),
+                    "cores": host_threads(), "kind": "port",
+                    "sample": f"numpy complex64 matrix DFT {N}^2 -> {M}^2 (two matmuls), all {len(lams)} lambda, "
+                              f"{cpu_model()}"},
+            "veil": {"value": 1.0 / veil_s, "unit": "images/s", "ms": round(veil_s * 1e3, 1), "cores": 1,
+                     "kind": "port",
+                     "sample": f"numpy complex64 fft2 convolution of the {W}x{H} 3-plane image on the {Pp}^2 grid"}}
+
+
+def precision_starburst_leg(ghost_planes, W, H, hw, hh, with_cpu=True):
     """Matrix-DFT of the 1024^2 precision pupil (9-blade rounded polygon,
     Perlin dust, 20 scratches) to a 2048^2 PSF at 31 wavelengths with
     lambda-scaled sampling, FP32 MFMA and 3xBF16 MFMA; per-lambda error vs
-    the FP64 (numpy complex128) matrix DFT on `check`; then the FFT veil
-    convolution of the 1920x1080 ghost image."""
+    the FP64 (numpy complex128) matrix DFT at every lambda; then the FFT veil
+    convolution of the 1920x1080 ghost image (warm: the kernel transform and
+    buffers of the first call are reused) with its HBM roofline, the LDS row
+    FFT's HBM roofline at sprite sizes, and the numpy CPU baselines."""
     from ghostlight_amd import FlareConfig, diffraction as dfr, hostprep, synthetic
     from oracle import diffraction_oracle as dor
     N, M, q = 1024, 2048, 2.0
@@ -140,31 +237,44 @@ def precision_starburst_leg(ghost_planes, W, H, hw, hh, check=(400.0, 550.0, 700
     flops = 8.0 * len(lams) * (N * N * M + M * M * N)
     out = {"pupil": f"{N}x{N} complex64, 9-blade rounded polygon, Perlin dust 0.9-1.0, 20 scratches",
            "psf": f"{M}x{M} per lambda, {len(lams)} lambda, q = {q} x lambda / 550 nm", "gflop": round(flops / 1e9, 1)}
-    idx = [int(np.argmin(np.abs(lams - c))) for c in check]
-    refs = {}
-    for i in idx:
-        refs[i] = np.abs(dor.mft(P, M, q * float(lams[i]) / 550.0)) ** 2
     for name, prec, peak in (("fp32", dfr.MDFT_FP32, 157.3), ("bf16x3", dfr.MDFT_BF16X3, 2500.0)):
         dfr.mft_psf(P, M, q, lams[:2], precision=prec)                 # warm the buffers
         psf, ms = dfr.mft_psf(P, M, q, lams, precision=prec, timing=True)
         tf = flops / (ms * 1e-3) / 1e12
         issued = tf * (3.0 if prec == dfr.MDFT_BF16X3 else 1.0)       # hi*hi + hi*lo + lo*hi products
+        errs = {}
+        for i, lam in enumerate(lams):
+            ref = np.abs(dor.mft(P, M, q * float(lam) / 550.0)) ** 2
+            errs[f"{float(lam):.1f}"] = float(dor.rel_l2(psf[i], ref))
+            del ref
         out[name] = {"ms": round(ms, 3), "tflops": round(tf, 2), "mfma_tflops_issued": round(issued, 2),
-                     "peak": peak, "frac": round(issued / peak, 4),
-                     "rel_l2_vs_fp64": {f"{float(lams[i]):.1f}": float(dor.rel_l2(psf[i], refs[i])) for i in idx}}
+                     "peak": peak, "frac": round(issued / peak, 4), "rel_l2_vs_fp64": errs,
+                     "rel_l2_vs_fp64_max": max(errs.values())}
+        print(f"bench: MDFT {name} {ms:.3f} ms, worst rel-L2 vs FP64 over {len(lams)} lambda "
+              f"{max(errs.values()):.2e}", file=sys.stderr, flush=True)
         del psf
     cfg = FlareConfig()
-    vo, vms = dfr.veil_convolve(ghost_planes, hw, hh, cfg, timing=True)
+    dfr.veil_convolve(ghost_planes, hw, hh, cfg)                       # kernel transform + buffers
+    vms = min(dfr.veil_convolve(ghost_planes, hw, hh, cfg, timing=True)[1] for _ in range(3))
+    vo = dfr.veil_convolve(ghost_planes, hw, hh, cfg)
     d = cfg.diffraction
     vref = dor.veil_convolve([np.asarray(x, np.float64) for x in ghost_planes], 2 * hw / W, 2 * hh / H,
                              d.veil_spread * hh, d.veil_falloff, d.veil_gain)
-    P2 = dor.veil_pad(W, H)
+    vb, P2 = veil_pass_bytes(W, H)
+    vgbs = vb / (vms * 1e-3) / 1e9
     out["veil_fft"] = {"image": f"{W}x{H} ghost image, 3 planes", "fft": f"{P2}x{P2}", "ms": round(vms, 3),
-                       "rel_l2_vs_fp64": float(dor.rel_l2(np.stack(vo), np.stack(vref)))}
+                       "rel_l2_vs_fp64": float(dor.rel_l2(np.stack(vo), np.stack(vref))),
+                       "roofline": {"bound": "hbm", "achieved": round(vgbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                                    "frac": round(vgbs / HBM_PEAK_GBS, 4), "bytes": vb}}
+    out["fft_rows"] = fft_rooflines()
+    print(f"bench: veil {vms:.3f} ms ({vgbs:.0f} GB/s on {vb / 1e9:.2f} GB), FFT {out['fft_rows']}",
+          file=sys.stderr, flush=True)
+    if with_cpu:
+        out["cpu_baseline"] = cpu_baseline_starburst(P, M, q, lams, ghost_planes, hw, hh)
     return out
 
 
-def sweep_leg(n_sources=120):
+def sweep_leg(n_sources=120, with_cpu=True):
     """Anamorphic source sweep: the 30-surface lens with 4 cylindrical
     surfaces (1.5x squeeze), seeded decenter/tilt, 192^2 pupil, 16 lambda,
     sources on a diagonal, each into its own 2048x858 frame, one launch."""
@@ -190,10 +300,39 @@ def sweep_leg(n_sources=120):
     _, st = render_ghosts(lens, src, W, H, cfg, hw, hh, pairs=pairs, boosts=boosts, ctx=ctx, device_out=True,
                           stats=True)
     ctx.close()
-    return {"sources": n_sources, "frame": f"{W}x{H}", "pairs_prefiltered": len(pairs), "render_ms": round(dt * 1e3, 2),
-            "first_render_ms": round(first * 1e3, 2), "trace_kernel_ms": round(kms, 2),
-            "segments": st["segments"], "segments_per_s": st["segments"] / dt, "frames_per_s": n_sources / dt,
-            "output_bytes": 3 * 4 * W * H * n_sources}
+    out = {"sources": n_sources, "frame": f"{W}x{H}", "pairs_prefiltered": len(pairs), "render_ms": round(dt * 1e3, 2),
+           "first_render_ms": round(first * 1e3, 2), "trace_kernel_ms": round(kms, 2),
+           "segments": st["segments"], "segments_per_s": st["segments"] / dt, "frames_per_s": n_sources / dt,
+           "output_bytes": 3 * 4 * W * H * n_sources}
+    if with_cpu:
+        out["cpu_baseline"] = cpu_baseline_sweep(lens, cfg, src, pairs, boosts, W, H, hw, hh)
+    return out
+
+
+def cpu_baseline_sweep(lens, cfg, src, pairs, boosts, W, H, hw, hh, target_s=10.0):
+    """The FP32 oracle (OpenMP over the host cores) on a bounded sample of
+    the sweep: 4 of its sources, the first k of its pairs, sized to about
+    target_s seconds; segments/s."""
+    from oracle import pyoracle
+    from ghostlight_amd import hostprep
+    from ghostlight_amd.config import to_params
+    spec = hostprep.spectral_table(cfg.spectral_samples)
+    C = hostprep.spectral_correction(spec)
+    p, _keep = to_params(cfg)
+    threads = host_threads()
+    s4 = src[np.linspace(0, len(src) - 1, 4).astype(int)]
+    k = 2
+    t0 = time.perf_counter()
+    pyoracle.render_ghosts(lens, pairs[:k], boosts[:k], s4, spec, p, hw, hh, W, H, correction=C, threads=threads)
+    dt = time.perf_counter() - t0
+    k2 = int(min(len(pairs), max(k, k * target_s / max(dt, 1e-3))))
+    t0 = time.perf_counter()
+    _, info = pyoracle.render_ghosts(lens, pairs[:k2], boosts[:k2], s4, spec, p, hw, hh, W, H, correction=C,
+                                     threads=threads)
+    dt = time.perf_counter() - t0
+    return {"value": info["segments"] / dt, "unit": "segments/s", "cores": threads, "kind": "port",
+            "sample": f"FP32 oracle render of 4 of the {len(src)} sources x the first {k2} of {len(pairs)} pairs "
+                      f"({info['segments']} segments in {dt:.2f} s, {cpu_model()})"}
 
 
 def psf_grid_leg():
@@ -220,23 +359,69 @@ def psf_grid_leg():
             "traces_per_s": traces / (out["kernel_ms"] * 1e-3), "dark_cells": int((out["status"] == 2).sum())}
 
 
+def served_ghost_error(req, out):
+    """rel-L2 of a served request's ghost layer against the FP64 and FP32
+    oracles' renders of the same lens, pair set, source and spectra, and the
+    FP32 oracle's own distance from the FP64 one."""
+    from oracle import pyoracle
+    from ghostlight_amd import hostprep
+    from ghostlight_amd.config import to_params
+    from ghostlight_amd.render import filter_ghost_pairs, source_rgb
+    from ghostlight_amd.source_map import solve_source_map
+    lens, cfg = req.lens, req.config
+    cal = lens.calibration()
+    hw = cfg.sensor_half_w if cfg.sensor_half_w > 0 else cal.sensor_half_w
+    hh = cfg.sensor_half_h if cfg.sensor_half_h > 0 else cal.sensor_half_h
+    smap = solve_source_map(lens, cal, (cfg.source_x - 0.5) * 2.0, (cfg.source_y - 0.5) * 2.0, hw, hh)
+    rgb = source_rgb(cfg)
+    src = np.array([[np.float32(smap.angle_x), np.float32(smap.angle_y), rgb[0], rgb[1], rgb[2]]], np.float32)
+    pairs, boosts = filter_ghost_pairs(lens, hw, hh, cfg)
+    spec = hostprep.spectral_table(cfg.spectral_samples)
+    p, _keep = to_params(cfg)
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    C = hostprep.spectral_correction(spec)
+    o64, _ = pyoracle.render_ghosts(lens, pairs, boosts, src, spec, p, hw, hh, req.width, req.height,
+                                    correction=C, dbl=True, threads=threads)
+    o32, _ = pyoracle.render_ghosts(lens, pairs, boosts, src, spec, p, hw, hh, req.width, req.height,
+                                    correction=C, threads=threads)
+    got = [out["ghost_r"], out["ghost_g"], out["ghost_b"]]
+    return pyoracle.rel_l2(got, o64), pyoracle.rel_l2(got, o32), pyoracle.rel_l2(o32, o64)
+
+
 def serving_leg(n_requests, rate_hz=40.0, slots=4):
     """Mixed-request serving stream (seeded lenses of 8-32 surfaces, 512^2 to
-    3840x2160, 8-31 wavelengths, 5-11 blades; ghosts + starburst + veil)."""
+    3840x2160, 8-31 wavelengths, 5-11 blades; ghosts + starburst + veil at
+    the default 1024^2 starburst grid).  The ghost layers of a sample of the
+    served requests (the first of each output size) are checked against the
+    FP64 oracle."""
     from ghostlight_amd.serving import (FlareServer, MixSpec, make_mix, run_one_at_a_time, run_stream,
                                         serving_lens_pool)
     pool = serving_lens_pool(5)
     mix = make_mix(MixSpec(n_requests=n_requests, rate_hz=rate_hz, seed=5), pool)
     warm = make_mix(MixSpec(n_requests=2 * slots, rate_hz=1e6, seed=99), pool)
     burst = [(0.0, req) for _, req in mix]
+    sample = {}
+    for i, (_, req) in enumerate(mix):
+        sample.setdefault((req.width, req.height), i)
+    keep = sorted(sample.values())
     with FlareServer(slots=slots) as srv:
         run_stream(srv, warm)
-        served = run_stream(srv, mix)
+        served = run_stream(srv, mix, keep=keep)
         saturated = run_stream(srv, burst)
+    outs = served.pop("outputs")
+    errors = []
+    for i in keep:
+        req = mix[i][1]
+        e64, e32, base = served_ghost_error(req, outs[i])
+        errors.append({"request": i, "size": f"{req.width}x{req.height}", "surfaces": req.lens.num_surfaces(),
+                       "lambda": req.config.spectral_samples, "ghost_rel_l2_vs_fp64": float(e64),
+                       "ghost_rel_l2_vs_fp32_oracle": float(e32), "fp32_oracle_vs_fp64": float(base)})
+    del outs
     plain = run_one_at_a_time(mix)
     r = lambda d: {k: (round(v, 3) if isinstance(v, float) else v) for k, v in d.items()}  # noqa: E731
-    return {"requests": n_requests, "rate_hz": rate_hz, "slots": slots, "served": r(served),
-            "served_all_at_once": r(saturated), "one_at_a_time": r(plain)}
+    return {"requests": n_requests, "rate_hz": rate_hz, "slots": slots, "starburst_grid": 1024,
+            "served": r(served), "served_all_at_once": r(saturated), "one_at_a_time": r(plain),
+            "served_error": errors, "served_error_max": max(x["ghost_rel_l2_vs_fp64"] for x in errors)}
 
 
 def main():
@@ -307,19 +492,22 @@ def main():
           f"tiled splat {splat_ms:.3f} ms, {live_pairs} pairs reach the sensor, "
           f"{st.get('traces_on_sensor', 0)} on-sensor traces", file=sys.stderr, flush=True)
 
-    roof = None
-    try:
-        ops = ops_per_segment(lens, pairs, cfg, src)
-        achieved = segs * ops / (kernel_ms * 1e-3) / 1e12
-        roof = {"bound": "valu", "achieved": round(achieved, 3), "peak": FP32_VECTOR_PEAK_TFLOPS,
-                "unit": "TFLOP/s", "frac": round(achieved / FP32_VECTOR_PEAK_TFLOPS, 4),
-                "traffic": TRACE_TRAFFIC_BYTES,
-                "kernel": "ghost_persist_kernel (trace + deposit records)", "kernel_ms": round(kernel_ms, 4),
-                "kernel_ms_in_graph": round(graph_kernel_ms, 4), "schedule": sched,
-                "splat_resolve_ms": round(splat_ms, 4),
-                "ops_per_segment": round(ops, 2), "segments_per_launch": segs}
-    except Exception as e:  # the oracle library is test infrastructure; report its absence
-        print(f"bench: roofline op count unavailable ({e})", file=sys.stderr)
+    sc = ctx.last_splat_counts()
+    ops = OPS_PER_SEGMENT
+    achieved = segs * ops / (kernel_ms * 1e-3) / 1e12
+    image_bytes = 3 * 4 * W * H
+    roof = {"bound": "valu", "achieved": round(achieved, 3), "peak": FP32_VECTOR_PEAK_TFLOPS,
+            "unit": "TFLOP/s", "frac": round(achieved / FP32_VECTOR_PEAK_TFLOPS, 4),
+            "traffic": TRACE_TRAFFIC_BYTES,
+            "kernel": "ghost_persist_kernel (trace + deposit records)", "kernel_ms": round(kernel_ms, 4),
+            "kernel_ms_in_graph": round(graph_kernel_ms, 4), "schedule": sched,
+            "ops_per_segment": round(ops, 2), "segments_per_launch": segs,
+            "splat": {"resolve_ms": round(splat_ms, 4), "records": sc["records"], "binned": sc["binned"],
+                      "chunks": sc["chunks"], "staged_tiles": sc["staged_tiles"],
+                      "record_bytes": 16 * sc["records"], "image_bytes": image_bytes,
+                      # trace writes the records; bin_count and bin_scatter read them, scatter writes the
+                      # binned copy, resolve reads it; plus the image's read-add-write
+                      "step_bytes": 16 * (3 * sc["records"] + 2 * sc["binned"]) + 2 * image_bytes}}
 
     cpu = prec = None
     if not a.no_cpu:
@@ -335,7 +523,7 @@ def main():
     if not a.no_starburst:
         print("bench: precision-starburst leg", file=sys.stderr, flush=True)
         ghost = render_ghosts(lens, src, W, H, cfg, hw, hh, pairs=pairs, boosts=boosts, ctx=ctx)
-        starburst = precision_starburst_leg(ghost, W, H, hw, hh)
+        starburst = precision_starburst_leg(ghost, W, H, hw, hh, with_cpu=not a.no_cpu)
     psf = None
     if not a.no_starburst:
         print("bench: PSF grid leg", file=sys.stderr, flush=True)
@@ -343,7 +531,7 @@ def main():
     sweep = None
     if a.sweep_sources > 0:
         print(f"bench: source-sweep leg, {a.sweep_sources} sources", file=sys.stderr, flush=True)
-        sweep = sweep_leg(a.sweep_sources)
+        sweep = sweep_leg(a.sweep_sources, with_cpu=not a.no_cpu)
     serving = None
     if a.serving_requests > 0:
         print(f"bench: serving leg, {a.serving_requests} requests", file=sys.stderr, flush=True)

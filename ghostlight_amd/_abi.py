@@ -255,6 +255,7 @@ SIGNATURES = {
                                       F32P, F32P, I32P, I32P]),
     "gl_fft2d_c2c_forward_inplace": (C.c_int, [VP, VP, C.c_int32]),
     "gl_fft2d_c2c_forward_host": (C.c_int, [VP, F32P, F32P, C.c_int32]),
+    "gl_fft2d_time": (C.c_int, [VP, C.c_int32, C.c_int32, C.POINTER(C.c_float)]),
     "gl_render_starburst": (C.c_int, [VP, VP, C.POINTER(gl_calibration), C.POINTER(gl_source), F32P, F32P, C.c_int32,
                                       C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_float, C.c_float,
                                       C.POINTER(gl_diffraction_params), F32P, F32P, F32P,

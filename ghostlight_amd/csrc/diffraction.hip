@@ -708,6 +708,29 @@ int gl_fft2d_c2c_forward_host(gl_ctx* ctx, const float* in, float* out, int32_t 
     return GL_OK;
 }
 
+int gl_fft2d_time(gl_ctx* ctx, int32_t N, int32_t reps, float* ms_per_transform) {
+    if (!ms_per_transform || reps < 1) return set_error(GL_ERR_INVALID, "null out or reps < 1");
+    int st;
+    gl_ctx* c = resolve_ctx(ctx, &st);
+    if (!c) return st;
+    std::lock_guard<std::mutex> g(c->mu);
+    if (N < 2 || (N & (N - 1)) || N > 16384) return set_error(GL_ERR_INVALID, "FFT size %d is not a power of two in [2, 16384]", N);
+    const size_t bytes = (size_t)N * N * sizeof(float2);
+    DevBuf& d = c->buf("fft_time_data");
+    if ((st = d.ensure(bytes))) return st;
+    GL_HIP(hipMemsetAsync(d.p, 0, bytes, c->stream));
+    if ((st = fft2d(c, d.as<float2>(), N, nullptr))) return st;
+    GL_HIP(hipEventRecord(c->ev0, c->stream));
+    for (int i = 0; i < reps; ++i)
+        if ((st = fft2d(c, d.as<float2>(), N, nullptr))) return st;
+    GL_HIP(hipEventRecord(c->ev1, c->stream));
+    GL_HIP(hipEventSynchronize(c->ev1));
+    float ms = 0.0f;
+    GL_HIP(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    *ms_per_transform = ms / (float)reps;
+    return GL_OK;
+}
+
 int gl_primary_survival(gl_ctx* ctx, const gl_lens* lens, const float* launch, int32_t n, float dx, float dy, float dz,
                         float z_spawn, float lam, unsigned char* out) {
     if (!lens || !launch || !out || n < 0) return set_error(GL_ERR_INVALID, "null argument");

@@ -152,6 +152,14 @@ def fft2d(a: np.ndarray, ctx=None) -> np.ndarray:
     return out
 
 
+def fft2d_time(N: int, reps: int = 20, ctx=None) -> float:
+    """Device ms of one in-place N x N forward transform (HIP events around
+    `reps` back-to-back transforms on the context's stream)."""
+    ms = C.c_float(0.0)
+    _abi.check(lib().gl_fft2d_time(ctx_handle(ctx), int(N), int(reps), C.byref(ms)), "fft2d_time")
+    return float(ms.value)
+
+
 def render_starburst_layer(lens, calib, sources, width, height, hw, hh, config, px=None, py=None, ctx=None):
     if px is None or py is None:
         px, py = source_pixels(lens, calib, sources, width, height, hw, hh)

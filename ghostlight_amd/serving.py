@@ -83,6 +83,8 @@ def estimate_bytes(req: FlareRequest) -> int:
     # context sizes the buffer from the previous render with 25% headroom
     rec = min(pairs * grid * ns, 1 << 24)
     b += int(rec * 16 * 2.5)
+    # fixed-point staging of the splat's multi-chunk tiles (24 KiB a 32^2 tile)
+    b += min(npx // 1024 + 1, int(rec * 1.5) // 8192 + 1) * 3 * 1024 * 8
     d = cfg.diffraction
     if d.starburst:
         N = max(256, int(d.starburst_grid))
@@ -282,7 +284,6 @@ def make_mix(spec: MixSpec, pool: Optional[List[OpticalSystem]] = None):
         cfg.aperture_blades = int(rng.integers(5, 12))
         cfg.diffraction.starburst = spec.starburst
         cfg.diffraction.veil = spec.veil
-        cfg.diffraction.starburst_grid = 512
         out.append((t, FlareRequest(lens=lens, width=w, height=h, config=cfg)))
     return out
 
@@ -293,9 +294,10 @@ def percentile(xs, q):
     return float(np.percentile(np.asarray(xs, np.float64), q))
 
 
-def run_stream(server: FlareServer, mix, time_scale: float = 1.0) -> Dict[str, float]:
+def run_stream(server: FlareServer, mix, time_scale: float = 1.0, keep=()) -> Dict[str, float]:
     """Submit the mix at its arrival times and wait for every result.
-    Returns throughput and latency percentiles (ms), overall and for 1080p."""
+    Returns throughput and latency percentiles (ms), overall and for 1080p;
+    with `keep` (request indices), also "outputs": {index: output dict}."""
     t0 = time.perf_counter()
     pend = []
     for (ta, req) in mix:
@@ -314,7 +316,8 @@ def run_stream(server: FlareServer, mix, time_scale: float = 1.0) -> Dict[str, f
     return {"requests": len(results), "seconds": span, "flares_per_s": len(results) / span,
             "p50_ms": percentile(lat, 50), "p99_ms": percentile(lat, 99),
             "p50_1080p_ms": percentile(lat1080, 50), "p99_1080p_ms": percentile(lat1080, 99),
-            "max_in_use_bytes": server.max_in_use, "wall_s": t1 - t0}
+            "max_in_use_bytes": server.max_in_use, "wall_s": t1 - t0,
+            **({"outputs": {i: results[i].output for i in keep}} if keep else {})}
 
 
 def run_one_at_a_time(mix, render_fn=None, ctx=None) -> Dict[str, float]:

@@ -513,6 +513,12 @@ int  gl_fft2d_c2c_forward_inplace(gl_ctx* ctx, void* d_data, int32_t N);
 /* Host-buffer convenience form of the same transform (tests, tools). */
 int  gl_fft2d_c2c_forward_host(gl_ctx* ctx, const float* in, float* out, int32_t N);
 
+/* Device time of the same transform (build-specific instrumentation, no
+ * reference counterpart): `reps` back-to-back in-place transforms of an
+ * N x N device array the context owns, bracketed by HIP events on ctx's
+ * stream after one untimed transform; *ms_per_transform = elapsed / reps. */
+int  gl_fft2d_time(gl_ctx* ctx, int32_t N, int32_t reps, float* ms_per_transform);
+
 /* render_starburst (diffraction.h:54-72): effective pupil (stop x cat's-eye
  * envelope), LDS radix FFT, PSF, lambda bake, downsample and area resample
  * (SPRITE) or the matrix-DFT window (MDFT).  px/py: each source's buffer-pixel
