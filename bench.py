@@ -33,10 +33,10 @@ FP32_VECTOR_PEAK_TFLOPS = 157.3   # MI355X published FP32 vector peak
 # tests/test_bench_constants.py recounts it.
 OPS_PER_SEGMENT = 116.08
 # HBM bytes per launch of the trace kernel on this workload, from rocprofv3
-# FETCH_SIZE / WRITE_SIZE runs scaled by a 1 GiB copy: 147 MiB read (lens,
-# tables, pupil grid, spill stack) + 961 MiB written (849 MiB of deposit
-# records, the rest the spill stack of eight-wave occupancy).
-TRACE_TRAFFIC_BYTES = 1.16e9
+# FETCH_SIZE / WRITE_SIZE runs scaled by an in-place 4096^2 complex64
+# transpose (128 MiB each way): 4.9 MiB read (lens, tables, pupil lattice,
+# work list) + 849 MiB written (the 55 M deposit records of 16 B).
+TRACE_TRAFFIC_BYTES = 8.95e8
 HBM_PEAK_GBS = 8000.0
 
 
@@ -275,6 +275,63 @@ def precision_starburst_leg(ghost_planes, W, H, hw, hh, with_cpu=True):
     if with_cpu:
         out["cpu_baseline"] = cpu_baseline_starburst(P, M, q, lams, ghost_planes, hw, hh)
     return out
+
+
+def plain_flare_leg(with_cpu=True, reps=5):
+    """The plain CPU-runnable flare: the 12-surface double-Gauss-like lens,
+    all two-bounce ghosts, 64^2 pupil grid, 8 wavelengths, a point source at
+    10 deg, 512^2 output, plus the 256^2 6-blade starburst; one render through
+    the public path (ghosts + starburst layer), median of `reps` after a
+    warm-up.  CPU baseline: the FP32 oracle's render of the ghost layer."""
+    from ghostlight_amd import Context, PointFlareConfig, synthetic
+    from ghostlight_amd.render import filter_ghost_pairs, render_sources
+    lens = synthetic.config1_lens()
+    cal = lens.calibration()
+    W = H = 512
+    cfg = PointFlareConfig(ray_grid=64, spectral_samples=8, aperture_blades=6, collect_stats=True)
+    cfg.diffraction.starburst = True
+    cfg.diffraction.starburst_grid = 256
+    cfg.diffraction.starburst_grid_cap = 256
+    src = np.array([[math.radians(10.0), 0.0, 1.0, 1.0, 1.0]], np.float32)
+    ctx = Context()
+    out = render_sources(src, W, H, lens, cal, cfg, ctx=ctx)
+    ts = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        out = render_sources(src, W, H, lens, cal, cfg, ctx=ctx)
+        ts.append((time.perf_counter() - t0) * 1e3)
+    ms = float(np.median(ts))
+    segs = int(out["stats"]["segments"])
+    res = {"lens": "12 surfaces, 6-blade stop, MgF2", "output": f"{W}x{H}", "ray_grid": 64, "lambda": 8,
+           "starburst_grid": 256, "render_ms": round(ms, 3), "segments": segs,
+           "segments_per_s": segs / (ms * 1e-3)}
+    if with_cpu:
+        from oracle import pyoracle
+        from ghostlight_amd import hostprep
+        from ghostlight_amd.config import to_params
+        hw, hh = cal.sensor_half_w, cal.sensor_half_h
+        pairs, boosts = filter_ghost_pairs(lens, hw, hh, cfg, ctx=ctx)
+        spec = hostprep.spectral_table(cfg.spectral_samples)
+        Cm = hostprep.spectral_correction(spec)
+        p, _keep = to_params(cfg)
+        threads = host_threads()
+        cts = []
+        for _ in range(3):
+            t0 = time.perf_counter()
+            o32, info = pyoracle.render_ghosts(lens, pairs, boosts, src, spec, p, hw, hh, W, H, correction=Cm,
+                                               threads=threads)
+            cts.append(time.perf_counter() - t0)
+        o64, _ = pyoracle.render_ghosts(lens, pairs, boosts, src, spec, p, hw, hh, W, H, correction=Cm, dbl=True,
+                                        threads=threads)
+        got = [out["ghost_r"], out["ghost_g"], out["ghost_b"]]
+        dt = float(np.median(cts))
+        res["cpu_baseline"] = {"value": info["segments"] / dt, "unit": "segments/s", "cores": threads, "kind": "port",
+                               "sample": f"FP32 oracle render of the whole ghost layer ({len(pairs)} pairs, "
+                                         f"{info['segments']} segments in {dt:.3f} s, {cpu_model()})"}
+        res["ghost_rel_l2_vs_fp32_oracle"] = pyoracle.rel_l2(got, o32)
+        res["ghost_rel_l2_vs_fp64"] = pyoracle.rel_l2(got, o64)
+    ctx.close()
+    return res
 
 
 def sweep_leg(n_sources=120, with_cpu=True):
@@ -532,6 +589,10 @@ def main():
     if not a.no_starburst:
         print("bench: PSF grid leg", file=sys.stderr, flush=True)
         psf = psf_grid_leg()
+    plain = None
+    if not a.no_starburst:
+        print("bench: plain 12-surface flare leg", file=sys.stderr, flush=True)
+        plain = plain_flare_leg(with_cpu=not a.no_cpu)
     sweep = None
     if a.sweep_sources > 0:
         print(f"bench: source-sweep leg, {a.sweep_sources} sources", file=sys.stderr, flush=True)
@@ -563,6 +624,7 @@ def main():
         "precision": prec,
         "precision_starburst": starburst,
         "psf_grid": psf,
+        "plain_flare": plain,
         "source_sweep": sweep,
         "serving": serving,
     }
