@@ -4,4 +4,4 @@ from ghostlight_amd import *  # noqa: F401,F403
 from ghostlight_amd import _cuda_available, _render_starburst_debug, _render_veil_debug  # noqa: F401
 from ghostlight_amd import _solve_source_map, _source_map_landing, _source_map_screen  # noqa: F401
 from ghostlight_amd import _aperture_edge_distance_debug, _aperture_sdf_debug, _hurb_sample_debug  # noqa: F401
-from ghostlight_amd import _gate_params_debug, _gate_sample_debug  # noqa: F401
+from ghostlight_amd import _gate_params_debug, _gate_sample_debug, _render_gate_debug  # noqa: F401

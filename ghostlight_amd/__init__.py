@@ -10,9 +10,11 @@ from .config import (BaffleShape, Baffle, DiffractionConfig, FlareConfig, GateCo
 from .lens import (ApertureImage, ApertureProfile, ApertureShape, Coating, CoatingModel, CylinderAxis, DispersionModel,
                    LensCalibration, OpticalSystem, Surface, SurfaceForm, TraceStatus, Vec3f, cross, dot)
 from .runtime import Context, device_available
-from .render import (GhostPair, Ray, TraceResult, enumerate_ghost_pairs, filter_ghost_pairs, render_ghosts,
-                     render_point_flare, render_source_flare, render_sources, solve_source_angle, trace_ghost_ray, trace_ghost_rays,
-                     trace_primary_ray)
+from .render import (GhostPair, Ray, RayPath, TraceEvent, TraceResult, enumerate_ghost_pairs, filter_ghost_pairs,
+                     render_ghosts, render_point_flare, render_source_flare, render_sources, solve_source_angle,
+                     trace_ghost_ray, trace_ghost_ray_diagnostic, trace_ghost_rays, trace_primary_ray,
+                     trace_primary_ray_diagnostic, trace_rays_diagnostic)
+from .render import _render_gate_debug
 from .hostprep import build_spectral_lambdas
 from .diffraction import _render_starburst_debug, _render_veil_debug, fft2d
 from .psf import PSFCellStatus, render_psf
@@ -36,4 +38,4 @@ __all__ = [n for n in dir() if not n.startswith("_")] + ["_cuda_available", "_re
                                                          "_source_map_landing", "_source_map_screen",
                                                          "_hurb_sample_debug", "_aperture_edge_distance_debug",
                                                          "_aperture_sdf_debug", "_gate_params_debug",
-                                                         "_gate_sample_debug"]
+                                                         "_gate_sample_debug", "_render_gate_debug"]

@@ -164,6 +164,18 @@ class gl_gate_resolved(C.Structure):
         ("n_scatter", C.c_int32), ("lobe", C.c_int32)]
 
 
+class gl_gate_debug(C.Structure):
+    _fields_ = [("gate", gl_gate_resolved), ("traces", C.c_uint64), ("scrapes", C.c_uint64), ("energy", C.c_double),
+                ("reach_mm", C.c_float), ("direct_r", C.POINTER(C.c_float)), ("direct_g", C.POINTER(C.c_float)),
+                ("direct_b", C.POINTER(C.c_float))]
+
+
+class gl_trace_event(C.Structure):
+    _fields_ = [("surface_index", C.c_int32), ("hit_point", C.c_float * 3), ("surface_normal", C.c_float * 3),
+                ("ior_before", C.c_float), ("ior_after", C.c_float), ("fresnel_weight", C.c_float),
+                ("status", C.c_int32), ("reflected", C.c_int32)]
+
+
 class gl_source_map(C.Structure):
     _fields_ = [("angle_x", C.c_float), ("angle_y", C.c_float), ("status", C.c_int32), ("residual_mm", C.c_float),
                 ("anchor_ax", C.c_float), ("anchor_ay", C.c_float), ("anchor_x", C.c_float), ("anchor_y", C.c_float),
@@ -253,6 +265,8 @@ SIGNATURES = {
                                            C.c_int32, I32P, C.POINTER(gl_ghost_stats)]),
     "gl_trace_ghost_rays": (C.c_int, [VP, VP, C.c_int32, C.c_int32, F32P, C.c_int32, C.c_int32, C.c_float,
                                       F32P, F32P, I32P, I32P]),
+    "gl_trace_rays_diagnostic": (C.c_int, [VP, VP, C.c_int32, C.c_int32, F32P, C.c_int32, C.c_int32, C.c_float,
+                                           C.c_int32, C.POINTER(gl_trace_event), I32P, F32P, F32P, I32P]),
     "gl_fft2d_c2c_forward_inplace": (C.c_int, [VP, VP, C.c_int32]),
     "gl_fft2d_c2c_forward_host": (C.c_int, [VP, F32P, F32P, C.c_int32]),
     "gl_fft2d_time": (C.c_int, [VP, C.c_int32, C.c_int32, C.POINTER(C.c_float)]),
@@ -273,6 +287,10 @@ SIGNATURES = {
     "gl_render_gate": (C.c_int, [VP, VP, C.POINTER(gl_source), C.c_int32, C.c_int32, C.c_int32, C.c_int32,
                                  C.c_int32, C.c_int32, C.c_int32, C.c_float, C.c_float, C.POINTER(gl_flare_params),
                                  C.POINTER(gl_gate_params), F32P, F32P, F32P, C.POINTER(C.c_uint64)]),
+    "gl_render_gate_debug": (C.c_int, [VP, VP, C.POINTER(gl_source), C.c_int32, C.c_int32, C.c_int32, C.c_int32,
+                                       C.c_int32, C.c_int32, C.c_int32, C.c_float, C.c_float,
+                                       C.POINTER(gl_flare_params), C.POINTER(gl_gate_params), F32P, F32P, F32P,
+                                       C.POINTER(gl_gate_debug)]),
     "gl_render_psf_tiles": (C.c_int, [VP, VP, C.POINTER(gl_psf_source), C.c_int32, C.POINTER(gl_flare_params),
                                       C.POINTER(gl_psf_params), F32P, F32P, F32P, F32P]),
     "gl_veil_convolve": (C.c_int, [VP, VP, VP, VP, C.c_int32, C.c_int32, C.c_float, C.c_float,

@@ -20,6 +20,8 @@
 #include <cmath>
 #include <cstring>
 
+static_assert(sizeof(gl_gate_debug) == 120, "gl_gate_debug is 120 bytes (ctypes mirrors it)");
+
 namespace gl {
 
 namespace {
@@ -52,6 +54,8 @@ struct GateArgs {
     int fmt_w, fmt_h, x0, y0, width, height;
     float *out_r, *out_g, *out_b;
     unsigned long long* counters;   // [traces, scrapes] or null
+    float *dir_r, *dir_g, *dir_b;   // direct image of the primary traces (telemetry) or null
+    unsigned* reach;                // largest |fold - edge| of a scrape, float bits (telemetry) or null
     GateDev g;
 };
 
@@ -143,6 +147,19 @@ __global__ __launch_bounds__(256) void gate_kernel(GateArgs A) {
     if (!isfinite(res.px) || !isfinite(res.py)) return;
     if (!(fabsf(d.z) > 1.0e-12f)) return;
     const float w_base = res.w * A.ray_weight;
+    // Telemetry: the direct image of the same traces, each landing added to
+    // the pixel it falls in (gate_render.cu:158-172).
+    if (A.dir_r) {
+        const float dpx = (res.px / (2.0f * A.half_w) + 0.5f) * (float)A.fmt_w + (float)A.x0;
+        const float dpy = (res.py / (2.0f * A.half_h) + 0.5f) * (float)A.fmt_h + (float)A.y0;
+        const int dxi = (int)floorf(dpx), dyi = (int)floorf(dpy);
+        if (dxi >= 0 && dxi < A.width && dyi >= 0 && dyi < A.height) {
+            const size_t pix = (size_t)dyi * A.width + dxi;
+            atomicAdd(&A.dir_r[pix], src.r * sp.y * w_base);
+            atomicAdd(&A.dir_g[pix], src.g * sp.z * w_base);
+            atomicAdd(&A.dir_b[pix], src.b * sp.w * w_base);
+        }
+    }
     const float sx = d.x / d.z, sy = d.y / d.z;
 
     for (int wall = 0; wall < 4; ++wall) {
@@ -167,6 +184,7 @@ __global__ __launch_bounds__(256) void gate_kernel(GateArgs A) {
         const float hit_other = p_other - s_other * back;
         if (hit_other <= o_lo || hit_other >= o_hi) continue;
         if (A.counters) atomicAdd(&A.counters[1], 1ull);
+        if (A.reach) atomicMax(A.reach, __float_as_uint(fabsf(fold - edge)));
         // Schlick to the wall normal: d is unit, so |n_axis| is the cosine.
         const float c = 1.0f - fminf(fmaxf(fabsf(n_axis), 0.0f), 1.0f);
         const float c2 = c * c;
@@ -202,10 +220,12 @@ __global__ __launch_bounds__(256) void gate_kernel(GateArgs A) {
     }
 }
 
-__global__ __launch_bounds__(256) void gate_zero_kernel(float* a, size_t n, unsigned long long* ctr) {
+__global__ __launch_bounds__(256) void gate_zero_kernel(float* a, size_t n, unsigned long long* ctr,
+                                                        unsigned* reach) {
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) a[i] = 0.0f;
     if (ctr && i < 2) ctr[i] = 0ull;
+    if (reach && i == 0) *reach = 0u;
 }
 
 // build_gpu_gate (gate.h:168-211): the opening from the effective sensor
@@ -236,16 +256,12 @@ GateDev resolve_gate(const gl_gate_params& gc, float half_w, float half_h) {
 }
 
 }  // namespace
-}  // namespace gl
 
-using namespace gl;
-
-extern "C" {
-
-int gl_render_gate(gl_ctx* ctx, const gl_lens* lens, const gl_source* sources, int32_t n_sources, int32_t width,
-                   int32_t height, int32_t fmt_w, int32_t fmt_h, int32_t fmt_x0, int32_t fmt_y0, float sensor_half_w,
-                   float sensor_half_h, const gl_flare_params* rp, const gl_gate_params* gp, float* out_r,
-                   float* out_g, float* out_b, uint64_t* counters_out) {
+// The pass behind gl_render_gate and gl_render_gate_debug (dbg may be null).
+static int render_gate(gl_ctx* ctx, const gl_lens* lens, const gl_source* sources, int32_t n_sources, int32_t width,
+                       int32_t height, int32_t fmt_w, int32_t fmt_h, int32_t fmt_x0, int32_t fmt_y0,
+                       float sensor_half_w, float sensor_half_h, const gl_flare_params* rp, const gl_gate_params* gp,
+                       float* out_r, float* out_g, float* out_b, uint64_t* counters_out, gl_gate_debug* dbg) {
     if (!lens || !rp || !gp || !out_r || !out_g || !out_b) return set_error(GL_ERR_INVALID, "null argument");
     if (width <= 0 || height <= 0 || fmt_w <= 0 || fmt_h <= 0)
         return set_error(GL_ERR_INVALID, "render_gate: degenerate output dimensions");
@@ -259,6 +275,17 @@ int gl_render_gate(gl_ctx* ctx, const gl_lens* lens, const gl_source* sources, i
     const size_t npx = (size_t)width * height;
     std::fill(out_r, out_r + npx, 0.0f); std::fill(out_g, out_g + npx, 0.0f); std::fill(out_b, out_b + npx, 0.0f);
     if (counters_out) counters_out[0] = counters_out[1] = 0;
+    if (dbg) {
+        std::memset(&dbg->gate, 0, sizeof dbg->gate);
+        dbg->traces = dbg->scrapes = 0;
+        dbg->energy = 0.0;
+        dbg->reach_mm = 0.0f;
+        if (dbg->direct_r) {
+            std::fill(dbg->direct_r, dbg->direct_r + npx, 0.0f);
+            std::fill(dbg->direct_g, dbg->direct_g + npx, 0.0f);
+            std::fill(dbg->direct_b, dbg->direct_b + npx, 0.0f);
+        }
+    }
     if (n_sources == 0) return GL_OK;
     int st;
     gl_ctx* c = resolve_ctx(ctx, &st);
@@ -291,16 +318,19 @@ int gl_render_gate(gl_ctx* ctx, const gl_lens* lens, const gl_source* sources, i
     DevBuf &bo = c->buf("gate_out"), &bc = c->buf("gate_ctr");
     if ((st = bs.ensure((size_t)n_sources * sizeof(gl_source))) || (st = bg.ensure(grid.size() * sizeof(gl_sample))) ||
         (st = bsp.ensure(spec4.size() * sizeof(float4))) || (st = bi.ensure(iat.size() * 4)) ||
-        (st = bo.ensure(3 * npx * 4)) || (st = bc.ensure(16)))
+        (st = bo.ensure((dbg && dbg->direct_r ? 6 : 3) * npx * 4)) || (st = bc.ensure(32)))
         return st;
     GL_HIP(hipMemcpyAsync(bs.p, sources, (size_t)n_sources * sizeof(gl_source), hipMemcpyHostToDevice, c->stream));
     GL_HIP(hipMemcpyAsync(bg.p, grid.data(), grid.size() * sizeof(gl_sample), hipMemcpyHostToDevice, c->stream));
     GL_HIP(hipMemcpyAsync(bsp.p, spec4.data(), spec4.size() * sizeof(float4), hipMemcpyHostToDevice, c->stream));
     GL_HIP(hipMemcpyAsync(bi.p, iat.data(), iat.size() * 4, hipMemcpyHostToDevice, c->stream));
     float* o = bo.as<float>();
-    unsigned long long* ctr = counters_out ? bc.as<unsigned long long>() : nullptr;
-    hipLaunchKernelGGL(gate_zero_kernel, dim3((unsigned)((3 * npx + 255) / 256)), dim3(256), 0, c->stream, o, 3 * npx,
-                       ctr);
+    const bool direct = dbg && dbg->direct_r;
+    const size_t planes = direct ? 6 : 3;
+    unsigned long long* ctr = (counters_out || dbg) ? bc.as<unsigned long long>() : nullptr;
+    unsigned* reach = dbg ? reinterpret_cast<unsigned*>(bc.as<char>() + 16) : nullptr;
+    hipLaunchKernelGGL(gate_zero_kernel, dim3((unsigned)((planes * npx + 255) / 256)), dim3(256), 0, c->stream, o,
+                       planes * npx, ctr, reach);
 
     GateArgs A;
     std::memset(&A, 0, sizeof A);
@@ -315,6 +345,8 @@ int gl_render_gate(gl_ctx* ctx, const gl_lens* lens, const gl_source* sources, i
     A.fmt_w = fmt_w; A.fmt_h = fmt_h; A.x0 = fmt_x0; A.y0 = fmt_y0; A.width = width; A.height = height;
     A.out_r = o; A.out_g = o + npx; A.out_b = o + 2 * npx;
     A.counters = ctr;
+    if (direct) { A.dir_r = o + 3 * npx; A.dir_g = o + 4 * npx; A.dir_b = o + 5 * npx; }
+    A.reach = reach;
     A.g = resolve_gate(*gp, sensor_half_w, sensor_half_h);
     const long long lanes = (long long)A.n_grid * ns;
     hipLaunchKernelGGL(gate_kernel, dim3((unsigned)((lanes + 255) / 256), (unsigned)n_sources), dim3(256), 0, c->stream,
@@ -323,10 +355,56 @@ int gl_render_gate(gl_ctx* ctx, const gl_lens* lens, const gl_source* sources, i
     GL_HIP(hipMemcpyAsync(out_r, o, npx * 4, hipMemcpyDeviceToHost, c->stream));
     GL_HIP(hipMemcpyAsync(out_g, o + npx, npx * 4, hipMemcpyDeviceToHost, c->stream));
     GL_HIP(hipMemcpyAsync(out_b, o + 2 * npx, npx * 4, hipMemcpyDeviceToHost, c->stream));
-    if (ctr) GL_HIP(hipMemcpyAsync(counters_out, ctr, 16, hipMemcpyDeviceToHost, c->stream));
+    unsigned long long ctr_h[2] = {0ull, 0ull};
+    unsigned reach_h = 0u;
+    if (ctr) GL_HIP(hipMemcpyAsync(ctr_h, ctr, 16, hipMemcpyDeviceToHost, c->stream));
+    if (reach) GL_HIP(hipMemcpyAsync(&reach_h, reach, 4, hipMemcpyDeviceToHost, c->stream));
+    if (direct) {
+        GL_HIP(hipMemcpyAsync(dbg->direct_r, o + 3 * npx, npx * 4, hipMemcpyDeviceToHost, c->stream));
+        GL_HIP(hipMemcpyAsync(dbg->direct_g, o + 4 * npx, npx * 4, hipMemcpyDeviceToHost, c->stream));
+        GL_HIP(hipMemcpyAsync(dbg->direct_b, o + 5 * npx, npx * 4, hipMemcpyDeviceToHost, c->stream));
+    }
     GL_HIP(hipStreamSynchronize(c->stream));
+    if (counters_out) { counters_out[0] = ctr_h[0]; counters_out[1] = ctr_h[1]; }
+    if (dbg) {
+        static_assert(sizeof(GateDev) == sizeof(gl_gate_resolved), "gl_gate_resolved mirrors GateDev");
+        std::memcpy(&dbg->gate, &A.g, sizeof A.g);
+        dbg->traces = ctr_h[0];
+        dbg->scrapes = ctr_h[1];
+        std::memcpy(&dbg->reach_mm, &reach_h, sizeof reach_h);
+        // The layer's energy as gate_render.cu:503-505 sums it.
+        double e = 0.0;
+        for (size_t i = 0; i < npx; ++i) e += out_r[i] + out_g[i] + out_b[i];
+        dbg->energy = e;
+    }
     return GL_OK;
 }
+
+}  // namespace gl
+
+using namespace gl;
+
+extern "C" {
+
+int gl_render_gate(gl_ctx* ctx, const gl_lens* lens, const gl_source* sources, int32_t n_sources, int32_t width,
+                   int32_t height, int32_t fmt_w, int32_t fmt_h, int32_t fmt_x0, int32_t fmt_y0, float sensor_half_w,
+                   float sensor_half_h, const gl_flare_params* rp, const gl_gate_params* gp, float* out_r,
+                   float* out_g, float* out_b, uint64_t* counters_out) {
+    return render_gate(ctx, lens, sources, n_sources, width, height, fmt_w, fmt_h, fmt_x0, fmt_y0, sensor_half_w,
+                       sensor_half_h, rp, gp, out_r, out_g, out_b, counters_out, nullptr);
+}
+
+int gl_render_gate_debug(gl_ctx* ctx, const gl_lens* lens, const gl_source* sources, int32_t n_sources,
+                         int32_t width, int32_t height, int32_t fmt_w, int32_t fmt_h, int32_t fmt_x0, int32_t fmt_y0,
+                         float sensor_half_w, float sensor_half_h, const gl_flare_params* rp,
+                         const gl_gate_params* gp, float* out_r, float* out_g, float* out_b, gl_gate_debug* dbg) {
+    if (!dbg) return set_error(GL_ERR_INVALID, "null debug record");
+    if (dbg->direct_r && (!dbg->direct_g || !dbg->direct_b))
+        return set_error(GL_ERR_INVALID, "direct planes: all three or none");
+    return render_gate(ctx, lens, sources, n_sources, width, height, fmt_w, fmt_h, fmt_x0, fmt_y0, sensor_half_w,
+                       sensor_half_h, rp, gp, out_r, out_g, out_b, nullptr, dbg);
+}
+
 
 int gl_gate_resolve(const gl_gate_params* gate, float half_w, float half_h, gl_gate_resolved* out) {
     if (!gate || !out) return set_error(GL_ERR_INVALID, "null argument");

@@ -19,7 +19,7 @@ import numpy as np
 
 from . import _abi, hostprep
 from .config import FlareConfig, PointFlareConfig, to_params
-from .lens import LensCalibration, OpticalSystem, TraceStatus
+from .lens import LensCalibration, OpticalSystem, TraceStatus, Vec3f
 from .runtime import ctx_handle, lib
 
 
@@ -178,10 +178,31 @@ class Ray:
 
 @dataclass
 class TraceResult:
-    """Landing of one traced ray (trace.h TraceResult)."""
-    position: tuple
+    """Landing of one traced ray (TraceResult, trace_event.h:39-43)."""
+    position: Vec3f
     weight: float
     status: TraceStatus
+
+
+@dataclass
+class TraceEvent:
+    """One surface a diagnostic trace reached (TraceEvent, trace_event.h:48-58)."""
+    surface_index: int
+    hit_point: Vec3f
+    surface_normal: Vec3f
+    ior_before: float
+    ior_after: float
+    fresnel_weight: float
+    status: TraceStatus
+    reflected: bool
+
+
+@dataclass
+class RayPath:
+    """A diagnostic trace: every surface event, then the landing (RayPath,
+    trace_event.h:61-64)."""
+    events: list
+    result: TraceResult
 
 
 @dataclass
@@ -199,19 +220,77 @@ def _one_ray(ray: Ray) -> np.ndarray:
     return r
 
 
+def _result(pos, w, st) -> TraceResult:
+    return TraceResult(Vec3f(float(pos[0]), float(pos[1]), float(pos[2])), float(w), TraceStatus(int(st)))
+
+
 def trace_ghost_ray(ray: Ray, lens: OpticalSystem, bounce_a: int, bounce_b: int, ctx=None) -> TraceResult:
     """One ghost trace (bind_trace.cpp:57-63; trace.cpp:171): the device
     tracer in FP32 with the CPU tracer's t >= 0 landing rule."""
-    if bounce_a < 0 or bounce_b < 0 or bounce_a >= bounce_b or bounce_b >= lens.num_surfaces():
-        raise ValueError("bounce indices must satisfy 0 <= bounce_a < bounce_b < num_surfaces")
+    _check_bounces(lens, bounce_a, bounce_b)
     pos, w, st, _ = trace_ghost_rays(lens, bounce_a, bounce_b, _one_ray(ray), t_ge0=True, ctx=ctx)
-    return TraceResult(tuple(float(x) for x in pos[0]), float(w[0]), TraceStatus(int(st[0])))
+    return _result(pos[0], w[0], st[0])
 
 
 def trace_primary_ray(ray: Ray, lens: OpticalSystem, ctx=None) -> TraceResult:
     """One primary trace (bind_trace.cpp:65-67; trace.cpp:243)."""
     pos, w, st, _ = trace_ghost_rays(lens, -1, -1, _one_ray(ray), t_ge0=True, ctx=ctx)
-    return TraceResult(tuple(float(x) for x in pos[0]), float(w[0]), TraceStatus(int(st[0])))
+    return _result(pos[0], w[0], st[0])
+
+
+def _check_bounces(lens: OpticalSystem, bounce_a: int, bounce_b: int) -> None:
+    if bounce_a < 0 or bounce_b < 0 or bounce_a >= bounce_b or bounce_b >= lens.num_surfaces():
+        raise ValueError("bounce indices must satisfy 0 <= bounce_a < bounce_b < num_surfaces")
+
+
+TRACE_EVENT_DTYPE = np.dtype([("surface_index", np.int32), ("hit_point", np.float32, 3),
+                              ("surface_normal", np.float32, 3), ("ior_before", np.float32),
+                              ("ior_after", np.float32), ("fresnel_weight", np.float32), ("status", np.int32),
+                              ("reflected", np.int32)])
+
+
+def trace_rays_diagnostic(lens: OpticalSystem, bounce_a: int, bounce_b: int, rays: np.ndarray, t_ge0: bool = True,
+                          fp64_radius_min: float = 0.0, ctx=None):
+    """Device diagnostic traces of (N, 7) rays: (events (N, 3 n_surfaces)
+    structured array of TRACE_EVENT_DTYPE, n_events (N,), pos (N, 3),
+    weight (N,), status (N,)); bounce_b < 0 traces the primary path."""
+    r = np.ascontiguousarray(rays, np.float32).reshape(-1, 7)
+    n = len(r)
+    m = max(1, 3 * lens.num_surfaces())
+    ev = np.zeros((max(1, n), m), TRACE_EVENT_DTYPE)
+    cnt = np.zeros(max(1, n), np.int32)
+    pos = np.zeros((max(1, n), 3), np.float32)
+    w = np.zeros(max(1, n), np.float32)
+    st = np.zeros(max(1, n), np.int32)
+    dl = lens.device_lens()
+    _abi.check(lib().gl_trace_rays_diagnostic(ctx_handle(ctx), dl.handle, int(bounce_a), int(bounce_b), _abi.fptr(r),
+                                              n, int(bool(t_ge0)), float(fp64_radius_min), m,
+                                              ev.ctypes.data_as(C.POINTER(_abi.gl_trace_event)), _abi.iptr(cnt),
+                                              _abi.fptr(pos), _abi.fptr(w), _abi.iptr(st)), "trace_rays_diagnostic")
+    return ev[:n], cnt[:n], pos[:n], w[:n], st[:n]
+
+
+def _ray_path(ev, cnt, pos, w, st) -> RayPath:
+    events = [TraceEvent(int(e["surface_index"]), Vec3f(*(float(x) for x in e["hit_point"])),
+                         Vec3f(*(float(x) for x in e["surface_normal"])), float(e["ior_before"]),
+                         float(e["ior_after"]), float(e["fresnel_weight"]), TraceStatus(int(e["status"])),
+                         bool(e["reflected"])) for e in ev[:int(cnt)]]
+    return RayPath(events, _result(pos, w, st))
+
+
+def trace_ghost_ray_diagnostic(ray: Ray, lens: OpticalSystem, bounce_a: int, bounce_b: int, ctx=None) -> RayPath:
+    """One ghost trace with its per-surface events (bind_trace.cpp:71-79;
+    trace.cpp:324-395), on the device."""
+    _check_bounces(lens, bounce_a, bounce_b)
+    ev, cnt, pos, w, st = trace_rays_diagnostic(lens, bounce_a, bounce_b, _one_ray(ray), ctx=ctx)
+    return _ray_path(ev[0], cnt[0], pos[0], w[0], st[0])
+
+
+def trace_primary_ray_diagnostic(ray: Ray, lens: OpticalSystem, ctx=None) -> RayPath:
+    """One primary trace with its per-surface events (bind_trace.cpp:81-86;
+    trace.cpp:264-320), on the device."""
+    ev, cnt, pos, w, st = trace_rays_diagnostic(lens, -1, -1, _one_ray(ray), ctx=ctx)
+    return _ray_path(ev[0], cnt[0], pos[0], w[0], st[0])
 
 
 def solve_source_angle(lens: OpticalSystem, calib: LensCalibration, ndc_x: float, ndc_y: float,
@@ -360,6 +439,55 @@ def render_gate_layer(sources, width: int, height: int, lens: OpticalSystem, con
     if counters:
         out["gate_counters"] = (int(ctr[0]), int(ctr[1]))
     return out
+
+
+_GATE_KEYS = ("x_pos", "x_neg", "y_pos", "y_neg", "t_mm", "zs_mm", "sig_wide", "sig_tight")
+
+
+def _render_gate_debug(w: int, h: int, lens: OpticalSystem, calibration: LensCalibration, config: PointFlareConfig,
+                       ctx=None) -> dict:
+    """The gate layer of one source with the pass's telemetry
+    (bind_renderers.cpp:288-334): gate_r/g/b, the direct image of the same
+    primary traces (direct_r/g/b), traces, scrapes, energy, the largest
+    realised specular reach, the predicted capture bands t / (2 N) and the
+    resolved gate.  The source is placed as the other debug entry points
+    place it (source_x / source_y through the source map).  With the gate
+    disabled the pass is a no-op: zero layer, no direct image, zero
+    telemetry."""
+    _dims("_render_gate_debug", w, h)
+    from .config import gate_params
+    from .diffraction import _debug_source
+    src, _px, _py, hw, hh = _debug_source(lens, calibration, config, w, h)
+    planes = [np.zeros((h, w), np.float32) for _ in range(3)]
+    d = {"width": int(w), "height": int(h)}
+    dbg = _abi.gl_gate_debug()
+    if config.gate.enabled:
+        direct = [np.zeros((h, w), np.float32) for _ in range(3)]
+        dbg.direct_r, dbg.direct_g, dbg.direct_b = (_abi.fptr(x) for x in direct)
+        sarr, n_src = _sources_array(src)
+        p, _keep = to_params(config)
+        gp = gate_params(config.gate)
+        _abi.check(lib().gl_render_gate_debug(ctx_handle(ctx), lens.device_lens().handle, sarr, n_src, w, h, w, h, 0,
+                                              0, float(hw), float(hh), C.byref(p), C.byref(gp),
+                                              _abi.fptr(planes[0]), _abi.fptr(planes[1]), _abi.fptr(planes[2]),
+                                              C.byref(dbg)), "render_gate")
+    d["gate_r"], d["gate_g"], d["gate_b"] = planes
+    if config.gate.enabled:
+        d["direct_r"], d["direct_g"], d["direct_b"] = direct
+    d["traces"] = int(dbg.traces)
+    d["scrapes"] = int(dbg.scrapes)
+    d["energy"] = float(dbg.energy)
+    d["reach_mm"] = float(dbg.reach_mm)
+    # gate_render.cu:507-510: the band t / (2 N) per axis, in float.
+    f = np.float32
+    t = f(dbg.gate.t_mm)
+    nx = f(calibration.f_number_x) if calibration.f_number_x > 0 else f(0.0)
+    ny = f(calibration.f_number_y) if calibration.f_number_y > 0 else f(0.0)
+    d["band_x_mm"] = float(t / (f(2.0) * nx)) if nx > 0 else 0.0
+    d["band_y_mm"] = float(t / (f(2.0) * ny)) if ny > 0 else 0.0
+    for k in _GATE_KEYS:
+        d[k] = float(getattr(dbg.gate, k))
+    return d
 
 
 def render_point_flare(width: int, height: int, lens: OpticalSystem, calibration: LensCalibration,

@@ -439,6 +439,35 @@ int  gl_trace_ghost_rays(gl_ctx* ctx, const gl_lens* lens, int32_t bounce_a,
                          float* out_pos, float* out_w, int32_t* out_status,
                          int32_t* out_segments);
 
+/* One surface interaction of a diagnostic trace (TraceEvent,
+ * trace_event.h:48-58): the hit point (the ray's origin when the surface was
+ * missed), the surface normal (zero when missed), the index on each side,
+ * the Fresnel factor of this surface (transmittance, or the reflectance at a
+ * ghost bounce; 0 when the ray died here), the status and the bounce flag. */
+typedef struct gl_trace_event {
+    int32_t surface_index;
+    float   hit_point[3];
+    float   surface_normal[3];
+    float   ior_before, ior_after;
+    float   fresnel_weight;
+    int32_t status;                     /* GL_TRACE_*                                 */
+    int32_t reflected;                  /* 1 at a ghost bounce                        */
+} gl_trace_event;
+
+/* The diagnostic overloads of trace_ghost_ray / trace_primary_ray
+ * (trace.cpp:264-395; the reference's trace_ghost_ray_diagnostic and
+ * trace_primary_ray_diagnostic, bind_trace.cpp:71-86) for n rays at once, one
+ * lane each on the device: the landing as gl_trace_ghost_rays, plus every
+ * surface the ray reached, in order, in events[i * max_events + k] for k <
+ * n_events[i].  bounce_b < 0 traces the primary path.  max_events must be at
+ * least 3 x the lens's surface count (the longest ghost path).  Bounce
+ * indices out of order give INVALID_INPUT, weight 0 and no events, as the
+ * reference's overload does. */
+int  gl_trace_rays_diagnostic(gl_ctx* ctx, const gl_lens* lens, int32_t bounce_a, int32_t bounce_b,
+                              const float* rays, int32_t n, int32_t t_ge0, float fp64_radius_min,
+                              int32_t max_events, gl_trace_event* events, int32_t* n_events,
+                              float* out_pos, float* out_w, int32_t* out_status);
+
 /* ---- diffraction: FFT seam, starburst, veil --------------------------------- */
 
 /* LensCalibration (lens_calibration.h:11-57), same field order. */
@@ -674,6 +703,30 @@ typedef struct gl_gate_resolved {
 
 int  gl_gate_resolve(const gl_gate_params* gate, float sensor_half_w, float sensor_half_h,
                      gl_gate_resolved* out);
+
+/* The gate pass's telemetry (GateDebug, gate_render.h:27-39; backs the
+ * reference's _render_gate_debug, bind_renderers.cpp:288-334): the resolved
+ * gate, traces and scrapes, the layer's energy (sum of r + g + b over the
+ * output, in double), the largest realised specular reach |fold - edge| of
+ * any scrape, and the direct image of the same primary traces (each landing's
+ * weight added to the pixel it falls in, width*height planes the caller
+ * provides; skipped when direct_r is NULL). */
+typedef struct gl_gate_debug {
+    gl_gate_resolved gate;
+    uint64_t traces, scrapes;
+    double   energy;
+    float    reach_mm;
+    float*   direct_r;
+    float*   direct_g;
+    float*   direct_b;
+} gl_gate_debug;
+
+/* gl_render_gate with the telemetry filled into *dbg (which must not be NULL). */
+int  gl_render_gate_debug(gl_ctx* ctx, const gl_lens* lens, const gl_source* sources, int32_t n_sources,
+                          int32_t width, int32_t height, int32_t fmt_w, int32_t fmt_h, int32_t fmt_x0,
+                          int32_t fmt_y0, float sensor_half_w, float sensor_half_h,
+                          const gl_flare_params* render, const gl_gate_params* gate,
+                          float* out_r, float* out_g, float* out_b, gl_gate_debug* dbg);
 
 /* n draws of the wall's scatter lobe in its own frame (along the wall
  * normal, along the wall, down the axis), draw i from the stream seeded

@@ -78,9 +78,16 @@ extern "C" {
 int orc_render_gate(const gl_surface* s, int n, const float* arena, const gl_source* src, int n_src,
                     const gl_sample* grid, int n_grid, const gl_spec* spec, int n_spec, const gl_gate_params* gp,
                     int jitter_seed, float fp64_rmin, float half_w, float half_h, int width, int height, int fmt_w,
-                    int fmt_h, int x0, int y0, float* out_r, float* out_g, float* out_b, uint64_t* counters) {
+                    int fmt_h, int x0, int y0, float* out_r, float* out_g, float* out_b, uint64_t* counters,
+                    float* dir_r, float* dir_g, float* dir_b, float* reach_out) {
+    // dir_* / reach_out (may be null): the pass's telemetry
+    // (gate_render.cu:158-172, 209-213): the direct image of the primary
+    // traces (each landing's weight in the pixel it falls in) and the largest
+    // realised |fold - edge| of a scrape.
     const size_t npx = (size_t)width * height;
     std::vector<double> R(npx, 0.0), G(npx, 0.0), B(npx, 0.0);
+    std::vector<double> DR(dir_r ? npx : 0, 0.0), DG(dir_r ? npx : 0, 0.0), DB(dir_r ? npx : 0, 0.0);
+    float reach = 0.0f;
     uint64_t traces = 0, scrapes = 0;
     if (n > 0 && n_grid > 0 && n_src > 0) {
         Lens L;
@@ -121,6 +128,17 @@ int orc_render_gate(const gl_surface* s, int n, const float* arena, const gl_sou
                     if (!std::isfinite(res.pos.x) || !std::isfinite(res.pos.y)) continue;
                     if (!(std::fabs(d.z) > 1.0e-12f)) continue;
                     const float w_base = res.weight * ray_weight;
+                    if (dir_r) {
+                        const float dpx = (res.pos.x / (2.0f * half_w) + 0.5f) * (float)fmt_w + (float)x0;
+                        const float dpy = (res.pos.y / (2.0f * half_h) + 0.5f) * (float)fmt_h + (float)y0;
+                        const int dxi = (int)std::floor(dpx), dyi = (int)std::floor(dpy);
+                        if (dxi >= 0 && dxi < width && dyi >= 0 && dyi < height) {
+                            const size_t pix = (size_t)dyi * width + dxi;
+                            DR[pix] += (double)(sc.r * spec[si].cmf_r * w_base);
+                            DG[pix] += (double)(sc.g * spec[si].cmf_g * w_base);
+                            DB[pix] += (double)(sc.b * spec[si].cmf_b * w_base);
+                        }
+                    }
                     const float sx = d.x / d.z, sy = d.y / d.z;
                     for (int wall = 0; wall < 4; ++wall) {
                         const int axis = wall >> 1;
@@ -143,6 +161,7 @@ int orc_render_gate(const gl_surface* s, int n, const float* arena, const gl_sou
                         const float hit_other = po - so * back;
                         if (hit_other <= lo || hit_other >= hi) continue;
                         ++scrapes;
+                        reach = std::fmax(reach, std::fabs((2.0f * edge - pa) - edge));
                         const float c = 1.0f - std::fmin(std::fmax(std::fabs(na), 0.0f), 1.0f);
                         const float c2 = c * c;
                         const float Rs = g.r0 + (1.0f - g.r0) * c2 * c2 * c;
@@ -189,6 +208,9 @@ int orc_render_gate(const gl_surface* s, int n, const float* arena, const gl_sou
         }
     }
     for (size_t i = 0; i < npx; ++i) { out_r[i] = (float)R[i]; out_g[i] = (float)G[i]; out_b[i] = (float)B[i]; }
+    if (dir_r)
+        for (size_t i = 0; i < npx; ++i) { dir_r[i] = (float)DR[i]; dir_g[i] = (float)DG[i]; dir_b[i] = (float)DB[i]; }
+    if (reach_out) *reach_out = reach;
     if (counters) { counters[0] = traces; counters[1] = scrapes; }
     return 0;
 }

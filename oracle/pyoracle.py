@@ -180,11 +180,12 @@ def ghost_stats(lens, pairs, boosts, sources, spec, params, half_w, half_h, widt
 
 
 def render_gate(lens, sources, grid, spec, gate, jitter_seed, half_w, half_h, width, height, fp64_rmin=0.0,
-                fmt=None):
+                fmt=None, telemetry=False):
     """The film-gate layer on the CPU: sources (N, 5), grid (K, 2) pupil
     samples, spec (S, 4) table already times d_lambda, gate a gl_gate_params,
     fmt = (fmt_w, fmt_h, x0, y0) frame window in the buffer (default: the
-    whole buffer).  Returns (r, g, b) planes and (traces, scrapes)."""
+    whole buffer).  Returns (r, g, b) planes and (traces, scrapes); with
+    telemetry also a dict with the direct image (direct_r/g/b) and reach_mm."""
     fw, fh, x0, y0 = fmt if fmt is not None else (width, height, 0, 0)
     from ghostlight_amd import _abi
     surfs, arena = lens.pack()
@@ -197,11 +198,18 @@ def render_gate(lens, sources, grid, spec, gate, jitter_seed, half_w, half_h, wi
     C.memmove(garr, g.ctypes.data, g.nbytes)
     sp = np.ascontiguousarray(spec, np.float32).reshape(-1, 4)
     planes = [np.zeros((height, width), np.float32) for _ in range(3)]
+    direct = [np.zeros((height, width), np.float32) for _ in range(3)] if telemetry else None
+    reach = C.c_float(0.0)
     ctr = np.zeros(2, np.uint64)
     lib().orc_render_gate(surfs, n, _f(arena), sarr, len(src), garr, len(g), _f(sp), len(sp), C.byref(gate),
                           int(jitter_seed), C.c_float(fp64_rmin), C.c_float(half_w), C.c_float(half_h), width, height,
                           fw, fh, x0, y0, _f(planes[0]), _f(planes[1]), _f(planes[2]),
-                          ctr.ctypes.data_as(C.POINTER(C.c_uint64)))
+                          ctr.ctypes.data_as(C.POINTER(C.c_uint64)),
+                          _f(direct[0]) if telemetry else None, _f(direct[1]) if telemetry else None,
+                          _f(direct[2]) if telemetry else None, C.byref(reach) if telemetry else None)
+    if telemetry:
+        return tuple(planes), (int(ctr[0]), int(ctr[1])), {"direct_r": direct[0], "direct_g": direct[1],
+                                                           "direct_b": direct[2], "reach_mm": float(reach.value)}
     return tuple(planes), (int(ctr[0]), int(ctr[1]))
 
 

@@ -16,6 +16,7 @@
 
 #include "../include/ghostlight_amd.h"
 
+#include <algorithm>
 #include <cstring>
 #include <vector>
 
@@ -192,6 +193,34 @@ int ref_trace_primary_rays(const gl_surface* s, int n, const float* arena, const
         Ray r; const float* q = rays + 7 * i;
         r.origin = Vec3f(q[0], q[1], q[2]); r.dir = Vec3f(q[3], q[4], q[5]); r.lambda = q[6];
         put(trace_primary_ray(r, L), pos + 3 * i, w + i, st + i);
+    }
+    return 0;
+}
+
+// The RayPath overloads of trace_ghost_ray / trace_primary_ray
+// (trace.cpp:264-395) for n rays; b < 0 traces the primary path.  Events go to
+// events[i * max_events + k], k < n_events[i] (truncated at max_events).
+int ref_trace_ray_paths(const gl_surface* s, int n, const float* arena, int a, int b, const float* rays, int n_rays,
+                        int max_events, gl_trace_event* events, int* n_events, float* pos, float* w, int* st) {
+    OpticalSystem L = make_lens(s, n, arena);
+    for (int i = 0; i < n_rays; ++i) {
+        Ray r; const float* q = rays + 7 * i;
+        r.origin = Vec3f(q[0], q[1], q[2]); r.dir = Vec3f(q[3], q[4], q[5]); r.lambda = q[6];
+        RayPath path;
+        const TraceResult res = b >= 0 ? trace_ghost_ray(r, L, a, b, path) : trace_primary_ray(r, L, path);
+        put(res, pos + 3 * i, w + i, st + i);
+        const int m = (int)std::min<size_t>(path.events.size(), (size_t)max_events);
+        n_events[i] = m;
+        for (int k = 0; k < m; ++k) {
+            const TraceEvent& e = path.events[k];
+            gl_trace_event& o = events[(size_t)i * max_events + k];
+            o.surface_index = e.surface_index;
+            o.hit_point[0] = e.hit_point.x; o.hit_point[1] = e.hit_point.y; o.hit_point[2] = e.hit_point.z;
+            o.surface_normal[0] = e.surface_normal.x; o.surface_normal[1] = e.surface_normal.y;
+            o.surface_normal[2] = e.surface_normal.z;
+            o.ior_before = e.ior_before; o.ior_after = e.ior_after; o.fresnel_weight = e.fresnel_weight;
+            o.status = (int)e.status; o.reflected = e.reflected ? 1 : 0;
+        }
     }
     return 0;
 }

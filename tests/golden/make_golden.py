@@ -441,6 +441,46 @@ def blade_goldens():
     print(f"wrote {path} ({os.path.getsize(path)} bytes, {len(out)} arrays)")
 
 
+def trace_path_goldens():
+    """The diagnostic traces (trace.cpp:264-395): per-surface events and the
+    landing of 96 rays through five pairs and the primary path of each
+    synthetic lens (and of the 12-surface one with an interior surface
+    muted), from the reference's own RayPath overloads."""
+    from ghostlight_amd.render import TRACE_EVENT_DTYPE
+    ref = pyoracle.ref()
+    if ref is None:
+        raise SystemExit("oracle/_ref/libglref.so not built (needs the reference sources)")
+    muted = synthetic.config1_lens()
+    muted.surfaces[5].is_active = False
+    lenses = {"c1": synthetic.config1_lens(), "c2": synthetic.config2_lens(), "c4": synthetic.config4_lens(),
+              "c1m": muted}
+    out = {}
+    for name, lens in lenses.items():
+        surfs, arena = lens.pack()
+        n = lens.num_surfaces()
+        out[f"lens_{name}_surfs"] = np.frombuffer(bytes(surfs), np.uint8).copy()
+        out[f"lens_{name}_arena"] = arena
+        rays = rays_for(lens, 96, seed=23, max_deg=12.0)
+        out[f"rays_{name}"] = rays
+        m = 3 * n
+        pairs = [(0, 1), (1, n - 1), (2, n // 2), (n // 3, n - 2), (n - 3, n - 1), (-1, -1)]
+        out[f"pairs_{name}"] = np.array(pairs, np.int32)
+        for (a, b) in pairs:
+            ev = np.zeros((len(rays), m), TRACE_EVENT_DTYPE)
+            cnt = np.zeros(len(rays), np.int32)
+            pos = np.zeros((len(rays), 3), np.float32)
+            w = np.zeros(len(rays), np.float32)
+            st = np.zeros(len(rays), np.int32)
+            ref.ref_trace_ray_paths(surfs, n, pyoracle._f(arena), a, b, pyoracle._f(rays), len(rays), m,
+                                    ev.ctypes.data_as(C.c_void_p), pyoracle._i(cnt), pyoracle._f(pos), pyoracle._f(w),
+                                    pyoracle._i(st))
+            key = f"path_{name}_{a}_{b}"
+            out[key + "_ev"], out[key + "_n"], out[key + "_pos"], out[key + "_w"], out[key + "_st"] = ev, cnt, pos, w, st
+    path = os.path.join(HERE, "trace_path_goldens.npz")
+    np.savez_compressed(path, **out)
+    print(f"wrote {path} ({os.path.getsize(path) / 1024:.0f} KiB, {len(out)} arrays)")
+
+
 if __name__ == "__main__":
     if len(sys.argv) > 1 and sys.argv[1] == "point_flare":
         point_flare_goldens()
@@ -452,6 +492,8 @@ if __name__ == "__main__":
         aperture_baseline_goldens()
     elif len(sys.argv) > 1 and sys.argv[1] == "blades":
         blade_goldens()
+    elif len(sys.argv) > 1 and sys.argv[1] == "trace_paths":
+        trace_path_goldens()
     else:
         main()
         point_flare_goldens()
@@ -459,3 +501,4 @@ if __name__ == "__main__":
         lens_file_goldens()
         aperture_baseline_goldens()
         blade_goldens()
+        trace_path_goldens()

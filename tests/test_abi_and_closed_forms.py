@@ -33,6 +33,13 @@ def test_library_exports_every_declared_symbol():
         assert s in _abi.SIGNATURES, f"no ctypes signature for {s}"
 
 
+def test_struct_sizes_match_the_library():
+    """The library static_asserts these sizes; the ctypes mirrors must agree."""
+    assert C.sizeof(_abi.gl_trace_event) == 48
+    assert C.sizeof(_abi.gl_gate_debug) == 120
+    assert C.sizeof(_abi.gl_gate_resolved) == 64
+
+
 def test_device_query_and_errors_without_device():
     lib = _abi.load_library()
     assert lib.gl_device_available() in (0, 1)
