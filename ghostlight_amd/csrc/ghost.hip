@@ -633,10 +633,10 @@ static int do_launch(Launch& Lc, float* out_r, float* out_g, float* out_b, float
     if ((st = upload(c, "src", Lc.src.data(), n_src * sizeof(SrcDev), &d_src))) return st;
     if ((st = upload(c, "iat", iat.data(), iat.size() * 4, &d_iat))) return st;
     if ((st = upload(c, "ibf", ibf.data(), ibf.size() * 4, &d_ibf))) return st;
-    std::vector<int2> lrows;
+    std::vector<int32_t> lrows;
     lambda_rows(L->host.data(), n, L->arena.data(), c->spec.data(), ns, lrows);
     void* d_lrow;
-    if ((st = upload(c, "lrow", lrows.data(), lrows.size() * sizeof(int2), &d_lrow))) return st;
+    if ((st = upload(c, "lrow", lrows.data(), lrows.size() * sizeof(int32_t), &d_lrow))) return st;
     // Refraction ratios per (lambda, surface): forward ibf / iat, backward
     // iat / ibf, the same IEEE quotients the tracer would form.
     std::vector<float2> rats((size_t)ns * n);

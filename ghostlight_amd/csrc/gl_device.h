@@ -70,7 +70,7 @@ struct DevSurf {
     float angle_ref_ior;
     int32_t tab_off, tab_n, sa_off, sa_nw;
     int32_t sa_na, n_asph;
-    float sa_a0, sa_inv_da;          // SF_SA_UNIFORM: first angle, 1 / spacing
+    float sa_a0, sa_inv_da, sa_da;   // SF_SA_UNIFORM: first angle, 1 / spacing, spacing
     float tint_r, tint_g, tint_b, tint_strength;
     float g_sigma, g_bg, g_peak, g_dx, g_dy;
     float asph[GL_MAX_ASPHERE_TERMS];
@@ -258,48 +258,68 @@ GLD float surface_weight(F3 dir, F3 nrm, float n1, float n2, const DevSurf GLC& 
             const float* wl = arena + s.sa_off;
             const float* an = wl + nw;
             const float* rr = an + na;
-            float ang = ref_angle_deg(cos_i, n1, s.angle_ref_ior);
+            const float ang = ref_angle_deg(cos_i, n1, s.angle_ref_ior);
             bool disc = false;
             const bool oob = s.discard != 0;
             const bool sorted = (s.flags & SF_SA_SORTED) != 0;
-            int iw = 0; float fw = 0.0f;
-            if (lr) {
-                const int2 e = ld_i2(lr);
-                if (e.x == -1) return 0.0f;
-                iw = e.x; fw = __int_as_float(e.y);
-            } else if (lam <= wl[0]) { if (oob && lam < wl[0]) disc = true; }
-            else if (lam >= wl[nw - 1]) { iw = nw > 1 ? nw - 2 : 0; fw = nw > 1 ? 1.0f : 0.0f; if (oob && lam > wl[nw - 1]) disc = true; }
-            else {
-                const int i = sorted ? lower_cell(wl, 1, 1, nw - 1, lam) : 1;
-                for (int j = i; j < nw; ++j) if (lam <= wl[j]) {
-                    iw = j - 1; float span = wl[j] - wl[j - 1];
-                    fw = span > 0.0f ? (lam - wl[j - 1]) / span : 0.0f; break; }
-            }
+            // Angle cell: ia, fa.
             int ia = 0; float fa = 0.0f;
-            if (ang <= an[0]) { if (oob && ang < an[0]) disc = true; }
+            if (s.flags & SF_SA_UNIFORM) {
+                // The host checked an[j] == a0 + j * da in float for every j,
+                // so the nodes are recomputed here (same float operations)
+                // instead of read: the first j with ang <= an[j] from an
+                // arithmetic guess and at most one step either way.
+                const float a0 = s.sa_a0, da = s.sa_da;
+                const float alast = a0 + (float)(na - 1) * da;
+                if (ang <= a0) { if (oob && ang < a0) disc = true; }
+                else if (ang >= alast) { ia = na > 1 ? na - 2 : 0; fa = na > 1 ? 1.0f : 0.0f; if (oob && ang > alast) disc = true; }
+                else {
+                    int j = 1 + (int)((ang - a0) * s.sa_inv_da);
+                    j = j < 1 ? 1 : (j > na - 1 ? na - 1 : j);
+                    if (j > 1 && ang <= a0 + (float)(j - 1) * da) --j;
+                    else if (j < na - 1 && ang > a0 + (float)j * da) ++j;
+                    ia = j - 1;
+                    const float lo_a = a0 + (float)(j - 1) * da, hi_a = a0 + (float)j * da;
+                    const float span = hi_a - lo_a;
+                    fa = span > 0.0f ? (ang - lo_a) * __builtin_amdgcn_rcpf(span) : 0.0f;
+                }
+            } else if (ang <= an[0]) { if (oob && ang < an[0]) disc = true; }
             else if (ang >= an[na - 1]) { ia = na > 1 ? na - 2 : 0; fa = na > 1 ? 1.0f : 0.0f; if (oob && ang > an[na - 1]) disc = true; }
-            else if (s.flags & SF_SA_UNIFORM) {
-                // First j in [1, na-1] with ang <= an[j]: an arithmetic guess,
-                // then exact steps against the table (at most one or two).
-                int j = 1 + (int)((ang - s.sa_a0) * s.sa_inv_da);
-                j = j < 1 ? 1 : (j > na - 1 ? na - 1 : j);
-                while (j > 1 && ang <= an[j - 1]) --j;
-                while (j < na - 1 && ang > an[j]) ++j;
-                ia = j - 1;
-                const float span = an[j] - an[j - 1];
-                fa = span > 0.0f ? (ang - an[j - 1]) * __builtin_amdgcn_rcpf(span) : 0.0f;
-            } else {
+            else {
                 const int i = sorted ? lower_cell(an, 1, 1, na - 1, ang) : 1;
                 for (int j = i; j < na; ++j) if (ang <= an[j]) {
                     ia = j - 1; float span = an[j] - an[j - 1];
                     fa = span > 0.0f ? (ang - an[j - 1]) * __builtin_amdgcn_rcpf(span) : 0.0f; break; }
             }
-            if (disc) return 0.0f;
-            const int iw1 = iw + 1 < nw ? iw + 1 : iw, ia1 = ia + 1 < na ? ia + 1 : ia;
-            float r00 = rr[iw * na + ia], r01 = rr[iw * na + ia1], r10 = rr[iw1 * na + ia], r11 = rr[iw1 * na + ia1];
-            // bilinear blend (weight only: fused multiply-adds)
-            const float lo = __builtin_fmaf(fa, r01 - r00, r00), hi = __builtin_fmaf(fa, r11 - r10, r10);
-            R = __builtin_fmaf(fw, hi - lo, lo);
+            if (lr) {
+                // The lambda row holds this wavelength's table blended along
+                // lambda on the host, node by node, as {R_q, R_q+1 - R_q}
+                // pairs at a self-relative offset (-1: the wavelength is
+                // discarded): one 8-byte gather instead of four table reads.
+                const int2 e = ld_i2(lr);
+                if (e.x == -1 || disc) return 0.0f;
+                // (entries and rows both start at even words: 8-byte aligned)
+                const unsigned long long GLC* row =
+                    reinterpret_cast<const unsigned long long GLC*>(reinterpret_cast<const float GLC*>(lr) + e.x);
+                const unsigned long long q = row[ia];
+                R = __builtin_fmaf(fa, __uint_as_float((unsigned)(q >> 32)), __uint_as_float((unsigned)q));
+            } else {
+                int iw = 0; float fw = 0.0f;
+                if (lam <= wl[0]) { if (oob && lam < wl[0]) disc = true; }
+                else if (lam >= wl[nw - 1]) { iw = nw > 1 ? nw - 2 : 0; fw = nw > 1 ? 1.0f : 0.0f; if (oob && lam > wl[nw - 1]) disc = true; }
+                else {
+                    const int i = sorted ? lower_cell(wl, 1, 1, nw - 1, lam) : 1;
+                    for (int j = i; j < nw; ++j) if (lam <= wl[j]) {
+                        iw = j - 1; float span = wl[j] - wl[j - 1];
+                        fw = span > 0.0f ? (lam - wl[j - 1]) / span : 0.0f; break; }
+                }
+                if (disc) return 0.0f;
+                const int iw1 = iw + 1 < nw ? iw + 1 : iw, ia1 = ia + 1 < na ? ia + 1 : ia;
+                float r00 = rr[iw * na + ia], r01 = rr[iw * na + ia1], r10 = rr[iw1 * na + ia], r11 = rr[iw1 * na + ia1];
+                // bilinear blend (weight only: fused multiply-adds)
+                const float lo = __builtin_fmaf(fa, r01 - r00, r00), hi = __builtin_fmaf(fa, r11 - r10, r10);
+                R = __builtin_fmaf(fw, hi - lo, lo);
+            }
         }
     } else {
         R = fresnel_R(cos_i, n1, n2);

@@ -131,9 +131,13 @@ bool lean_lens(const std::vector<DevSurf>& ds);
 // every active surface is spectral-angular or a bare interface,
 // CC_AIRY | CC_FRESNEL when every active surface is SIMPLE, else CC_ALL.
 int coating_classes(const std::vector<DevSurf>& ds);
-// Per-lambda coating lookups of the lambda-only part of surface_weight, one
-// int2 per (lambda, surface): see LambdaRow in gl_device.h.
-void lambda_rows(const gl_surface* s, int n, const float* arena, const gl_spec* spec, int ns, std::vector<int2>& out);
+// Per-lambda coating lookups of the lambda-only part of surface_weight: one
+// int2 entry per (lambda, surface) -- {-2, R bits} for a lambda-only value,
+// {-1, 0} for a discarded wavelength, {offset, 0} for a spectral-angular
+// table whose lambda-blended {R, dR} row sits `offset` words after the entry
+// -- followed by those rows, in one buffer of 4-byte words.
+void lambda_rows(const gl_surface* s, int n, const float* arena, const gl_spec* spec, int ns,
+                 std::vector<int32_t>& buf);
 DevMask build_mask(const gl_surface* s, int n, int blades_override, float rotation_deg);
 
 // host preparation (hostprep.cpp)

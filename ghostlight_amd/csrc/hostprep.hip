@@ -405,6 +405,7 @@ void build_devsurfs(const gl_surface* s, int n, float fp64_rmin, const float* ar
                     f |= SF_SA_UNIFORM;
                     d.sa_a0 = an[0];
                     d.sa_inv_da = 1.0f / da;
+                    d.sa_da = da;
                 }
             }
         }
@@ -429,8 +430,10 @@ void build_devsurfs(const gl_surface* s, int n, float fp64_rmin, const float* ar
 //   SPECTRAL          x = -2, y = bits of the interpolated reflectance
 //   ARTIST            x = -2, y = bits of the tinted reflectance
 //   out of range with discard: x = -1 (weight 0);  other models: x = -3.
-void lambda_rows(const gl_surface* s, int n, const float* arena, const gl_spec* spec, int ns, std::vector<int2>& out) {
-    out.assign((size_t)ns * n, make_int2(-3, 0));
+void lambda_rows(const gl_surface* s, int n, const float* arena, const gl_spec* spec, int ns, std::vector<int32_t>& buf) {
+    std::vector<int2> out((size_t)ns * n, make_int2(-3, 0));
+    std::vector<float> rows;                 // SPECTRAL_ANGULAR rows, appended after the entries
+    const size_t rows_at = 2 * out.size();   // in 4-byte words
     auto bits = [](float f) { int i; std::memcpy(&i, &f, 4); return i; };
     for (int si = 0; si < ns; ++si) {
         const float lam = spec[si].lambda;
@@ -447,7 +450,27 @@ void lambda_rows(const gl_surface* s, int n, const float* arena, const gl_spec* 
                 else for (int i = 1; i < nw; ++i) if (lam <= wl[i]) {
                     iw = i - 1; const float span = wl[i] - wl[i - 1];
                     fw = span > 0.0f ? (lam - wl[i - 1]) / span : 0.0f; break; }
-                e = disc ? make_int2(-1, 0) : make_int2(iw, bits(fw));
+                if (disc) {
+                    e = make_int2(-1, 0);
+                } else {
+                    // This wavelength's table blended along lambda, node by
+                    // node, stored as {R_q, R_q+1 - R_q} (the last node's
+                    // slope is 0: the device clamps the upper node there).
+                    const int na = g.sa_n_angles;
+                    const float* rr = arena + g.sa_offset + nw + na;
+                    const int iw1 = iw + 1 < nw ? iw + 1 : iw;
+                    std::vector<float> bl(na);
+                    for (int q = 0; q < na; ++q) {
+                        const float r0 = rr[iw * na + q], r1 = rr[iw1 * na + q];
+                        bl[q] = std::fma(fw, r1 - r0, r0);
+                    }
+                    const size_t at = rows_at + rows.size();
+                    for (int q = 0; q < na; ++q) {
+                        rows.push_back(bl[q]);
+                        rows.push_back(q + 1 < na ? bl[q + 1] - bl[q] : 0.0f);
+                    }
+                    e = make_int2((int)(at - 2 * ((size_t)si * n + k)), 0);
+                }
             } else if (g.coating_model == GL_COAT_SPECTRAL && g.table_count > 0) {
                 const float* t = arena + g.table_offset;
                 const int tn = g.table_count;
@@ -476,6 +499,9 @@ void lambda_rows(const gl_surface* s, int n, const float* arena, const gl_spec* 
             }
         }
     }
+    buf.resize(rows_at + rows.size());
+    std::memcpy(buf.data(), out.data(), out.size() * sizeof(int2));
+    if (!rows.empty()) std::memcpy(buf.data() + rows_at, rows.data(), rows.size() * sizeof(float));
 }
 
 double now_ms() {
