@@ -266,23 +266,18 @@ GLD float surface_weight(F3 dir, F3 nrm, float n1, float n2, const DevSurf GLC& 
             int ia = 0; float fa = 0.0f;
             if (s.flags & SF_SA_UNIFORM) {
                 // The host checked an[j] == a0 + j * da in float for every j,
-                // so the nodes are recomputed here (same float operations)
-                // instead of read: the first j with ang <= an[j] from an
-                // arithmetic guess and at most one step either way.
+                // so the cell comes from arithmetic: ia = floor((ang - a0) /
+                // da) clamped to the table, fa measured from the recomputed
+                // node (same float operations as the host's check) and
+                // clamped to [0, 1].  Where the reference's scan would pick
+                // the cell below a node that the floor puts above it, both
+                // interpolate to that node's value.
                 const float a0 = s.sa_a0, da = s.sa_da;
-                const float alast = a0 + (float)(na - 1) * da;
-                if (ang <= a0) { if (oob && ang < a0) disc = true; }
-                else if (ang >= alast) { ia = na > 1 ? na - 2 : 0; fa = na > 1 ? 1.0f : 0.0f; if (oob && ang > alast) disc = true; }
-                else {
-                    int j = 1 + (int)((ang - a0) * s.sa_inv_da);
-                    j = j < 1 ? 1 : (j > na - 1 ? na - 1 : j);
-                    if (j > 1 && ang <= a0 + (float)(j - 1) * da) --j;
-                    else if (j < na - 1 && ang > a0 + (float)j * da) ++j;
-                    ia = j - 1;
-                    const float lo_a = a0 + (float)(j - 1) * da, hi_a = a0 + (float)j * da;
-                    const float span = hi_a - lo_a;
-                    fa = span > 0.0f ? (ang - lo_a) * __builtin_amdgcn_rcpf(span) : 0.0f;
-                }
+                const int top = na > 1 ? na - 2 : 0;
+                const float t = fmaxf((ang - a0) * s.sa_inv_da, 0.0f);
+                ia = min((int)t, top);
+                fa = fminf(fmaxf((ang - (a0 + (float)ia * da)) * s.sa_inv_da, 0.0f), na > 1 ? 1.0f : 0.0f);
+                if (oob && (ang < a0 || ang > a0 + (float)(na - 1) * da)) disc = true;
             } else if (ang <= an[0]) { if (oob && ang < an[0]) disc = true; }
             else if (ang >= an[na - 1]) { ia = na > 1 ? na - 2 : 0; fa = na > 1 ? 1.0f : 0.0f; if (oob && ang > an[na - 1]) disc = true; }
             else {
