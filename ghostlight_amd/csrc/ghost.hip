@@ -231,11 +231,12 @@ __device__ __forceinline__ void deposit_range(const GhostArgs& A, int ps, int gi
             const float lam = spf[0];
             uint32_t hs = 0u;
             const F3 dl = H ? hurb_entry<H>(A, sr, o0, d0, lam, hray, si, hs) : d0;
-            TraceRes res = trace_ghost<true, H, LF, CM>(as_const(A.S), A.n, as_const(A.poly), A.arena, pr.surf_a,
-                                                    pr.surf_b, o0, dl, lam, as_const(A.iat) + (size_t)si * A.n,
-                                                    as_const(A.ibf) + (size_t)si * A.n, false,
-                                                    A.lrow ? as_const(A.lrow) + (size_t)si * A.n : nullptr, &A.hurb,
-                                                    &hs, A.rat ? as_const(A.rat) + (size_t)si * A.n : nullptr);
+            TraceRes res = trace_ghost<true, H, LF, CM, true>(as_const(A.S), A.n, as_const(A.poly), A.arena,
+                                                          pr.surf_a, pr.surf_b, o0, dl, lam,
+                                                          as_const(A.iat) + (size_t)si * A.n,
+                                                          as_const(A.ibf) + (size_t)si * A.n, false,
+                                                          A.lrow ? as_const(A.lrow) + (size_t)si * A.n : nullptr,
+                                                          &A.hurb, &hs, as_const(A.rat) + (size_t)si * A.n);
             if (res.status != GL_TRACE_OK) break;
             if (!isfinite(res.px) || !isfinite(res.py)) break;
             float base = res.w * w_ray * A.gain * pr.area_boost;
@@ -361,9 +362,10 @@ __global__ __launch_bounds__(kBlock) void ghost_kernel(GhostArgs A) {
             if (STATS) ++n_tr;
             uint32_t hs = 0u;
             const F3 dl = H ? hurb_entry<H>(A, sr, o0, d0, spf[0], hray, si, hs) : d0;
-            TraceRes res = trace_ghost<false, H>(as_const(A.S), A.n, as_const(A.poly), A.arena, pr.surf_a, pr.surf_b,
-                                                 o0, dl, spf[0], as_const(A.iat) + (size_t)si * A.n,
-                                                 as_const(A.ibf) + (size_t)si * A.n, false, nullptr, &A.hurb, &hs);
+            TraceRes res = trace_ghost<false, H, false, CC_ALL, true>(
+                as_const(A.S), A.n, as_const(A.poly), A.arena, pr.surf_a, pr.surf_b, o0, dl, spf[0],
+                as_const(A.iat) + (size_t)si * A.n, as_const(A.ibf) + (size_t)si * A.n, false, nullptr, &A.hurb, &hs,
+                as_const(A.rat) + (size_t)si * A.n);
             n_seg += (unsigned long long)res.segs;
             if (!STATS) continue;
             if (res.status != GL_TRACE_OK) continue;
@@ -742,16 +744,19 @@ static int do_launch(Launch& Lc, float* out_r, float* out_g, float* out_b, float
                 if (!rp.empty()) {
                     std::vector<float4> s3(rns);
                     std::vector<float> a3((size_t)rns * n), b3((size_t)rns * n);
+                    std::vector<float2> r3((size_t)rns * n);
                     for (int k = 0; k < rns; ++k) {
                         const gl_spec& q = c->spec[li[k]];
                         s3[k] = make_float4(q.lambda, q.cmf_r, q.cmf_g, q.cmf_b);
                         std::copy_n(&iat[(size_t)li[k] * n], n, &a3[(size_t)k * n]);
                         std::copy_n(&ibf[(size_t)li[k] * n], n, &b3[(size_t)k * n]);
+                        std::copy_n(&rats[(size_t)li[k] * n], n, &r3[(size_t)k * n]);
                     }
-                    void *d_s3, *d_a3, *d_b3;
+                    void *d_s3, *d_a3, *d_b3, *d_r3;
                     if ((st = upload(c, "spec3", s3.data(), rns * sizeof(float4), &d_s3)) ||
                         (st = upload(c, "iat3", a3.data(), a3.size() * 4, &d_a3)) ||
-                        (st = upload(c, "ibf3", b3.data(), b3.size() * 4, &d_b3)))
+                        (st = upload(c, "ibf3", b3.data(), b3.size() * 4, &d_b3)) ||
+                        (st = upload(c, "rat3", r3.data(), r3.size() * sizeof(float2), &d_r3)))
                         return st;
                     GL_HIP(hipMemcpyAsync(brf.p, refine.data(), n_ps, hipMemcpyHostToDevice, c->stream));
                     GL_HIP(hipMemcpyAsync(bg.p, rp.data(), rp.size() * sizeof(gl_sample), hipMemcpyHostToDevice, c->stream));
@@ -759,6 +764,7 @@ static int do_launch(Launch& Lc, float* out_r, float* out_g, float* out_b, float
                     R.grid = bg.as<gl_sample>(); R.n_grid = (int)rp.size();
                     R.spec = static_cast<const float4*>(d_s3); R.n_spec = rns;
                     R.iat = static_cast<const float*>(d_a3); R.ibf = static_cast<const float*>(d_b3);
+                    R.rat = static_cast<const float2*>(d_r3);
                     R.pair_active = nullptr; R.ps_refine = brf.as<unsigned char>();
                     if ((st = launch<true, false, false>(R, n_ps, R.n_grid, c->stream))) return st;
                 }

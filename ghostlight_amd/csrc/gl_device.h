@@ -13,8 +13,10 @@
 // (trace_core.h, trace_cuda.h, fresnel.h; see citations per function) and
 // the whole file is compiled with -ffp-contract=off and IEEE division and
 // square root, so statuses, hit/vignette masks and landing positions are
-// bit-identical to the FP32 CPU oracle.  The only non-IEEE functions are the
-// coating's cos/asin/exp, which scale weights, never masks.
+// bit-identical to the FP32 CPU oracle.  Arithmetic that feeds only a weight
+// (Fresnel and coating amplitudes, the coating-table angle, cos/asin/exp)
+// uses hardware reciprocal and square root estimates instead: weights move
+// by a few ulp, never a mask.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -100,6 +102,14 @@ GLD F3 normalized(F3 v) {  // vec3.h:67-71
     float l = sqrtf(v.x * v.x + v.y * v.y + v.z * v.z);
     return (l > 1e-12f) ? mul(v, 1.0f / l) : f3(0.f, 0.f, 0.f);
 }
+GLD float sqrt_rn_big(float x);
+// normalized() for a vector of length near 1 (a reflected unit direction):
+// |v|^2 stays far above the small-subnormal range, so sqrt_rn_big gives
+// sqrtf's bits.
+GLD F3 normalized_unit(F3 v) {
+    float l = sqrt_rn_big(v.x * v.x + v.y * v.y + v.z * v.z);
+    return (l > 1e-12f) ? mul(v, 1.0f / l) : f3(0.f, 0.f, 0.f);
+}
 template <class P> GLD F3 m_mul(P m, F3 v) {   // trace_core.h:33-39
     return f3(m[0] * v.x + m[1] * v.y + m[2] * v.z, m[3] * v.x + m[4] * v.y + m[5] * v.z,
               m[6] * v.x + m[7] * v.y + m[8] * v.z);
@@ -110,31 +120,55 @@ template <class P> GLD F3 m_tmul(P m, F3 v) {  // trace_core.h:41-47
 }
 
 // ---- Fresnel and coatings (fresnel.h:69-178) -----------------------------
-GLD float fresnel_R(float cos_i, float n1, float n2) {
+// Correctly rounded square root for arguments that are 0, at least 2^-96,
+// +inf or NaN.  The hardware estimate (v_sqrt_f32, within 1 ulp) is moved to
+// the neighbour whose residual x - s * s' has the right sign: the IEEE square
+// root's own rounding step without its input scaling, which is the identity
+// on this range, so the bits equal sqrtf's.  Used where the argument is
+// bounded away from the small-subnormal range by construction.
+GLD float sqrt_rn_big(float x) {
+    const float s = __builtin_amdgcn_sqrtf(x);
+    const float sm = __int_as_float(__float_as_int(s) - 1), sp = __int_as_float(__float_as_int(s) + 1);
+    const float rm = __builtin_fmaf(-sm, s, x), rp = __builtin_fmaf(-sp, s, x);
+    const float r = rm <= 0.0f ? sm : s;
+    return rp > 0.0f ? sp : r;
+}
+// Weight-only quotient: reciprocal estimate (1 ulp) times the numerator.
+GLD float wdiv(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
+
+// eta = n1 / n2, IEEE-rounded (the refraction's own ratio), so the total
+// reflection test sees the same bits as a division here.  cos_t is exact
+// (1 - sin2_t >= 2^-24), so the amplitudes' numerators carry no extra error
+// into the cancellation of nearly matched indices; only the two quotients use
+// the reciprocal estimate (a few ulp on the weight).
+GLD float fresnel_R(float cos_i, float n1, float n2, float eta) {
     cos_i = fabsf(cos_i);
-    float eta = n1 / n2;
     float sin2_t = eta * eta * (1.0f - cos_i * cos_i);
     if (sin2_t >= 1.0f) return 1.0f;
-    float cos_t = sqrtf(1.0f - sin2_t);
-    float rs = (n1 * cos_i - n2 * cos_t) / (n1 * cos_i + n2 * cos_t);
-    float rp = (n2 * cos_i - n1 * cos_t) / (n2 * cos_i + n1 * cos_t);
+    float cos_t = sqrt_rn_big(1.0f - sin2_t);
+    float rs = wdiv(n1 * cos_i - n2 * cos_t, n1 * cos_i + n2 * cos_t);
+    float rp = wdiv(n2 * cos_i - n1 * cos_t, n2 * cos_i + n1 * cos_t);
     return 0.5f * (rs * rs + rp * rp);
 }
 GLD float clamp01(float r) { return fminf(fmaxf(r, 0.0f), 1.0f); }
-GLD float coating_R(float cos_i, float n1, float n2, float cn, float d_nm, float lam) {
+// Single-layer (Airy) coating.  The interference sum cancels for an AR
+// layer, so the phase and the two amplitudes keep IEEE quotients; the square
+// roots are exact (arguments >= 2^-24) and only the final ratio, whose
+// numerator is already formed, uses the reciprocal estimate.
+GLD float coating_R(float cos_i, float n1, float n2, float cn, float d_nm, float lam, float eta) {
     float sin2_c = (n1 / cn) * (n1 / cn) * (1.0f - cos_i * cos_i);
-    if (sin2_c >= 1.0f) return fresnel_R(cos_i, n1, n2);
-    float cos_c = sqrtf(1.0f - sin2_c);
+    if (sin2_c >= 1.0f) return fresnel_R(cos_i, n1, n2, eta);
+    float cos_c = sqrt_rn_big(1.0f - sin2_c);
     float delta = 2.0f * 3.14159265358979323846f * cn * d_nm * cos_c / lam;
     float r01 = (n1 * cos_i - cn * cos_c) / (n1 * cos_i + cn * cos_c);
     float sin2_2 = (cn / n2) * (cn / n2) * (1.0f - cos_c * cos_c);
-    if (sin2_2 >= 1.0f) return fresnel_R(cos_i, n1, n2);
-    float cos_2 = sqrtf(1.0f - sin2_2);
+    if (sin2_2 >= 1.0f) return fresnel_R(cos_i, n1, n2, eta);
+    float cos_2 = sqrt_rn_big(1.0f - sin2_2);
     float r12 = (cn * cos_c - n2 * cos_2) / (cn * cos_c + n2 * cos_2);
     float c2d = cosf(2.0f * delta);
     float num = r01 * r01 + r12 * r12 + 2.0f * r01 * r12 * c2d;
     float den = 1.0f + r01 * r01 * r12 * r12 + 2.0f * r01 * r12 * c2d;
-    return clamp01(num / den);
+    return clamp01(wdiv(num, den));
 }
 // The coating-table angle feeds only the weight (never a hit, vignette or
 // segment decision), so it is computed with the hardware square root and
@@ -177,7 +211,7 @@ GLD float lerp1d(const float* t, int n, float key, bool disc, bool* out_disc, bo
     if (sorted) {
         const int i = lower_cell(t, 2, 1, n - 1, key);
         float span = t[2 * i] - t[2 * (i - 1)];
-        float f = (span > 0.0f) ? (key - t[2 * (i - 1)]) / span : 0.0f;
+        float f = (span > 0.0f) ? wdiv(key - t[2 * (i - 1)], span) : 0.0f;
         return t[2 * (i - 1) + 1] + f * (t[2 * i + 1] - t[2 * (i - 1) + 1]);
     }
     for (int i = 1; i < n; ++i)
@@ -205,29 +239,29 @@ enum CoatClass : int {
 // of the weight, precomputed on the host with the same float operations), or
 // nullptr to evaluate it here.  CM: the coating classes the lens uses.
 template <int CM = CC_ALL>
-GLD float surface_weight(F3 dir, F3 nrm, float n1, float n2, const DevSurf GLC& s, const float* __restrict__ arena,
-                         float lam, bool reflecting, const int2 GLC* lr) {
+GLD float surface_weight(F3 dir, F3 nrm, float n1, float n2, float eta, const DevSurf GLC& s,
+                         const float* __restrict__ arena, float lam, bool reflecting, const int2 GLC* lr) {
     float cos_i = fabsf(dot3(nrm, dir));
     float R;
     const int m = s.coat_model;
     if (CM == CC_FRESNEL) {
-        R = fresnel_R(cos_i, n1, n2);
+        R = fresnel_R(cos_i, n1, n2, eta);
     } else if (CM == (CC_SA | CC_FRESNEL) && m != GL_COAT_SPECTRAL_ANGULAR) {
-        R = fresnel_R(cos_i, n1, n2);
+        R = fresnel_R(cos_i, n1, n2, eta);
     } else if (CM == (CC_AIRY | CC_FRESNEL) && m == GL_COAT_SIMPLE && s.ar_layers <= 0) {
-        R = fresnel_R(cos_i, n1, n2);
+        R = fresnel_R(cos_i, n1, n2, eta);
     } else if (CM == (CC_AIRY | CC_FRESNEL)) {
         const float mg = 1.38f;
         const float qw = 550.0f / (4.0f * mg);
-        R = coating_R(cos_i, n1, n2, mg, qw, lam);
+        R = coating_R(cos_i, n1, n2, mg, qw, lam, eta);
         for (int i = 1; i < s.ar_layers; ++i) R *= 0.25f;
         R = clamp01(R);
     } else if (m == GL_COAT_SIMPLE) {
-        if (s.ar_layers <= 0) R = fresnel_R(cos_i, n1, n2);
+        if (s.ar_layers <= 0) R = fresnel_R(cos_i, n1, n2, eta);
         else {
             const float mg = 1.38f;
             const float qw = 550.0f / (4.0f * mg);
-            R = coating_R(cos_i, n1, n2, mg, qw, lam);
+            R = coating_R(cos_i, n1, n2, mg, qw, lam, eta);
             for (int i = 1; i < s.ar_layers; ++i) R *= 0.25f;
             R = clamp01(R);
         }
@@ -240,7 +274,7 @@ GLD float surface_weight(F3 dir, F3 nrm, float n1, float n2, const DevSurf GLC& 
         float sum = br + bg + bb;
         R = clamp01(s.tint_strength * (s.tint_r * br + s.tint_g * bg + s.tint_b * bb) / sum);
     } else if (m == GL_COAT_SPECTRAL || m == GL_COAT_ANGULAR) {
-        if (s.tab_n <= 0) R = fresnel_R(cos_i, n1, n2);
+        if (s.tab_n <= 0) R = fresnel_R(cos_i, n1, n2, eta);
         else if (m == GL_COAT_SPECTRAL && lr) {
             const int2 e = ld_i2(lr);
             if (e.x == -1) return 0.0f;
@@ -253,7 +287,7 @@ GLD float surface_weight(F3 dir, F3 nrm, float n1, float n2, const DevSurf GLC& 
         }
     } else if (m == GL_COAT_SPECTRAL_ANGULAR) {
         const int nw = s.sa_nw, na = s.sa_na;
-        if (nw <= 0 || na <= 0) R = fresnel_R(cos_i, n1, n2);
+        if (nw <= 0 || na <= 0) R = fresnel_R(cos_i, n1, n2, eta);
         else {
             const float* wl = arena + s.sa_off;
             const float* an = wl + nw;
@@ -317,7 +351,7 @@ GLD float surface_weight(F3 dir, F3 nrm, float n1, float n2, const DevSurf GLC& 
             }
         }
     } else {
-        R = fresnel_R(cos_i, n1, n2);
+        R = fresnel_R(cos_i, n1, n2, eta);
     }
     R = clamp01(R);
     return reflecting ? R : (1.0f - R);
@@ -582,31 +616,35 @@ GLD bool intersect(F3 o, F3 d, const DevSurf GLC& s, const float GLC* poly, floa
 }
 
 // n_ratio = n1 / n2, IEEE-rounded (by the caller, or from the lambda row).
+// Both square roots take arguments bounded away from the small-subnormal
+// range (1 - sin2_t >= 2^-24 since sin2_t < 1; sq >= 1e-18), so sqrt_rn_big
+// gives sqrtf's bits.
 GLD bool refract_ratio(F3 dir, F3 nrm, float n_ratio, F3& out) {   // trace_core.h:533-552
     float cos_i = -dot3(nrm, dir);
     float sin2_t = n_ratio * n_ratio * (1.0f - cos_i * cos_i);
     if (sin2_t >= 1.0f) return false;
-    float cos_t = sqrtf(1.0f - sin2_t);
+    float cos_t = sqrt_rn_big(1.0f - sin2_t);
     F3 tmp = add(mul(dir, n_ratio), mul(nrm, n_ratio * cos_i - cos_t));
     float sq = tmp.x * tmp.x + tmp.y * tmp.y + tmp.z * tmp.z;
     if (sq < 1e-18f) return false;
-    float inv = 1.0f / sqrtf(sq);
+    float inv = 1.0f / sqrt_rn_big(sq);
     out = mul(tmp, inv);
     return true;
 }
 GLD bool refract(F3 dir, F3 nrm, float n1, float n2, F3& out) { return refract_ratio(dir, nrm, n1 / n2, out); }
-GLD F3 reflect(F3 dir, F3 nrm) { return normalized(sub(dir, mul(nrm, 2.0f * dot3(dir, nrm)))); }
+GLD F3 reflect(F3 dir, F3 nrm) { return normalized_unit(sub(dir, mul(nrm, 2.0f * dot3(dir, nrm)))); }
 
 // One surface interaction (trace_cuda.h:226-361 loop bodies).  Returns false
 // when the ray died; res.status is set to TIR for total internal reflection.
 // W = false skips the Fresnel / coating / attenuator weights (callers that
 // need only landings, statuses and segment counts).
-// ratio: n1 / n2 for a refraction when the caller has it from a table
-// (negative: divide here).  LF: no IMAGE apertures and no attenuators.
-template <bool W, bool LF = false, int CM = CC_ALL>
+// ratio: n1 / n2 from the caller's table when RT (the same IEEE quotient);
+// otherwise it is divided here, and only where it is used.  LF: no IMAGE
+// apertures and no attenuators.
+template <bool W, bool LF = false, int CM = CC_ALL, bool RT = false>
 GLD bool step(const DevSurf GLC& s, const float GLC* poly, const float* __restrict__ arena, F3& o, F3& d,
               float lam, float n2, bool reflecting, float reflect_ior, float& cur, TraceRes& res,
-              const int2 GLC* lr, float ratio = -1.0f) {
+              const int2 GLC* lr, float ratio = 0.0f) {
     float t; F3 nrm;
     ++res.segs;
     if (!intersect<LF>(o, d, s, poly, t, nrm)) return false;
@@ -616,16 +654,17 @@ GLD bool step(const DevSurf GLC& s, const float GLC* poly, const float* __restri
     const float n1 = cur;
     if (reflecting) {
         if (W) {
-            res.w *= surface_weight<CM>(d, nrm, n1, n2, s, arena, lam, true, lr);
+            res.w *= surface_weight<CM>(d, nrm, n1, n2, RT ? ratio : n1 / n2, s, arena, lam, true, lr);
             if (!LF) res.w *= attenuator(hit, s);
         }
         d = reflect(d, nrm);
         cur = reflect_ior;
     } else {
         F3 nd;
-        if (!refract_ratio(d, nrm, ratio >= 0.0f ? ratio : n1 / n2, nd)) { res.status = GL_TRACE_TIR; return false; }
+        const float eta = RT ? ratio : n1 / n2;
+        if (!refract_ratio(d, nrm, eta, nd)) { res.status = GL_TRACE_TIR; return false; }
         if (W) {
-            res.w *= surface_weight<CM>(d, nrm, n1, n2, s, arena, lam, false, lr);
+            res.w *= surface_weight<CM>(d, nrm, n1, n2, eta, s, arena, lam, false, lr);
             if (!LF) res.w *= attenuator(hit, s);
         }
         d = nd;
@@ -865,13 +904,14 @@ GLD void to_sensor(F3 o, F3 d, bool t_ge0, TraceRes& res) {   // trace_cuda.h:36
 // kick (hurb_apply) drawn from the per-trace stream `hs`
 // (d_trace_ghost_ray<HURB>, trace_cuda.h:248-361).
 //
-// rat (may be null): this wavelength's refraction ratios, {ibf[k] / iat[k],
-// iat[k] / ibf[k]} per surface, divided on the host in IEEE float.  The index
+// rat (read when RT; null otherwise): this wavelength's refraction ratios,
+// {ibf[k] / iat[k], iat[k] / ibf[k]} per surface, divided on the host in IEEE
+// float; they also serve as the Fresnel eta.  The index
 // on the incident side of surface k is ibf[k] on the two forward legs and
 // iat[k] on the backward one (muted surfaces are skipped, so the medium
 // carried between surfaces is exactly that), so the ratio the refraction
 // needs is rat[k].x forward and rat[k].y backward.
-template <bool W = true, bool H = false, bool LF = false, int CM = CC_ALL>
+template <bool W = true, bool H = false, bool LF = false, int CM = CC_ALL, bool RT = false>
 GLD TraceRes trace_ghost(const DevSurf GLC* S, int n, const float GLC* poly, const float* __restrict__ arena,
                          int a, int b, F3 o, F3 d, float lam, const float GLC* iat, const float GLC* ibf,
                          bool t_ge0, const int2 GLC* lrow = nullptr, const HurbDev* hb = nullptr,
@@ -893,12 +933,13 @@ GLD TraceRes trace_ghost(const DevSurf GLC* S, int n, const float GLC* poly, con
         }
         const DevSurf GLC& s = S[k];
         if (!(s.flags & SF_ACTIVE)) continue;
-        float ratio = -1.0f;
-        if (rat) {
+        float ratio = 0.0f;
+        if (RT) {
             const float GLC* rk = reinterpret_cast<const float GLC*>(rat + k);
             ratio = (i >= end1 && i < end2) ? rk[1] : rk[0];
         }
-        if (!step<W, LF, CM>(s, poly, arena, o, d, lam, n2, refl, rior, cur, res, lrow ? lrow + k : nullptr, ratio))
+        if (!step<W, LF, CM, RT>(s, poly, arena, o, d, lam, n2, refl, rior, cur, res, lrow ? lrow + k : nullptr,
+                                 ratio))
             return res;
         if (H) {
             F3 en;

@@ -59,7 +59,7 @@ __device__ bool diag_step(const DevSurf GLC& s, int k, const float GLC* poly, co
     }
     const float n1 = cur;
     if (reflecting) {
-        const float fw = surface_weight<CC_ALL>(d, nrm, n1, n2, s, arena, lam, true, nullptr);
+        const float fw = surface_weight<CC_ALL>(d, nrm, n1, n2, n1 / n2, s, arena, lam, true, nullptr);
         res.w *= fw;
         res.w *= attenuator(hit, s);
         sink.put(k, hit, nrm, n1, n2, fw, true, GL_TRACE_OK);
@@ -72,7 +72,7 @@ __device__ bool diag_step(const DevSurf GLC& s, int k, const float GLC* poly, co
             res.status = GL_TRACE_TIR;
             return false;
         }
-        const float fw = surface_weight<CC_ALL>(d, nrm, n1, n2, s, arena, lam, false, nullptr);
+        const float fw = surface_weight<CC_ALL>(d, nrm, n1, n2, n1 / n2, s, arena, lam, false, nullptr);
         res.w *= fw;
         res.w *= attenuator(hit, s);
         sink.put(k, hit, nrm, n1, n2, fw, false, GL_TRACE_OK);
@@ -130,6 +130,32 @@ __global__ __launch_bounds__(64) void diag_kernel(const DevSurf* __restrict__ S,
     w[i] = res.w;
     st[i] = res.status;
     n_events[i] = sink.n;
+}
+
+// Every float bit pattern in [lo, hi] through sqrt_rn_big and through the
+// correctly rounded sqrtf (this unit is built with IEEE square root); counts
+// the patterns whose bits differ and keeps the smallest one.
+__global__ __launch_bounds__(256) void sqrt_check_kernel(uint32_t lo, uint32_t hi, unsigned long long* bad,
+                                                        unsigned* first) {
+    unsigned long long nbad = 0;
+    unsigned fb = 0xFFFFFFFFu;
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t u = (uint64_t)lo + (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; u <= (uint64_t)hi; u += stride) {
+        const float x = __uint_as_float((uint32_t)u);
+        const float a = sqrt_rn_big(x), b = sqrtf(x);
+        if (__float_as_uint(a) != __float_as_uint(b)) {
+            ++nbad;
+            fb = min(fb, (uint32_t)u);
+        }
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        nbad += __shfl_xor(nbad, off, 64);
+        fb = min(fb, (unsigned)__shfl_xor((int)fb, off, 64));
+    }
+    if ((threadIdx.x & 63) == 0 && nbad) {
+        atomicAdd(bad, nbad);
+        atomicMin(first, fb);
+    }
 }
 
 }  // namespace
@@ -190,6 +216,30 @@ int gl_trace_rays_diagnostic(gl_ctx* ctx, const gl_lens* lens, int32_t a, int32_
     GL_HIP(hipMemcpyAsync(out_w, bw.p, (size_t)n_rays * 4, hipMemcpyDeviceToHost, c->stream));
     GL_HIP(hipMemcpyAsync(out_status, bs.p, (size_t)n_rays * 4, hipMemcpyDeviceToHost, c->stream));
     GL_HIP(hipStreamSynchronize(c->stream));
+    return GL_OK;
+}
+
+int gl_selftest_sqrt_rn(gl_ctx* ctx, uint32_t lo_bits, uint32_t hi_bits, uint64_t* mismatches,
+                        uint32_t* first_mismatch) {
+    if (!mismatches || !first_mismatch) return set_error(GL_ERR_INVALID, "null argument");
+    if (lo_bits > hi_bits) return set_error(GL_ERR_INVALID, "empty bit range");
+    int st;
+    gl_ctx* c = resolve_ctx(ctx, &st);
+    if (!c) return st;
+    std::lock_guard<std::mutex> g(c->mu);
+    DevBuf& b = c->buf("sqrt_check");
+    if ((st = b.ensure(16))) return st;
+    const uint32_t init[4] = {0u, 0u, 0xFFFFFFFFu, 0u};
+    GL_HIP(hipMemcpyAsync(b.p, init, 16, hipMemcpyHostToDevice, c->stream));
+    unsigned long long* d_bad = b.as<unsigned long long>();
+    unsigned* d_first = reinterpret_cast<unsigned*>(b.as<char>() + 8);
+    hipLaunchKernelGGL(sqrt_check_kernel, dim3(8192), dim3(256), 0, c->stream, lo_bits, hi_bits, d_bad, d_first);
+    GL_HIP(hipGetLastError());
+    uint32_t out[4];
+    GL_HIP(hipMemcpyAsync(out, b.p, 16, hipMemcpyDeviceToHost, c->stream));
+    GL_HIP(hipStreamSynchronize(c->stream));
+    *mismatches = (uint64_t)out[0] | ((uint64_t)out[1] << 32);
+    *first_mismatch = out[2];
     return GL_OK;
 }
 

@@ -454,6 +454,17 @@ typedef struct gl_trace_event {
     int32_t reflected;                  /* 1 at a ghost bounce                        */
 } gl_trace_event;
 
+/* Self-check of the tracer's square root (no counterpart in the reference):
+ * every float whose bits lie in [lo_bits, hi_bits] goes through the
+ * tracer's corrected hardware square root and through IEEE sqrtf on the
+ * device; *mismatches receives how many differ in any bit and
+ * *first_mismatch the smallest such bit pattern (0xFFFFFFFF: none).  The
+ * tracer uses that form only on arguments of 0, at least 2^-96, +inf or NaN,
+ * so [0x0F800000, 0x7F800000] (2^-96 to +inf) and 0 must give no mismatch;
+ * a NaN argument gives a NaN either way, its payload may differ. */
+int  gl_selftest_sqrt_rn(gl_ctx* ctx, uint32_t lo_bits, uint32_t hi_bits, uint64_t* mismatches,
+                         uint32_t* first_mismatch);
+
 /* The diagnostic overloads of trace_ghost_ray / trace_primary_ray
  * (trace.cpp:264-395; the reference's trace_ghost_ray_diagnostic and
  * trace_primary_ray_diagnostic, bind_trace.cpp:71-86) for n rays at once, one

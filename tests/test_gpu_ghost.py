@@ -322,3 +322,17 @@ def test_frame_per_source_matches_single_renders():
         assert float(sum(x.sum() for x in one)) > 0
         assert e <= 1e-6
         assert all(np.array_equal(fr[c][i], one[c]) for c in range(3)), f"frame {i} differs from its single render"
+
+
+def test_tracer_square_root_bits_equal_ieee():
+    """The refraction and reflection use a corrected hardware square root on
+    arguments of 0 or at least 2^-96; every float there (and +inf) must give
+    sqrtf's bits."""
+    import ctypes as C
+    from ghostlight_amd import _abi
+    from ghostlight_amd.runtime import lib
+    bad, first = C.c_uint64(), C.c_uint32()
+    for lo, hi in [(0x0F800000, 0x7F800000), (0x00000000, 0x00000000), (0x80000000, 0x80000000)]:
+        print(f"sqrt self-check over [{lo:#010x}, {hi:#010x}]", flush=True)
+        _abi.check(lib().gl_selftest_sqrt_rn(None, lo, hi, C.byref(bad), C.byref(first)), "gl_selftest_sqrt_rn")
+        assert bad.value == 0, f"{bad.value} mismatches, first at {first.value:#010x}"
