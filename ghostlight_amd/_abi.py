@@ -268,6 +268,8 @@ SIGNATURES = {
     "gl_trace_rays_diagnostic": (C.c_int, [VP, VP, C.c_int32, C.c_int32, F32P, C.c_int32, C.c_int32, C.c_float,
                                            C.c_int32, C.POINTER(gl_trace_event), I32P, F32P, F32P, I32P]),
     "gl_selftest_sqrt_rn": (C.c_int, [VP, C.c_uint32, C.c_uint32, C.POINTER(C.c_uint64), C.POINTER(C.c_uint32)]),
+    "gl_selftest_rn": (C.c_int, [VP, C.c_int32, C.c_uint32, C.c_uint32, C.POINTER(C.c_uint64),
+                                 C.POINTER(C.c_uint32)]),
     "gl_fft2d_c2c_forward_inplace": (C.c_int, [VP, VP, C.c_int32]),
     "gl_fft2d_c2c_forward_host": (C.c_int, [VP, F32P, F32P, C.c_int32]),
     "gl_fft2d_time": (C.c_int, [VP, C.c_int32, C.c_int32, C.POINTER(C.c_float)]),

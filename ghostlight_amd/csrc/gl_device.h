@@ -65,6 +65,7 @@ struct DevBaffles {
 struct DevSurf {
     float z, R, sa, aspect;          // vertex z, radius, semi-aperture, X stretch
     float dx, dy, inv_absR, conic;   // decenter, 1/|R| (float division), conic K
+    float cz, R2, sa2, pad0;         // z + R, R * R, sa * sa: the same float operations, done once on the host
     float rot[9];
     int32_t flags, form, nblades, poly_off;   // poly_off: index into the normals array
     float apothem;                   // polygon apothem (mm), rounded from FP64
@@ -105,10 +106,11 @@ GLD F3 normalized(F3 v) {  // vec3.h:67-71
 GLD float sqrt_rn_big(float x);
 // normalized() for a vector of length near 1 (a reflected unit direction):
 // |v|^2 stays far above the small-subnormal range, so sqrt_rn_big gives
-// sqrtf's bits.
+// sqrtf's bits, and rcp_rn the IEEE reciprocal of the length.
+GLD float rcp_rn(float x);
 GLD F3 normalized_unit(F3 v) {
     float l = sqrt_rn_big(v.x * v.x + v.y * v.y + v.z * v.z);
-    return (l > 1e-12f) ? mul(v, 1.0f / l) : f3(0.f, 0.f, 0.f);
+    return (l > 1e-12f) ? mul(v, rcp_rn(l)) : f3(0.f, 0.f, 0.f);
 }
 template <class P> GLD F3 m_mul(P m, F3 v) {   // trace_core.h:33-39
     return f3(m[0] * v.x + m[1] * v.y + m[2] * v.z, m[3] * v.x + m[4] * v.y + m[5] * v.z,
@@ -132,6 +134,16 @@ GLD float sqrt_rn_big(float x) {
     const float rm = __builtin_fmaf(-sm, s, x), rp = __builtin_fmaf(-sp, s, x);
     const float r = rm <= 0.0f ? sm : s;
     return rp > 0.0f ? sp : r;
+}
+// Correctly rounded reciprocal for normal x whose reciprocal is normal
+// (2^-126 <= |x| <= 2^126): the hardware estimate (v_rcp_f32, within 1 ulp)
+// refined by one Newton step whose residual 1 - x * y the fused
+// multiply-add forms exactly.  Equal to 1.0f / x in every bit on that range
+// (checked over every float of it by gl_selftest_rn).
+GLD float rcp_rn(float x) {
+    const float y = __builtin_amdgcn_rcpf(x);
+    const float e = __builtin_fmaf(-x, y, 1.0f);
+    return __builtin_fmaf(e, y, y);
 }
 // Weight-only quotient: reciprocal estimate (1 ulp) times the numerator.
 GLD float wdiv(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
@@ -234,14 +246,14 @@ enum CoatClass : int {
     CC_ALL = 31,
 };
 
-// surface_weight (trace_core.h:635-783).
+// surface_weight (trace_core.h:635-783).  cos_i = |dot(nrm, dir)|, the
+// caller's (intersect hands it over with the normal).
 // lr: this surface's entry of the per-lambda lookup row (the lambda-only part
 // of the weight, precomputed on the host with the same float operations), or
 // nullptr to evaluate it here.  CM: the coating classes the lens uses.
 template <int CM = CC_ALL>
-GLD float surface_weight(F3 dir, F3 nrm, float n1, float n2, float eta, const DevSurf GLC& s,
+GLD float surface_weight(float cos_i, float n1, float n2, float eta, const DevSurf GLC& s,
                          const float* __restrict__ arena, float lam, bool reflecting, const int2 GLC* lr) {
-    float cos_i = fabsf(dot3(nrm, dir));
     float R;
     const int m = s.coat_model;
     if (CM == CC_FRESNEL) {
@@ -439,7 +451,7 @@ GLD bool check_aperture(F3 hit, const DevSurf GLC& s, const float GLC* poly) {
     float hx = (!LF && (s.flags & SF_ASPECT)) ? hit.x / s.aspect : hit.x;
     float hy = hit.y;
     float h2 = hx * hx + hy * hy;
-    if (!(h2 <= s.sa * s.sa)) return false;
+    if (!(h2 <= s.sa2)) return false;
     if (!LF && (s.flags & SF_DEFORMED)) return blade_inside(s, hx, hy, h2);
     if (s.flags & SF_POLYGON) {
         const float GLC* pc = poly + s.poly_off;
@@ -485,10 +497,18 @@ GLD float asph_dsag(float r, float inv_R, float K, const DevSurf GLC& s) {    //
 
 // intersect_surface (trace_core.h:299-525).  Returns false on miss / clip.
 // LF: every surface is flat or spherical, untransformed and solved in FP32.
+// dn_out = dot(n_out, d) in the caller's frame, with the bits the caller's
+// own dot product would give: the refraction, reflection and coating weight
+// take their incidence cosine from it instead of forming it again.  Where
+// the normal was flipped to face the ray, dot(-n, d) = -dot(n, d) exactly
+// (negation commutes with every rounding; a zero partial sum changes sign
+// only where dot(n, d) itself could not have been > 0).
 template <bool LF = false>
-GLD bool intersect(F3 o, F3 d, const DevSurf GLC& s, const float GLC* poly, float& t_out, F3& n_out) {
+GLD bool intersect_dn(F3 o, F3 d, const DevSurf GLC& s, const float GLC* poly, float& t_out, F3& n_out,
+                      float& dn_out) {
     const float zv = s.z;
     const bool tr = !LF && (s.flags & SF_TRANSFORMED) != 0;
+    const F3 dw = d;
     if (tr) {
         o = add(m_tmul(s.rot, sub(o, f3(s.dx, s.dy, zv))), f3(0.f, 0.f, zv));
         d = m_tmul(s.rot, d);
@@ -502,11 +522,12 @@ GLD bool intersect(F3 o, F3 d, const DevSurf GLC& s, const float GLC* poly, floa
         t_out = t;
         n_out = f3(0.f, 0.f, (d.z > 0.0f) ? -1.0f : 1.0f);
         if (tr) n_out = m_mul(s.rot, n_out);
+        dn_out = dot3(n_out, dw);
         return true;
     }
     if (s.form == GL_FORM_SPHERE) {
         const float R = s.R;
-        F3 ctr = f3(0.f, 0.f, zv + R);
+        F3 ctr = f3(0.f, 0.f, s.cz);
         float t;
         if (!LF && (s.flags & SF_FP64)) {
             // FP64 quadratic for large |R| (sag cancellation in |oc|^2 - R^2).
@@ -529,19 +550,22 @@ GLD bool intersect(F3 o, F3 d, const DevSurf GLC& s, const float GLC* poly, floa
             else return false;
             t = (float)td;
         } else {
+            // a = |d|^2 is within a few ulp of 1 (unit directions), where
+            // 0.5 * rcp_rn(a) is the IEEE quotient 0.5 / a.
             F3 oc = sub(o, ctr);
-            float a = dot3(d, d), b = 2.0f * dot3(oc, d), c = dot3(oc, oc) - R * R;
+            float a = dot3(d, d), b = 2.0f * dot3(oc, d), c = dot3(oc, oc) - s.R2;
             float disc = b * b - 4.0f * a * c;
             if (disc < 0.0f) return false;
-            float sd = sqrtf(disc), inv_2a = 0.5f / a;
+            float sd = sqrtf(disc), inv_2a = 0.5f * rcp_rn(a);
             if (!pick_root((-b - sd) * inv_2a, (-b + sd) * inv_2a, o.z, d.z, zv, t)) return false;
         }
         F3 hit = add(o, mul(d, t));
         if (!check_aperture<LF>(hit, s, poly)) return false;
         F3 nrm = mul(sub(hit, ctr), s.inv_absR);
-        if (dot3(nrm, d) > 0.0f) nrm = neg(nrm);
-        if (tr) nrm = m_mul(s.rot, nrm);
-        t_out = t; n_out = nrm;
+        float dn = dot3(nrm, d);
+        if (dn > 0.0f) { nrm = neg(nrm); dn = -dn; }
+        if (tr) { nrm = m_mul(s.rot, nrm); dn = dot3(nrm, dw); }
+        t_out = t; n_out = nrm; dn_out = dn;
         return true;
     }
     if (LF) return false;
@@ -567,7 +591,7 @@ GLD bool intersect(F3 o, F3 d, const DevSurf GLC& s, const float GLC* poly, floa
         nrm = mul(nrm, 1.0f / nlen);
         if (dot3(nrm, d) > 0.0f) nrm = neg(nrm);
         if (tr) nrm = m_mul(s.rot, nrm);
-        t_out = t; n_out = nrm;
+        t_out = t; n_out = nrm; dn_out = dot3(nrm, dw);
         return true;
     }
     // FORM_ASPHERE: Newton on the sag residual from the spherical guess.
@@ -610,29 +634,40 @@ GLD bool intersect(F3 o, F3 d, const DevSurf GLC& s, const float GLC* poly, floa
         nrm = mul(nrm, 1.0f / nlen);
         if (dot3(nrm, d) > 0.0f) nrm = neg(nrm);
         if (tr) nrm = m_mul(s.rot, nrm);
-        t_out = t; n_out = nrm;
+        t_out = t; n_out = nrm; dn_out = dot3(nrm, dw);
         return true;
     }
+}
+template <bool LF = false>
+GLD bool intersect(F3 o, F3 d, const DevSurf GLC& s, const float GLC* poly, float& t_out, F3& n_out) {
+    float dn;
+    return intersect_dn<LF>(o, d, s, poly, t_out, n_out, dn);
 }
 
 // n_ratio = n1 / n2, IEEE-rounded (by the caller, or from the lambda row).
 // Both square roots take arguments bounded away from the small-subnormal
 // range (1 - sin2_t >= 2^-24 since sin2_t < 1; sq >= 1e-18), so sqrt_rn_big
 // gives sqrtf's bits.
-GLD bool refract_ratio(F3 dir, F3 nrm, float n_ratio, F3& out) {   // trace_core.h:533-552
-    float cos_i = -dot3(nrm, dir);
+// cos_i = -dot(nrm, dir), the caller's.  |tmp| is within rounding of 1 and
+// at least 1e-9, so rcp_rn gives the IEEE reciprocal of its length.
+GLD bool refract_cos(F3 dir, F3 nrm, float cos_i, float n_ratio, F3& out) {   // trace_core.h:533-552
     float sin2_t = n_ratio * n_ratio * (1.0f - cos_i * cos_i);
     if (sin2_t >= 1.0f) return false;
     float cos_t = sqrt_rn_big(1.0f - sin2_t);
     F3 tmp = add(mul(dir, n_ratio), mul(nrm, n_ratio * cos_i - cos_t));
     float sq = tmp.x * tmp.x + tmp.y * tmp.y + tmp.z * tmp.z;
     if (sq < 1e-18f) return false;
-    float inv = 1.0f / sqrt_rn_big(sq);
+    float inv = rcp_rn(sqrt_rn_big(sq));
     out = mul(tmp, inv);
     return true;
 }
+GLD bool refract_ratio(F3 dir, F3 nrm, float n_ratio, F3& out) {
+    return refract_cos(dir, nrm, -dot3(nrm, dir), n_ratio, out);
+}
 GLD bool refract(F3 dir, F3 nrm, float n1, float n2, F3& out) { return refract_ratio(dir, nrm, n1 / n2, out); }
-GLD F3 reflect(F3 dir, F3 nrm) { return normalized_unit(sub(dir, mul(nrm, 2.0f * dot3(dir, nrm)))); }
+// dn = dot(dir, nrm) (the product is commutative, so dot(nrm, dir) too).
+GLD F3 reflect_dn(F3 dir, F3 nrm, float dn) { return normalized_unit(sub(dir, mul(nrm, 2.0f * dn))); }
+GLD F3 reflect(F3 dir, F3 nrm) { return reflect_dn(dir, nrm, dot3(dir, nrm)); }
 
 // One surface interaction (trace_cuda.h:226-361 loop bodies).  Returns false
 // when the ray died; res.status is set to TIR for total internal reflection.
@@ -645,26 +680,26 @@ template <bool W, bool LF = false, int CM = CC_ALL, bool RT = false>
 GLD bool step(const DevSurf GLC& s, const float GLC* poly, const float* __restrict__ arena, F3& o, F3& d,
               float lam, float n2, bool reflecting, float reflect_ior, float& cur, TraceRes& res,
               const int2 GLC* lr, float ratio = 0.0f) {
-    float t; F3 nrm;
+    float t, dn; F3 nrm;
     ++res.segs;
-    if (!intersect<LF>(o, d, s, poly, t, nrm)) return false;
+    if (!intersect_dn<LF>(o, d, s, poly, t, nrm, dn)) return false;
     F3 hit = add(o, mul(d, t));
     o = hit;
     if (!LF && (s.flags & SF_IMAGE) && !image_passes(s, arena, hit)) return false;
     const float n1 = cur;
     if (reflecting) {
         if (W) {
-            res.w *= surface_weight<CM>(d, nrm, n1, n2, RT ? ratio : n1 / n2, s, arena, lam, true, lr);
+            res.w *= surface_weight<CM>(fabsf(dn), n1, n2, RT ? ratio : n1 / n2, s, arena, lam, true, lr);
             if (!LF) res.w *= attenuator(hit, s);
         }
-        d = reflect(d, nrm);
+        d = reflect_dn(d, nrm, dn);
         cur = reflect_ior;
     } else {
         F3 nd;
         const float eta = RT ? ratio : n1 / n2;
-        if (!refract_ratio(d, nrm, eta, nd)) { res.status = GL_TRACE_TIR; return false; }
+        if (!refract_cos(d, nrm, -dn, eta, nd)) { res.status = GL_TRACE_TIR; return false; }
         if (W) {
-            res.w *= surface_weight<CM>(d, nrm, n1, n2, eta, s, arena, lam, false, lr);
+            res.w *= surface_weight<CM>(fabsf(dn), n1, n2, eta, s, arena, lam, false, lr);
             if (!LF) res.w *= attenuator(hit, s);
         }
         d = nd;

@@ -352,6 +352,11 @@ void build_devsurfs(const gl_surface* s, int n, float fp64_rmin, const float* ar
         d.z = g.z; d.R = g.radius; d.sa = g.semi_aperture; d.aspect = g.aperture_aspect;
         d.dx = g.decenter_x; d.dy = g.decenter_y;
         d.inv_absR = (std::fabs(g.radius) > 0.0f) ? 1.0f / std::fabs(g.radius) : 0.0f;
+        // The tracer's sphere centre z, squared radius and squared
+        // semi-aperture, rounded once here with the same float operations.
+        d.cz = g.z + g.radius;
+        d.R2 = g.radius * g.radius;
+        d.sa2 = g.semi_aperture * g.semi_aperture;
         d.conic = g.conic_k;
         std::memcpy(d.rot, g.rot, sizeof d.rot);
         int f = 0;

@@ -59,7 +59,7 @@ __device__ bool diag_step(const DevSurf GLC& s, int k, const float GLC* poly, co
     }
     const float n1 = cur;
     if (reflecting) {
-        const float fw = surface_weight<CC_ALL>(d, nrm, n1, n2, n1 / n2, s, arena, lam, true, nullptr);
+        const float fw = surface_weight<CC_ALL>(fabsf(dot3(nrm, d)), n1, n2, n1 / n2, s, arena, lam, true, nullptr);
         res.w *= fw;
         res.w *= attenuator(hit, s);
         sink.put(k, hit, nrm, n1, n2, fw, true, GL_TRACE_OK);
@@ -72,7 +72,7 @@ __device__ bool diag_step(const DevSurf GLC& s, int k, const float GLC* poly, co
             res.status = GL_TRACE_TIR;
             return false;
         }
-        const float fw = surface_weight<CC_ALL>(d, nrm, n1, n2, n1 / n2, s, arena, lam, false, nullptr);
+        const float fw = surface_weight<CC_ALL>(fabsf(dot3(nrm, d)), n1, n2, n1 / n2, s, arena, lam, false, nullptr);
         res.w *= fw;
         res.w *= attenuator(hit, s);
         sink.put(k, hit, nrm, n1, n2, fw, false, GL_TRACE_OK);
@@ -132,17 +132,25 @@ __global__ __launch_bounds__(64) void diag_kernel(const DevSurf* __restrict__ S,
     n_events[i] = sink.n;
 }
 
-// Every float bit pattern in [lo, hi] through sqrt_rn_big and through the
-// correctly rounded sqrtf (this unit is built with IEEE square root); counts
-// the patterns whose bits differ and keeps the smallest one.
-__global__ __launch_bounds__(256) void sqrt_check_kernel(uint32_t lo, uint32_t hi, unsigned long long* bad,
-                                                        unsigned* first) {
+// Every float bit pattern in [lo, hi] through one of the tracer's fast
+// correctly rounded forms and through the IEEE operation it stands for (this
+// unit is built with IEEE division and square root); counts the patterns
+// whose bits differ and keeps the smallest one.
+//   OP 0: sqrt_rn_big(x) against sqrtf(x)
+//   OP 1: rcp_rn(x) against 1.0f / x
+//   OP 2: 0.5f * rcp_rn(x) against 0.5f / x (the sphere's 1 / 2a)
+template <int OP>
+__global__ __launch_bounds__(256) void rn_check_kernel(uint32_t lo, uint32_t hi, unsigned long long* bad,
+                                                      unsigned* first) {
     unsigned long long nbad = 0;
     unsigned fb = 0xFFFFFFFFu;
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
     for (uint64_t u = (uint64_t)lo + (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; u <= (uint64_t)hi; u += stride) {
         const float x = __uint_as_float((uint32_t)u);
-        const float a = sqrt_rn_big(x), b = sqrtf(x);
+        float a, b;
+        if (OP == 0) { a = sqrt_rn_big(x); b = sqrtf(x); }
+        else if (OP == 1) { a = rcp_rn(x); b = 1.0f / x; }
+        else { a = 0.5f * rcp_rn(x); b = 0.5f / x; }
         if (__float_as_uint(a) != __float_as_uint(b)) {
             ++nbad;
             fb = min(fb, (uint32_t)u);
@@ -219,21 +227,25 @@ int gl_trace_rays_diagnostic(gl_ctx* ctx, const gl_lens* lens, int32_t a, int32_
     return GL_OK;
 }
 
-int gl_selftest_sqrt_rn(gl_ctx* ctx, uint32_t lo_bits, uint32_t hi_bits, uint64_t* mismatches,
-                        uint32_t* first_mismatch) {
+int gl_selftest_rn(gl_ctx* ctx, int32_t op, uint32_t lo_bits, uint32_t hi_bits, uint64_t* mismatches,
+                   uint32_t* first_mismatch) {
     if (!mismatches || !first_mismatch) return set_error(GL_ERR_INVALID, "null argument");
     if (lo_bits > hi_bits) return set_error(GL_ERR_INVALID, "empty bit range");
+    if (op < 0 || op > 2) return set_error(GL_ERR_INVALID, "op must be 0 (sqrt), 1 (reciprocal) or 2 (0.5 / x)");
     int st;
     gl_ctx* c = resolve_ctx(ctx, &st);
     if (!c) return st;
     std::lock_guard<std::mutex> g(c->mu);
-    DevBuf& b = c->buf("sqrt_check");
+    DevBuf& b = c->buf("rn_check");
     if ((st = b.ensure(16))) return st;
     const uint32_t init[4] = {0u, 0u, 0xFFFFFFFFu, 0u};
     GL_HIP(hipMemcpyAsync(b.p, init, 16, hipMemcpyHostToDevice, c->stream));
     unsigned long long* d_bad = b.as<unsigned long long>();
     unsigned* d_first = reinterpret_cast<unsigned*>(b.as<char>() + 8);
-    hipLaunchKernelGGL(sqrt_check_kernel, dim3(8192), dim3(256), 0, c->stream, lo_bits, hi_bits, d_bad, d_first);
+    const dim3 grid(8192), block(256);
+    if (op == 0) hipLaunchKernelGGL(rn_check_kernel<0>, grid, block, 0, c->stream, lo_bits, hi_bits, d_bad, d_first);
+    else if (op == 1) hipLaunchKernelGGL(rn_check_kernel<1>, grid, block, 0, c->stream, lo_bits, hi_bits, d_bad, d_first);
+    else hipLaunchKernelGGL(rn_check_kernel<2>, grid, block, 0, c->stream, lo_bits, hi_bits, d_bad, d_first);
     GL_HIP(hipGetLastError());
     uint32_t out[4];
     GL_HIP(hipMemcpyAsync(out, b.p, 16, hipMemcpyDeviceToHost, c->stream));
@@ -241,6 +253,11 @@ int gl_selftest_sqrt_rn(gl_ctx* ctx, uint32_t lo_bits, uint32_t hi_bits, uint64_
     *mismatches = (uint64_t)out[0] | ((uint64_t)out[1] << 32);
     *first_mismatch = out[2];
     return GL_OK;
+}
+
+int gl_selftest_sqrt_rn(gl_ctx* ctx, uint32_t lo_bits, uint32_t hi_bits, uint64_t* mismatches,
+                        uint32_t* first_mismatch) {
+    return gl_selftest_rn(ctx, 0, lo_bits, hi_bits, mismatches, first_mismatch);
 }
 
 }  // extern "C"

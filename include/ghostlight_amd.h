@@ -464,6 +464,15 @@ typedef struct gl_trace_event {
  * a NaN argument gives a NaN either way, its payload may differ. */
 int  gl_selftest_sqrt_rn(gl_ctx* ctx, uint32_t lo_bits, uint32_t hi_bits, uint64_t* mismatches,
                          uint32_t* first_mismatch);
+/* The same check for each of the tracer's fast correctly rounded forms
+ * (no counterpart in the reference): op 0 = the square root above, op 1 =
+ * the refined hardware reciprocal against 1.0f / x, op 2 = half of it
+ * against 0.5f / x.  The tracer uses the reciprocal forms only on normal
+ * arguments whose quotient is normal, so [0x00800000, 0x7E800000] (op 1),
+ * [0x00800000, 0x7DFFFFFF] (op 2) and their negative images must give no
+ * mismatch. */
+int  gl_selftest_rn(gl_ctx* ctx, int32_t op, uint32_t lo_bits, uint32_t hi_bits, uint64_t* mismatches,
+                    uint32_t* first_mismatch);
 
 /* The diagnostic overloads of trace_ghost_ray / trace_primary_ray
  * (trace.cpp:264-395; the reference's trace_ghost_ray_diagnostic and

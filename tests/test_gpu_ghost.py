@@ -336,3 +336,21 @@ def test_tracer_square_root_bits_equal_ieee():
         print(f"sqrt self-check over [{lo:#010x}, {hi:#010x}]", flush=True)
         _abi.check(lib().gl_selftest_sqrt_rn(None, lo, hi, C.byref(bad), C.byref(first)), "gl_selftest_sqrt_rn")
         assert bad.value == 0, f"{bad.value} mismatches, first at {first.value:#010x}"
+
+
+def test_tracer_reciprocal_bits_equal_ieee():
+    """The sphere's 1 / 2a and the refraction's 1 / |v| use the refined
+    hardware reciprocal on normal arguments with normal reciprocals; every
+    float of that range, of either sign, must give the IEEE quotient's bits
+    (for 0.5 / x up to 2^125, where the quotient stays normal)."""
+    import ctypes as C
+    from ghostlight_amd import _abi
+    from ghostlight_amd.runtime import lib
+    bad, first = C.c_uint64(), C.c_uint32()
+    ranges = {1: [(0x00800000, 0x7E800000), (0x80800000, 0xFE800000)],
+              2: [(0x00800000, 0x7DFFFFFF), (0x80800000, 0xFDFFFFFF)]}
+    for op in (1, 2):
+        for lo, hi in ranges[op]:
+            print(f"reciprocal self-check op {op} over [{lo:#010x}, {hi:#010x}]", flush=True)
+            _abi.check(lib().gl_selftest_rn(None, op, lo, hi, C.byref(bad), C.byref(first)), "gl_selftest_rn")
+            assert bad.value == 0, f"op {op}: {bad.value} mismatches, first at {first.value:#010x}"
