@@ -207,9 +207,26 @@ __device__ __forceinline__ F3 hurb_entry(const GhostArgs& A, const SrcDev& sr, F
     return d0;
 }
 
+// p with its value hidden from the optimiser: what is read through the
+// result is loaded where it is used (scalar loads from the kernel argument
+// segment) instead of being hoisted and held in registers across the trace.
+template <class T> __device__ __forceinline__ const T GLC* fresh(const T GLC* p) {
+    asm volatile("" : "+s"(p));
+    return p;
+}
+
+// The launch's GhostArgs where the kernel receives it: every kernel that
+// deposits takes GhostArgs as its first argument, at offset 0 of the kernel
+// argument segment.  (Taking the parameter's address instead would make the
+// compiler copy the whole struct to scratch.)
+__device__ __forceinline__ const GhostArgs GLC* kernel_args() {
+    return (const GhostArgs GLC*)__builtin_amdgcn_kernarg_segment_ptr();
+}
+
 template <bool H, bool LF = false, int CM = CC_ALL>
 __device__ __forceinline__ void deposit_range(const GhostArgs& A, int ps, int gi, int si0, int si1, float4* stage,
                                               int& staged, int lane) {
+    const GhostArgs GLC* Ap = kernel_args();
     const int pair_idx = ps / A.n_src, src_idx = ps % A.n_src;
     const gl_pair pr = A.pairs[pair_idx];
     float u = 0.f, v = 0.f, w_ray = 0.f;
@@ -239,18 +256,22 @@ __device__ __forceinline__ void deposit_range(const GhostArgs& A, int ps, int gi
                                                           &A.hurb, &hs, as_const(A.rat) + (size_t)si * A.n);
             if (res.status != GL_TRACE_OK) break;
             if (!isfinite(res.px) || !isfinite(res.py)) break;
-            float base = res.w * w_ray * A.gain * pr.area_boost;
+            // After the trace: arguments re-read, not carried through it.
+            const GhostArgs GLC& B = *fresh(Ap);
+            const SrcDev GLC& srb = as_const(B.src)[src_idx];
+            float base = res.w * w_ray * B.gain * as_const(B.pairs)[pair_idx].area_boost;
             if (base < 1e-14f) break;
-            float px = (res.px / (2.0f * A.sw) + 0.5f) * A.fmt_w + A.x0;
-            float py = (res.py / (2.0f * A.sh) + 0.5f) * A.fmt_h + A.y0;
+            float px = (res.px / (2.0f * B.sw) + 0.5f) * B.fmt_w + B.x0;
+            float py = (res.py / (2.0f * B.sh) + 0.5f) * B.fmt_h + B.y0;
             if (!isfinite(px) || !isfinite(py)) break;
             px = fmaxf(-2.0e9f, fminf(2.0e9f, px));
             py = fmaxf(-2.0e9f, fminf(2.0e9f, py));
             const int x0 = (int)floorf(px - 0.5f), y0 = (int)floorf(py - 0.5f);
-            if (x0 + 1 < 0 || x0 >= A.width || y0 + 1 < 0 || y0 >= A.height) break;   // footprint off the image
-            const float cr = spf[1], cg = spf[2], cb = spf[3];
-            const float P = sr.r * cr + sr.g * cg + sr.b * cb;
-            const int frame = A.frame_mode == 2 ? pair_idx : (A.frame_mode ? src_idx : 0);
+            if (x0 + 1 < 0 || x0 >= B.width || y0 + 1 < 0 || y0 >= B.height) break;   // footprint off the image
+            const float GLC* spc = as_const(reinterpret_cast<const float*>(B.spec)) + 4 * si;
+            const float cr = spc[1], cg = spc[2], cb = spc[3];
+            const float P = srb.r * cr + srb.g * cg + srb.b * cb;
+            const int frame = B.frame_mode == 2 ? pair_idx : (B.frame_mode ? src_idx : 0);
             rec = make_float4(px, py, P * base, __int_as_float(si | (frame << 8)));
             dep = true;
         } while (false);
@@ -314,7 +335,9 @@ __global__ void persist_reset_kernel(PersistQ Q) {
 
 // STATS = probe / diagnostic pass (counters, survivor bbox); otherwise the
 // render (SPLAT) and/or the segment count (COUNT).
-template <bool STATS, bool SPLAT, bool COUNT, bool H>
+// LF: the lens fits the tracer's lean path (lean_lens), as for the
+// persistent kernel.
+template <bool STATS, bool SPLAT, bool COUNT, bool H, bool LF = false>
 __global__ __launch_bounds__(kBlock) void ghost_kernel(GhostArgs A) {
     const int ps = blockIdx.x;
     const int pair_idx = ps / A.n_src, src_idx = ps % A.n_src;
@@ -342,7 +365,7 @@ __global__ __launch_bounds__(kBlock) void ghost_kernel(GhostArgs A) {
         __shared__ float4 stage_all[kBlock / 64][kStage];
         float4* stage = stage_all[threadIdx.x >> 6];
         int staged = 0;                              // wave-uniform
-        deposit_range<H>(A, ps, gi, 0, A.n_spec, stage, staged, lane);
+        deposit_range<H, LF>(A, ps, gi, 0, A.n_spec, stage, staged, lane);
         if (staged) flush_stage(A, stage, staged, lane);
         return;
     }
@@ -362,7 +385,7 @@ __global__ __launch_bounds__(kBlock) void ghost_kernel(GhostArgs A) {
             if (STATS) ++n_tr;
             uint32_t hs = 0u;
             const F3 dl = H ? hurb_entry<H>(A, sr, o0, d0, spf[0], hray, si, hs) : d0;
-            TraceRes res = trace_ghost<false, H, false, CC_ALL, true>(
+            TraceRes res = trace_ghost<false, H, LF, CC_ALL, true>(
                 as_const(A.S), A.n, as_const(A.poly), A.arena, pr.surf_a, pr.surf_b, o0, dl, spf[0],
                 as_const(A.iat) + (size_t)si * A.n, as_const(A.ibf) + (size_t)si * A.n, false, nullptr, &A.hurb, &hs,
                 as_const(A.rat) + (size_t)si * A.n);
@@ -524,14 +547,16 @@ static int lam_group(int ns) {
     return g;
 }
 
+// lean: the lens fits the lean tracer (no HURB launch takes that form).
 template <bool STATS, bool SPLAT, bool COUNT>
-static int launch(const GhostArgs& A_in, int n_ps, int n_samples, hipStream_t s) {
+static int launch(const GhostArgs& A_in, int n_ps, int n_samples, hipStream_t s, bool lean) {
     if (n_ps <= 0 || n_samples <= 0) return GL_OK;
     GhostArgs A = A_in;
     A.lam_group = STATS ? lam_group(A.n_spec) : 0;
     const int per_block = A.lam_group > 0 ? kBlock / A.lam_group : kBlock;
     dim3 grid((unsigned)n_ps, (unsigned)((n_samples + per_block - 1) / per_block), 1);
     if (A.hurb.on) hipLaunchKernelGGL((ghost_kernel<STATS, SPLAT, COUNT, true>), grid, dim3(kBlock), 0, s, A);
+    else if (lean) hipLaunchKernelGGL((ghost_kernel<STATS, SPLAT, COUNT, false, true>), grid, dim3(kBlock), 0, s, A);
     else hipLaunchKernelGGL((ghost_kernel<STATS, SPLAT, COUNT, false>), grid, dim3(kBlock), 0, s, A);
     GL_HIP(hipGetLastError());
     return GL_OK;
@@ -715,7 +740,7 @@ static int do_launch(Launch& Lc, float* out_r, float* out_g, float* out_b, float
             P.grid = bg.as<gl_sample>(); P.n_grid = (int)probe.size();
             P.pair_active = (pass == 0 || want_conc) ? nullptr : bm.as<unsigned char>();
             P.ps_refine = nullptr;
-            if ((st = launch<true, false, false>(P, n_ps, P.n_grid, c->stream))) return st;
+            if ((st = launch<true, false, false>(P, n_ps, P.n_grid, c->stream, D.lean))) return st;
             if (pass == 0) {
                 GL_HIP(hipStreamSynchronize(c->stream));
                 pass0_ok = true;
@@ -766,7 +791,7 @@ static int do_launch(Launch& Lc, float* out_r, float* out_g, float* out_b, float
                     R.iat = static_cast<const float*>(d_a3); R.ibf = static_cast<const float*>(d_b3);
                     R.rat = static_cast<const float2*>(d_r3);
                     R.pair_active = nullptr; R.ps_refine = brf.as<unsigned char>();
-                    if ((st = launch<true, false, false>(R, n_ps, R.n_grid, c->stream))) return st;
+                    if ((st = launch<true, false, false>(R, n_ps, R.n_grid, c->stream, D.lean))) return st;
                 }
             }
         }
@@ -984,7 +1009,7 @@ static int do_launch(Launch& Lc, float* out_r, float* out_g, float* out_b, float
                 else
                     hipLaunchKernelGGL((ghost_persist_kernel<false, CC_ALL>), gw, bw, 0, c->stream, A, Q);
                 GL_HIP(hipGetLastError());
-            } else if ((e = launch<false, true, false>(A, n_ps, n_grid, c->stream))) {
+            } else if ((e = launch<false, true, false>(A, n_ps, n_grid, c->stream, D.lean))) {
                 return e;
             }
             if (events) GL_HIP(hipEventRecord(c->ev1, c->stream));
@@ -1096,11 +1121,11 @@ static int do_launch(Launch& Lc, float* out_r, float* out_g, float* out_b, float
         GL_HIP(hipMemsetAsync(bcn.p, 0, 3 * n_pairs * 8, c->stream));
         GhostArgs Cg = A;
         Cg.segs = bsg.as<unsigned long long>();
-        if ((st = launch<false, false, true>(Cg, n_ps, n_grid, c->stream))) return st;
+        if ((st = launch<false, false, true>(Cg, n_ps, n_grid, c->stream, D.lean))) return st;
         GhostArgs Sg = A;
         Sg.pair_active = nullptr; Sg.conc.rects = nullptr; Sg.bbox = nullptr; Sg.segs = nullptr;
         Sg.cnt = bcn.as<unsigned long long>();
-        if ((st = launch<true, false, false>(Sg, n_ps, n_grid, c->stream))) return st;
+        if ((st = launch<true, false, false>(Sg, n_ps, n_grid, c->stream, D.lean))) return st;
         std::vector<unsigned long long> h(3 * n_pairs);
         unsigned long long segs = 0;
         GL_HIP(hipMemcpyAsync(h.data(), bcn.p, 3 * n_pairs * 8, hipMemcpyDeviceToHost, c->stream));
