@@ -73,6 +73,7 @@ struct GhostArgs {
     const float* __restrict__ ibf;
     const unsigned char* __restrict__ pair_active;
     const unsigned char* __restrict__ ps_refine;
+    const unsigned* __restrict__ refine_n;      // refine pass: slots the mask holds (run only for 1..kRefineMaxEntries)
     ConcDev conc;
     int* __restrict__ bbox;                     // STATS: mins | maxs | hits
     unsigned long long* __restrict__ cnt;       // STATS: traces | survived | on-sensor (per pair)
@@ -357,6 +358,10 @@ __global__ __launch_bounds__(kBlock) void ghost_kernel(GhostArgs A) {
     // Whole-block exits are uniform; per-lane validity stays a flag so the
     // wave reductions below see every lane.
     if (A.pair_active != nullptr && A.pair_active[pair_idx] == 0) return;
+    if (A.refine_n != nullptr) {
+        const unsigned nr = *A.refine_n;
+        if (nr == 0u || nr > (unsigned)kRefineMaxEntries) return;
+    }
     if (A.ps_refine != nullptr && A.ps_refine[ps] == 0) return;
     const gl_pair pr = A.pairs[pair_idx];
     if (pr.surf_a < 0 || pr.surf_b <= pr.surf_a || pr.surf_b >= A.n) return;
@@ -434,6 +439,24 @@ __global__ __launch_bounds__(kBlock) void ghost_kernel(GhostArgs A) {
         const unsigned long long sg = wave_sum_u64(n_seg);
         if ((threadIdx.x & 63) == 0 && sg) atomicAdd(A.segs, sg);
     }
+}
+
+// Refine mask of the (pair, source) slots: hits below `hmax`, and how many
+// (one workgroup of 1024 lanes).
+__global__ __launch_bounds__(1024) void refine_mask_kernel(const int* __restrict__ hits, int n, int hmax,
+                                                           unsigned char* __restrict__ mask, unsigned* __restrict__ count) {
+    __shared__ unsigned s_n;
+    if (threadIdx.x == 0) s_n = 0u;
+    __syncthreads();
+    unsigned k = 0;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+        const bool r = hits[i] < hmax;
+        mask[i] = r ? 1 : 0;
+        k += r ? 1u : 0u;
+    }
+    atomicAdd(&s_n, k);
+    __syncthreads();
+    if (threadIdx.x == 0) *count = s_n;
 }
 
 __global__ void fill3_kernel(float* __restrict__ a, float* __restrict__ b, float* __restrict__ c, size_t n) {
@@ -620,8 +643,6 @@ static int do_launch(Launch& Lc, float* out_r, float* out_g, float* out_b, float
     if ((st = prepare_tables(c, L, p))) return st;
     const int n_grid = (int)c->grid.size();
     const int ns = (int)c->spec.size();
-    std::vector<float> iat((size_t)ns * n), ibf((size_t)ns * n);
-    ior_tables(L->host.data(), n, c->spec.data(), ns, iat.data(), ibf.data());
     const double t_prep1 = now_ms();
 
     // Frame mode: one output frame per source (or per pair), frame-major planes.
@@ -655,21 +676,55 @@ static int do_launch(Launch& Lc, float* out_r, float* out_g, float* out_b, float
     const double t_up0 = now_ms();
     LensDev D;
     if ((st = lens_sync(L, p->fp64_radius_min, c->stream, &D))) return st;
-    void *d_pairs, *d_src, *d_iat, *d_ibf, *d_grid = c->buf("grid").p, *d_spec = c->buf("spec").p;
+    void *d_pairs, *d_src, *d_grid = c->buf("grid").p, *d_spec = c->buf("spec").p;
     if ((st = upload(c, "pairs", Lc.pairs.data(), n_pairs * sizeof(gl_pair), &d_pairs))) return st;
     if ((st = upload(c, "src", Lc.src.data(), n_src * sizeof(SrcDev), &d_src))) return st;
-    if ((st = upload(c, "iat", iat.data(), iat.size() * 4, &d_iat))) return st;
-    if ((st = upload(c, "ibf", ibf.data(), ibf.size() * 4, &d_ibf))) return st;
-    std::vector<int32_t> lrows;
-    lambda_rows(L->host.data(), n, L->arena.data(), c->spec.data(), ns, lrows);
-    void* d_lrow;
-    if ((st = upload(c, "lrow", lrows.data(), lrows.size() * sizeof(int32_t), &d_lrow))) return st;
-    // Refraction ratios per (lambda, surface): forward ibf / iat, backward
-    // iat / ibf, the same IEEE quotients the tracer would form.
-    std::vector<float2> rats((size_t)ns * n);
-    for (size_t q = 0; q < rats.size(); ++q) rats[q] = make_float2(ibf[q] / iat[q], iat[q] / ibf[q]);
-    void* d_rat;
-    if ((st = upload(c, "rat", rats.data(), rats.size() * sizeof(float2), &d_rat))) return st;
+    // Tables of this (lens, spectral table): per-lambda indices after / before
+    // each surface, coating lookup rows, refraction ratios (forward ibf / iat,
+    // backward iat / ibf: the same IEEE quotients the tracer would form) and
+    // their 3-lambda slices for the refine probe.  Built and uploaded only
+    // when the lens or the spectral table changes.
+    const int rns = std::min(3, ns);
+    const int li[3] = {0, ns / 2, ns - 1};
+    DevBuf &b_iat = c->buf("iat"), &b_ibf = c->buf("ibf"), &b_lrow = c->buf("lrow"), &b_rat = c->buf("rat");
+    DevBuf &b_s3 = c->buf("spec3"), &b_a3 = c->buf("iat3"), &b_b3 = c->buf("ibf3"), &b_r3 = c->buf("rat3");
+    {
+        std::vector<uint8_t> tk;
+        key_append(tk, L->uid);
+        tk.insert(tk.end(), c->spec_key.begin(), c->spec_key.end());
+        if (tk != c->ltab_key || !b_iat.p || !b_ibf.p || !b_lrow.p || !b_rat.p || !b_s3.p || !b_a3.p || !b_b3.p ||
+            !b_r3.p) {
+            c->ltab_key.clear();
+            std::vector<float> iat((size_t)ns * n), ibf((size_t)ns * n);
+            ior_tables(L->host.data(), n, c->spec.data(), ns, iat.data(), ibf.data());
+            std::vector<int32_t> lrows;
+            lambda_rows(L->host.data(), n, L->arena.data(), c->spec.data(), ns, lrows);
+            std::vector<float2> rats((size_t)ns * n);
+            for (size_t q = 0; q < rats.size(); ++q) rats[q] = make_float2(ibf[q] / iat[q], iat[q] / ibf[q]);
+            std::vector<float4> s3(rns);
+            std::vector<float> a3((size_t)rns * n), b3((size_t)rns * n);
+            std::vector<float2> r3((size_t)rns * n);
+            for (int k = 0; k < rns; ++k) {
+                const gl_spec& q = c->spec[li[k]];
+                s3[k] = make_float4(q.lambda, q.cmf_r, q.cmf_g, q.cmf_b);
+                std::copy_n(&iat[(size_t)li[k] * n], n, &a3[(size_t)k * n]);
+                std::copy_n(&ibf[(size_t)li[k] * n], n, &b3[(size_t)k * n]);
+                std::copy_n(&rats[(size_t)li[k] * n], n, &r3[(size_t)k * n]);
+            }
+            void* d;
+            if ((st = upload(c, "iat", iat.data(), iat.size() * 4, &d)) ||
+                (st = upload(c, "ibf", ibf.data(), ibf.size() * 4, &d)) ||
+                (st = upload(c, "lrow", lrows.data(), lrows.size() * sizeof(int32_t), &d)) ||
+                (st = upload(c, "rat", rats.data(), rats.size() * sizeof(float2), &d)) ||
+                (st = upload(c, "spec3", s3.data(), rns * sizeof(float4), &d)) ||
+                (st = upload(c, "iat3", a3.data(), a3.size() * 4, &d)) ||
+                (st = upload(c, "ibf3", b3.data(), b3.size() * 4, &d)) ||
+                (st = upload(c, "rat3", r3.data(), r3.size() * sizeof(float2), &d)))
+                return st;
+            c->ltab_key = tk;
+        }
+    }
+    void *d_iat = b_iat.p, *d_ibf = b_ibf.p, *d_lrow = b_lrow.p, *d_rat = b_rat.p;
     if (!c->grid_uploaded || !d_grid || c->buf("grid").cap < n_grid * sizeof(gl_sample)) {
         if ((st = upload(c, "grid", c->grid.data(), n_grid * sizeof(gl_sample), &d_grid))) return st;
         c->grid_uploaded = true;
@@ -712,12 +767,39 @@ static int do_launch(Launch& Lc, float* out_r, float* out_g, float* out_b, float
     std::vector<float4> rects;
     std::vector<unsigned char> active(n_pairs, 1);
     if ((p->cull_dead_pairs || want_conc) && probes_allowed) {
-        DevBuf &bc = c->buf("probe_cnt"), &bb = c->buf("probe_bbox"), &bg = c->buf("probe_grid");
-        DevBuf &bm = c->buf("pair_mask"), &brf = c->buf("refine_mask");
-        if ((st = bc.ensure(3 * n_pairs * 8)) || (st = bb.ensure((size_t)n_ps * 5 * 4)) ||
-            (st = bg.ensure((size_t)kRefineGrid * kRefineGrid * sizeof(gl_sample))) || (st = bm.ensure(n_pairs)) ||
-            (st = brf.ensure(n_ps)))
+        DevBuf &bc = c->buf("probe_cnt"), &bb = c->buf("probe_bbox");
+        DevBuf &bm = c->buf("pair_mask"), &brf = c->buf("refine_mask"), &bnr = c->buf("refine_n");
+        if ((st = bc.ensure(3 * n_pairs * 8)) || (st = bb.ensure((size_t)n_ps * 5 * 4)) || (st = bm.ensure(n_pairs)) ||
+            (st = brf.ensure(n_ps)) || (st = bnr.ensure(4)))
             return st;
+        // Probe sample sets of this lens: three jittered 40^2 passes (stored
+        // one after another) and the 128^2 refine grid, built on the host
+        // once per (lens, blade override, rotation) and kept on the device.
+        DevBuf &bpa = c->buf("probe_all"), &brg = c->buf("refine_grid");
+        {
+            std::vector<uint8_t> pk;
+            key_append(pk, L->uid); key_append(pk, p->aperture_blades); key_append(pk, p->aperture_rotation);
+            if (pk != c->probe_key || !bpa.p || !brg.p || (int)c->probe_n.size() != kProbePasses) {
+                c->probe_key.clear();
+                std::vector<gl_sample> all;
+                c->probe_n.assign(kProbePasses, 0);
+                for (int pass = 0; pass < kProbePasses; ++pass) {
+                    const std::vector<gl_sample> probe = build_samples(L->host.data(), n, kProbeGrid, 1,
+                                                                       (int)(0x9E3779B9u * (unsigned)(pass + 1)),
+                                                                       p->aperture_blades, p->aperture_rotation);
+                    c->probe_n[pass] = (int)probe.size();
+                    all.insert(all.end(), probe.begin(), probe.end());
+                }
+                const std::vector<gl_sample> rp = build_samples(L->host.data(), n, kRefineGrid, 1, (int)0xB5297A4Du,
+                                                                p->aperture_blades, p->aperture_rotation);
+                c->refine_n = (int)rp.size();
+                void* d;
+                if ((st = upload(c, "probe_all", all.data(), std::max<size_t>(1, all.size()) * sizeof(gl_sample), &d)) ||
+                    (st = upload(c, "refine_grid", rp.data(), std::max<size_t>(1, rp.size()) * sizeof(gl_sample), &d)))
+                    return st;
+                c->probe_key = pk;
+            }
+        }
         GL_HIP(hipMemsetAsync(bc.p, 0, 3 * n_pairs * 8, c->stream));
         if (want_conc) {
             GL_HIP(hipMemsetAsync(bb.p, 0x7F, (size_t)n_ps * 2 * 4, c->stream));
@@ -728,80 +810,69 @@ static int do_launch(Launch& Lc, float* out_r, float* out_g, float* out_b, float
         P.cnt = bc.as<unsigned long long>();
         P.bbox = want_conc ? bb.as<int>() : nullptr;
         P.segs = nullptr;
-        std::vector<unsigned char> cand(n_pairs, 0);
+        P.ps_refine = nullptr;
         std::vector<unsigned long long> cnt(3 * n_pairs);
         bool pass0_ok = false;
-        for (int pass = 0; pass < kProbePasses; ++pass) {
-            std::vector<gl_sample> probe = build_samples(L->host.data(), n, kProbeGrid, 1,
-                                                         (int)(0x9E3779B9u * (unsigned)(pass + 1)),
-                                                         p->aperture_blades, p->aperture_rotation);
-            if (probe.empty()) { if (pass == 0) break; else continue; }
-            GL_HIP(hipMemcpyAsync(bg.p, probe.data(), probe.size() * sizeof(gl_sample), hipMemcpyHostToDevice, c->stream));
-            P.grid = bg.as<gl_sample>(); P.n_grid = (int)probe.size();
-            P.pair_active = (pass == 0 || want_conc) ? nullptr : bm.as<unsigned char>();
-            P.ps_refine = nullptr;
+        const gl_sample* probe_all = bpa.as<gl_sample>();
+        if (c->probe_n[0] > 0 && want_conc && !A.hurb.on) {
+            // With concentration every pass probes every pair, and the pass
+            // statistics are integer sums, minima and maxima: the three
+            // passes run as one launch over their concatenated samples with
+            // the same counters.  (HURB seeds its kicks with the sample index,
+            // so HURB probes keep one launch per pass.)
+            int total = 0;
+            for (int pass = 0; pass < kProbePasses; ++pass) total += c->probe_n[pass];
+            P.grid = probe_all; P.n_grid = total;
+            P.pair_active = nullptr;
             if ((st = launch<true, false, false>(P, n_ps, P.n_grid, c->stream, D.lean))) return st;
-            if (pass == 0) {
-                GL_HIP(hipStreamSynchronize(c->stream));
-                pass0_ok = true;
-                if (!want_conc) {
-                    GL_HIP(d2h_sync(c, cnt.data(), bc.p, 3 * n_pairs * 8));
-                    int nc = 0;
-                    for (int q = 0; q < n_pairs; ++q)
-                        if (cnt[q] > 0 && cnt[2 * n_pairs + q] == 0) { cand[q] = 1; ++nc; }
-                    if (nc == 0) break;
-                    GL_HIP(hipMemcpyAsync(bm.p, cand.data(), n_pairs, hipMemcpyHostToDevice, c->stream));
+            pass0_ok = true;
+        } else {
+            std::vector<unsigned char> cand(n_pairs, 0);
+            size_t off = 0;
+            for (int pass = 0; pass < kProbePasses; ++pass) {
+                const int np = c->probe_n[pass];
+                if (np == 0) { if (pass == 0) break; else continue; }
+                P.grid = probe_all + off; P.n_grid = np;
+                off += (size_t)np;
+                P.pair_active = (pass == 0 || want_conc) ? nullptr : bm.as<unsigned char>();
+                if ((st = launch<true, false, false>(P, n_ps, P.n_grid, c->stream, D.lean))) return st;
+                if (pass == 0) {
+                    pass0_ok = true;
+                    if (!want_conc) {
+                        GL_HIP(d2h_sync(c, cnt.data(), bc.p, 3 * n_pairs * 8));
+                        int nc = 0;
+                        for (int q = 0; q < n_pairs; ++q)
+                            if (cnt[q] > 0 && cnt[2 * n_pairs + q] == 0) { cand[q] = 1; ++nc; }
+                        if (nc == 0) break;
+                        GL_HIP(hipMemcpyAsync(bm.p, cand.data(), n_pairs, hipMemcpyHostToDevice, c->stream));
+                    }
                 }
             }
         }
-        if (want_conc && pass0_ok) {
-            std::vector<int> hits(n_ps);
-            GL_HIP(hipMemcpyAsync(hits.data(), bb.as<int>() + 4 * n_ps, n_ps * 4, hipMemcpyDeviceToHost, c->stream));
-            GL_HIP(hipStreamSynchronize(c->stream));
-            std::vector<unsigned char> refine(n_ps, 0);
-            int n_ref = 0;
-            for (int i = 0; i < n_ps; ++i) if (hits[i] < kRefineHitsMax) { refine[i] = 1; ++n_ref; }
-            const int rns = std::min(3, ns);
-            const int li[3] = {0, ns / 2, ns - 1};
-            if (n_ref > 0 && n_ref <= kRefineMaxEntries) {
-                std::vector<gl_sample> rp = build_samples(L->host.data(), n, kRefineGrid, 1, (int)0xB5297A4Du,
-                                                          p->aperture_blades, p->aperture_rotation);
-                if (!rp.empty()) {
-                    std::vector<float4> s3(rns);
-                    std::vector<float> a3((size_t)rns * n), b3((size_t)rns * n);
-                    std::vector<float2> r3((size_t)rns * n);
-                    for (int k = 0; k < rns; ++k) {
-                        const gl_spec& q = c->spec[li[k]];
-                        s3[k] = make_float4(q.lambda, q.cmf_r, q.cmf_g, q.cmf_b);
-                        std::copy_n(&iat[(size_t)li[k] * n], n, &a3[(size_t)k * n]);
-                        std::copy_n(&ibf[(size_t)li[k] * n], n, &b3[(size_t)k * n]);
-                        std::copy_n(&rats[(size_t)li[k] * n], n, &r3[(size_t)k * n]);
-                    }
-                    void *d_s3, *d_a3, *d_b3, *d_r3;
-                    if ((st = upload(c, "spec3", s3.data(), rns * sizeof(float4), &d_s3)) ||
-                        (st = upload(c, "iat3", a3.data(), a3.size() * 4, &d_a3)) ||
-                        (st = upload(c, "ibf3", b3.data(), b3.size() * 4, &d_b3)) ||
-                        (st = upload(c, "rat3", r3.data(), r3.size() * sizeof(float2), &d_r3)))
-                        return st;
-                    GL_HIP(hipMemcpyAsync(brf.p, refine.data(), n_ps, hipMemcpyHostToDevice, c->stream));
-                    GL_HIP(hipMemcpyAsync(bg.p, rp.data(), rp.size() * sizeof(gl_sample), hipMemcpyHostToDevice, c->stream));
-                    GhostArgs R = P;
-                    R.grid = bg.as<gl_sample>(); R.n_grid = (int)rp.size();
-                    R.spec = static_cast<const float4*>(d_s3); R.n_spec = rns;
-                    R.iat = static_cast<const float*>(d_a3); R.ibf = static_cast<const float*>(d_b3);
-                    R.rat = static_cast<const float2*>(d_r3);
-                    R.pair_active = nullptr; R.ps_refine = brf.as<unsigned char>();
-                    if ((st = launch<true, false, false>(R, n_ps, R.n_grid, c->stream, D.lean))) return st;
-                }
-            }
+        if (want_conc && pass0_ok && c->refine_n > 0) {
+            // Refine the (pair, source) slots with few hits at three
+            // wavelengths on the 128^2 grid; the slot mask and its count are
+            // formed on the device, and the refine pass runs only when 1 to
+            // kRefineMaxEntries slots qualify (checked by its workgroups).
+            hipLaunchKernelGGL(refine_mask_kernel, dim3(1), dim3(1024), 0, c->stream, bb.as<int>() + 4 * n_ps, n_ps,
+                               kRefineHitsMax, brf.as<unsigned char>(), bnr.as<unsigned>());
+            GL_HIP(hipGetLastError());
+            GhostArgs R = P;
+            R.grid = brg.as<gl_sample>(); R.n_grid = c->refine_n;
+            R.spec = b_s3.as<float4>(); R.n_spec = rns;
+            R.iat = b_a3.as<float>(); R.ibf = b_b3.as<float>();
+            R.rat = b_r3.as<float2>();
+            R.pair_active = nullptr; R.ps_refine = brf.as<unsigned char>();
+            R.refine_n = bnr.as<unsigned>();
+            if ((st = launch<true, false, false>(R, n_ps, R.n_grid, c->stream, D.lean))) return st;
         }
         if (pass0_ok) {
+            std::vector<int> bbh(want_conc ? (size_t)n_ps * 5 : 0);
             GL_HIP(hipMemcpyAsync(cnt.data(), bc.p, 3 * n_pairs * 8, hipMemcpyDeviceToHost, c->stream));
+            if (want_conc) GL_HIP(hipMemcpyAsync(bbh.data(), bb.p, (size_t)n_ps * 5 * 4, hipMemcpyDeviceToHost, c->stream));
             GL_HIP(hipStreamSynchronize(c->stream));
             for (int q = 0; q < n_pairs; ++q) active[q] = cnt[2 * n_pairs + q] > 0 ? 1 : 0;
             if (want_conc) {
-                std::vector<int> bbh((size_t)n_ps * 5);
-                GL_HIP(d2h_sync(c, bbh.data(), bb.p, (size_t)n_ps * 5 * 4));
                 rects.assign(n_ps, make_float4(2.f, 2.f, -2.f, -2.f));
                 const float dil = kConcDilateCells * (2.0f / (float)kProbeGrid);
                 auto dec = [](int i) { float f; std::memcpy(&f, &i, 4); return f - 2.0f; };

@@ -70,6 +70,13 @@ struct gl_ctx {
     bool spec_do_correct = false;
     bool grid_uploaded = false, spec_uploaded = false;
     float veil_baked_p = -1.0f;          // falloff the cached veil reference holds
+    // Ghost-render tables of one (lens, spectral table) -- per-lambda IOR,
+    // coating lookup rows, refraction ratios and their 3-lambda refine
+    // slices -- and the probe sample sets of one (lens, blade override,
+    // rotation), on the device while their inputs stay the same.
+    std::vector<uint8_t> ltab_key, probe_key;
+    std::vector<int> probe_n;            // samples in each probe pass (passes concatenated)
+    int refine_n = 0;                    // samples in the refine grid
     int max_lds = 0;                     // per-workgroup LDS bytes (device attribute)
     gl::DevBuf& buf(const char* name) { return bufs[name]; }
 };
@@ -103,6 +110,7 @@ struct LensDev {
 // use and never modified, so concurrent renders on one lens need the mutex
 // only while a variant is looked up or created.
 struct gl_lens {
+    uint64_t uid = 0;                        // unique per created lens: keys the contexts' per-lens caches
     std::vector<gl_surface> host;
     std::vector<float> arena;
     int device = -1;

@@ -3,6 +3,7 @@
 #include "gl_runtime.h"
 
 #include <algorithm>
+#include <atomic>
 
 #include <cstdio>
 
@@ -241,6 +242,8 @@ int gl_ctx_trim(gl_ctx* ctx) {
     c->work_host.clear();
     c->work_key.clear();
     c->vconv_key.clear();
+    c->ltab_key.clear();
+    c->probe_key.clear();
     c->grid_uploaded = false;
     c->spec_uploaded = false;
     c->veil_baked_p = -1.0f;
@@ -295,7 +298,9 @@ int gl_lens_create(const gl_surface* s, int32_t n, const float* arena, int32_t a
              (long long)g.image_offset + (long long)g.image_width * g.image_height > (long long)arena_len))
             return gl::set_error(GL_ERR_INVALID, "surface %d: aperture bitmap outside the arena", i);
     }
+    static std::atomic<uint64_t> next_uid{1};
     gl_lens* L = new gl_lens();
+    L->uid = next_uid.fetch_add(1);
     L->host.assign(s, s + n);
     if (arena && arena_len > 0) L->arena.assign(arena, arena + arena_len);
     else L->arena.assign(1, 0.0f);

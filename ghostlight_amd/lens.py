@@ -355,7 +355,18 @@ class OpticalSystem:
         if n > _abi.GL_MAX_SURFACES:
             raise ValueError(f"lens has {n} surfaces; at most {_abi.GL_MAX_SURFACES} supported")
         arr = (_abi.gl_surface * max(n, 1))()
-        arena: List[float] = []
+        # The arena as float32 chunks, joined once at the end; `used` is its
+        # running length (the offset of the next chunk).
+        chunks: List[np.ndarray] = []
+        used = 0
+
+        def put(a) -> int:
+            nonlocal used
+            a = np.asarray(a, np.float32).ravel()
+            off = used
+            chunks.append(a)
+            used += a.size
+            return off
         stop_index = next((i for i, s in enumerate(self.surfaces) if s.is_stop), -1)
         for i, s in enumerate(self.surfaces):
             g = arr[i]
@@ -386,14 +397,16 @@ class OpticalSystem:
             g.table_offset = g.table_count = g.sa_offset = g.sa_n_wavelengths = g.sa_n_angles = 0
             if c.table is not None and len(c.table):
                 t = np.asarray(c.table, np.float32).reshape(-1, 2)
-                g.table_offset, g.table_count = len(arena), len(t)
-                arena.extend(t.ravel().tolist())
+                g.table_count = len(t)
+                g.table_offset = put(t)
             if c.sa_r is not None:
-                wl = np.asarray(c.sa_wavelengths, np.float32)
-                an = np.asarray(c.sa_angles, np.float32)
+                wl = np.asarray(c.sa_wavelengths, np.float32).ravel()
+                an = np.asarray(c.sa_angles, np.float32).ravel()
                 r = np.asarray(c.sa_r, np.float32).reshape(len(wl), len(an))
-                g.sa_offset, g.sa_n_wavelengths, g.sa_n_angles = len(arena), len(wl), len(an)
-                arena.extend(wl.tolist()); arena.extend(an.tolist()); arena.extend(r.ravel().tolist())
+                g.sa_n_wavelengths, g.sa_n_angles = len(wl), len(an)
+                g.sa_offset = put(wl)
+                put(an)
+                put(r)
             g.aperture_semi_diameter = float(s.aperture_semi_diameter)
             g.image_offset = g.image_width = g.image_height = g.image_sdf = g.image_sdf_offset = 0
             img = self.aperture_images[i] if i < len(self.aperture_images) else None
@@ -402,8 +415,8 @@ class OpticalSystem:
                 px = np.asarray(img.pixels, np.float32).ravel()
                 if img.width <= 0 or img.height <= 0 or len(px) != img.width * img.height:
                     raise ValueError(f"surface {i}: aperture image dimensions do not match its pixel data")
-                g.image_offset, g.image_width, g.image_height = len(arena), int(img.width), int(img.height)
-                arena.extend(px.tolist())
+                g.image_width, g.image_height = int(img.width), int(img.height)
+                g.image_offset = put(px)
                 if s.is_stop and i == stop_index:
                     # the stop's bitmap also carries its edge distance field for HURB
                     # (find_sdf_target_surface / bake_aperture_sdf, aperture_sdf.cpp:94-189)
@@ -411,9 +424,9 @@ class OpticalSystem:
                     if s.aperture_semi_diameter > 0 and _abi.load_library().gl_bake_aperture_sdf(
                             _abi.fptr(px), int(img.width), int(img.height), float(s.aperture_semi_diameter),
                             float(s.aperture_aspect), _abi.fptr(sdf)) == 0:
-                        g.image_sdf, g.image_sdf_offset = 1, len(arena)
-                        arena.extend(sdf.tolist())
-        return arr, np.asarray(arena if arena else [0.0], np.float32)
+                        g.image_sdf = 1
+                        g.image_sdf_offset = put(sdf)
+        return arr, (np.concatenate(chunks) if used else np.zeros(1, np.float32))
 
     @classmethod
     def from_packed(cls, surfaces, arena=None, name: str = "") -> "OpticalSystem":
