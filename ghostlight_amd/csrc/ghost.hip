@@ -441,6 +441,18 @@ __global__ __launch_bounds__(kBlock) void ghost_kernel(GhostArgs A) {
     }
 }
 
+// The counters a render reads back, gathered into one block for one copy:
+// records made and binned, resolve chunks | staged tiles << 32, and the
+// device wall clock at the persistent trace's first start and last finish.
+__global__ void render_summary_kernel(SplatBufs B, int n_tiles, const unsigned long long* __restrict__ clk,
+                                      unsigned long long* __restrict__ out) {
+    out[0] = *B.rec_count;
+    out[1] = *B.bin_total;
+    out[2] = (unsigned long long)B.chunk_start[n_tiles] | ((unsigned long long)*B.mc_count << 32);
+    out[3] = clk ? clk[0] : 0ull;
+    out[4] = clk ? clk[1] : 0ull;
+}
+
 // Refine mask of the (pair, source) slots: hits below `hmax`, and how many
 // (one workgroup of 1024 lanes).
 __global__ __launch_bounds__(1024) void refine_mask_kernel(const int* __restrict__ hits, int n, int hmax,
@@ -536,6 +548,43 @@ static int upload(gl_ctx* c, const char* name, const void* src, size_t bytes, vo
     int st = b.ensure(bytes);
     if (st) return st;
     if (bytes) GL_HIP(hipMemcpyAsync(b.p, src, bytes, hipMemcpyHostToDevice, c->stream));
+    *dptr = b.p;
+    return GL_OK;
+}
+
+// Pinned staging (gl_ctx::pin): pin_begin sizes it at the start of a render
+// (no transfer of the context is in flight then), pin_take hands out 64-byte
+// aligned slices, upload_pinned stages a small upload through one (falling
+// back to a pageable copy if the slice does not fit).
+static int pin_begin(gl_ctx* c, size_t bytes) {
+    c->pin_used = 0;
+    if (bytes <= c->pin_cap) return GL_OK;
+    std::lock_guard<std::mutex> gm(device_mem_mutex());
+    if (c->pin) (void)hipHostFree(c->pin);
+    c->pin = nullptr;
+    c->pin_cap = 0;
+    const size_t cap = std::max<size_t>(bytes, (size_t)64 << 10);
+    void* h = nullptr;
+    if (hipHostMalloc(&h, cap, hipHostMallocDefault) != hipSuccess || !h)
+        return set_error(GL_ERR_NO_MEMORY, "hipHostMalloc of %zu bytes of staging failed", cap);
+    c->pin = static_cast<char*>(h);
+    c->pin_cap = cap;
+    return GL_OK;
+}
+static void* pin_take(gl_ctx* c, size_t bytes) {
+    const size_t off = (c->pin_used + 63) & ~(size_t)63;
+    if (!c->pin || off + bytes > c->pin_cap) return nullptr;
+    c->pin_used = off + bytes;
+    return c->pin + off;
+}
+static int upload_pinned(gl_ctx* c, const char* name, const void* src, size_t bytes, void** dptr) {
+    void* h = bytes ? pin_take(c, bytes) : nullptr;
+    if (!h) return upload(c, name, src, bytes, dptr);
+    DevBuf& b = c->buf(name);
+    int st = b.ensure(bytes);
+    if (st) return st;
+    std::memcpy(h, src, bytes);
+    GL_HIP(hipMemcpyAsync(b.p, h, bytes, hipMemcpyHostToDevice, c->stream));
     *dptr = b.p;
     return GL_OK;
 }
@@ -674,11 +723,19 @@ static int do_launch(Launch& Lc, float* out_r, float* out_g, float* out_b, float
     if (n_pairs == 0 || n_src == 0 || n_grid == 0) return finish_empty();
 
     const double t_up0 = now_ms();
+    const int n_ps = n_pairs * n_src;
+    {
+        const size_t slack = 64 * 12;
+        const size_t pin_bytes = (size_t)n_pairs * sizeof(gl_pair) + (size_t)n_src * sizeof(SrcDev) +
+                                 3 * (size_t)n_pairs + (size_t)n_ps * (sizeof(float4) + sizeof(int2) + 5 * 4) +
+                                 3 * (size_t)n_pairs * 8 + 2 * 64 + slack;
+        if ((st = pin_begin(c, pin_bytes))) return st;
+    }
     LensDev D;
     if ((st = lens_sync(L, p->fp64_radius_min, c->stream, &D))) return st;
     void *d_pairs, *d_src, *d_grid = c->buf("grid").p, *d_spec = c->buf("spec").p;
-    if ((st = upload(c, "pairs", Lc.pairs.data(), n_pairs * sizeof(gl_pair), &d_pairs))) return st;
-    if ((st = upload(c, "src", Lc.src.data(), n_src * sizeof(SrcDev), &d_src))) return st;
+    if ((st = upload_pinned(c, "pairs", Lc.pairs.data(), n_pairs * sizeof(gl_pair), &d_pairs))) return st;
+    if ((st = upload_pinned(c, "src", Lc.src.data(), n_src * sizeof(SrcDev), &d_src))) return st;
     // Tables of this (lens, spectral table): per-lambda indices after / before
     // each surface, coating lookup rows, refraction ratios (forward ibf / iat,
     // backward iat / ibf: the same IEEE quotients the tracer would form) and
@@ -762,7 +819,6 @@ static int do_launch(Launch& Lc, float* out_r, float* out_g, float* out_b, float
     // ---- probe / cull / concentrate (ghost_render.cu:969-1238)
     const bool probes_allowed = n_pairs > 1 && ns != 3 && !Lc.frame_by_pair;
     const bool want_conc = p->concentrate_samples && probes_allowed;
-    const int n_ps = n_pairs * n_src;
     bool cull_ready = false, rect_ready = false;
     std::vector<float4> rects;
     std::vector<unsigned char> active(n_pairs, 1);
@@ -844,7 +900,8 @@ static int do_launch(Launch& Lc, float* out_r, float* out_g, float* out_b, float
                         for (int q = 0; q < n_pairs; ++q)
                             if (cnt[q] > 0 && cnt[2 * n_pairs + q] == 0) { cand[q] = 1; ++nc; }
                         if (nc == 0) break;
-                        GL_HIP(hipMemcpyAsync(bm.p, cand.data(), n_pairs, hipMemcpyHostToDevice, c->stream));
+                        void* d;
+                        if ((st = upload_pinned(c, "pair_mask", cand.data(), n_pairs, &d))) return st;
                     }
                 }
             }
@@ -868,9 +925,15 @@ static int do_launch(Launch& Lc, float* out_r, float* out_g, float* out_b, float
         }
         if (pass0_ok) {
             std::vector<int> bbh(want_conc ? (size_t)n_ps * 5 : 0);
-            GL_HIP(hipMemcpyAsync(cnt.data(), bc.p, 3 * n_pairs * 8, hipMemcpyDeviceToHost, c->stream));
-            if (want_conc) GL_HIP(hipMemcpyAsync(bbh.data(), bb.p, (size_t)n_ps * 5 * 4, hipMemcpyDeviceToHost, c->stream));
+            void* h_cnt = pin_take(c, 3 * (size_t)n_pairs * 8);
+            void* h_bbh = want_conc ? pin_take(c, (size_t)n_ps * 5 * 4) : nullptr;
+            GL_HIP(hipMemcpyAsync(h_cnt ? h_cnt : cnt.data(), bc.p, 3 * n_pairs * 8, hipMemcpyDeviceToHost, c->stream));
+            if (want_conc)
+                GL_HIP(hipMemcpyAsync(h_bbh ? h_bbh : bbh.data(), bb.p, (size_t)n_ps * 5 * 4, hipMemcpyDeviceToHost,
+                                      c->stream));
             GL_HIP(hipStreamSynchronize(c->stream));
+            if (h_cnt) std::memcpy(cnt.data(), h_cnt, 3 * (size_t)n_pairs * 8);
+            if (h_bbh) std::memcpy(bbh.data(), h_bbh, (size_t)n_ps * 5 * 4);
             for (int q = 0; q < n_pairs; ++q) active[q] = cnt[2 * n_pairs + q] > 0 ? 1 : 0;
             if (want_conc) {
                 rects.assign(n_ps, make_float4(2.f, 2.f, -2.f, -2.f));
@@ -898,7 +961,8 @@ static int do_launch(Launch& Lc, float* out_r, float* out_g, float* out_b, float
                 rect_ready = true;
             }
         }
-        GL_HIP(hipMemcpyAsync(bm.p, active.data(), n_pairs, hipMemcpyHostToDevice, c->stream));
+        void* d;
+        if ((st = upload_pinned(c, "pair_mask", active.data(), n_pairs, &d))) return st;
         cull_ready = p->cull_dead_pairs != 0;
     }
 
@@ -906,7 +970,7 @@ static int do_launch(Launch& Lc, float* out_r, float* out_g, float* out_b, float
     std::vector<int2> budgets;
     if (rect_ready) {
         void* d_rects;
-        if ((st = upload(c, "rects", rects.data(), n_ps * sizeof(float4), &d_rects))) return st;
+        if ((st = upload_pinned(c, "rects", rects.data(), n_ps * sizeof(float4), &d_rects))) return st;
         A.conc.rects = static_cast<const float4*>(d_rects);
         A.conc.mask = build_mask(L->host.data(), n, p->aperture_blades, p->aperture_rotation);
         A.conc.jitter = p->pupil_jitter;
@@ -941,7 +1005,7 @@ static int do_launch(Launch& Lc, float* out_r, float* out_g, float* out_b, float
                 }
             }
             void* d_b;
-            if ((st = upload(c, "budgets", budgets.data(), n_ps * sizeof(int2), &d_b))) return st;
+            if ((st = upload_pinned(c, "budgets", budgets.data(), n_ps * sizeof(int2), &d_b))) return st;
             A.conc.budgets = static_cast<const int2*>(d_b);
             A.conc.inv_aref = 1.0f / a_ref;
             if (stats && stats->ps_cap >= n_ps && stats->ps_budget)
@@ -1060,6 +1124,8 @@ static int do_launch(Launch& Lc, float* out_r, float* out_g, float* out_b, float
         SplatBufs sb;
         std::memset(&sb, 0, sizeof sb);
         if ((st = splat_prepare(c, sg, want, sb))) return st;
+        DevBuf& b_sum = c->buf("render_summary");
+        if ((st = b_sum.ensure(64))) return st;
         A.rec = sb.rec; A.rec_cap = sb.rec_cap; A.rec_count = sb.rec_count;
         // The whole device sequence of one render after the probes: no host
         // synchronisation and no memset nodes, so it can be captured.
@@ -1092,12 +1158,16 @@ static int do_launch(Launch& Lc, float* out_r, float* out_g, float* out_b, float
                                    A.out_r, A.out_g, A.out_b, npx, C[0], C[1], C[2], C[3], C[4], C[5], C[6], C[7], C[8]);
                 GL_HIP(hipGetLastError());
             }
+            hipLaunchKernelGGL(render_summary_kernel, dim3(1), dim3(1), 0, c->stream, sb, sg.n_tiles,
+                               persistent ? Q.clk : nullptr, b_sum.as<unsigned long long>());
+            GL_HIP(hipGetLastError());
             return GL_OK;
         };
         if (use_graph) {
             std::vector<uint8_t> gk;
             key_append(gk, A); key_append(gk, sb); key_append(gk, Q); key_append(gk, workers); key_append(gk, sg);
             key_append(gk, c->spec_do_correct); key_append(gk, c->spec_C); key_append(gk, npx);
+            key_append(gk, b_sum.p);
             if (!c->graph || gk != c->graph_key) {
                 c->graph_key.clear();
                 hipGraph_t gph = nullptr;
@@ -1124,19 +1194,18 @@ static int do_launch(Launch& Lc, float* out_r, float* out_g, float* out_b, float
         } else if ((st = enqueue(true))) {
             return st;
         }
-        unsigned long long counts[4] = {0, 0, 0, 0};
-        unsigned chunks[2] = {0u, 0u};
-        GL_HIP(hipMemcpyAsync(&counts[0], sb.rec_count, 8, hipMemcpyDeviceToHost, c->stream));
-        GL_HIP(hipMemcpyAsync(&counts[1], sb.bin_total, 8, hipMemcpyDeviceToHost, c->stream));
-        GL_HIP(hipMemcpyAsync(&chunks[0], sb.chunk_start + sg.n_tiles, 4, hipMemcpyDeviceToHost, c->stream));
-        GL_HIP(hipMemcpyAsync(&chunks[1], sb.mc_count, 4, hipMemcpyDeviceToHost, c->stream));
-        if (persistent) GL_HIP(hipMemcpyAsync(&counts[2], Q.clk, 16, hipMemcpyDeviceToHost, c->stream));
+        unsigned long long sum_local[5] = {0, 0, 0, 0, 0};
+        unsigned long long* h_sum = static_cast<unsigned long long*>(pin_take(c, sizeof sum_local));
+        GL_HIP(hipMemcpyAsync(h_sum ? h_sum : sum_local, b_sum.p, sizeof sum_local, hipMemcpyDeviceToHost, c->stream));
         if (out_r) {
             GL_HIP(hipMemcpyAsync(out_r, A.out_r, npx * 4, hipMemcpyDeviceToHost, c->stream));
             GL_HIP(hipMemcpyAsync(out_g, A.out_g, npx * 4, hipMemcpyDeviceToHost, c->stream));
             GL_HIP(hipMemcpyAsync(out_b, A.out_b, npx * 4, hipMemcpyDeviceToHost, c->stream));
         }
         GL_HIP(hipStreamSynchronize(c->stream));
+        if (h_sum) std::memcpy(sum_local, h_sum, sizeof sum_local);
+        const unsigned long long counts[4] = {sum_local[0], sum_local[1], sum_local[3], sum_local[4]};
+        const unsigned chunks[2] = {(unsigned)(sum_local[2] & 0xFFFFFFFFull), (unsigned)(sum_local[2] >> 32)};
         c->splat_hint = counts[0];
         c->splat_counts[0] = counts[0]; c->splat_counts[1] = counts[1];
         c->splat_counts[2] = chunks[0]; c->splat_counts[3] = chunks[1];

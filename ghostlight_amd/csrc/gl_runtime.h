@@ -77,6 +77,12 @@ struct gl_ctx {
     std::vector<uint8_t> ltab_key, probe_key;
     std::vector<int> probe_n;            // samples in each probe pass (passes concatenated)
     int refine_n = 0;                    // samples in the refine grid
+    // Pinned host staging of a ghost render's small transfers (pairs,
+    // sources, masks, rects, budgets up; probe and render counters down):
+    // slices are handed out in order within one render, which ends with a
+    // stream synchronisation, so the next render starts on an idle buffer.
+    char* pin = nullptr;
+    size_t pin_cap = 0, pin_used = 0;
     int max_lds = 0;                     // per-workgroup LDS bytes (device attribute)
     gl::DevBuf& buf(const char* name) { return bufs[name]; }
 };

@@ -193,6 +193,7 @@ void gl_ctx_destroy(gl_ctx* c) {
         {
             std::lock_guard<std::mutex> gm(gl::device_mem_mutex());
             if (c->graph) (void)hipGraphExecDestroy(c->graph);
+            if (c->pin) (void)hipHostFree(c->pin);
             if (c->stream) (void)hipStreamDestroy(c->stream);
         }
     }
@@ -233,10 +234,13 @@ int gl_ctx_trim(gl_ctx* ctx) {
     GL_HIP(hipStreamSynchronize(c->stream));
     for (auto& kv : c->bufs) kv.second.release();
     c->bufs.clear();
-    if (c->graph) {
+    {
         std::lock_guard<std::mutex> gm(gl::device_mem_mutex());
-        (void)hipGraphExecDestroy(c->graph);
+        if (c->graph) (void)hipGraphExecDestroy(c->graph);
         c->graph = nullptr;
+        if (c->pin) (void)hipHostFree(c->pin);
+        c->pin = nullptr;
+        c->pin_cap = c->pin_used = 0;
     }
     c->graph_key.clear();
     c->work_host.clear();

@@ -36,6 +36,25 @@ def _sources_array(sources) -> "C.Array":
     return arr, len(s)
 
 
+_PAIR_DTYPE = np.dtype([("a", np.int32), ("b", np.int32), ("boost", np.float32)])
+
+
+def _pairs_array(pairs, boosts) -> "C.Array":
+    """gl_pair array of (surf_a, surf_b) pairs and their area boosts."""
+    n = len(pairs)
+    arr = (_abi.gl_pair * max(1, n))()
+    if n:
+        ab = np.asarray(pairs, np.int64).reshape(n, 2)
+        rec = np.empty(n, _PAIR_DTYPE)
+        rec["a"], rec["b"] = ab[:, 0], ab[:, 1]
+        bo = np.asarray(boosts, np.float64).ravel()
+        if bo.size != n:
+            raise ValueError(f"{bo.size} area boosts for {n} ghost pairs")
+        rec["boost"] = bo
+        C.memmove(arr, rec.ctypes.data, rec.nbytes)
+    return arr
+
+
 class _Stats:
     """Owner of the caller-side arrays of gl_ghost_stats."""
 
@@ -123,9 +142,7 @@ def render_ghosts(lens: OpticalSystem, sources, width: int, height: int, config:
         pairs, boosts = filter_ghost_pairs(lens, sensor_half_w, sensor_half_h, config, ctx)
     if boosts is None:
         boosts = [1.0] * len(pairs)
-    parr = (_abi.gl_pair * max(1, len(pairs)))()
-    for i, (ab, bo) in enumerate(zip(pairs, boosts)):
-        parr[i].surf_a, parr[i].surf_b, parr[i].area_boost = int(ab[0]), int(ab[1]), float(bo)
+    parr = _pairs_array(pairs, boosts)
     sarr, n_src = _sources_array(sources)
     fw, fh, x0, y0 = fmt if fmt is not None else (width, height, 0, 0)
     p, _keep = to_params(config)
