@@ -34,6 +34,7 @@ struct SplatBufs {
     unsigned* mc_tile;                    // per staging slot: the tile
     unsigned* mc_count;                   // tiles with several chunks
     unsigned long long* stage;            // int64 fixed-point planes of those tiles (kept zero between renders)
+    unsigned* blk_hist;                   // LDS form: each binning workgroup's tile counts of its record slice
     unsigned stage_tiles;                 // staging slots allocated
 };
 

@@ -8,12 +8,14 @@
 //
 //   bin_count    per-workgroup LDS histogram of records over screen tiles
 //                (a bilinear footprint touches 1, 2 or 4 tiles), merged into
-//                per-tile counts; also the largest |power x colour weight|
-//                a tile receives
+//                per-tile counts and also kept per workgroup for the scatter;
+//                also the largest |power x colour weight| a tile receives
 //   bin_scan     one workgroup: exclusive scan -> tile starts / cursors, the
 //                work-chunk list (a tile with many records is split into
 //                chunks) and a staging slot per tile with several chunks
-//   bin_scatter  copies each record into its tiles' ranges
+//   bin_scatter  copies each record into its tiles' ranges (each workgroup
+//                takes the same record slice as in bin_count and reserves its
+//                ranges from that slice's stored histogram)
 //   resolve      persistent workgroups pull chunks from a queue and sum the
 //                chunk's bilinear contributions into an LDS tile; a tile with
 //                one chunk is written out by that workgroup (read-add-write),
@@ -135,11 +137,15 @@ __global__ __launch_bounds__(kBinThreads) void bin_count_kernel(SplatBufs B, Spl
     }
     if (g.lds_hist) {
         __syncthreads();
-        for (int t = threadIdx.x; t < g.n_tiles; t += blockDim.x)
-            if (hist[t]) {
-                atomicAdd(&B.tile_count[t], hist[t]);
+        unsigned* mine = B.blk_hist + (size_t)blockIdx.x * g.n_tiles;
+        for (int t = threadIdx.x; t < g.n_tiles; t += blockDim.x) {
+            const unsigned c = hist[t];
+            mine[t] = c;
+            if (c) {
+                atomicAdd(&B.tile_count[t], c);
                 atomicMax(&B.tile_mag[t], lmag[t]);
             }
+        }
     }
 }
 
@@ -218,19 +224,12 @@ __global__ __launch_bounds__(kBinThreads) void bin_scatter_kernel(SplatBufs B, S
         }
         return;
     }
-    unsigned* hist = smem;                       // counts, then per-block base offsets
+    unsigned* hist = smem;                       // per-tile cursor within this block's ranges
     unsigned long long* base = reinterpret_cast<unsigned long long*>(smem + ((g.n_tiles + 1) & ~1));
-    for (int t = threadIdx.x; t < g.n_tiles; t += blockDim.x) hist[t] = 0u;
-    __syncthreads();
-    for (unsigned long long i = r0 + threadIdx.x; i < r1; i += blockDim.x) {
-        int tx0, tx1, ty0, ty1, fb;
-        if (!tiles_of(B.rec[i], g, tx0, tx1, ty0, ty1, fb)) continue;
-        for (int ty = ty0; ty <= ty1; ++ty)
-            for (int tx = tx0; tx <= tx1; ++tx) atomicAdd(&hist[fb + ty * g.tiles_x + tx], 1u);
-    }
-    __syncthreads();
+    // This block's slice counts, stored by bin_count for the same slice.
+    const unsigned* mine = B.blk_hist + (size_t)blockIdx.x * g.n_tiles;
     for (int t = threadIdx.x; t < g.n_tiles; t += blockDim.x) {
-        const unsigned c = hist[t];
+        const unsigned c = mine[t];
         base[t] = c ? atomicAdd(&B.tile_cursor[t], (unsigned long long)c) : 0ull;
         hist[t] = 0u;
     }
@@ -401,10 +400,12 @@ int splat_prepare(gl_ctx* c, const SplatGeom& g, unsigned long long rec_cap, Spl
     if ((st = brec.ensure((size_t)rec_cap * sizeof(float4))) || (st = bbin.ensure((size_t)bin_cap * sizeof(float4))))
         return st;
     // tile arrays: start/cursor (u64), scalars, count (u32), chunk_start
-    // (u32, n+1), magnitude, staging slot, staged tile (u32 each)
+    // (u32, n+1), magnitude, staging slot, staged tile (u32 each), then the
+    // binning workgroups' slice histograms (LDS form)
     const size_t nt = (size_t)g.n_tiles;
     const size_t bytes = nt * 8 * 2 + 64 + nt * 4 + (nt + 1) * 4 + nt * 4 * 3;
-    if ((st = bt.ensure(bytes + 256))) return st;
+    const size_t blk_bytes = g.lds_hist ? (size_t)kBinBlocks * nt * 4 : 0;
+    if ((st = bt.ensure(bytes + 256 + blk_bytes))) return st;
     char* p = bt.as<char>();
     B.rec = brec.as<float4>();
     B.rec_cap = brec.cap / sizeof(float4);
@@ -421,6 +422,7 @@ int splat_prepare(gl_ctx* c, const SplatGeom& g, unsigned long long rec_cap, Spl
     B.tile_mag = B.mc_count + 1;
     B.mc_slot = B.tile_mag + nt;
     B.mc_tile = B.mc_slot + nt;
+    B.blk_hist = g.lds_hist ? reinterpret_cast<unsigned*>(p + ((bytes + 255) & ~(size_t)255)) : nullptr;
     // Staging for tiles with several chunks: each holds more than kChunk of
     // the binned records, so there are at most bin_cap / kChunk of them (a
     // render whose binning overflows is redone with larger buffers; its
