@@ -153,14 +153,15 @@ GLD float wdiv(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
 // (1 - sin2_t >= 2^-24), so the amplitudes' numerators carry no extra error
 // into the cancellation of nearly matched indices; only the two quotients use
 // the reciprocal estimate (a few ulp on the weight).
+// Total reflection (sin2_t >= 1) gives 1; the amplitudes are formed for
+// every lane and the result selected, so the lanes of a wave do not branch.
 GLD float fresnel_R(float cos_i, float n1, float n2, float eta) {
     cos_i = fabsf(cos_i);
     float sin2_t = eta * eta * (1.0f - cos_i * cos_i);
-    if (sin2_t >= 1.0f) return 1.0f;
     float cos_t = sqrt_rn_big(1.0f - sin2_t);
     float rs = wdiv(n1 * cos_i - n2 * cos_t, n1 * cos_i + n2 * cos_t);
     float rp = wdiv(n2 * cos_i - n1 * cos_t, n2 * cos_i + n1 * cos_t);
-    return 0.5f * (rs * rs + rp * rp);
+    return (sin2_t >= 1.0f) ? 1.0f : 0.5f * (rs * rs + rp * rp);
 }
 GLD float clamp01(float r) { return fminf(fmaxf(r, 0.0f), 1.0f); }
 // Single-layer (Airy) coating.  The interference sum cancels for an AR
@@ -191,7 +192,6 @@ GLD float coating_R(float cos_i, float n1, float n2, float cn, float d_nm, float
 GLD float ref_angle_deg(float cos_i, float n1, float aref) {   // trace_core.h:614-621
     const float sin_i = __builtin_amdgcn_sqrtf(fmaxf(0.0f, 1.0f - cos_i * cos_i));
     const float s = (aref > 0.0f) ? (n1 * sin_i * __builtin_amdgcn_rcpf(aref)) : sin_i;
-    if (s >= 1.0f) return 90.0f;
     float q = __builtin_fmaf(s, -0.0012624911f, 0.0066700901f);
     q = __builtin_fmaf(s, q, -0.0170881256f);
     q = __builtin_fmaf(s, q, 0.0308918810f);
@@ -199,8 +199,9 @@ GLD float ref_angle_deg(float cos_i, float n1, float aref) {   // trace_core.h:6
     q = __builtin_fmaf(s, q, 0.0889789874f);
     q = __builtin_fmaf(s, q, -0.2145988016f);
     q = __builtin_fmaf(s, q, 1.5707963050f);
-    return __builtin_fmaf(-__builtin_amdgcn_sqrtf(1.0f - s), q, 1.57079632679489662f) *
-           (180.0f / 3.14159265358979323846f);
+    const float ang = __builtin_fmaf(-__builtin_amdgcn_sqrtf(1.0f - s), q, 1.57079632679489662f) *
+                      (180.0f / 3.14159265358979323846f);
+    return (s >= 1.0f) ? 90.0f : ang;      // selected, not branched
 }
 GLD int2 ld_i2(const int2 GLC* p) {
     const int GLC* q = reinterpret_cast<const int GLC*>(p);
@@ -669,6 +670,91 @@ GLD bool refract(F3 dir, F3 nrm, float n1, float n2, F3& out) { return refract_r
 GLD F3 reflect_dn(F3 dir, F3 nrm, float dn) { return normalized_unit(sub(dir, mul(nrm, 2.0f * dn))); }
 GLD F3 reflect(F3 dir, F3 nrm) { return reflect_dn(dir, nrm, dot3(dir, nrm)); }
 
+// One surface interaction of the lean path (flat or FP32 sphere, untransformed,
+// round or polygon aperture, no bitmap or attenuator): the same float
+// operations as intersect_dn / check_aperture / refract_cos below, written
+// so that a lane's failure is a flag rather than a branch.  Every lane forms
+// the whole interaction and the lanes whose ray died leave at one point at
+// the end (status VIGNETTED for a miss or clip, TIR for a failed
+// refraction); what a dead lane computed past its failure is discarded.
+// With one divergent exit per surface the compiler keeps the ray state in
+// place instead of merging it at each early return.
+template <bool W, int CM, bool RT>
+GLD bool step_lean(const DevSurf GLC& s, const float GLC* poly, const float* __restrict__ arena, F3& o, F3& d,
+                   float lam, float n2, bool reflecting, float reflect_ior, float& cur, TraceRes& res,
+                   const int2 GLC* lr, float ratio) {
+    ++res.segs;
+    const float zv = s.z;
+    const bool flat = (s.flags & SF_FLAT) != 0;          // uniform
+    float t;
+    bool ok;
+    if (flat) {                                          // intersect_surface, plane
+        ok = !(fabsf(d.z) < 1e-12f);
+        t = (zv - o.z) / d.z;
+        ok = ok && (t > 1e-6f);
+    } else {                                             // sphere: nearer root to the vertex plane
+        const F3 oc = sub(o, f3(0.f, 0.f, s.cz));
+        const float a = dot3(d, d), b = 2.0f * dot3(oc, d), c = dot3(oc, oc) - s.R2;
+        const float disc = b * b - 4.0f * a * c;
+        const float sd = sqrtf(disc), inv_2a = 0.5f * rcp_rn(a);
+        const float t1 = (-b - sd) * inv_2a, t2 = (-b + sd) * inv_2a;
+        const bool c1 = t1 > 1e-6f, c2 = t2 > 1e-6f;
+        const float z1 = o.z + t1 * d.z, z2 = o.z + t2 * d.z;
+        const bool near1 = fabsf(z1 - zv) < fabsf(z2 - zv);
+        t = c1 ? ((c2 && !near1) ? t2 : t1) : t2;
+        ok = !(disc < 0.0f) && (c1 || c2);
+    }
+    const F3 hit = add(o, mul(d, t));
+    const float h2 = hit.x * hit.x + hit.y * hit.y;      // check_aperture
+    ok = ok && (h2 <= s.sa2);
+    if (s.flags & SF_POLYGON) {
+        const float GLC* pc = poly + s.poly_off;
+        const float GLC* ps = pc + GL_MAX_BLADES;
+        float m = hit.x * pc[0] + hit.y * ps[0];
+        for (int k = 1; k < s.nblades; ++k) {
+            const float e = hit.x * pc[k] + hit.y * ps[k];
+            m = e > m ? e : m;
+        }
+        ok = ok && (m <= s.apothem);
+    }
+    F3 nrm;
+    float dn;
+    if (flat) {
+        nrm = f3(0.f, 0.f, (d.z > 0.0f) ? -1.0f : 1.0f);
+        dn = dot3(nrm, d);
+    } else {
+        nrm = mul(sub(hit, f3(0.f, 0.f, s.cz)), s.inv_absR);
+        dn = dot3(nrm, d);
+        const bool flip = dn > 0.0f;
+        nrm = flip ? neg(nrm) : nrm;
+        dn = flip ? -dn : dn;
+    }
+    o = hit;
+    const float n1 = cur;
+    bool bent = true;
+    if (reflecting) {                                    // uniform
+        if (W) res.w *= surface_weight<CM>(fabsf(dn), n1, n2, RT ? ratio : n1 / n2, s, arena, lam, true, lr);
+        d = reflect_dn(d, nrm, dn);
+        cur = reflect_ior;
+    } else {                                             // refract_cos
+        const float eta = RT ? ratio : n1 / n2;
+        const float ci = -dn;
+        const float sin2_t = eta * eta * (1.0f - ci * ci);
+        const float cos_t = sqrt_rn_big(1.0f - sin2_t);
+        const F3 tmp = add(mul(d, eta), mul(nrm, eta * ci - cos_t));
+        const float sq = tmp.x * tmp.x + tmp.y * tmp.y + tmp.z * tmp.z;
+        bent = !(sin2_t >= 1.0f) && !(sq < 1e-18f);
+        if (W) res.w *= surface_weight<CM>(fabsf(dn), n1, n2, eta, s, arena, lam, false, lr);
+        d = mul(tmp, rcp_rn(sqrt_rn_big(sq)));
+        cur = n2;
+    }
+    if (!(ok && bent)) {
+        res.status = ok ? GL_TRACE_TIR : GL_TRACE_VIGNETTED;
+        return false;
+    }
+    return true;
+}
+
 // One surface interaction (trace_cuda.h:226-361 loop bodies).  Returns false
 // when the ray died; res.status is set to TIR for total internal reflection.
 // W = false skips the Fresnel / coating / attenuator weights (callers that
@@ -680,6 +766,7 @@ template <bool W, bool LF = false, int CM = CC_ALL, bool RT = false>
 GLD bool step(const DevSurf GLC& s, const float GLC* poly, const float* __restrict__ arena, F3& o, F3& d,
               float lam, float n2, bool reflecting, float reflect_ior, float& cur, TraceRes& res,
               const int2 GLC* lr, float ratio = 0.0f) {
+    if constexpr (LF) return step_lean<W, CM, RT>(s, poly, arena, o, d, lam, n2, reflecting, reflect_ior, cur, res, lr, ratio);
     float t, dn; F3 nrm;
     ++res.segs;
     if (!intersect_dn<LF>(o, d, s, poly, t, nrm, dn)) return false;
