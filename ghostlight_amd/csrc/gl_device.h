@@ -135,6 +135,18 @@ GLD float sqrt_rn_big(float x) {
     const float r = rm <= 0.0f ? sm : s;
     return rp > 0.0f ? sp : r;
 }
+// Correctly rounded square root for x >= 0 (any finite or infinite value,
+// subnormals included; NaN gives NaN): arguments below 2^-96 are scaled by
+// 2^32 before sqrt_rn_big and the root by 2^-16 after -- exact power-of-two
+// steps -- as the IEEE square root's own expansion does.  Equal to sqrtf in
+// every bit on [0, +inf] (checked over every float of it by gl_selftest_rn);
+// sqrtf's separate handling of +-0 and +inf is not needed, sqrt_rn_big
+// returns them.
+GLD float sqrt_rn_pos(float x) {
+    const bool tiny = x < 0x1p-96f;
+    const float r = sqrt_rn_big(tiny ? x * 0x1p32f : x);
+    return tiny ? r * 0x1p-16f : r;
+}
 // Correctly rounded reciprocal for normal x whose reciprocal is normal
 // (2^-126 <= |x| <= 2^126): the hardware estimate (v_rcp_f32, within 1 ulp)
 // refined by one Newton step whose residual 1 - x * y the fused
@@ -696,7 +708,9 @@ GLD bool step_lean(const DevSurf GLC& s, const float GLC* poly, const float* __r
         const F3 oc = sub(o, f3(0.f, 0.f, s.cz));
         const float a = dot3(d, d), b = 2.0f * dot3(oc, d), c = dot3(oc, oc) - s.R2;
         const float disc = b * b - 4.0f * a * c;
-        const float sd = sqrtf(disc), inv_2a = 0.5f * rcp_rn(a);
+        // sqrt_rn_pos: sqrtf's bits wherever disc >= 0 (a lane with disc < 0
+        // is dead and its root unused)
+        const float sd = sqrt_rn_pos(disc), inv_2a = 0.5f * rcp_rn(a);
         const float t1 = (-b - sd) * inv_2a, t2 = (-b + sd) * inv_2a;
         const bool c1 = t1 > 1e-6f, c2 = t2 > 1e-6f;
         const float z1 = o.z + t1 * d.z, z2 = o.z + t2 * d.z;

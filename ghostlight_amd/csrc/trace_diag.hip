@@ -139,6 +139,7 @@ __global__ __launch_bounds__(64) void diag_kernel(const DevSurf* __restrict__ S,
 //   OP 0: sqrt_rn_big(x) against sqrtf(x)
 //   OP 1: rcp_rn(x) against 1.0f / x
 //   OP 2: 0.5f * rcp_rn(x) against 0.5f / x (the sphere's 1 / 2a)
+//   OP 3: sqrt_rn_pos(x) against sqrtf(x)
 template <int OP>
 __global__ __launch_bounds__(256) void rn_check_kernel(uint32_t lo, uint32_t hi, unsigned long long* bad,
                                                       unsigned* first) {
@@ -150,7 +151,8 @@ __global__ __launch_bounds__(256) void rn_check_kernel(uint32_t lo, uint32_t hi,
         float a, b;
         if (OP == 0) { a = sqrt_rn_big(x); b = sqrtf(x); }
         else if (OP == 1) { a = rcp_rn(x); b = 1.0f / x; }
-        else { a = 0.5f * rcp_rn(x); b = 0.5f / x; }
+        else if (OP == 2) { a = 0.5f * rcp_rn(x); b = 0.5f / x; }
+        else { a = sqrt_rn_pos(x); b = sqrtf(x); }
         if (__float_as_uint(a) != __float_as_uint(b)) {
             ++nbad;
             fb = min(fb, (uint32_t)u);
@@ -231,7 +233,8 @@ int gl_selftest_rn(gl_ctx* ctx, int32_t op, uint32_t lo_bits, uint32_t hi_bits, 
                    uint32_t* first_mismatch) {
     if (!mismatches || !first_mismatch) return set_error(GL_ERR_INVALID, "null argument");
     if (lo_bits > hi_bits) return set_error(GL_ERR_INVALID, "empty bit range");
-    if (op < 0 || op > 2) return set_error(GL_ERR_INVALID, "op must be 0 (sqrt), 1 (reciprocal) or 2 (0.5 / x)");
+    if (op < 0 || op > 3)
+        return set_error(GL_ERR_INVALID, "op must be 0 (sqrt), 1 (reciprocal), 2 (0.5 / x) or 3 (sqrt of x >= 0)");
     int st;
     gl_ctx* c = resolve_ctx(ctx, &st);
     if (!c) return st;
@@ -245,7 +248,8 @@ int gl_selftest_rn(gl_ctx* ctx, int32_t op, uint32_t lo_bits, uint32_t hi_bits, 
     const dim3 grid(8192), block(256);
     if (op == 0) hipLaunchKernelGGL(rn_check_kernel<0>, grid, block, 0, c->stream, lo_bits, hi_bits, d_bad, d_first);
     else if (op == 1) hipLaunchKernelGGL(rn_check_kernel<1>, grid, block, 0, c->stream, lo_bits, hi_bits, d_bad, d_first);
-    else hipLaunchKernelGGL(rn_check_kernel<2>, grid, block, 0, c->stream, lo_bits, hi_bits, d_bad, d_first);
+    else if (op == 2) hipLaunchKernelGGL(rn_check_kernel<2>, grid, block, 0, c->stream, lo_bits, hi_bits, d_bad, d_first);
+    else hipLaunchKernelGGL(rn_check_kernel<3>, grid, block, 0, c->stream, lo_bits, hi_bits, d_bad, d_first);
     GL_HIP(hipGetLastError());
     uint32_t out[4];
     GL_HIP(hipMemcpyAsync(out, b.p, 16, hipMemcpyDeviceToHost, c->stream));

@@ -93,7 +93,8 @@ def render_starburst(lens: OpticalSystem, calibration: LensCalibration, sources,
         dbg = _abi.gl_starburst_debug()
         dbg.grid_cap = cap
         dbg.psf, dbg.sprite_rgb, dbg.pupil = _abi.fptr(psf), _abi.fptr(spr), _abi.fptr(pup)
-    _abi.check(lib().gl_render_starburst(ctx_handle(ctx), lens.device_lens().handle, C.byref(cal), sarr, _abi.fptr(px),
+    dl = lens.device_lens()
+    _abi.check(lib().gl_render_starburst(ctx_handle(ctx), dl.handle, C.byref(cal), sarr, _abi.fptr(px),
                                          _abi.fptr(py), n, width, height, fw, fh, float(sensor_half_w),
                                          float(sensor_half_h), C.byref(p), _abi.fptr(planes[0]), _abi.fptr(planes[1]),
                                          _abi.fptr(planes[2]), C.byref(dbg) if dbg is not None else None),
@@ -254,7 +255,8 @@ def primary_survival(lens: OpticalSystem, launch_xy, direction, z_spawn: float, 
     if len(xy) == 0:
         return out.astype(bool)
     d = np.asarray(direction, np.float32)
-    _abi.check(lib().gl_primary_survival(ctx_handle(ctx), lens.device_lens().handle, _abi.fptr(xy), len(xy),
+    dl = lens.device_lens()
+    _abi.check(lib().gl_primary_survival(ctx_handle(ctx), dl.handle, _abi.fptr(xy), len(xy),
                                          float(d[0]), float(d[1]), float(d[2]), float(z_spawn), float(lambda_nm),
                                          out.ctypes.data_as(C.POINTER(C.c_ubyte))), "primary_survival")
     return out.astype(bool)

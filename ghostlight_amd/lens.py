@@ -10,6 +10,7 @@ from __future__ import annotations
 
 import enum
 import math
+import threading
 from dataclasses import dataclass, field
 from typing import List, Optional
 
@@ -18,6 +19,9 @@ import ctypes as C
 import numpy as np
 
 from . import _abi
+
+# Serialises the creation of device lenses (OpticalSystem.device_lens).
+_DEVICE_LENS_LOCK = threading.Lock()
 
 
 class SurfaceForm(enum.IntEnum):
@@ -483,13 +487,21 @@ class OpticalSystem:
         return h.digest()
 
     def device_lens(self):
-        """Device-resident handle for this prescription (re-created on change)."""
+        """Device-resident handle for this prescription (re-created on change).
+
+        Callers keep the returned object while they use its handle: a
+        replaced DeviceLens frees its device lens when the last reference
+        goes.  Creation is serialised, so threads rendering the same lens
+        share one DeviceLens instead of replacing each other's."""
         from . import runtime
         key = self.pack_key()
-        if self._dev is None or self._dev_key != key:
-            self._dev = runtime.DeviceLens(self)
-            self._dev_key = key
-        return self._dev
+        with _DEVICE_LENS_LOCK:
+            dev = self._dev
+            if dev is None or self._dev_key != key:
+                dev = runtime.DeviceLens(self)
+                self._dev = dev
+                self._dev_key = key
+        return dev
 
     # ---- renderer wrappers (optical_system.py:334-429) -------------------
     def calibration(self, d_line_nm: float = 587.56) -> "LensCalibration":

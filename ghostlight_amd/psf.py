@@ -250,7 +250,8 @@ def deep_pupil(lens, ax, ay, targets=None, ctx=None) -> np.ndarray:
     ang = np.ascontiguousarray(np.stack([np.asarray(ax, np.float32), np.asarray(ay, np.float32)], 1))
     out = np.zeros((len(ang), 6), np.float32)
     tg = None if targets is None else np.ascontiguousarray(targets, np.float32).reshape(-1, 2)
-    _abi.check(lib().gl_psf_deep_pupil(ctx_handle(ctx), lens.device_lens().handle, len(ang), _abi.fptr(ang),
+    dl = lens.device_lens()
+    _abi.check(lib().gl_psf_deep_pupil(ctx_handle(ctx), dl.handle, len(ang), _abi.fptr(ang),
                                        _abi.fptr(tg) if tg is not None else None, int(tg is not None),
                                        _abi.fptr(out)), "render_psf")
     return out
@@ -295,7 +296,8 @@ def render_psf(sources, lens: OpticalSystem, config: PSFConfig, ctx=None, timing
     W, H = cfg.grid_nx * cfg.tile_w, cfg.grid_ny * cfg.tile_h
     planes = [np.zeros((H, W), np.float32) for _ in range(3)]
     ms = C.c_float(0.0)
-    _abi.check(lib().gl_render_psf_tiles(ctx_handle(ctx), lens.device_lens().handle, arr, n, C.byref(rp),
+    dl = lens.device_lens()
+    _abi.check(lib().gl_render_psf_tiles(ctx_handle(ctx), dl.handle, arr, n, C.byref(rp),
                                          C.byref(pp), _abi.fptr(planes[0]), _abi.fptr(planes[1]),
                                          _abi.fptr(planes[2]), C.byref(ms)), "render_psf")
     out = {"composite_w": W, "composite_h": H, "tile_w": cfg.tile_w, "tile_h": cfg.tile_h,

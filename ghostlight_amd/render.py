@@ -448,7 +448,8 @@ def render_gate_layer(sources, width: int, height: int, lens: OpticalSystem, con
     gp = gate_params(config.gate)
     planes = [np.zeros((height, width), np.float32) for _ in range(3)]
     ctr = (C.c_uint64 * 2)()
-    _abi.check(lib().gl_render_gate(ctx_handle(ctx), lens.device_lens().handle, sarr, n_src, width, height, width,
+    dl = lens.device_lens()
+    _abi.check(lib().gl_render_gate(ctx_handle(ctx), dl.handle, sarr, n_src, width, height, width,
                                     height, 0, 0, float(sensor_half_w), float(sensor_half_h), C.byref(p),
                                     C.byref(gp), _abi.fptr(planes[0]), _abi.fptr(planes[1]), _abi.fptr(planes[2]),
                                     ctr), "render_gate")
@@ -484,7 +485,8 @@ def _render_gate_debug(w: int, h: int, lens: OpticalSystem, calibration: LensCal
         sarr, n_src = _sources_array(src)
         p, _keep = to_params(config)
         gp = gate_params(config.gate)
-        _abi.check(lib().gl_render_gate_debug(ctx_handle(ctx), lens.device_lens().handle, sarr, n_src, w, h, w, h, 0,
+        dl = lens.device_lens()
+        _abi.check(lib().gl_render_gate_debug(ctx_handle(ctx), dl.handle, sarr, n_src, w, h, w, h, 0,
                                               0, float(hw), float(hh), C.byref(p), C.byref(gp),
                                               _abi.fptr(planes[0]), _abi.fptr(planes[1]), _abi.fptr(planes[2]),
                                               C.byref(dbg)), "render_gate")

@@ -32,7 +32,8 @@ def calibrate_lens(lens: OpticalSystem, d_line_nm: float = D_LINE_NM, ctx=None) 
     """Covered field (90% of the axial spawn-disc throughput), image circle
     (5%), paraxial focal lengths, entrance pupil and f-numbers per axis."""
     out = _abi.gl_calibration()
-    _abi.check(lib().gl_calibrate_lens(ctx_handle(ctx), lens.device_lens().handle, float(d_line_nm),
+    dl = lens.device_lens()
+    _abi.check(lib().gl_calibrate_lens(ctx_handle(ctx), dl.handle, float(d_line_nm),
                                        float(getattr(lens, "focal_length", 0.0) or 0.0), C.byref(out)),
                "calibrate_lens")
     return LensCalibration(**{f: float(getattr(out, f)) for f in LensCalibration.FIELDS})
@@ -69,7 +70,8 @@ def solve_source_map(lens: OpticalSystem, calib: LensCalibration, ndc_x: float, 
     (mm) to the field angle whose spawn-disc survivor mean lands there."""
     out = _abi.gl_source_map()
     cs = _calib_struct(calib)
-    _abi.check(lib().gl_solve_source_angle(ctx_handle(ctx), lens.device_lens().handle, C.byref(cs), float(ndc_x),
+    dl = lens.device_lens()
+    _abi.check(lib().gl_solve_source_angle(ctx_handle(ctx), dl.handle, C.byref(cs), float(ndc_x),
                                            float(ndc_y), float(eff_half_w), float(eff_half_h), C.byref(out)),
                "solve_source_angle")
     return SourceMapSolve(angle_x=out.angle_x, angle_y=out.angle_y, status=STATUS_NAMES.get(out.status, "fallback"),
@@ -98,7 +100,8 @@ def _source_map_landing(lens: OpticalSystem, calibration: LensCalibration, angle
     angle, or None past the image circle."""
     x, y, ok = C.c_float(), C.c_float(), C.c_int32()
     cs = _calib_struct(calibration)
-    _abi.check(lib().gl_source_map_landing(ctx_handle(ctx), lens.device_lens().handle, C.byref(cs), float(angle_x),
+    dl = lens.device_lens()
+    _abi.check(lib().gl_source_map_landing(ctx_handle(ctx), dl.handle, C.byref(cs), float(angle_x),
                                            float(angle_y), C.byref(x), C.byref(y), C.byref(ok)),
                "source_map_landing")
     return (x.value, y.value) if ok.value else None

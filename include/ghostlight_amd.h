@@ -467,7 +467,8 @@ int  gl_selftest_sqrt_rn(gl_ctx* ctx, uint32_t lo_bits, uint32_t hi_bits, uint64
 /* The same check for each of the tracer's fast correctly rounded forms
  * (no counterpart in the reference): op 0 = the square root above, op 1 =
  * the refined hardware reciprocal against 1.0f / x, op 2 = half of it
- * against 0.5f / x.  The tracer uses the reciprocal forms only on normal
+ * against 0.5f / x, op 3 = the square root of any x >= 0 (scaled below
+ * 2^-96) against sqrtf, which must agree on all of [0x00000000, 0x7F800000].  The tracer uses the reciprocal forms only on normal
  * arguments whose quotient is normal, so [0x00800000, 0x7E800000] (op 1),
  * [0x00800000, 0x7DFFFFFF] (op 2) and their negative images must give no
  * mismatch. */

@@ -339,7 +339,9 @@ def test_tracer_square_root_bits_equal_ieee():
 
 
 def test_tracer_reciprocal_bits_equal_ieee():
-    """The sphere's 1 / 2a and the refraction's 1 / |v| use the refined
+    """The sphere's root of the discriminant uses a scaled corrected square
+    root (every float >= 0 must give sqrtf's bits); its 1 / 2a and the
+    refraction's 1 / |v| use the refined
     hardware reciprocal on normal arguments with normal reciprocals; every
     float of that range, of either sign, must give the IEEE quotient's bits
     (for 0.5 / x up to 2^125, where the quotient stays normal)."""
@@ -348,8 +350,9 @@ def test_tracer_reciprocal_bits_equal_ieee():
     from ghostlight_amd.runtime import lib
     bad, first = C.c_uint64(), C.c_uint32()
     ranges = {1: [(0x00800000, 0x7E800000), (0x80800000, 0xFE800000)],
-              2: [(0x00800000, 0x7DFFFFFF), (0x80800000, 0xFDFFFFFF)]}
-    for op in (1, 2):
+              2: [(0x00800000, 0x7DFFFFFF), (0x80800000, 0xFDFFFFFF)],
+              3: [(0x00000000, 0x7F800000)]}
+    for op in (1, 2, 3):
         for lo, hi in ranges[op]:
             print(f"reciprocal self-check op {op} over [{lo:#010x}, {hi:#010x}]", flush=True)
             _abi.check(lib().gl_selftest_rn(None, op, lo, hi, C.byref(bad), C.byref(first)), "gl_selftest_rn")
