@@ -110,16 +110,21 @@ __device__ __forceinline__ unsigned record_mag(const float4& r, const float4* __
     return __float_as_uint(fabsf(r.z) * m);
 }
 
+// LDS: per-workgroup histograms in LDS (g.lds_hist), else straight into the
+// global tile counters.  The two forms are separate instantiations so that
+// the LDS one issues LDS atomics (a pointer that may be either would make
+// every atomic a flat one).
+template <bool LDS>
 __global__ __launch_bounds__(kBinThreads) void bin_count_kernel(SplatBufs B, SplatGeom g,
                                                                 const float4* __restrict__ spec) {
     extern __shared__ unsigned hist[];            // counts, then magnitudes (LDS form)
     unsigned* lmag = hist + g.n_tiles;
-    if (g.lds_hist) {
+    if (LDS) {
         for (int t = threadIdx.x; t < 2 * g.n_tiles; t += blockDim.x) hist[t] = 0u;
         __syncthreads();
     }
-    unsigned* h = g.lds_hist ? hist : B.tile_count;
-    unsigned* hm = g.lds_hist ? lmag : B.tile_mag;
+    unsigned* h = LDS ? hist : B.tile_count;
+    unsigned* hm = LDS ? lmag : B.tile_mag;
     const unsigned long long n = n_records(B);
     const unsigned long long per = (n + gridDim.x - 1) / gridDim.x;
     const unsigned long long r0 = per * blockIdx.x, r1 = min(n, r0 + per);
@@ -135,7 +140,7 @@ __global__ __launch_bounds__(kBinThreads) void bin_count_kernel(SplatBufs B, Spl
                 atomicMax(&hm[t], m);
             }
     }
-    if (g.lds_hist) {
+    if (LDS) {
         __syncthreads();
         unsigned* mine = B.blk_hist + (size_t)blockIdx.x * g.n_tiles;
         for (int t = threadIdx.x; t < g.n_tiles; t += blockDim.x) {
@@ -450,7 +455,8 @@ int splat_resolve(hipStream_t s, const SplatGeom& g, const SplatBufs& B, const f
                   float* out_b) {
     const size_t hist_bytes = g.lds_hist ? (size_t)g.n_tiles * 8 : 0;
     const size_t scat_bytes = g.lds_hist ? (size_t)((g.n_tiles + 1) & ~1) * 4 + (size_t)g.n_tiles * 8 : 0;
-    hipLaunchKernelGGL(bin_count_kernel, dim3(kBinBlocks), dim3(kBinThreads), hist_bytes, s, B, g, spec);
+    if (g.lds_hist) hipLaunchKernelGGL(bin_count_kernel<true>, dim3(kBinBlocks), dim3(kBinThreads), hist_bytes, s, B, g, spec);
+    else hipLaunchKernelGGL(bin_count_kernel<false>, dim3(kBinBlocks), dim3(kBinThreads), 0, s, B, g, spec);
     hipLaunchKernelGGL(bin_scan_kernel, dim3(1), dim3(1024), 0, s, B, g);
     hipLaunchKernelGGL(bin_scatter_kernel, dim3(kBinBlocks), dim3(kBinThreads), scat_bytes, s, B, g);
     const int TS = 1 << g.tile_shift;
