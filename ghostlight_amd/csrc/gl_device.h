@@ -1088,7 +1088,8 @@ GLD TraceRes trace_ghost(const DevSurf GLC* S, int n, const float GLC* poly, con
 }
 
 // d_trace_primary_ray (trace_cuda.h:111-172).
-template <bool W = true>
+// LF: the lens fits the lean path (lean_lens).
+template <bool W = true, bool LF = false>
 GLD TraceRes trace_primary(const DevSurf GLC* S, int n, const float GLC* poly, const float* __restrict__ arena,
                            F3 o, F3 d, float lam, const float GLC* iat, bool t_ge0, F3* out_dir) {
     TraceRes res; res.status = GL_TRACE_VIGNETTED; res.w = 1.0f; res.segs = 0; res.px = res.py = res.pz = 0.f;
@@ -1096,7 +1097,7 @@ GLD TraceRes trace_primary(const DevSurf GLC* S, int n, const float GLC* poly, c
     for (int k = 0; k < n; ++k) {
         const DevSurf GLC& s = S[k];
         if (!(s.flags & SF_ACTIVE)) continue;
-        if (!step<W>(s, poly, arena, o, d, lam, iat[k], false, 0.0f, cur, res, nullptr)) return res;
+        if (!step<W, LF>(s, poly, arena, o, d, lam, iat[k], false, 0.0f, cur, res, nullptr)) return res;
     }
     to_sensor(o, d, t_ge0, res);
     if (out_dir) *out_dir = d;

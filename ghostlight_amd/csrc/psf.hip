@@ -54,7 +54,8 @@ __device__ __forceinline__ void psf_add(const PsfArgs& A, float* tile, int x0t, 
     }
 }
 
-template <bool LDS>
+// LF: the lens fits the tracer's lean path (lean_lens).
+template <bool LDS, bool LF>
 __global__ __launch_bounds__(kPsfBlock) void psf_kernel(PsfArgs A) {
     extern __shared__ float tile[];
     const int si_src = blockIdx.x;
@@ -76,7 +77,7 @@ __global__ __launch_bounds__(kPsfBlock) void psf_kernel(PsfArgs A) {
         const float src_avg = (s.r + s.g + s.b) * (1.0f / 3.0f);
         for (int si = 0; si < A.n_spec; ++si) {
             const float4 sp = A.spec[si];
-            TraceRes res = trace_primary<true>(as_const(A.S), A.n, as_const(A.poly), A.arena, o, dir, sp.x,
+            TraceRes res = trace_primary<true, LF>(as_const(A.S), A.n, as_const(A.poly), A.arena, o, dir, sp.x,
                                                as_const(A.iat) + (size_t)si * A.n, false, nullptr);
             if (res.status != GL_TRACE_OK) continue;
             if (!isfinite(res.px) || !isfinite(res.py)) continue;
@@ -113,14 +114,17 @@ __global__ __launch_bounds__(kPsfBlock) void psf_kernel(PsfArgs A) {
                 const int cx = (int)floorf(px), cy = (int)floorf(py);
                 const float ox = px - ((float)cx + 0.5f), oy = py - ((float)cy + 0.5f);
                 const int R = A.radius_px;
-                float wsum = 0.0f;
-                for (int dy = -R; dy <= R; ++dy) {
-                    const float dyr = (float)dy - oy;
-                    for (int dx = -R; dx <= R; ++dx) {
-                        const float dxr = (float)dx - ox;
-                        wsum += expf(-(dxr * dxr + dyr * dyr) * inv_2sig2);
-                    }
+                // The normalising sum factors into row and column sums
+                // (exp(-(a + b)) = exp(-a) exp(-b)): 2(2R+1) exponentials
+                // instead of (2R+1)^2; it differs from the cell-by-cell sum by
+                // rounding only (a few 1e-8 relative).
+                float sx = 0.0f, sy = 0.0f;
+                for (int q = -R; q <= R; ++q) {
+                    const float dxr = (float)q - ox, dyr = (float)q - oy;
+                    sx += expf(-(dxr * dxr) * inv_2sig2);
+                    sy += expf(-(dyr * dyr) * inv_2sig2);
                 }
+                const float wsum = sx * sy;
                 const float inv_wsum = (wsum > 1e-20f) ? (1.0f / wsum) : 0.0f;
                 for (int dy = -R; dy <= R; ++dy) {
                     const int yi = cy + dy;
@@ -244,10 +248,11 @@ int gl_render_psf_tiles(gl_ctx* ctx, const gl_lens* lens, const gl_psf_source* s
     GL_HIP(hipEventRecord(c->ev0, c->stream));
     int lds_max = 0;
     if (hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, 0) != hipSuccess) lds_max = 65536;
-    if (tile_bytes <= std::min<size_t>(kPsfMaxLdsTile, (size_t)lds_max))
-        hipLaunchKernelGGL(psf_kernel<true>, grid_dim, dim3(kPsfBlock), tile_bytes, c->stream, A);
-    else
-        hipLaunchKernelGGL(psf_kernel<false>, grid_dim, dim3(kPsfBlock), 0, c->stream, A);
+    const bool lds = tile_bytes <= std::min<size_t>(kPsfMaxLdsTile, (size_t)lds_max);
+    if (lds && D.lean) hipLaunchKernelGGL((psf_kernel<true, true>), grid_dim, dim3(kPsfBlock), tile_bytes, c->stream, A);
+    else if (lds) hipLaunchKernelGGL((psf_kernel<true, false>), grid_dim, dim3(kPsfBlock), tile_bytes, c->stream, A);
+    else if (D.lean) hipLaunchKernelGGL((psf_kernel<false, true>), grid_dim, dim3(kPsfBlock), 0, c->stream, A);
+    else hipLaunchKernelGGL((psf_kernel<false, false>), grid_dim, dim3(kPsfBlock), 0, c->stream, A);
     GL_HIP(hipGetLastError());
     GL_HIP(hipEventRecord(c->ev1, c->stream));
     GL_HIP(hipMemcpyAsync(out_r, o, npx * 4, hipMemcpyDeviceToHost, c->stream));
