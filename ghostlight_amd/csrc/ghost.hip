@@ -224,7 +224,7 @@ __device__ __forceinline__ const GhostArgs GLC* kernel_args() {
     return (const GhostArgs GLC*)__builtin_amdgcn_kernarg_segment_ptr();
 }
 
-template <bool H, bool LF = false, int CM = CC_ALL>
+template <bool H, int LF = 0, int CM = CC_ALL>
 __device__ __forceinline__ void deposit_range(const GhostArgs& A, int ps, int gi, int si0, int si1, float4* stage,
                                               int& staged, int lane) {
     const GhostArgs GLC* Ap = kernel_args();
@@ -302,9 +302,9 @@ struct PersistQ {
 
 // Eight waves per SIMD: the trace is VALU-bound with long dependent chains,
 // and the extra waves hide more latency than the few spilled registers cost.
-// LF: the lens fits the tracer's lean path (lean_lens); CM: the coating
+// LF: the lens's lean level (lean_level); CM: the coating
 // classes its surfaces use (coating_classes).
-template <bool LF, int CM>
+template <int LF, int CM>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8, 8)))
 void ghost_persist_kernel(GhostArgs A, PersistQ Q) {
     __shared__ float4 stage_all[kBlock / 64][kStage];
@@ -336,9 +336,9 @@ __global__ void persist_reset_kernel(PersistQ Q) {
 
 // STATS = probe / diagnostic pass (counters, survivor bbox); otherwise the
 // render (SPLAT) and/or the segment count (COUNT).
-// LF: the lens fits the tracer's lean path (lean_lens), as for the
+// LF: the lens's lean level (lean_level), as for the
 // persistent kernel.
-template <bool STATS, bool SPLAT, bool COUNT, bool H, bool LF = false>
+template <bool STATS, bool SPLAT, bool COUNT, bool H, int LF = 0>
 __global__ __launch_bounds__(kBlock) void ghost_kernel(GhostArgs A) {
     const int ps = blockIdx.x;
     const int pair_idx = ps / A.n_src, src_idx = ps % A.n_src;
@@ -619,16 +619,17 @@ static int lam_group(int ns) {
     return g;
 }
 
-// lean: the lens fits the lean tracer (no HURB launch takes that form).
+// lean: the lens's lean level (no HURB launch takes a lean form).
 template <bool STATS, bool SPLAT, bool COUNT>
-static int launch(const GhostArgs& A_in, int n_ps, int n_samples, hipStream_t s, bool lean) {
+static int launch(const GhostArgs& A_in, int n_ps, int n_samples, hipStream_t s, int lean) {
     if (n_ps <= 0 || n_samples <= 0) return GL_OK;
     GhostArgs A = A_in;
     A.lam_group = STATS ? lam_group(A.n_spec) : 0;
     const int per_block = A.lam_group > 0 ? kBlock / A.lam_group : kBlock;
     dim3 grid((unsigned)n_ps, (unsigned)((n_samples + per_block - 1) / per_block), 1);
     if (A.hurb.on) hipLaunchKernelGGL((ghost_kernel<STATS, SPLAT, COUNT, true>), grid, dim3(kBlock), 0, s, A);
-    else if (lean) hipLaunchKernelGGL((ghost_kernel<STATS, SPLAT, COUNT, false, true>), grid, dim3(kBlock), 0, s, A);
+    else if (lean == 1) hipLaunchKernelGGL((ghost_kernel<STATS, SPLAT, COUNT, false, 1>), grid, dim3(kBlock), 0, s, A);
+    else if (lean == 2) hipLaunchKernelGGL((ghost_kernel<STATS, SPLAT, COUNT, false, 2>), grid, dim3(kBlock), 0, s, A);
     else hipLaunchKernelGGL((ghost_kernel<STATS, SPLAT, COUNT, false>), grid, dim3(kBlock), 0, s, A);
     GL_HIP(hipGetLastError());
     return GL_OK;
@@ -1137,14 +1138,20 @@ static int do_launch(Launch& Lc, float* out_r, float* out_g, float* out_b, float
             if (events) GL_HIP(hipEventRecord(c->ev0, c->stream));
             if (persistent) {
                 const dim3 gw((unsigned)workers), bw(kBlock);
-                if (D.lean && D.coat_classes == (CC_SA | CC_FRESNEL))
-                    hipLaunchKernelGGL((ghost_persist_kernel<true, CC_SA | CC_FRESNEL>), gw, bw, 0, c->stream, A, Q);
-                else if (D.lean && D.coat_classes == (CC_AIRY | CC_FRESNEL))
-                    hipLaunchKernelGGL((ghost_persist_kernel<true, CC_AIRY | CC_FRESNEL>), gw, bw, 0, c->stream, A, Q);
-                else if (D.lean)
-                    hipLaunchKernelGGL((ghost_persist_kernel<true, CC_ALL>), gw, bw, 0, c->stream, A, Q);
+                if (D.lean == 1 && D.coat_classes == (CC_SA | CC_FRESNEL))
+                    hipLaunchKernelGGL((ghost_persist_kernel<1, CC_SA | CC_FRESNEL>), gw, bw, 0, c->stream, A, Q);
+                else if (D.lean == 1 && D.coat_classes == (CC_AIRY | CC_FRESNEL))
+                    hipLaunchKernelGGL((ghost_persist_kernel<1, CC_AIRY | CC_FRESNEL>), gw, bw, 0, c->stream, A, Q);
+                else if (D.lean == 1)
+                    hipLaunchKernelGGL((ghost_persist_kernel<1, CC_ALL>), gw, bw, 0, c->stream, A, Q);
+                else if (D.lean == 2 && D.coat_classes == (CC_SA | CC_FRESNEL))
+                    hipLaunchKernelGGL((ghost_persist_kernel<2, CC_SA | CC_FRESNEL>), gw, bw, 0, c->stream, A, Q);
+                else if (D.lean == 2 && D.coat_classes == (CC_AIRY | CC_FRESNEL))
+                    hipLaunchKernelGGL((ghost_persist_kernel<2, CC_AIRY | CC_FRESNEL>), gw, bw, 0, c->stream, A, Q);
+                else if (D.lean == 2)
+                    hipLaunchKernelGGL((ghost_persist_kernel<2, CC_ALL>), gw, bw, 0, c->stream, A, Q);
                 else
-                    hipLaunchKernelGGL((ghost_persist_kernel<false, CC_ALL>), gw, bw, 0, c->stream, A, Q);
+                    hipLaunchKernelGGL((ghost_persist_kernel<0, CC_ALL>), gw, bw, 0, c->stream, A, Q);
                 GL_HIP(hipGetLastError());
             } else if ((e = launch<false, true, false>(A, n_ps, n_grid, c->stream, D.lean))) {
                 return e;

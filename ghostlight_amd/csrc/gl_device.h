@@ -457,8 +457,9 @@ GLD bool image_passes(const DevSurf GLC& s, const float* __restrict__ arena, F3 
 // check_aperture (trace_core.h:133-170); the regular polygon is the
 // half-plane intersection max_k(h . n_k) <= apothem (same set as the
 // reference's r <= apothem / cos(offset)).
-// LF ("lean features"): the lens has no stretched and no deformed apertures
-// (see kLeanFlags), so those branches are compiled out.
+// LF ("lean features"): the lens has no stretched and no deformed apertures,
+// so those branches are compiled out.  (The tracer's own LF instantiation
+// takes step_lean, which handles stretched apertures itself.)
 template <bool LF = false>
 GLD bool check_aperture(F3 hit, const DevSurf GLC& s, const float GLC* poly) {
     float hx = (!LF && (s.flags & SF_ASPECT)) ? hit.x / s.aspect : hit.x;
@@ -509,7 +510,8 @@ GLD float asph_dsag(float r, float inv_R, float K, const DevSurf GLC& s) {    //
 }
 
 // intersect_surface (trace_core.h:299-525).  Returns false on miss / clip.
-// LF: every surface is flat or spherical, untransformed and solved in FP32.
+// LF: every surface is flat or spherical, untransformed and solved in FP32
+// (the tracer's LF instantiation takes step_lean instead).
 // dn_out = dot(n_out, d) in the caller's frame, with the bits the caller's
 // own dot product would give: the refraction, reflection and coating weight
 // take their incidence cosine from it instead of forming it again.  Where
@@ -682,51 +684,84 @@ GLD bool refract(F3 dir, F3 nrm, float n1, float n2, F3& out) { return refract_r
 GLD F3 reflect_dn(F3 dir, F3 nrm, float dn) { return normalized_unit(sub(dir, mul(nrm, 2.0f * dn))); }
 GLD F3 reflect(F3 dir, F3 nrm) { return reflect_dn(dir, nrm, dot3(dir, nrm)); }
 
-// One surface interaction of the lean path (flat or FP32 sphere, untransformed,
-// round or polygon aperture, no bitmap or attenuator): the same float
-// operations as intersect_dn / check_aperture / refract_cos below, written
-// so that a lane's failure is a flag rather than a branch.  Every lane forms
-// the whole interaction and the lanes whose ray died leave at one point at
-// the end (status VIGNETTED for a miss or clip, TIR for a failed
-// refraction); what a dead lane computed past its failure is discarded.
-// With one divergent exit per surface the compiler keeps the ray state in
-// place instead of merging it at each early return.
-template <bool W, int CM, bool RT>
+// One surface interaction of the lean path (flat, FP32 sphere or cylinder,
+// decentred / tilted or not, round, stretched or polygon aperture, no
+// bitmap, deformed blade or attenuator): the same float operations as
+// intersect_dn / check_aperture / refract_cos below, written so that a lane's
+// failure is a flag rather than a branch.  Every lane forms the whole
+// interaction and the lanes whose ray died leave at one point at the end
+// (status VIGNETTED for a miss or clip, TIR for a failed refraction); what a
+// dead lane computed past its failure is discarded.  With one divergent exit
+// per surface the compiler keeps the ray state in place instead of merging
+// it at each early return.  The surface kind, the transform and the
+// stretched aperture are tests of the surface record: uniform over the wave.
+// MED = false compiles out the cylinder, the transform and the stretched
+// aperture (lenses of flat and spherical surfaces only, lean level 1);
+// MED = true keeps them (lean level 2).
+template <bool W, int CM, bool RT, bool MED>
 GLD bool step_lean(const DevSurf GLC& s, const float GLC* poly, const float* __restrict__ arena, F3& o, F3& d,
                    float lam, float n2, bool reflecting, float reflect_ior, float& cur, TraceRes& res,
                    const int2 GLC* lr, float ratio) {
     ++res.segs;
     const float zv = s.z;
-    const bool flat = (s.flags & SF_FLAT) != 0;          // uniform
+    const int fl = s.flags;
+    const bool flat = (fl & SF_FLAT) != 0;                 // uniform
+    const bool cyl = MED && !flat && s.form == GL_FORM_CYLINDRICAL;
+    const bool tr = MED && (fl & SF_TRANSFORMED) != 0;
+    // The interaction is solved in the surface's frame (ol, dl); the hit that
+    // moves the ray is formed from the caller's o and d with the same t.
+    F3 ol = o, dl = d;
+    if (tr) {
+        ol = add(m_tmul(s.rot, sub(o, f3(s.dx, s.dy, zv))), f3(0.f, 0.f, zv));
+        dl = m_tmul(s.rot, d);
+    }
     float t;
     bool ok;
-    if (flat) {                                          // intersect_surface, plane
-        ok = !(fabsf(d.z) < 1e-12f);
-        t = (zv - o.z) / d.z;
+    if (flat) {                                            // intersect_surface, plane
+        ok = !(fabsf(dl.z) < 1e-12f);
+        t = (zv - ol.z) / dl.z;
         ok = ok && (t > 1e-6f);
-    } else {                                             // sphere: nearer root to the vertex plane
-        const F3 oc = sub(o, f3(0.f, 0.f, s.cz));
-        const float a = dot3(d, d), b = 2.0f * dot3(oc, d), c = dot3(oc, oc) - s.R2;
+    } else if (cyl) {                                      // cylinder about x or y
+        const float vx = (fl & SF_CYL_Y) ? 1.0f : 0.0f, vy = (fl & SF_CYL_Y) ? 0.0f : 1.0f;
+        const float R = s.R;
+        const float oc_v = ol.x * vx + ol.y * vy;
+        const float oc_z = ol.z - zv - R;
+        const float dv = dl.x * vx + dl.y * vy;
+        const float qa = dv * dv + dl.z * dl.z;
+        const float qb = 2.0f * (oc_v * dv + oc_z * dl.z);
+        const float qc = oc_v * oc_v + oc_z * oc_z - R * R;
+        const float disc = qb * qb - 4.0f * qa * qc;
+        const float sd = sqrtf(disc), inv_2a = 0.5f / qa;
+        const float t1 = (-qb - sd) * inv_2a, t2 = (-qb + sd) * inv_2a;
+        const bool c1 = t1 > 1e-6f, c2 = t2 > 1e-6f;
+        const float z1 = ol.z + t1 * dl.z, z2 = ol.z + t2 * dl.z;
+        const bool near1 = fabsf(z1 - zv) < fabsf(z2 - zv);
+        t = c1 ? ((c2 && !near1) ? t2 : t1) : t2;
+        ok = !(disc < 0.0f) && (c1 || c2);
+    } else {                                               // sphere: nearer root to the vertex plane
+        const F3 oc = sub(ol, f3(0.f, 0.f, s.cz));
+        const float a = dot3(dl, dl), b = 2.0f * dot3(oc, dl), c = dot3(oc, oc) - s.R2;
         const float disc = b * b - 4.0f * a * c;
         // sqrt_rn_pos: sqrtf's bits wherever disc >= 0 (a lane with disc < 0
-        // is dead and its root unused)
+        // is dead and its root unused); a = |dl|^2 is within rounding of 1
         const float sd = sqrt_rn_pos(disc), inv_2a = 0.5f * rcp_rn(a);
         const float t1 = (-b - sd) * inv_2a, t2 = (-b + sd) * inv_2a;
         const bool c1 = t1 > 1e-6f, c2 = t2 > 1e-6f;
-        const float z1 = o.z + t1 * d.z, z2 = o.z + t2 * d.z;
+        const float z1 = ol.z + t1 * dl.z, z2 = ol.z + t2 * dl.z;
         const bool near1 = fabsf(z1 - zv) < fabsf(z2 - zv);
         t = c1 ? ((c2 && !near1) ? t2 : t1) : t2;
         ok = !(disc < 0.0f) && (c1 || c2);
     }
-    const F3 hit = add(o, mul(d, t));
-    const float h2 = hit.x * hit.x + hit.y * hit.y;      // check_aperture
+    const F3 hl = add(ol, mul(dl, t));                     // the hit in the surface's frame
+    const float hx = (MED && (fl & SF_ASPECT)) ? hl.x / s.aspect : hl.x, hy = hl.y;   // check_aperture
+    const float h2 = hx * hx + hy * hy;
     ok = ok && (h2 <= s.sa2);
-    if (s.flags & SF_POLYGON) {
+    if (fl & SF_POLYGON) {
         const float GLC* pc = poly + s.poly_off;
         const float GLC* ps = pc + GL_MAX_BLADES;
-        float m = hit.x * pc[0] + hit.y * ps[0];
+        float m = hx * pc[0] + hy * ps[0];
         for (int k = 1; k < s.nblades; ++k) {
-            const float e = hit.x * pc[k] + hit.y * ps[k];
+            const float e = hx * pc[k] + hy * ps[k];
             m = e > m ? e : m;
         }
         ok = ok && (m <= s.apothem);
@@ -734,23 +769,37 @@ GLD bool step_lean(const DevSurf GLC& s, const float GLC* poly, const float* __r
     F3 nrm;
     float dn;
     if (flat) {
-        nrm = f3(0.f, 0.f, (d.z > 0.0f) ? -1.0f : 1.0f);
+        nrm = f3(0.f, 0.f, (dl.z > 0.0f) ? -1.0f : 1.0f);
+        if (tr) nrm = m_mul(s.rot, nrm);
         dn = dot3(nrm, d);
     } else {
-        nrm = mul(sub(hit, f3(0.f, 0.f, s.cz)), s.inv_absR);
-        dn = dot3(nrm, d);
+        if (cyl) {
+            const float vx = (fl & SF_CYL_Y) ? 1.0f : 0.0f, vy = (fl & SF_CYL_Y) ? 0.0f : 1.0f;
+            const float hv = hl.x * vx + hl.y * vy;
+            nrm = f3(hv * vx, hv * vy, hl.z - zv - s.R);
+            const float nlen = sqrtf(nrm.x * nrm.x + nrm.y * nrm.y + nrm.z * nrm.z);
+            ok = ok && !(nlen < 1e-12f);
+            nrm = mul(nrm, 1.0f / nlen);
+        } else {
+            nrm = mul(sub(hl, f3(0.f, 0.f, s.cz)), s.inv_absR);
+        }
+        dn = dot3(nrm, dl);
         const bool flip = dn > 0.0f;
         nrm = flip ? neg(nrm) : nrm;
         dn = flip ? -dn : dn;
+        if (tr) {
+            nrm = m_mul(s.rot, nrm);
+            dn = dot3(nrm, d);
+        }
     }
-    o = hit;
+    o = tr ? add(o, mul(d, t)) : hl;
     const float n1 = cur;
     bool bent = true;
-    if (reflecting) {                                    // uniform
+    if (reflecting) {                                      // uniform
         if (W) res.w *= surface_weight<CM>(fabsf(dn), n1, n2, RT ? ratio : n1 / n2, s, arena, lam, true, lr);
         d = reflect_dn(d, nrm, dn);
         cur = reflect_ior;
-    } else {                                             // refract_cos
+    } else {                                               // refract_cos
         const float eta = RT ? ratio : n1 / n2;
         const float ci = -dn;
         const float sin2_t = eta * eta * (1.0f - ci * ci);
@@ -774,16 +823,19 @@ GLD bool step_lean(const DevSurf GLC& s, const float GLC* poly, const float* __r
 // W = false skips the Fresnel / coating / attenuator weights (callers that
 // need only landings, statuses and segment counts).
 // ratio: n1 / n2 from the caller's table when RT (the same IEEE quotient);
-// otherwise it is divided here, and only where it is used.  LF: no IMAGE
-// apertures and no attenuators.
-template <bool W, bool LF = false, int CM = CC_ALL, bool RT = false>
+// otherwise it is divided here, and only where it is used.  LF: the lens's
+// lean level (lean_level); 1 and 2 take the predicated step_lean.
+template <bool W, int LF = 0, int CM = CC_ALL, bool RT = false>
 GLD bool step(const DevSurf GLC& s, const float GLC* poly, const float* __restrict__ arena, F3& o, F3& d,
               float lam, float n2, bool reflecting, float reflect_ior, float& cur, TraceRes& res,
               const int2 GLC* lr, float ratio = 0.0f) {
-    if constexpr (LF) return step_lean<W, CM, RT>(s, poly, arena, o, d, lam, n2, reflecting, reflect_ior, cur, res, lr, ratio);
+    if constexpr (LF != 0) {
+        return step_lean<W, CM, RT, LF == 2>(s, poly, arena, o, d, lam, n2, reflecting, reflect_ior, cur, res, lr,
+                                            ratio);
+    } else {
     float t, dn; F3 nrm;
     ++res.segs;
-    if (!intersect_dn<LF>(o, d, s, poly, t, nrm, dn)) return false;
+    if (!intersect_dn<false>(o, d, s, poly, t, nrm, dn)) return false;
     F3 hit = add(o, mul(d, t));
     o = hit;
     if (!LF && (s.flags & SF_IMAGE) && !image_passes(s, arena, hit)) return false;
@@ -807,6 +859,7 @@ GLD bool step(const DevSurf GLC& s, const float GLC* poly, const float* __restri
         cur = n2;
     }
     return true;
+    }
 }
 
 // ---- front-of-lens baffles (baffle_stack_blocks / baffle_stack_edge_distance,
@@ -1047,7 +1100,7 @@ GLD void to_sensor(F3 o, F3 d, bool t_ge0, TraceRes& res) {   // trace_cuda.h:36
 // iat[k] on the backward one (muted surfaces are skipped, so the medium
 // carried between surfaces is exactly that), so the ratio the refraction
 // needs is rat[k].x forward and rat[k].y backward.
-template <bool W = true, bool H = false, bool LF = false, int CM = CC_ALL, bool RT = false>
+template <bool W = true, bool H = false, int LF = 0, int CM = CC_ALL, bool RT = false>
 GLD TraceRes trace_ghost(const DevSurf GLC* S, int n, const float GLC* poly, const float* __restrict__ arena,
                          int a, int b, F3 o, F3 d, float lam, const float GLC* iat, const float GLC* ibf,
                          bool t_ge0, const int2 GLC* lrow = nullptr, const HurbDev* hb = nullptr,
@@ -1088,8 +1141,8 @@ GLD TraceRes trace_ghost(const DevSurf GLC* S, int n, const float GLC* poly, con
 }
 
 // d_trace_primary_ray (trace_cuda.h:111-172).
-// LF: the lens fits the lean path (lean_lens).
-template <bool W = true, bool LF = false>
+// LF: the lean level of the lens (lean_level: 0 generic, 1 or 2 predicated).
+template <bool W = true, int LF = 0>
 GLD TraceRes trace_primary(const DevSurf GLC* S, int n, const float GLC* poly, const float* __restrict__ arena,
                            F3 o, F3 d, float lam, const float GLC* iat, bool t_ge0, F3* out_dir) {
     TraceRes res; res.status = GL_TRACE_VIGNETTED; res.w = 1.0f; res.segs = 0; res.px = res.py = res.pz = 0.f;

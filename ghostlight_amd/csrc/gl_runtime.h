@@ -106,7 +106,7 @@ struct LensDev {
     const DevSurf* surf = nullptr;
     const float* poly = nullptr;             // per polygon surface: c[32] | s[32]
     const float* arena = nullptr;
-    bool lean = false;                       // every active surface fits the tracer's lean path
+    int lean = 0;                            // the lens's lean level (lean_level)
     int coat_classes = CC_ALL;               // coating-weight specialisation (coating_classes)
 };
 }  // namespace gl
@@ -125,7 +125,7 @@ struct gl_lens {
         float fp64_rmin;
         gl::DevSurf* d_surf;
         float* d_poly;
-        bool lean;
+        int lean;
         int coat_classes;
     };
     std::vector<Variant> variants;
@@ -137,10 +137,13 @@ gl_ctx* resolve_ctx(gl_ctx* ctx, int* status);
 int lens_sync(gl_lens* lens, float fp64_rmin, hipStream_t stream, LensDev* out);
 void build_devsurfs(const gl_surface* s, int n, float fp64_rmin, const float* arena, int arena_len,
                     std::vector<DevSurf>& out, std::vector<float>& poly);
-// True when every active surface is flat or spherical, untransformed, solved
-// in FP32, with a round or regular-polygon aperture of aspect 1 and no
-// bitmap or attenuator: the tracer's lean instantiation covers the lens.
-bool lean_lens(const std::vector<DevSurf>& ds);
+// The tracer instantiation that covers a lens: 1 when every active surface
+// is flat or spherical, untransformed, solved in FP32, with a round or
+// regular-polygon aperture of aspect 1; 2 when cylinders, decentred / tilted
+// surfaces or stretched apertures are present as well; else 0 (the generic
+// tracer: FP64 radii, aspheres, deformed blades, bitmaps or attenuators).
+// Levels 1 and 2 take the predicated surface step.
+int lean_level(const std::vector<DevSurf>& ds);
 // The coating-weight specialisation a lens can use: CC_SA | CC_FRESNEL when
 // every active surface is spectral-angular or a bare interface,
 // CC_AIRY | CC_FRESNEL when every active surface is SIMPLE, else CC_ALL.

@@ -54,8 +54,8 @@ __device__ __forceinline__ void psf_add(const PsfArgs& A, float* tile, int x0t, 
     }
 }
 
-// LF: the lens fits the tracer's lean path (lean_lens).
-template <bool LDS, bool LF>
+// LF: the lens's lean level (lean_level).
+template <bool LDS, int LF>
 __global__ __launch_bounds__(kPsfBlock) void psf_kernel(PsfArgs A) {
     extern __shared__ float tile[];
     const int si_src = blockIdx.x;
@@ -249,10 +249,13 @@ int gl_render_psf_tiles(gl_ctx* ctx, const gl_lens* lens, const gl_psf_source* s
     int lds_max = 0;
     if (hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, 0) != hipSuccess) lds_max = 65536;
     const bool lds = tile_bytes <= std::min<size_t>(kPsfMaxLdsTile, (size_t)lds_max);
-    if (lds && D.lean) hipLaunchKernelGGL((psf_kernel<true, true>), grid_dim, dim3(kPsfBlock), tile_bytes, c->stream, A);
-    else if (lds) hipLaunchKernelGGL((psf_kernel<true, false>), grid_dim, dim3(kPsfBlock), tile_bytes, c->stream, A);
-    else if (D.lean) hipLaunchKernelGGL((psf_kernel<false, true>), grid_dim, dim3(kPsfBlock), 0, c->stream, A);
-    else hipLaunchKernelGGL((psf_kernel<false, false>), grid_dim, dim3(kPsfBlock), 0, c->stream, A);
+    if (lds && D.lean == 1) hipLaunchKernelGGL((psf_kernel<true, 1>), grid_dim, dim3(kPsfBlock), tile_bytes, c->stream, A);
+    else if (lds && D.lean == 2)
+        hipLaunchKernelGGL((psf_kernel<true, 2>), grid_dim, dim3(kPsfBlock), tile_bytes, c->stream, A);
+    else if (lds) hipLaunchKernelGGL((psf_kernel<true, 0>), grid_dim, dim3(kPsfBlock), tile_bytes, c->stream, A);
+    else if (D.lean == 1) hipLaunchKernelGGL((psf_kernel<false, 1>), grid_dim, dim3(kPsfBlock), 0, c->stream, A);
+    else if (D.lean == 2) hipLaunchKernelGGL((psf_kernel<false, 2>), grid_dim, dim3(kPsfBlock), 0, c->stream, A);
+    else hipLaunchKernelGGL((psf_kernel<false, 0>), grid_dim, dim3(kPsfBlock), 0, c->stream, A);
     GL_HIP(hipGetLastError());
     GL_HIP(hipEventRecord(c->ev1, c->stream));
     GL_HIP(hipMemcpyAsync(out_r, o, npx * 4, hipMemcpyDeviceToHost, c->stream));

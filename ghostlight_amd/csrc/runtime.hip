@@ -108,7 +108,7 @@ int lens_sync(gl_lens* L, float fp64_rmin, hipStream_t stream, LensDev* out) {
         std::vector<DevSurf> ds;
         std::vector<float> poly;
         build_devsurfs(L->host.data(), (int)L->host.size(), fp64_rmin, L->arena.data(), (int)L->arena.size(), ds, poly);
-        gl_lens::Variant nv{fp64_rmin, nullptr, nullptr, lean_lens(ds), coating_classes(ds)};
+        gl_lens::Variant nv{fp64_rmin, nullptr, nullptr, lean_level(ds), coating_classes(ds)};
         {
             std::lock_guard<std::mutex> ga(device_mem_mutex());
             GL_HIP(hipMalloc(&nv.d_surf, std::max<size_t>(1, ds.size()) * sizeof(DevSurf)));
@@ -147,15 +147,18 @@ int coating_classes(const std::vector<DevSurf>& ds) {
     return CC_ALL;
 }
 
-bool lean_lens(const std::vector<DevSurf>& ds) {
-    const int32_t heavy = SF_TRANSFORMED | SF_FP64 | SF_ASPECT | SF_DEFORMED | SF_IMAGE;
+int lean_level(const std::vector<DevSurf>& ds) {
+    const int32_t heavy = SF_FP64 | SF_DEFORMED | SF_IMAGE;
+    int level = 1;
     for (const DevSurf& d : ds) {
         if (!(d.flags & SF_ACTIVE)) continue;
-        if (d.flags & heavy) return false;
-        if (!(d.flags & SF_FLAT) && d.form != GL_FORM_SPHERE) return false;
-        if (d.coat_model == GL_COAT_ATTENUATOR_GAUSS) return false;
+        if (d.flags & heavy) return 0;
+        if (d.coat_model == GL_COAT_ATTENUATOR_GAUSS) return 0;
+        const bool flat = (d.flags & SF_FLAT) != 0;
+        if (!flat && d.form != GL_FORM_SPHERE && d.form != GL_FORM_CYLINDRICAL) return 0;
+        if ((d.flags & (SF_TRANSFORMED | SF_ASPECT)) || (!flat && d.form == GL_FORM_CYLINDRICAL)) level = 2;
     }
-    return true;
+    return level;
 }
 
 }  // namespace gl
