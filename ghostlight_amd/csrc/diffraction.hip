@@ -689,6 +689,9 @@ int gl_fft2d_c2c_forward_inplace(gl_ctx* ctx, void* d, int32_t N) {
     gl_ctx* c = resolve_ctx(ctx, &st);
     if (!c) return st;
     if (!d) return set_error(GL_ERR_INVALID, "null data");
+    // The long-row form keeps its ping-pong plane and twiddles in the
+    // context's buffer cache: one call at a time per context.
+    std::lock_guard<std::mutex> g(c->mu);
     return fft2d(c, static_cast<float2*>(d), N, nullptr);
 }
 

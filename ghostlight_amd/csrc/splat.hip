@@ -53,7 +53,7 @@ namespace gl {
 static constexpr int kBinBlocks = 1024;
 static constexpr int kBinThreads = 256;
 static constexpr int kChunk = 8192;          // records per resolve work item
-static constexpr int kResolveBlocks = 1024;
+static constexpr int kResolveBlocks = 1536;     // six 24 KiB LDS tiles per CU
 static constexpr int kMaxTiles = 8192;       // LDS: 8 B a tile in bin_count, 12 B in bin_scatter
 static constexpr int kFinishBlocks = 512;
 
@@ -87,6 +87,29 @@ __device__ __forceinline__ bool tiles_of(const float4& r, const SplatGeom& g, in
     tx0 = xa >> g.tile_shift; tx1 = xb >> g.tile_shift;
     ty0 = ya >> g.tile_shift; ty1 = yb >> g.tile_shift;
     return true;
+}
+
+// Records a thread loads before it processes them in the binning passes:
+// several loads in flight per lane.  A thread still takes its records in the
+// order r0 + tid, r0 + tid + blockDim, ..., so the passes see the same
+// per-thread sequence as a plain strided loop.
+static constexpr int kBinUnroll = 4;
+
+template <class F>
+__device__ __forceinline__ void for_records(const float4* __restrict__ rec, unsigned long long r0,
+                                            unsigned long long r1, F&& f) {
+    const unsigned long long step = blockDim.x;
+    for (unsigned long long i0 = r0 + threadIdx.x; i0 < r1; i0 += kBinUnroll * step) {
+        float4 rr[kBinUnroll];
+#pragma unroll
+        for (int u = 0; u < kBinUnroll; ++u) {
+            const unsigned long long i = i0 + (unsigned long long)u * step;
+            if (i < r1) rr[u] = rec[i];
+        }
+#pragma unroll
+        for (int u = 0; u < kBinUnroll; ++u)
+            if (i0 + (unsigned long long)u * step < r1) f(rr[u]);
+    }
 }
 
 __device__ __forceinline__ unsigned long long n_records(const SplatBufs& B) {
@@ -128,10 +151,9 @@ __global__ __launch_bounds__(kBinThreads) void bin_count_kernel(SplatBufs B, Spl
     const unsigned long long n = n_records(B);
     const unsigned long long per = (n + gridDim.x - 1) / gridDim.x;
     const unsigned long long r0 = per * blockIdx.x, r1 = min(n, r0 + per);
-    for (unsigned long long i = r0 + threadIdx.x; i < r1; i += blockDim.x) {
-        const float4 r = B.rec[i];
+    for_records(B.rec, r0, r1, [&](const float4& r) {
         int tx0, tx1, ty0, ty1, base;
-        if (!tiles_of(r, g, tx0, tx1, ty0, ty1, base)) continue;
+        if (!tiles_of(r, g, tx0, tx1, ty0, ty1, base)) return;
         const unsigned m = record_mag(r, spec);
         for (int ty = ty0; ty <= ty1; ++ty)
             for (int tx = tx0; tx <= tx1; ++tx) {
@@ -139,7 +161,7 @@ __global__ __launch_bounds__(kBinThreads) void bin_count_kernel(SplatBufs B, Spl
                 atomicAdd(&h[t], 1u);
                 atomicMax(&hm[t], m);
             }
-    }
+    });
     if (LDS) {
         __syncthreads();
         unsigned* mine = B.blk_hist + (size_t)blockIdx.x * g.n_tiles;
@@ -239,17 +261,16 @@ __global__ __launch_bounds__(kBinThreads) void bin_scatter_kernel(SplatBufs B, S
         hist[t] = 0u;
     }
     __syncthreads();
-    for (unsigned long long i = r0 + threadIdx.x; i < r1; i += blockDim.x) {
-        const float4 r = B.rec[i];
+    for_records(B.rec, r0, r1, [&](const float4& r) {
         int tx0, tx1, ty0, ty1, fb;
-        if (!tiles_of(r, g, tx0, tx1, ty0, ty1, fb)) continue;
+        if (!tiles_of(r, g, tx0, tx1, ty0, ty1, fb)) return;
         for (int ty = ty0; ty <= ty1; ++ty)
             for (int tx = tx0; tx <= tx1; ++tx) {
                 const int t = fb + ty * g.tiles_x + tx;
                 const unsigned long long slot = base[t] + atomicAdd(&hist[t], 1u);
                 if (slot < B.bin_cap) B.binned[slot] = r;
             }
-    }
+    });
 }
 
 // Fixed-point exponent of a tile: every contribution is below 2^e (e from the
