@@ -261,16 +261,17 @@ __global__ __launch_bounds__(kBinThreads) void bin_scatter_kernel(SplatBufs B, S
         hist[t] = 0u;
     }
     __syncthreads();
-    for_records(B.rec, r0, r1, [&](const float4& r) {
+    for (unsigned long long i = r0 + threadIdx.x; i < r1; i += blockDim.x) {
+        const float4 r = B.rec[i];
         int tx0, tx1, ty0, ty1, fb;
-        if (!tiles_of(r, g, tx0, tx1, ty0, ty1, fb)) return;
+        if (!tiles_of(r, g, tx0, tx1, ty0, ty1, fb)) continue;
         for (int ty = ty0; ty <= ty1; ++ty)
             for (int tx = tx0; tx <= tx1; ++tx) {
                 const int t = fb + ty * g.tiles_x + tx;
                 const unsigned long long slot = base[t] + atomicAdd(&hist[t], 1u);
                 if (slot < B.bin_cap) B.binned[slot] = r;
             }
-    });
+    }
 }
 
 // Fixed-point exponent of a tile: every contribution is below 2^e (e from the
