@@ -265,6 +265,8 @@ SIGNATURES = {
                                            C.c_int32, I32P, C.POINTER(gl_ghost_stats)]),
     "gl_trace_ghost_rays": (C.c_int, [VP, VP, C.c_int32, C.c_int32, F32P, C.c_int32, C.c_int32, C.c_float,
                                       F32P, F32P, I32P, I32P]),
+    "gl_trace_ghost_rays_level": (C.c_int, [VP, VP, C.c_int32, C.c_int32, F32P, C.c_int32, C.c_int32, C.c_float,
+                                            C.c_int32, F32P, F32P, I32P, I32P]),
     "gl_trace_rays_diagnostic": (C.c_int, [VP, VP, C.c_int32, C.c_int32, F32P, C.c_int32, C.c_int32, C.c_float,
                                            C.c_int32, C.POINTER(gl_trace_event), I32P, F32P, F32P, I32P]),
     "gl_selftest_sqrt_rn": (C.c_int, [VP, C.c_uint32, C.c_uint32, C.POINTER(C.c_uint64), C.POINTER(C.c_uint32)]),

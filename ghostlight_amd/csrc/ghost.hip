@@ -520,20 +520,28 @@ __global__ void filter_probe_kernel(const DevSurf* __restrict__ S, const float* 
     out[(size_t)p * 67 + t] = make_float4(r.px, r.py, r.w, (float)r.status);
 }
 
-// Single-ray traces for tests and probes.
+// Single-ray traces for tests and probes.  <0, CC_ALL, false> is the generic
+// surface step; the lean levels (1, 2) with a coating class and RT take the
+// per-ray refraction ratios (rat) and coating rows (lrow) exactly as the
+// render kernels take the per-lambda ones, so they run the instantiation a
+// render of this lens runs.
+template <int LF, int CM, bool RT>
 __global__ void ray_kernel(const DevSurf* __restrict__ S, const float* __restrict__ poly, const float* __restrict__ arena,
                            int n, int a, int b, const float* __restrict__ rays, int n_rays, const float* __restrict__ iat,
-                           const float* __restrict__ ibf, int t_ge0, float* __restrict__ pos, float* __restrict__ w,
-                           int* __restrict__ st, int* __restrict__ segs) {
+                           const float* __restrict__ ibf, const float2* __restrict__ rat, const int2* __restrict__ lrow,
+                           int t_ge0, float* __restrict__ pos, float* __restrict__ w, int* __restrict__ st,
+                           int* __restrict__ segs) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n_rays) return;
     const float* q = rays + 7 * (size_t)i;
+    const size_t row = (size_t)i * n;
     // Per-ray IOR slices: lane-varying, so plain global pointers.
-    TraceRes r = (b >= 0) ? trace_ghost(as_const(S), n, as_const(poly), arena, a, b, f3(q[0], q[1], q[2]),
-                                        f3(q[3], q[4], q[5]), q[6], as_const(iat + (size_t)i * n),
-                                        as_const(ibf + (size_t)i * n), t_ge0 != 0)
-                          : trace_primary(as_const(S), n, as_const(poly), arena, f3(q[0], q[1], q[2]),
-                                          f3(q[3], q[4], q[5]), q[6], as_const(iat + (size_t)i * n), t_ge0 != 0, nullptr);
+    TraceRes r = (b >= 0) ? trace_ghost<true, false, LF, CM, RT>(
+                                as_const(S), n, as_const(poly), arena, a, b, f3(q[0], q[1], q[2]), f3(q[3], q[4], q[5]),
+                                q[6], as_const(iat + row), as_const(ibf + row), t_ge0 != 0,
+                                RT ? as_const(lrow + row) : nullptr, nullptr, nullptr, RT ? as_const(rat + row) : nullptr)
+                          : trace_primary<true, LF>(as_const(S), n, as_const(poly), arena, f3(q[0], q[1], q[2]),
+                                                    f3(q[3], q[4], q[5]), q[6], as_const(iat + row), t_ge0 != 0, nullptr);
     pos[3 * i] = r.px; pos[3 * i + 1] = r.py; pos[3 * i + 2] = r.pz;
     w[i] = r.w; st[i] = r.status;
     if (segs) segs[i] = r.segs;
@@ -1499,10 +1507,11 @@ int gl_render_ghost_pipeline(gl_ctx* ctx, const gl_lens* lens, const gl_source* 
                                   height, fmt_w, fmt_h, fx0, fy0, p, out_r, out_g, out_b, nullptr, stats);
 }
 
-int gl_trace_ghost_rays(gl_ctx* ctx, const gl_lens* lens, int32_t a, int32_t b, const float* rays, int32_t n_rays,
-                        int32_t t_ge0, float fp64_rmin, float* out_pos, float* out_w, int32_t* out_status,
-                        int32_t* out_segments) {
+int gl_trace_ghost_rays_level(gl_ctx* ctx, const gl_lens* lens, int32_t a, int32_t b, const float* rays,
+                              int32_t n_rays, int32_t t_ge0, float fp64_rmin, int32_t level, float* out_pos,
+                              float* out_w, int32_t* out_status, int32_t* out_segments) {
     if (!lens || !rays || n_rays < 0 || !out_pos || !out_w || !out_status) return set_error(GL_ERR_INVALID, "null argument");
+    if (level < -1 || level > 2) return set_error(GL_ERR_INVALID, "level must be -1, 0, 1 or 2");
     if (n_rays == 0) return GL_OK;
     const int n = (int)lens->host.size();
     if (b >= 0 && (a < 0 || b <= a || b >= n)) {
@@ -1516,24 +1525,53 @@ int gl_trace_ghost_rays(gl_ctx* ctx, const gl_lens* lens, int32_t a, int32_t b, 
     gl_lens* L = const_cast<gl_lens*>(lens);
     LensDev D;
     if ((st = lens_sync(L, fp64_rmin, c->stream, &D))) return st;
-    // Per-ray IOR slices (each ray carries its own wavelength).
+    const int lf = level < 0 ? D.lean : level;
+    // A lean level handles the surfaces of its own level and below only;
+    // D.lean == 0 is a lens only the generic step handles.
+    if (lf != 0 && (D.lean == 0 || lf < D.lean))
+        return set_error(GL_ERR_INVALID, "lean level %d does not handle this lens (its level is %d)", lf, D.lean);
+    // Per-ray tables (each ray carries its own wavelength): indices after and
+    // before each surface, and for the lean levels the refraction ratios and
+    // coating rows the render tables hold per lambda.
+    std::vector<gl_spec> sp((size_t)n_rays);
+    for (int i = 0; i < n_rays; ++i) sp[i] = gl_spec{rays[7 * (size_t)i + 6], 0, 0, 0};
     std::vector<float> iat((size_t)n_rays * n), ibf((size_t)n_rays * n);
-    for (int i = 0; i < n_rays; ++i) {
-        gl_spec sp{rays[7 * (size_t)i + 6], 0, 0, 0};
-        ior_tables(L->host.data(), n, &sp, 1, &iat[(size_t)i * n], &ibf[(size_t)i * n]);
-    }
-    void *d_r, *d_a, *d_b;
+    ior_tables(L->host.data(), n, sp.data(), n_rays, iat.data(), ibf.data());
+    void *d_r, *d_a, *d_b, *d_rat = nullptr, *d_lrow = nullptr;
     if ((st = upload(c, "r_rays", rays, (size_t)n_rays * 7 * 4, &d_r)) ||
         (st = upload(c, "r_iat", iat.data(), iat.size() * 4, &d_a)) ||
         (st = upload(c, "r_ibf", ibf.data(), ibf.size() * 4, &d_b)))
         return st;
+    if (lf != 0) {
+        std::vector<float2> rats(iat.size());
+        for (size_t q = 0; q < rats.size(); ++q) rats[q] = make_float2(ibf[q] / iat[q], iat[q] / ibf[q]);
+        std::vector<int32_t> lrows;
+        lambda_rows(L->host.data(), n, L->arena.data(), sp.data(), n_rays, lrows);
+        if ((st = upload(c, "r_rat", rats.data(), rats.size() * sizeof(float2), &d_rat)) ||
+            (st = upload(c, "r_lrow", lrows.data(), lrows.size() * sizeof(int32_t), &d_lrow)))
+            return st;
+    }
     DevBuf &bp = c->buf("r_pos"), &bw = c->buf("r_w"), &bs = c->buf("r_st"), &bg = c->buf("r_seg");
     if ((st = bp.ensure((size_t)n_rays * 12)) || (st = bw.ensure((size_t)n_rays * 4)) ||
         (st = bs.ensure((size_t)n_rays * 4)) || (st = bg.ensure((size_t)n_rays * 4)))
         return st;
-    hipLaunchKernelGGL(ray_kernel, dim3((unsigned)((n_rays + 255) / 256)), dim3(256), 0, c->stream, D.surf, D.poly,
-                       D.arena, n, a, b, static_cast<const float*>(d_r), n_rays, static_cast<const float*>(d_a),
-                       static_cast<const float*>(d_b), t_ge0, bp.as<float>(), bw.as<float>(), bs.as<int>(), bg.as<int>());
+    const dim3 gr((unsigned)((n_rays + 255) / 256)), bl(256);
+    const float *fr = static_cast<const float*>(d_r), *fa = static_cast<const float*>(d_a),
+                *fb = static_cast<const float*>(d_b);
+    const float2* rt = static_cast<const float2*>(d_rat);
+    const int2* lr = static_cast<const int2*>(d_lrow);
+    const int cc = D.coat_classes;
+#define GL_RAY_LAUNCH(LF_, CM_, RT_)                                                                                 \
+    hipLaunchKernelGGL((ray_kernel<LF_, CM_, RT_>), gr, bl, 0, c->stream, D.surf, D.poly, D.arena, n, a, b, fr,      \
+                       n_rays, fa, fb, rt, lr, t_ge0, bp.as<float>(), bw.as<float>(), bs.as<int>(), bg.as<int>())
+    if (lf == 1 && cc == (CC_SA | CC_FRESNEL)) GL_RAY_LAUNCH(1, CC_SA | CC_FRESNEL, true);
+    else if (lf == 1 && cc == (CC_AIRY | CC_FRESNEL)) GL_RAY_LAUNCH(1, CC_AIRY | CC_FRESNEL, true);
+    else if (lf == 1) GL_RAY_LAUNCH(1, CC_ALL, true);
+    else if (lf == 2 && cc == (CC_SA | CC_FRESNEL)) GL_RAY_LAUNCH(2, CC_SA | CC_FRESNEL, true);
+    else if (lf == 2 && cc == (CC_AIRY | CC_FRESNEL)) GL_RAY_LAUNCH(2, CC_AIRY | CC_FRESNEL, true);
+    else if (lf == 2) GL_RAY_LAUNCH(2, CC_ALL, true);
+    else GL_RAY_LAUNCH(0, CC_ALL, false);
+#undef GL_RAY_LAUNCH
     GL_HIP(hipGetLastError());
     GL_HIP(hipMemcpyAsync(out_pos, bp.p, (size_t)n_rays * 12, hipMemcpyDeviceToHost, c->stream));
     GL_HIP(hipMemcpyAsync(out_w, bw.p, (size_t)n_rays * 4, hipMemcpyDeviceToHost, c->stream));
@@ -1541,6 +1579,13 @@ int gl_trace_ghost_rays(gl_ctx* ctx, const gl_lens* lens, int32_t a, int32_t b, 
     if (out_segments) GL_HIP(hipMemcpyAsync(out_segments, bg.p, (size_t)n_rays * 4, hipMemcpyDeviceToHost, c->stream));
     GL_HIP(hipStreamSynchronize(c->stream));
     return GL_OK;
+}
+
+int gl_trace_ghost_rays(gl_ctx* ctx, const gl_lens* lens, int32_t a, int32_t b, const float* rays, int32_t n_rays,
+                        int32_t t_ge0, float fp64_rmin, float* out_pos, float* out_w, int32_t* out_status,
+                        int32_t* out_segments) {
+    return gl_trace_ghost_rays_level(ctx, lens, a, b, rays, n_rays, t_ge0, fp64_rmin, 0, out_pos, out_w, out_status,
+                                     out_segments);
 }
 
 }  // extern "C"

@@ -168,10 +168,12 @@ def render_ghosts(lens: OpticalSystem, sources, width: int, height: int, config:
 
 
 def trace_ghost_rays(lens: OpticalSystem, bounce_a: int, bounce_b: int, rays: np.ndarray, t_ge0: bool = True,
-                     fp64_radius_min: float = 0.0, ctx=None):
+                     fp64_radius_min: float = 0.0, ctx=None, level: int = 0):
     """Device traces of (N, 7) rays [ox, oy, oz, dx, dy, dz, lambda]; bounce_b < 0
     traces the primary path.  Returns (pos (N,3), weight (N,), status (N,),
-    segments (N,))."""
+    segments (N,)).  level picks the device tracer's instantiation: 0 the
+    generic surface step, 1 or 2 a predicated lean step (not below the level
+    the lens needs), -1 the one a render of this lens runs."""
     r = np.ascontiguousarray(rays, np.float32).reshape(-1, 7)
     n = len(r)
     pos = np.zeros((n, 3), np.float32)
@@ -179,9 +181,10 @@ def trace_ghost_rays(lens: OpticalSystem, bounce_a: int, bounce_b: int, rays: np
     st = np.zeros(n, np.int32)
     sg = np.zeros(n, np.int32)
     dl = lens.device_lens()
-    _abi.check(lib().gl_trace_ghost_rays(ctx_handle(ctx), dl.handle, int(bounce_a), int(bounce_b), _abi.fptr(r), n,
-                                         int(bool(t_ge0)), float(fp64_radius_min), _abi.fptr(pos), _abi.fptr(w),
-                                         _abi.iptr(st), _abi.iptr(sg)), "trace_ghost_rays")
+    _abi.check(lib().gl_trace_ghost_rays_level(ctx_handle(ctx), dl.handle, int(bounce_a), int(bounce_b),
+                                               _abi.fptr(r), n, int(bool(t_ge0)), float(fp64_radius_min), int(level),
+                                               _abi.fptr(pos), _abi.fptr(w), _abi.iptr(st), _abi.iptr(sg)),
+               "trace_ghost_rays")
     return pos, w, st, sg
 
 

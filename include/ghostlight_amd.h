@@ -439,6 +439,23 @@ int  gl_trace_ghost_rays(gl_ctx* ctx, const gl_lens* lens, int32_t bounce_a,
                          float* out_pos, float* out_w, int32_t* out_status,
                          int32_t* out_segments);
 
+/* gl_trace_ghost_rays with the device tracer's instantiation chosen:
+ * level 0 is the generic surface step (what gl_trace_ghost_rays runs);
+ * 1 and 2 are the predicated lean steps (1: flat and spherical surfaces,
+ * untransformed, round or polygon apertures; 2: adds tilted or decentred
+ * surfaces, cylinders and stretched apertures), with the refraction ratios
+ * and coating rows taken from per-ray tables as a render takes its
+ * per-lambda ones; -1 is the level a render of this lens runs.  A lean level
+ * below the one the lens needs, or any lean level for a lens only the generic
+ * step handles (aspheres, FP64 radii, image surfaces), is GL_ERR_INVALID.  Same results at every
+ * level (the tests hold them bit-equal); no counterpart in the reference
+ * beyond trace_ghost_ray / trace_primary_ray (trace.h). */
+int  gl_trace_ghost_rays_level(gl_ctx* ctx, const gl_lens* lens, int32_t bounce_a,
+                               int32_t bounce_b, const float* rays, int32_t n,
+                               int32_t t_ge0, float fp64_radius_min, int32_t level,
+                               float* out_pos, float* out_w, int32_t* out_status,
+                               int32_t* out_segments);
+
 /* One surface interaction of a diagnostic trace (TraceEvent,
  * trace_event.h:48-58): the hit point (the ray's origin when the surface was
  * missed), the surface normal (zero when missed), the index on each side,

@@ -40,17 +40,47 @@ def _random_rays(lens, n, seed, max_deg=14.0):
     return rays
 
 
+def _lean_levels(lens):
+    """The device tracer levels that accept this lens: 0 (generic) always, a
+    lean level only from the one the lens needs upward."""
+    rays = _random_rays(lens, 1, seed=0)
+    ok = [0]
+    for level in (1, 2):
+        try:
+            trace_ghost_rays(lens, 0, 1, rays, level=level)
+            ok.append(level)
+        except ValueError as e:
+            assert "does not handle this lens" in str(e)
+    return ok
+
+
+@pytest.mark.parametrize("name", ["c1", "c2", "c4"])
+def test_lean_levels_accepted(name):
+    # These lenses are spheres and flats (c4 adds cylinders), so a render
+    # runs a predicated lean step on them; level 2 handles all that level 1 does.
+    levels = _lean_levels(LENSES[name])
+    assert 2 in levels
+    assert levels == ([0, 2] if name == "c4" else [0, 1, 2]), levels
+    with pytest.raises(ValueError):
+        trace_ghost_rays(LENSES[name], 0, 1, _random_rays(LENSES[name], 1, seed=0), level=3)
+
+
 @pytest.mark.parametrize("name", ["c1", "c2", "c4"])
 @pytest.mark.parametrize("t_ge0", [False, True])
-def test_single_ray_trace_bitexact(name, t_ge0):
+@pytest.mark.parametrize("level", [0, -1, 1, 2])
+def test_single_ray_trace_bitexact(name, t_ge0, level):
+    """Every tracer instantiation (generic step, each predicated lean level
+    with the lens's coating class) lands each ray bit-equal to the oracle."""
     lens = LENSES[name]
+    if level > 0 and level not in _lean_levels(lens):
+        pytest.skip(f"lean level {level} does not handle lens {name}")
     n_s = lens.num_surfaces()
     rays = _random_rays(lens, 600, seed=sum(map(ord, name)))
     pairs = [(0, 1), (1, n_s - 1), (2, n_s // 2), (n_s // 3, n_s - 2), (n_s - 3, n_s - 1)]
     for (a, b) in pairs + [(-1, -1)]:
-        pos, w, st, sg = trace_ghost_rays(lens, a, b, rays, t_ge0=t_ge0, fp64_radius_min=0.0)
+        pos, w, st, sg = trace_ghost_rays(lens, a, b, rays, t_ge0=t_ge0, fp64_radius_min=0.0, level=level)
         opos, ow, ost, osg = pyoracle.trace_ghost(lens, a, b, rays, dbl=False, t_ge0=t_ge0)
-        print(f"{name} pair ({a},{b}): {int((st == 0).sum())} OK of {len(st)}", flush=True)
+        print(f"{name} level {level} pair ({a},{b}): {int((st == 0).sum())} OK of {len(st)}", flush=True)
         assert np.array_equal(st, ost), f"status mismatch for pair {(a, b)}"
         assert np.array_equal(sg, osg), "segment counts differ"
         ok = st == 0
