@@ -224,6 +224,62 @@ __device__ __forceinline__ const GhostArgs GLC* kernel_args() {
     return (const GhostArgs GLC*)__builtin_amdgcn_kernarg_segment_ptr();
 }
 
+// One (pupil sample, wavelength) trace of a deposit pass: true with the
+// deposit record in rec when the landing deposits (ghost_render.cu:372-462).
+// Only the wavelength is needed during the trace; the colour weights and the
+// launch's scalars are read after it (through fresh(Ap)), so they hold no
+// registers across the surface loop.
+template <bool H, int LF, int CM>
+__device__ __forceinline__ bool trace_record(const GhostArgs& A, const GhostArgs GLC* Ap, const gl_pair& pr,
+                                             int pair_idx, int src_idx, const SrcDev& sr, F3 o0, F3 d0, float w_ray,
+                                             uint32_t hray, int si, float4& rec) {
+    const float GLC* spf = as_const(reinterpret_cast<const float*>(A.spec)) + 4 * si;
+    const float lam = spf[0];
+    uint32_t hs = 0u;
+    const F3 dl = H ? hurb_entry<H>(A, sr, o0, d0, lam, hray, si, hs) : d0;
+    TraceRes res = trace_ghost<true, H, LF, CM, true>(as_const(A.S), A.n, as_const(A.poly), A.arena, pr.surf_a,
+                                                  pr.surf_b, o0, dl, lam, as_const(A.iat) + (size_t)si * A.n,
+                                                  as_const(A.ibf) + (size_t)si * A.n, false,
+                                                  A.lrow ? as_const(A.lrow) + (size_t)si * A.n : nullptr, &A.hurb,
+                                                  &hs, as_const(A.rat) + (size_t)si * A.n);
+    if (res.status != GL_TRACE_OK) return false;
+    if (!isfinite(res.px) || !isfinite(res.py)) return false;
+    const GhostArgs GLC& B = *fresh(Ap);
+    const SrcDev GLC& srb = as_const(B.src)[src_idx];
+    float base = res.w * w_ray * B.gain * as_const(B.pairs)[pair_idx].area_boost;
+    if (base < 1e-14f) return false;
+    float px = (res.px / (2.0f * B.sw) + 0.5f) * B.fmt_w + B.x0;
+    float py = (res.py / (2.0f * B.sh) + 0.5f) * B.fmt_h + B.y0;
+    if (!isfinite(px) || !isfinite(py)) return false;
+    px = fmaxf(-2.0e9f, fminf(2.0e9f, px));
+    py = fmaxf(-2.0e9f, fminf(2.0e9f, py));
+    const int x0 = (int)floorf(px - 0.5f), y0 = (int)floorf(py - 0.5f);
+    if (x0 + 1 < 0 || x0 >= B.width || y0 + 1 < 0 || y0 >= B.height) return false;   // footprint off the image
+    const float GLC* spc = as_const(reinterpret_cast<const float*>(B.spec)) + 4 * si;
+    const float cr = spc[1], cg = spc[2], cb = spc[3];
+    const float P = srb.r * cr + srb.g * cg + srb.b * cb;
+    const int frame = B.frame_mode == 2 ? pair_idx : (B.frame_mode ? src_idx : 0);
+    rec = make_float4(px, py, P * base, __int_as_float(si | (frame << 8)));
+    return true;
+}
+
+// Stage the records of the wave's depositing lanes (a ballot and a prefix
+// count give each its slot), flushing first when they would overflow.
+__device__ __forceinline__ void stage_records(const GhostArgs& A, float4* stage, int& staged, int lane, bool dep,
+                                              const float4& rec) {
+    const unsigned long long bal = __ballot(dep);
+    const int cnt = (int)__popcll(bal);
+    if (cnt) {
+        if (staged + cnt > kStage) { flush_stage(A, stage, staged, lane); staged = 0; }
+        if (dep) {
+            const int slot = staged + (int)__builtin_amdgcn_mbcnt_hi((unsigned)(bal >> 32),
+                                                                    __builtin_amdgcn_mbcnt_lo((unsigned)bal, 0u));
+            stage[slot] = rec;
+        }
+        staged += cnt;
+    }
+}
+
 template <bool H, int LF = 0, int CM = CC_ALL>
 __device__ __forceinline__ void deposit_range(const GhostArgs& A, int ps, int gi, int si0, int si1, float4* stage,
                                               int& staged, int lane) {
@@ -239,54 +295,47 @@ __device__ __forceinline__ void deposit_range(const GhostArgs& A, int ps, int gi
     if (A.baffles.n > 0 && live && baffles_block(A.baffles, o0.x, o0.y, sr.bx, sr.by)) live = false;
     const uint32_t hray = H ? hurb_ray_seed(A, sr, gi, pair_idx) : 0u;
     for (int si = si0; si < si1; ++si) {
-        bool dep = false;
         float4 rec = make_float4(0.f, 0.f, 0.f, 0.f);
-        if (live) do {
-            // Only the wavelength is needed during the trace; the colour
-            // weights are read after it, so they hold no registers across
-            // the surface loop.
-            const float GLC* spf = as_const(reinterpret_cast<const float*>(A.spec)) + 4 * si;
-            const float lam = spf[0];
-            uint32_t hs = 0u;
-            const F3 dl = H ? hurb_entry<H>(A, sr, o0, d0, lam, hray, si, hs) : d0;
-            TraceRes res = trace_ghost<true, H, LF, CM, true>(as_const(A.S), A.n, as_const(A.poly), A.arena,
-                                                          pr.surf_a, pr.surf_b, o0, dl, lam,
-                                                          as_const(A.iat) + (size_t)si * A.n,
-                                                          as_const(A.ibf) + (size_t)si * A.n, false,
-                                                          A.lrow ? as_const(A.lrow) + (size_t)si * A.n : nullptr,
-                                                          &A.hurb, &hs, as_const(A.rat) + (size_t)si * A.n);
-            if (res.status != GL_TRACE_OK) break;
-            if (!isfinite(res.px) || !isfinite(res.py)) break;
-            // After the trace: arguments re-read, not carried through it.
-            const GhostArgs GLC& B = *fresh(Ap);
-            const SrcDev GLC& srb = as_const(B.src)[src_idx];
-            float base = res.w * w_ray * B.gain * as_const(B.pairs)[pair_idx].area_boost;
-            if (base < 1e-14f) break;
-            float px = (res.px / (2.0f * B.sw) + 0.5f) * B.fmt_w + B.x0;
-            float py = (res.py / (2.0f * B.sh) + 0.5f) * B.fmt_h + B.y0;
-            if (!isfinite(px) || !isfinite(py)) break;
-            px = fmaxf(-2.0e9f, fminf(2.0e9f, px));
-            py = fmaxf(-2.0e9f, fminf(2.0e9f, py));
-            const int x0 = (int)floorf(px - 0.5f), y0 = (int)floorf(py - 0.5f);
-            if (x0 + 1 < 0 || x0 >= B.width || y0 + 1 < 0 || y0 >= B.height) break;   // footprint off the image
-            const float GLC* spc = as_const(reinterpret_cast<const float*>(B.spec)) + 4 * si;
-            const float cr = spc[1], cg = spc[2], cb = spc[3];
-            const float P = srb.r * cr + srb.g * cg + srb.b * cb;
-            const int frame = B.frame_mode == 2 ? pair_idx : (B.frame_mode ? src_idx : 0);
-            rec = make_float4(px, py, P * base, __int_as_float(si | (frame << 8)));
-            dep = true;
-        } while (false);
-        const unsigned long long bal = __ballot(dep);
-        const int cnt = (int)__popcll(bal);
-        if (cnt) {
-            if (staged + cnt > kStage) { flush_stage(A, stage, staged, lane); staged = 0; }
-            if (dep) {
-                const int slot = staged + (int)__builtin_amdgcn_mbcnt_hi((unsigned)(bal >> 32),
-                                                                        __builtin_amdgcn_mbcnt_lo((unsigned)bal, 0u));
-                stage[slot] = rec;
-            }
-            staged += cnt;
-        }
+        const bool dep = live && trace_record<H, LF, CM>(A, Ap, pr, pair_idx, src_idx, sr, o0, d0, w_ray, hray, si, rec);
+        stage_records(A, stage, staged, lane, dep, rec);
+    }
+}
+
+// The same traces with the wavelengths across lanes: the wave's 64 pupil
+// samples (one per lane, as deposit_range takes them) times the L = si1 - si0
+// wavelengths are 64 L (sample, wavelength) traces, run in L passes of 64;
+// trace q = 64 p + lane of pass p is sample lane q / L at wavelength
+// si0 + q % L, its origin and weight fetched from that sample's lane.  The
+// lanes of one pass hold a few neighbouring samples at consecutive
+// wavelengths, which mostly meet the same fate, so a pass lasts as long as
+// its longest few samples rather than the longest of 64.
+template <int LF, int CM>
+__device__ __forceinline__ void deposit_lanes(const GhostArgs& A, int ps, int gi, int si0, int si1, float4* stage,
+                                              int& staged, int lane) {
+    const GhostArgs GLC* Ap = kernel_args();
+    const int pair_idx = ps / A.n_src, src_idx = ps % A.n_src;
+    const gl_pair pr = A.pairs[pair_idx];
+    float u = 0.f, v = 0.f, w_ray = 0.f;
+    bool live = gi < A.n_grid;
+    if (live) live = sample_at(A, ps, gi, u, v, w_ray);
+    const SrcDev sr = A.src[src_idx];
+    const float ox = u * A.front_R + (-20.0f * sr.bx), oy = v * A.front_R + (-20.0f * sr.by);
+    if (A.baffles.n > 0 && live && baffles_block(A.baffles, ox, oy, sr.bx, sr.by)) live = false;
+    const F3 d0 = f3(sr.dx, sr.dy, sr.dz);
+    const int L = si1 - si0;
+    if (L <= 0) return;
+    // (sl, r) = (q / L, q % L), advanced by 64 = dq L + mq per pass.
+    const int dq = 64 / L, mq = 64 - dq * L;
+    int sl = lane / L, r = lane - (lane / L) * L;
+    for (int p = 0; p < L; ++p) {
+        const float qx = __shfl(ox, sl, 64), qy = __shfl(oy, sl, 64), qw = __shfl(w_ray, sl, 64);
+        const bool ql = __shfl((int)live, sl, 64) != 0;
+        float4 rec = make_float4(0.f, 0.f, 0.f, 0.f);
+        const bool dep = ql && trace_record<false, LF, CM>(A, Ap, pr, pair_idx, src_idx, sr, f3(qx, qy, A.start_z),
+                                                          d0, qw, 0u, si0 + r, rec);
+        stage_records(A, stage, staged, lane, dep, rec);
+        sl += dq; r += mq;
+        if (r >= L) { r -= L; ++sl; }
     }
 }
 
@@ -321,7 +370,7 @@ void ghost_persist_kernel(GhostArgs A, PersistQ Q) {
         __syncthreads();
         if (item >= n_items) break;
         const int4 w = Q.work[item];
-        deposit_range<false, LF, CM>(A, w.x, w.y * kBlock + (int)threadIdx.x, w.z, w.w, stage, staged, lane);
+        deposit_lanes<LF, CM>(A, w.x, w.y * kBlock + (int)threadIdx.x, w.z, w.w, stage, staged, lane);
     }
     if (staged) flush_stage(A, stage, staged, lane);
     __syncthreads();
