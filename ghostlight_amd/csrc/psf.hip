@@ -17,7 +17,7 @@
 
 namespace gl {
 
-static constexpr int kPsfBlock = 256;
+static constexpr int kPsfBlock = 1024;
 static constexpr size_t kPsfMaxLdsTile = 128 * 1024;
 
 struct PsfArgs {
@@ -126,15 +126,25 @@ __global__ __launch_bounds__(kPsfBlock) void psf_kernel(PsfArgs A) {
                 }
                 const float wsum = sx * sy;
                 const float inv_wsum = (wsum > 1e-20f) ? (1.0f / wsum) : 0.0f;
+                // Cell weights from the hardware base-2 exponential:
+                // exp(t) = 2^(t log2 e), within a few 1e-7 relative for the
+                // |t| <= 2 (2.5 + 1)^2 / 2 this footprint reaches.  Each lane
+                // walks its row from its own column (lane mod (2R + 1)), so
+                // the wave's LDS atomics spread over the footprint instead
+                // of landing on the same few cells at once.
+                const int W2 = 2 * R + 1;
+                const int dx0 = -R + (int)(threadIdx.x & 63) % W2;
                 for (int dy = -R; dy <= R; ++dy) {
                     const int yi = cy + dy;
                     if (yi < ty_min || yi >= ty_max || yi < 0 || yi >= A.comp_h) continue;
                     const float dyr = (float)dy - oy;
-                    for (int dx = -R; dx <= R; ++dx) {
+                    int dx = dx0;
+                    for (int j = 0; j < W2; ++j) {
                         const float dxr = (float)dx - ox;
-                        const float wt = expf(-(dxr * dxr + dyr * dyr) * inv_2sig2) * inv_wsum;
-                        if (wt < 1e-6f) continue;
-                        put(cx + dx, yi, wt);
+                        const float t = -(dxr * dxr + dyr * dyr) * inv_2sig2;
+                        const float wt = __builtin_amdgcn_exp2f(t * 1.44269504088896341f) * inv_wsum;
+                        if (wt >= 1e-6f) put(cx + dx, yi, wt);
+                        dx = dx == R ? -R : dx + 1;
                     }
                 }
             }
