@@ -80,6 +80,7 @@ struct GhostArgs {
     unsigned long long* __restrict__ segs;      // optional segment counter
     const int2* __restrict__ lrow;              // SPLAT: per-(lambda, surface) coating lookups
     const float2* __restrict__ rat;             // per-(lambda, surface) refraction ratios (trace_ghost)
+    const LamRec* __restrict__ lrec;            // iat, ibf, ratios and lrow interleaved (trace_ghost_rec)
     int lam_group;                              // STATS: lanes per sample (one lambda each), 0 = loop
     int frame_mode;                             // SPLAT: records carry a frame: 1 = source, 2 = pair
     float4* __restrict__ rec;                   // SPLAT: deposit records
@@ -229,19 +230,28 @@ __device__ __forceinline__ const GhostArgs GLC* kernel_args() {
 // Only the wavelength is needed during the trace; the colour weights and the
 // launch's scalars are read after it (through fresh(Ap)), so they hold no
 // registers across the surface loop.
-template <bool H, int LF, int CM>
+// REC: the wavelength's tables come from the interleaved records (the
+// lambda-across-lanes form, no HURB).
+template <bool H, int LF, int CM, bool REC = false>
 __device__ __forceinline__ bool trace_record(const GhostArgs& A, const GhostArgs GLC* Ap, const gl_pair& pr,
                                              int pair_idx, int src_idx, const SrcDev& sr, F3 o0, F3 d0, float w_ray,
                                              uint32_t hray, int si, float4& rec) {
     const float GLC* spf = as_const(reinterpret_cast<const float*>(A.spec)) + 4 * si;
     const float lam = spf[0];
-    uint32_t hs = 0u;
-    const F3 dl = H ? hurb_entry<H>(A, sr, o0, d0, lam, hray, si, hs) : d0;
-    TraceRes res = trace_ghost<true, H, LF, CM, true>(as_const(A.S), A.n, as_const(A.poly), A.arena, pr.surf_a,
-                                                  pr.surf_b, o0, dl, lam, as_const(A.iat) + (size_t)si * A.n,
-                                                  as_const(A.ibf) + (size_t)si * A.n, false,
-                                                  A.lrow ? as_const(A.lrow) + (size_t)si * A.n : nullptr, &A.hurb,
-                                                  &hs, as_const(A.rat) + (size_t)si * A.n);
+    TraceRes res;
+    if constexpr (REC) {
+        static_assert(!H, "the record form has no HURB");
+        res = trace_ghost_rec<LF, CM>(as_const(A.S), A.n, as_const(A.poly), A.arena, pr.surf_a, pr.surf_b, o0, d0,
+                                      lam, as_const(A.lrec) + (size_t)si * A.n, false);
+    } else {
+        uint32_t hs = 0u;
+        const F3 dl = H ? hurb_entry<H>(A, sr, o0, d0, lam, hray, si, hs) : d0;
+        res = trace_ghost<true, H, LF, CM, true>(as_const(A.S), A.n, as_const(A.poly), A.arena, pr.surf_a, pr.surf_b,
+                                                 o0, dl, lam, as_const(A.iat) + (size_t)si * A.n,
+                                                 as_const(A.ibf) + (size_t)si * A.n, false,
+                                                 A.lrow ? as_const(A.lrow) + (size_t)si * A.n : nullptr, &A.hurb, &hs,
+                                                 as_const(A.rat) + (size_t)si * A.n);
+    }
     if (res.status != GL_TRACE_OK) return false;
     if (!isfinite(res.px) || !isfinite(res.py)) return false;
     const GhostArgs GLC& B = *fresh(Ap);
@@ -331,8 +341,8 @@ __device__ __forceinline__ void deposit_lanes(const GhostArgs& A, int ps, int gi
         const float qx = __shfl(ox, sl, 64), qy = __shfl(oy, sl, 64), qw = __shfl(w_ray, sl, 64);
         const bool ql = __shfl((int)live, sl, 64) != 0;
         float4 rec = make_float4(0.f, 0.f, 0.f, 0.f);
-        const bool dep = ql && trace_record<false, LF, CM>(A, Ap, pr, pair_idx, src_idx, sr, f3(qx, qy, A.start_z),
-                                                          d0, qw, 0u, si0 + r, rec);
+        const bool dep = ql && trace_record<false, LF, CM, true>(A, Ap, pr, pair_idx, src_idx, sr,
+                                                                f3(qx, qy, A.start_z), d0, qw, 0u, si0 + r, rec);
         stage_records(A, stage, staged, lane, dep, rec);
         sl += dq; r += mq;
         if (r >= L) { r -= L; ++sl; }
@@ -570,25 +580,29 @@ __global__ void filter_probe_kernel(const DevSurf* __restrict__ S, const float* 
 }
 
 // Single-ray traces for tests and probes.  <0, CC_ALL, false> is the generic
-// surface step; the lean levels (1, 2) with a coating class and RT take the
-// per-ray refraction ratios (rat) and coating rows (lrow) exactly as the
-// render kernels take the per-lambda ones, so they run the instantiation a
+// surface step; the lean levels (1, 2) with a coating class and RT read each
+// ray's wavelength tables from interleaved records (lrec) exactly as the
+// persistent render kernel reads the per-lambda ones, so they run the trace a
 // render of this lens runs.
 template <int LF, int CM, bool RT>
 __global__ void ray_kernel(const DevSurf* __restrict__ S, const float* __restrict__ poly, const float* __restrict__ arena,
                            int n, int a, int b, const float* __restrict__ rays, int n_rays, const float* __restrict__ iat,
-                           const float* __restrict__ ibf, const float2* __restrict__ rat, const int2* __restrict__ lrow,
-                           int t_ge0, float* __restrict__ pos, float* __restrict__ w, int* __restrict__ st,
+                           const float* __restrict__ ibf, const LamRec* __restrict__ lrec, int t_ge0,
+                           float* __restrict__ pos, float* __restrict__ w, int* __restrict__ st,
                            int* __restrict__ segs) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n_rays) return;
     const float* q = rays + 7 * (size_t)i;
     const size_t row = (size_t)i * n;
     // Per-ray IOR slices: lane-varying, so plain global pointers.
-    TraceRes r = (b >= 0) ? trace_ghost<true, false, LF, CM, RT>(
-                                as_const(S), n, as_const(poly), arena, a, b, f3(q[0], q[1], q[2]), f3(q[3], q[4], q[5]),
-                                q[6], as_const(iat + row), as_const(ibf + row), t_ge0 != 0,
-                                RT ? as_const(lrow + row) : nullptr, nullptr, nullptr, RT ? as_const(rat + row) : nullptr)
+    TraceRes r;
+    if (b >= 0 && RT)
+        r = trace_ghost_rec<LF, CM>(as_const(S), n, as_const(poly), arena, a, b, f3(q[0], q[1], q[2]),
+                                    f3(q[3], q[4], q[5]), q[6], as_const(lrec + row), t_ge0 != 0);
+    else
+        r = (b >= 0) ? trace_ghost<true, false, LF, CM, false>(as_const(S), n, as_const(poly), arena, a, b,
+                                                               f3(q[0], q[1], q[2]), f3(q[3], q[4], q[5]), q[6],
+                                                               as_const(iat + row), as_const(ibf + row), t_ge0 != 0)
                           : trace_primary<true, LF>(as_const(S), n, as_const(poly), arena, f3(q[0], q[1], q[2]),
                                                     f3(q[3], q[4], q[5]), q[6], as_const(iat + row), t_ge0 != 0, nullptr);
     pos[3 * i] = r.px; pos[3 * i + 1] = r.py; pos[3 * i + 2] = r.pz;
@@ -690,6 +704,32 @@ static int launch(const GhostArgs& A_in, int n_ps, int n_samples, hipStream_t s,
     else hipLaunchKernelGGL((ghost_kernel<STATS, SPLAT, COUNT, false>), grid, dim3(kBlock), 0, s, A);
     GL_HIP(hipGetLastError());
     return GL_OK;
+}
+
+// The interleaved per-(lambda, surface) records (LamRec) from the separate
+// tables: lrows is lambda_rows' output, the iat.size() entries followed by the
+// coating rows.  The rows follow the records here, and each entry's
+// self-relative row offset is re-based to its record's lrow field (word
+// 8q + 4 of the record array; rows start on a whole record, so they stay
+// 8-byte aligned).
+static std::vector<LamRec> lam_records(const std::vector<float>& iat, const std::vector<float>& ibf,
+                                       const std::vector<float2>& rats, const std::vector<int32_t>& lrows) {
+    const size_t m = iat.size();
+    const size_t row_words = lrows.size() - 2 * m;
+    std::vector<LamRec> out(m + (row_words + 7) / 8);
+    for (size_t q = 0; q < m; ++q) {
+        LamRec& r = out[q];
+        r.iat = iat[q]; r.ibf = ibf[q]; r.rfw = rats[q].x; r.rbw = rats[q].y;
+        int2 e = make_int2(lrows[2 * q], lrows[2 * q + 1]);
+        if (e.x >= 0) {      // a row: old word 2q + e.x, past the 2m entry words
+            const size_t row = 8 * m + (2 * q + (size_t)e.x - 2 * m);
+            e.x = (int)(row - (8 * q + 4));
+        }
+        r.lrow = e;
+        r.pad = make_int2(0, 0);
+    }
+    std::memcpy(reinterpret_cast<int32_t*>(out.data()) + 8 * m, lrows.data() + 2 * m, row_words * sizeof(int32_t));
+    return out;
 }
 
 struct Launch {
@@ -802,13 +842,14 @@ static int do_launch(Launch& Lc, float* out_r, float* out_g, float* out_b, float
     const int rns = std::min(3, ns);
     const int li[3] = {0, ns / 2, ns - 1};
     DevBuf &b_iat = c->buf("iat"), &b_ibf = c->buf("ibf"), &b_lrow = c->buf("lrow"), &b_rat = c->buf("rat");
+    DevBuf& b_lrec = c->buf("lrec");
     DevBuf &b_s3 = c->buf("spec3"), &b_a3 = c->buf("iat3"), &b_b3 = c->buf("ibf3"), &b_r3 = c->buf("rat3");
     {
         std::vector<uint8_t> tk;
         key_append(tk, L->uid);
         tk.insert(tk.end(), c->spec_key.begin(), c->spec_key.end());
-        if (tk != c->ltab_key || !b_iat.p || !b_ibf.p || !b_lrow.p || !b_rat.p || !b_s3.p || !b_a3.p || !b_b3.p ||
-            !b_r3.p) {
+        if (tk != c->ltab_key || !b_iat.p || !b_ibf.p || !b_lrow.p || !b_rat.p || !b_lrec.p || !b_s3.p || !b_a3.p ||
+            !b_b3.p || !b_r3.p) {
             c->ltab_key.clear();
             std::vector<float> iat((size_t)ns * n), ibf((size_t)ns * n);
             ior_tables(L->host.data(), n, c->spec.data(), ns, iat.data(), ibf.data());
@@ -816,6 +857,7 @@ static int do_launch(Launch& Lc, float* out_r, float* out_g, float* out_b, float
             lambda_rows(L->host.data(), n, L->arena.data(), c->spec.data(), ns, lrows);
             std::vector<float2> rats((size_t)ns * n);
             for (size_t q = 0; q < rats.size(); ++q) rats[q] = make_float2(ibf[q] / iat[q], iat[q] / ibf[q]);
+            const std::vector<LamRec> lrec = lam_records(iat, ibf, rats, lrows);
             std::vector<float4> s3(rns);
             std::vector<float> a3((size_t)rns * n), b3((size_t)rns * n);
             std::vector<float2> r3((size_t)rns * n);
@@ -831,6 +873,7 @@ static int do_launch(Launch& Lc, float* out_r, float* out_g, float* out_b, float
                 (st = upload(c, "ibf", ibf.data(), ibf.size() * 4, &d)) ||
                 (st = upload(c, "lrow", lrows.data(), lrows.size() * sizeof(int32_t), &d)) ||
                 (st = upload(c, "rat", rats.data(), rats.size() * sizeof(float2), &d)) ||
+                (st = upload(c, "lrec", lrec.data(), lrec.size() * sizeof(LamRec), &d)) ||
                 (st = upload(c, "spec3", s3.data(), rns * sizeof(float4), &d)) ||
                 (st = upload(c, "iat3", a3.data(), a3.size() * 4, &d)) ||
                 (st = upload(c, "ibf3", b3.data(), b3.size() * 4, &d)) ||
@@ -865,6 +908,7 @@ static int do_launch(Launch& Lc, float* out_r, float* out_g, float* out_b, float
     A.iat = static_cast<const float*>(d_iat); A.ibf = static_cast<const float*>(d_ibf);
     A.lrow = static_cast<const int2*>(d_lrow);
     A.rat = static_cast<const float2*>(d_rat);
+    A.lrec = static_cast<const LamRec*>(b_lrec.p);
     A.conc.rects = nullptr;
     A.hurb.on = p->hurb ? 1 : 0;
     A.hurb.dist = p->hurb_kick;
@@ -1586,7 +1630,7 @@ int gl_trace_ghost_rays_level(gl_ctx* ctx, const gl_lens* lens, int32_t a, int32
     for (int i = 0; i < n_rays; ++i) sp[i] = gl_spec{rays[7 * (size_t)i + 6], 0, 0, 0};
     std::vector<float> iat((size_t)n_rays * n), ibf((size_t)n_rays * n);
     ior_tables(L->host.data(), n, sp.data(), n_rays, iat.data(), ibf.data());
-    void *d_r, *d_a, *d_b, *d_rat = nullptr, *d_lrow = nullptr;
+    void *d_r, *d_a, *d_b, *d_rec = nullptr;
     if ((st = upload(c, "r_rays", rays, (size_t)n_rays * 7 * 4, &d_r)) ||
         (st = upload(c, "r_iat", iat.data(), iat.size() * 4, &d_a)) ||
         (st = upload(c, "r_ibf", ibf.data(), ibf.size() * 4, &d_b)))
@@ -1596,9 +1640,8 @@ int gl_trace_ghost_rays_level(gl_ctx* ctx, const gl_lens* lens, int32_t a, int32
         for (size_t q = 0; q < rats.size(); ++q) rats[q] = make_float2(ibf[q] / iat[q], iat[q] / ibf[q]);
         std::vector<int32_t> lrows;
         lambda_rows(L->host.data(), n, L->arena.data(), sp.data(), n_rays, lrows);
-        if ((st = upload(c, "r_rat", rats.data(), rats.size() * sizeof(float2), &d_rat)) ||
-            (st = upload(c, "r_lrow", lrows.data(), lrows.size() * sizeof(int32_t), &d_lrow)))
-            return st;
+        const std::vector<LamRec> lrec = lam_records(iat, ibf, rats, lrows);
+        if ((st = upload(c, "r_lrec", lrec.data(), lrec.size() * sizeof(LamRec), &d_rec))) return st;
     }
     DevBuf &bp = c->buf("r_pos"), &bw = c->buf("r_w"), &bs = c->buf("r_st"), &bg = c->buf("r_seg");
     if ((st = bp.ensure((size_t)n_rays * 12)) || (st = bw.ensure((size_t)n_rays * 4)) ||
@@ -1607,12 +1650,11 @@ int gl_trace_ghost_rays_level(gl_ctx* ctx, const gl_lens* lens, int32_t a, int32
     const dim3 gr((unsigned)((n_rays + 255) / 256)), bl(256);
     const float *fr = static_cast<const float*>(d_r), *fa = static_cast<const float*>(d_a),
                 *fb = static_cast<const float*>(d_b);
-    const float2* rt = static_cast<const float2*>(d_rat);
-    const int2* lr = static_cast<const int2*>(d_lrow);
+    const LamRec* lq = static_cast<const LamRec*>(d_rec);
     const int cc = D.coat_classes;
 #define GL_RAY_LAUNCH(LF_, CM_, RT_)                                                                                 \
     hipLaunchKernelGGL((ray_kernel<LF_, CM_, RT_>), gr, bl, 0, c->stream, D.surf, D.poly, D.arena, n, a, b, fr,      \
-                       n_rays, fa, fb, rt, lr, t_ge0, bp.as<float>(), bw.as<float>(), bs.as<int>(), bg.as<int>())
+                       n_rays, fa, fb, lq, t_ge0, bp.as<float>(), bw.as<float>(), bs.as<int>(), bg.as<int>())
     if (lf == 1 && cc == (CC_SA | CC_FRESNEL)) GL_RAY_LAUNCH(1, CC_SA | CC_FRESNEL, true);
     else if (lf == 1 && cc == (CC_AIRY | CC_FRESNEL)) GL_RAY_LAUNCH(1, CC_AIRY | CC_FRESNEL, true);
     else if (lf == 1) GL_RAY_LAUNCH(1, CC_ALL, true);

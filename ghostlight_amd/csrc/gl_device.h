@@ -85,6 +85,18 @@ struct DevSurf {
     int32_t sdf_off;                 // image stop: [d, nx, ny, 0] texels at arena[sdf_off]; -1 none
 };
 
+// One wavelength's tables for one surface, interleaved so that a trace with
+// a lane-varying wavelength reaches all of them from one address per surface:
+// the indices after (iat) and before (ibf) the surface, the refraction ratios
+// forward (ibf / iat) and backward (iat / ibf), and the coating-row entry of
+// lambda_rows, whose self-relative row offset counts from this lrow field.
+struct __attribute__((aligned(16))) LamRec {
+    float iat, ibf, rfw, rbw;
+    int2 lrow;
+    int2 pad;
+};
+static_assert(sizeof(LamRec) == 32, "LamRec is two 16-byte loads");
+
 struct TraceRes {
     float px, py, pz, w;
     int32_t status, segs;
@@ -1135,6 +1147,41 @@ GLD TraceRes trace_ghost(const DevSurf GLC* S, int n, const float GLC* poly, con
             const float ed = surface_edge(s, o, en, arena);
             hurb_apply(*hb, d, en, ed, lam, *hs);
         }
+    }
+    to_sensor(o, d, t_ge0, res);
+    return res;
+}
+
+// The same trace reading this wavelength's tables from interleaved records
+// (LamRec, one per surface): the index, ratio and coating row of surface k
+// come from one address, which is what a lane-varying wavelength needs.
+// Always the refraction-ratio (RT) form.
+template <int LF, int CM>
+GLD TraceRes trace_ghost_rec(const DevSurf GLC* S, int n, const float GLC* poly, const float* __restrict__ arena,
+                             int a, int b, F3 o, F3 d, float lam, const LamRec GLC* rec, bool t_ge0) {
+    TraceRes res; res.status = GL_TRACE_VIGNETTED; res.w = 1.0f; res.segs = 0; res.px = res.py = res.pz = 0.f;
+    float cur = 1.0f;
+    const float refl_a = rec[a].iat;
+    const int end1 = b + 1, end2 = end1 + (b - a), end3 = end2 + (n - 1 - a);
+    for (int i = 0; i < end3; ++i) {
+        int k;
+        float rior;
+        bool refl, back;
+        if (i < end1) {
+            k = i; refl = (k == b); rior = cur; back = false;
+        } else if (i < end2) {
+            k = b - 1 - (i - end1); refl = (k == a); rior = refl_a; back = true;
+        } else {
+            k = a + 1 + (i - end2); refl = false; rior = 0.0f; back = false;
+        }
+        const DevSurf GLC& s = S[k];
+        if (!(s.flags & SF_ACTIVE)) continue;
+        const LamRec GLC& q = rec[k];
+        const float GLC* t4 = reinterpret_cast<const float GLC*>(&q);
+        const float n2 = back ? t4[1] : t4[0];
+        const float ratio = back ? t4[3] : t4[2];
+        if (!step<true, LF, CM, true>(s, poly, arena, o, d, lam, n2, refl, rior, cur, res, &q.lrow, ratio))
+            return res;
     }
     to_sensor(o, d, t_ge0, res);
     return res;
