@@ -89,6 +89,54 @@ __device__ __forceinline__ bool tiles_of(const float4& r, const SplatGeom& g, in
     return true;
 }
 
+// The up-to-four tiles of a record's footprint box as a fixed sequence of
+// slots j = 0..3: (tx0, ty0), (tx1, ty0), (tx0, ty1), (tx1, ty1), a slot
+// valid only where its corner is a new tile.
+__device__ __forceinline__ bool tile_slot(int j, int tx0, int tx1, int ty0, int ty1, int base, int tiles_x, int& t) {
+    const int tx = (j & 1) ? tx1 : tx0, ty = (j & 2) ? ty1 : ty0;
+    const bool ok = ((j & 1) == 0 || tx1 > tx0) && ((j & 2) == 0 || ty1 > ty0);
+    t = base + ty * tiles_x + tx;
+    return ok;
+}
+
+// Wave-aggregated global tile updates: the active lanes that hold the same
+// tile t form one group, served by one atomic from its lowest lane.  The
+// groups are taken one at a time (the lowest pending lane's tile first); the
+// loop runs once per distinct tile among the wave's lanes.  COUNT adds the
+// group's size to cnt[t] and raises mag[t] to the group's largest m; CURSOR
+// reserves the group's slots in cur[t] and returns each lane's own slot.
+template <bool CURSOR>
+__device__ __forceinline__ unsigned long long wave_tile_update(bool valid, int t, unsigned m, unsigned* cnt,
+                                                               unsigned* mag, unsigned long long* cur) {
+    const int lane = (int)(threadIdx.x & 63);
+    unsigned long long pending = __ballot(valid);
+    unsigned long long slot = 0;
+    while (pending) {
+        const int leader = __builtin_ctzll(pending);
+        const int tl = __shfl(t, leader, 64);
+        const bool mine = valid && t == tl;
+        const unsigned long long grp = __ballot(mine);
+        const unsigned n = (unsigned)__popcll(grp);
+        if (CURSOR) {
+            unsigned long long b = 0;
+            if (lane == leader) b = atomicAdd(&cur[tl], (unsigned long long)n);
+            b = __shfl(b, leader, 64);
+            const unsigned rank = __builtin_amdgcn_mbcnt_hi((unsigned)(grp >> 32),
+                                                            __builtin_amdgcn_mbcnt_lo((unsigned)grp, 0u));
+            if (mine) slot = b + rank;
+        } else {
+            unsigned mm = mine ? m : 0u;
+            for (int o = 32; o > 0; o >>= 1) mm = max(mm, (unsigned)__shfl_xor((int)mm, o, 64));
+            if (lane == leader) {
+                atomicAdd(&cnt[tl], n);
+                atomicMax(&mag[tl], mm);
+            }
+        }
+        pending &= ~grp;
+    }
+    return slot;
+}
+
 // Records a thread loads before it processes them in the binning passes:
 // several loads in flight per lane.  A thread still takes its records in the
 // order r0 + tid, r0 + tid + blockDim, ..., so the passes see the same
@@ -151,17 +199,42 @@ __global__ __launch_bounds__(kBinThreads) void bin_count_kernel(SplatBufs B, Spl
     const unsigned long long n = n_records(B);
     const unsigned long long per = (n + gridDim.x - 1) / gridDim.x;
     const unsigned long long r0 = per * blockIdx.x, r1 = min(n, r0 + per);
-    for_records(B.rec, r0, r1, [&](const float4& r) {
-        int tx0, tx1, ty0, ty1, base;
-        if (!tiles_of(r, g, tx0, tx1, ty0, ty1, base)) return;
-        const unsigned m = record_mag(r, spec);
-        for (int ty = ty0; ty <= ty1; ++ty)
-            for (int tx = tx0; tx <= tx1; ++tx) {
-                const int t = base + ty * g.tiles_x + tx;
-                atomicAdd(&h[t], 1u);
-                atomicMax(&hm[t], m);
+    if (LDS) {
+        for_records(B.rec, r0, r1, [&](const float4& r) {
+            int tx0, tx1, ty0, ty1, base;
+            if (!tiles_of(r, g, tx0, tx1, ty0, ty1, base)) return;
+            const unsigned m = record_mag(r, spec);
+            for (int ty = ty0; ty <= ty1; ++ty)
+                for (int tx = tx0; tx <= tx1; ++tx) {
+                    const int t = base + ty * g.tiles_x + tx;
+                    atomicAdd(&h[t], 1u);
+                    atomicMax(&hm[t], m);
+                }
+        });
+    } else {
+        // Global counters, one atomic per distinct tile of the wave.  The
+        // loop bound is uniform across the workgroup, so every lane of a wave
+        // is active at each aggregation step (lanes past the range hold no
+        // tile).
+        const unsigned long long span = (r1 > r0) ? r1 - r0 : 0ull;
+        const unsigned long long steps = (span + blockDim.x - 1) / blockDim.x;
+        for (unsigned long long k = 0; k < steps; ++k) {
+            const unsigned long long i = r0 + k * blockDim.x + threadIdx.x;
+            int tx0 = 0, tx1 = 0, ty0 = 0, ty1 = 0, base = 0;
+            bool in = false;
+            unsigned m = 0u;
+            if (i < r1) {
+                const float4 r = B.rec[i];
+                in = tiles_of(r, g, tx0, tx1, ty0, ty1, base);
+                if (in) m = record_mag(r, spec);
             }
-    });
+            for (int j = 0; j < 4; ++j) {
+                int t = 0;
+                const bool ok = in && tile_slot(j, tx0, tx1, ty0, ty1, base, g.tiles_x, t);
+                wave_tile_update<false>(ok, t, m, h, hm, nullptr);
+            }
+        }
+    }
     if (LDS) {
         __syncthreads();
         unsigned* mine = B.blk_hist + (size_t)blockIdx.x * g.n_tiles;
@@ -238,16 +311,26 @@ __global__ __launch_bounds__(kBinThreads) void bin_scatter_kernel(SplatBufs B, S
     const unsigned long long per = (n + gridDim.x - 1) / gridDim.x;
     const unsigned long long r0 = per * blockIdx.x, r1 = min(n, r0 + per);
     if (!g.lds_hist) {
-        // Global form: one cursor reservation per (record, tile).
-        for (unsigned long long i = r0 + threadIdx.x; i < r1; i += blockDim.x) {
-            const float4 r = B.rec[i];
-            int tx0, tx1, ty0, ty1, fb;
-            if (!tiles_of(r, g, tx0, tx1, ty0, ty1, fb)) continue;
-            for (int ty = ty0; ty <= ty1; ++ty)
-                for (int tx = tx0; tx <= tx1; ++tx) {
-                    const unsigned long long slot = atomicAdd(&B.tile_cursor[fb + ty * g.tiles_x + tx], 1ull);
-                    if (slot < B.bin_cap) B.binned[slot] = r;
-                }
+        // Global form: cursor reservations aggregated per distinct tile of
+        // the wave (the loop bound is uniform across the workgroup, so whole
+        // waves run each step together).
+        const unsigned long long span = (r1 > r0) ? r1 - r0 : 0ull;
+        const unsigned long long steps = (span + blockDim.x - 1) / blockDim.x;
+        for (unsigned long long k = 0; k < steps; ++k) {
+            const unsigned long long i = r0 + k * blockDim.x + threadIdx.x;
+            float4 r = make_float4(0.f, 0.f, 0.f, 0.f);
+            int tx0 = 0, tx1 = 0, ty0 = 0, ty1 = 0, fb = 0;
+            bool in = false;
+            if (i < r1) {
+                r = B.rec[i];
+                in = tiles_of(r, g, tx0, tx1, ty0, ty1, fb);
+            }
+            for (int j = 0; j < 4; ++j) {
+                int t = 0;
+                const bool ok = in && tile_slot(j, tx0, tx1, ty0, ty1, fb, g.tiles_x, t);
+                const unsigned long long slot = wave_tile_update<true>(ok, t, 0u, nullptr, nullptr, B.tile_cursor);
+                if (ok && slot < B.bin_cap) B.binned[slot] = r;
+            }
         }
         return;
     }
