@@ -311,6 +311,12 @@ __device__ __forceinline__ void deposit_range(const GhostArgs& A, int ps, int gi
     }
 }
 
+// Whether the lambda-across-lanes trace reads the interleaved records: where
+// the coating weight reads coating rows (a third per-lambda table), one
+// record address serves all three; with only Airy and Fresnel weights the two
+// separate tables measured faster.
+template <int CM> static constexpr bool kRecForm = CM != (CC_AIRY | CC_FRESNEL);
+
 // The same traces with the wavelengths across lanes: the wave's 64 pupil
 // samples (one per lane, as deposit_range takes them) times the L = si1 - si0
 // wavelengths are 64 L (sample, wavelength) traces, run in L passes of 64;
@@ -341,8 +347,9 @@ __device__ __forceinline__ void deposit_lanes(const GhostArgs& A, int ps, int gi
         const float qx = __shfl(ox, sl, 64), qy = __shfl(oy, sl, 64), qw = __shfl(w_ray, sl, 64);
         const bool ql = __shfl((int)live, sl, 64) != 0;
         float4 rec = make_float4(0.f, 0.f, 0.f, 0.f);
-        const bool dep = ql && trace_record<false, LF, CM, true>(A, Ap, pr, pair_idx, src_idx, sr,
-                                                                f3(qx, qy, A.start_z), d0, qw, 0u, si0 + r, rec);
+        const bool dep = ql && trace_record<false, LF, CM, kRecForm<CM>>(A, Ap, pr, pair_idx, src_idx, sr,
+                                                                         f3(qx, qy, A.start_z), d0, qw, 0u, si0 + r,
+                                                                         rec);
         stage_records(A, stage, staged, lane, dep, rec);
         sl += dq; r += mq;
         if (r >= L) { r -= L; ++sl; }
@@ -581,13 +588,15 @@ __global__ void filter_probe_kernel(const DevSurf* __restrict__ S, const float* 
 
 // Single-ray traces for tests and probes.  <0, CC_ALL, false> is the generic
 // surface step; the lean levels (1, 2) with a coating class and RT read each
-// ray's wavelength tables from interleaved records (lrec) exactly as the
-// persistent render kernel reads the per-lambda ones, so they run the trace a
-// render of this lens runs.
+// ray's wavelength tables as the persistent render kernel reads the
+// per-lambda ones -- interleaved records (lrec) where kRecForm<CM>, else the
+// separate ratio (rat) and coating-row (lrow) tables -- so they run the trace
+// a render of this lens runs.
 template <int LF, int CM, bool RT>
 __global__ void ray_kernel(const DevSurf* __restrict__ S, const float* __restrict__ poly, const float* __restrict__ arena,
                            int n, int a, int b, const float* __restrict__ rays, int n_rays, const float* __restrict__ iat,
-                           const float* __restrict__ ibf, const LamRec* __restrict__ lrec, int t_ge0,
+                           const float* __restrict__ ibf, const LamRec* __restrict__ lrec,
+                           const float2* __restrict__ rat, const int2* __restrict__ lrow, int t_ge0,
                            float* __restrict__ pos, float* __restrict__ w, int* __restrict__ st,
                            int* __restrict__ segs) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -596,9 +605,14 @@ __global__ void ray_kernel(const DevSurf* __restrict__ S, const float* __restric
     const size_t row = (size_t)i * n;
     // Per-ray IOR slices: lane-varying, so plain global pointers.
     TraceRes r;
-    if (b >= 0 && RT)
+    if (b >= 0 && RT && kRecForm<CM>)
         r = trace_ghost_rec<LF, CM>(as_const(S), n, as_const(poly), arena, a, b, f3(q[0], q[1], q[2]),
                                     f3(q[3], q[4], q[5]), q[6], as_const(lrec + row), t_ge0 != 0);
+    else if (b >= 0 && RT)
+        r = trace_ghost<true, false, LF, CM, true>(as_const(S), n, as_const(poly), arena, a, b, f3(q[0], q[1], q[2]),
+                                                   f3(q[3], q[4], q[5]), q[6], as_const(iat + row),
+                                                   as_const(ibf + row), t_ge0 != 0, as_const(lrow + row), nullptr,
+                                                   nullptr, as_const(rat + row));
     else
         r = (b >= 0) ? trace_ghost<true, false, LF, CM, false>(as_const(S), n, as_const(poly), arena, a, b,
                                                                f3(q[0], q[1], q[2]), f3(q[3], q[4], q[5]), q[6],
@@ -1630,7 +1644,7 @@ int gl_trace_ghost_rays_level(gl_ctx* ctx, const gl_lens* lens, int32_t a, int32
     for (int i = 0; i < n_rays; ++i) sp[i] = gl_spec{rays[7 * (size_t)i + 6], 0, 0, 0};
     std::vector<float> iat((size_t)n_rays * n), ibf((size_t)n_rays * n);
     ior_tables(L->host.data(), n, sp.data(), n_rays, iat.data(), ibf.data());
-    void *d_r, *d_a, *d_b, *d_rec = nullptr;
+    void *d_r, *d_a, *d_b, *d_rec = nullptr, *d_rat = nullptr, *d_lrow = nullptr;
     if ((st = upload(c, "r_rays", rays, (size_t)n_rays * 7 * 4, &d_r)) ||
         (st = upload(c, "r_iat", iat.data(), iat.size() * 4, &d_a)) ||
         (st = upload(c, "r_ibf", ibf.data(), ibf.size() * 4, &d_b)))
@@ -1641,7 +1655,10 @@ int gl_trace_ghost_rays_level(gl_ctx* ctx, const gl_lens* lens, int32_t a, int32
         std::vector<int32_t> lrows;
         lambda_rows(L->host.data(), n, L->arena.data(), sp.data(), n_rays, lrows);
         const std::vector<LamRec> lrec = lam_records(iat, ibf, rats, lrows);
-        if ((st = upload(c, "r_lrec", lrec.data(), lrec.size() * sizeof(LamRec), &d_rec))) return st;
+        if ((st = upload(c, "r_lrec", lrec.data(), lrec.size() * sizeof(LamRec), &d_rec)) ||
+            (st = upload(c, "r_rat", rats.data(), rats.size() * sizeof(float2), &d_rat)) ||
+            (st = upload(c, "r_lrow", lrows.data(), lrows.size() * sizeof(int32_t), &d_lrow)))
+            return st;
     }
     DevBuf &bp = c->buf("r_pos"), &bw = c->buf("r_w"), &bs = c->buf("r_st"), &bg = c->buf("r_seg");
     if ((st = bp.ensure((size_t)n_rays * 12)) || (st = bw.ensure((size_t)n_rays * 4)) ||
@@ -1651,10 +1668,12 @@ int gl_trace_ghost_rays_level(gl_ctx* ctx, const gl_lens* lens, int32_t a, int32
     const float *fr = static_cast<const float*>(d_r), *fa = static_cast<const float*>(d_a),
                 *fb = static_cast<const float*>(d_b);
     const LamRec* lq = static_cast<const LamRec*>(d_rec);
+    const float2* rt = static_cast<const float2*>(d_rat);
+    const int2* lr = static_cast<const int2*>(d_lrow);
     const int cc = D.coat_classes;
 #define GL_RAY_LAUNCH(LF_, CM_, RT_)                                                                                 \
     hipLaunchKernelGGL((ray_kernel<LF_, CM_, RT_>), gr, bl, 0, c->stream, D.surf, D.poly, D.arena, n, a, b, fr,      \
-                       n_rays, fa, fb, lq, t_ge0, bp.as<float>(), bw.as<float>(), bs.as<int>(), bg.as<int>())
+                       n_rays, fa, fb, lq, rt, lr, t_ge0, bp.as<float>(), bw.as<float>(), bs.as<int>(), bg.as<int>())
     if (lf == 1 && cc == (CC_SA | CC_FRESNEL)) GL_RAY_LAUNCH(1, CC_SA | CC_FRESNEL, true);
     else if (lf == 1 && cc == (CC_AIRY | CC_FRESNEL)) GL_RAY_LAUNCH(1, CC_AIRY | CC_FRESNEL, true);
     else if (lf == 1) GL_RAY_LAUNCH(1, CC_ALL, true);
