@@ -27,6 +27,7 @@ static constexpr int kRefineGrid = 128;
 static constexpr int kRefineHitsMax = 16;
 static constexpr int kRefineMaxEntries = 4096;
 static constexpr int kStage = 256;            // deposit records staged per wave in LDS
+static constexpr long long kProbeBlocks = 16384;   // workgroups a probe launch aims at
 
 // Persistent-launch schedule candidates the autotuner times: lambda range
 // per work item (ns / lam_div), workgroups per CU, longest-first order.
@@ -82,6 +83,7 @@ struct GhostArgs {
     const float2* __restrict__ rat;             // per-(lambda, surface) refraction ratios (trace_ghost)
     const LamRec* __restrict__ lrec;            // iat, ibf, ratios and lrow interleaved (trace_ghost_rec)
     int lam_group;                              // STATS: lanes per sample (one lambda each), 0 = loop
+    int probe_iters;                            // STATS: sample chunks per workgroup (>= 1)
     int frame_mode;                             // SPLAT: records carry a frame: 1 = source, 2 = pair
     float4* __restrict__ rec;                   // SPLAT: deposit records
     unsigned long long rec_cap;
@@ -412,14 +414,15 @@ __global__ __launch_bounds__(kBlock) void ghost_kernel(GhostArgs A) {
     // STATS with lam_group = G > 0: G lanes per pupil sample, one wavelength
     // each (the probe passes are short; spreading lambda fills the device).
     const int G = STATS ? A.lam_group : 0;
+    const int chunk0 = (int)blockIdx.y * (STATS ? A.probe_iters : 1);   // first sample chunk of this workgroup
     int gi, si0 = 0, si1 = A.n_spec;
     if (STATS && G > 0) {
         const int Gs = G > 0 ? G : 1;                // keeps the non-STATS instantiations free of a zero divisor
-        gi = (int)blockIdx.y * (kBlock / Gs) + (int)(threadIdx.x >> 6) * (64 / Gs) + lane / Gs;
+        gi = chunk0 * (kBlock / Gs) + (int)(threadIdx.x >> 6) * (64 / Gs) + lane / Gs;
         si0 = lane % Gs;
         si1 = si0 < A.n_spec ? si0 + 1 : si0;
     } else {
-        gi = (int)blockIdx.y * kBlock + (int)threadIdx.x;
+        gi = chunk0 * kBlock + (int)threadIdx.x;
     }
     // Whole-block exits are uniform; per-lane validity stays a flag so the
     // wave reductions below see every lane.
@@ -440,16 +443,28 @@ __global__ __launch_bounds__(kBlock) void ghost_kernel(GhostArgs A) {
         if (staged) flush_stage(A, stage, staged, lane);
         return;
     }
-    float u = 0.f, v = 0.f, w_ray = 0.f;
-    bool live = gi < A.n_grid;
-    if (live) live = sample_at(A, ps, gi, u, v, w_ray);
-
     unsigned long long n_tr = 0, n_sv = 0, n_on = 0, n_seg = 0;
+    // STATS: a workgroup takes probe_iters consecutive sample chunks (chunk
+    // blockIdx.y * iters + it) and keeps its counts, its on-sensor box and
+    // its hits in registers across them, so the per-pair counters and the
+    // per-slot box take one set of atomics per wave instead of one per chunk.
+    const int iters = STATS ? A.probe_iters : 1;
+    const int chunk_lanes = (STATS && G > 0) ? kBlock / G : kBlock;
+    float bumn = 3.0f, bvmn = 3.0f, bumx = -3.0f, bvmx = -3.0f;
+    int hits_acc = 0;
+    unsigned long long any_on = 0;
+    for (int it = 0; it < iters; ++it) {
+    if (it > 0) gi += chunk_lanes;
+    const unsigned long long on0 = n_on;
+    float u = 0.f, v = 0.f, w_ray = 0.f;
+    int gs = gi;
+    bool live = gs < A.n_grid;
+    if (live) live = sample_at(A, ps, gs, u, v, w_ray);
     const SrcDev sr = A.src[src_idx];
     const F3 o0 = f3(u * A.front_R + (-20.0f * sr.bx), v * A.front_R + (-20.0f * sr.by), A.start_z);
     const F3 d0 = f3(sr.dx, sr.dy, sr.dz);
     if (A.baffles.n > 0 && live && baffles_block(A.baffles, o0.x, o0.y, sr.bx, sr.by)) live = false;
-    const uint32_t hray = H ? hurb_ray_seed(A, sr, gi, pair_idx) : 0u;
+    const uint32_t hray = H ? hurb_ray_seed(A, sr, gs, pair_idx) : 0u;
     if (live)
         for (int si = si0; si < si1; ++si) {
             const float GLC* spf = as_const(reinterpret_cast<const float*>(A.spec)) + 4 * si;
@@ -470,6 +485,21 @@ __global__ __launch_bounds__(kBlock) void ghost_kernel(GhostArgs A) {
             if (isfinite(spx) && isfinite(spy) && spx >= 0.0f && spx < (float)A.width && spy >= 0.0f &&
                 spy < (float)A.height) ++n_on;
         }
+    if (STATS && A.bbox != nullptr) {
+        // This chunk's box and hits (samples with any wavelength on the sensor).
+        const bool on = n_on > on0;
+        const unsigned long long bal = __ballot(on);
+        int hits = (int)__popcll(bal);
+        if (G > 0) {
+            hits = 0;
+            const unsigned long long gm = G == 64 ? ~0ull : ((1ull << G) - 1ull);
+            for (int g0 = 0; g0 < 64; g0 += G) hits += ((bal >> g0) & gm) != 0ull ? 1 : 0;
+        }
+        hits_acc += hits;
+        any_on |= bal;
+        if (on) { bumn = fminf(bumn, u); bvmn = fminf(bvmn, v); bumx = fmaxf(bumx, u); bvmx = fmaxf(bvmx, v); }
+    }
+    }
     if (STATS) {
         const unsigned long long t = wave_sum_u64(n_tr), s = wave_sum_u64(n_sv), o = wave_sum_u64(n_on);
         const unsigned long long sg = wave_sum_u64(n_seg);
@@ -481,24 +511,15 @@ __global__ __launch_bounds__(kBlock) void ghost_kernel(GhostArgs A) {
             if (A.segs && sg) atomicAdd(A.segs, sg);
         }
         if (A.bbox != nullptr) {
-            const bool on = n_on > 0;
-            const unsigned long long bal = __ballot(on);
-            // hits = pupil samples with any wavelength on the sensor
-            int hits = (int)__popcll(bal);
-            if (G > 0) {
-                hits = 0;
-                const unsigned long long gm = G == 64 ? ~0ull : ((1ull << G) - 1ull);
-                for (int g0 = 0; g0 < 64; g0 += G) hits += ((bal >> g0) & gm) != 0ull ? 1 : 0;
-            }
-            const float umn = wave_min(on ? u : 3.0f), vmn = wave_min(on ? v : 3.0f);
-            const float umx = wave_max(on ? u : -3.0f), vmx = wave_max(on ? v : -3.0f);
-            if (lane0 && bal) {
+            const float umn = wave_min(bumn), vmn = wave_min(bvmn);
+            const float umx = wave_max(bumx), vmx = wave_max(bvmx);
+            if (lane0 && any_on) {
                 const int n_ps = (int)gridDim.x;
                 atomicMin(A.bbox + 2 * ps, enc_pupil(umn));
                 atomicMin(A.bbox + 2 * ps + 1, enc_pupil(vmn));
                 atomicMax(A.bbox + 2 * n_ps + 2 * ps, enc_pupil(umx));
                 atomicMax(A.bbox + 2 * n_ps + 2 * ps + 1, enc_pupil(vmx));
-                atomicAdd(A.bbox + 4 * n_ps + ps, hits);
+                atomicAdd(A.bbox + 4 * n_ps + ps, hits_acc);
             }
         }
     } else if (COUNT) {
@@ -711,7 +732,17 @@ static int launch(const GhostArgs& A_in, int n_ps, int n_samples, hipStream_t s,
     GhostArgs A = A_in;
     A.lam_group = STATS ? lam_group(A.n_spec) : 0;
     const int per_block = A.lam_group > 0 ? kBlock / A.lam_group : kBlock;
-    dim3 grid((unsigned)n_ps, (unsigned)((n_samples + per_block - 1) / per_block), 1);
+    const long long chunks = (n_samples + per_block - 1) / per_block;
+    // Probe passes: enough chunks per workgroup that the launch holds about
+    // kProbeBlocks workgroups (the counters and boxes then take one set of
+    // atomics per wave for all of them); other passes one chunk each.
+    int iters = 1;
+    if (STATS) {
+        const long long want = ((long long)n_ps * chunks + kProbeBlocks - 1) / kProbeBlocks;
+        iters = (int)std::max(1ll, std::min(chunks, want));
+    }
+    A.probe_iters = iters;
+    dim3 grid((unsigned)n_ps, (unsigned)((chunks + iters - 1) / iters), 1);
     if (A.hurb.on) hipLaunchKernelGGL((ghost_kernel<STATS, SPLAT, COUNT, true>), grid, dim3(kBlock), 0, s, A);
     else if (lean == 1) hipLaunchKernelGGL((ghost_kernel<STATS, SPLAT, COUNT, false, 1>), grid, dim3(kBlock), 0, s, A);
     else if (lean == 2) hipLaunchKernelGGL((ghost_kernel<STATS, SPLAT, COUNT, false, 2>), grid, dim3(kBlock), 0, s, A);
