@@ -34,10 +34,10 @@ FP32_VECTOR_PEAK_TFLOPS = 157.3   # MI355X published FP32 vector peak
 OPS_PER_SEGMENT = 116.08
 # HBM bytes per launch of the trace kernel on this workload, from rocprofv3
 # FETCH_SIZE / WRITE_SIZE runs scaled by an in-place 4096^2 complex64
-# transpose (128 MiB each way): 62 MiB read (lens, per-lane wavelength
-# tables, pupil lattice, work list) + 855 MiB written (the 55 M deposit
-# records of 16 B).
-TRACE_TRAFFIC_BYTES = 9.61e8
+# transpose (128 MiB each way): 4.8 MiB read (lens, wavelength records,
+# pupil lattice, work list) + 849 MiB written (the 55 M deposit records of
+# 16 B).
+TRACE_TRAFFIC_BYTES = 8.95e8
 HBM_PEAK_GBS = 8000.0
 
 
