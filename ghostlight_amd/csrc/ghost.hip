@@ -30,9 +30,13 @@ static constexpr int kStage = 256;            // deposit records staged per wave
 static constexpr long long kProbeBlocks = 16384;   // workgroups a probe launch aims at
 
 // Persistent-launch schedule candidates the autotuner times: lambda range
-// per work item (ns / lam_div), workgroups per CU, longest-first order.
+// per work item (ns / lam_div), workgroups per CU, longest-first order.  The
+// lambda range is also the number of wavelengths one pass of 64 lanes holds
+// (deposit_lanes), which decides how coherent a pass is against how many
+// distinct per-lambda table rows it gathers: the 24-surface 31-lambda render
+// is fastest at 4 wavelengths a pass, the 30-surface 16-lambda sweep at 16.
 struct Sched { int lam_div, blocks_per_cu, longest_first; };
-static constexpr Sched kSched[] = {{1, 4, 1}, {1, 8, 1}, {2, 8, 1}, {4, 8, 1}, {1, 8, 0}, {2, 4, 1}};
+static constexpr Sched kSched[] = {{1, 8, 1}, {2, 8, 1}, {4, 8, 1}, {8, 8, 1}, {16, 8, 1}, {8, 8, 0}};
 static constexpr int kNumSched = (int)(sizeof(kSched) / sizeof(kSched[0]));
 
 // hseed: the source-angle part of the per-ray HURB seed, float bits of
