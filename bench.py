@@ -346,10 +346,11 @@ def sweep_leg(n_sources=120, with_cpu=True):
     hw, hh = synthetic.sensor_for(lens, W, H, half_field_deg=12.0)
     t = np.radians(np.linspace(-14.0, 14.0, n_sources))
     src = np.stack([t, t * (H / W), np.ones(n_sources), np.ones(n_sources), np.ones(n_sources)], 1).astype(np.float32)
-    # Schedule candidate 0 (all 16 wavelengths in one work item, so a pass of
-    # 64 lanes holds four samples x 16 wavelengths): the fastest candidate on
-    # this lens, fixed so that the first render needs no autotuning trials.
-    cfg = FlareConfig(ray_grid=192, spectral_samples=16, frame_per_source=True, batch_mode=1, schedule=0)
+    # Schedule candidate 1 (all 16 wavelengths in one work item, so a pass of
+    # 64 lanes holds four samples x 16 wavelengths; eight workgroups per CU):
+    # the fastest candidate on this lens, fixed so that the first render needs
+    # no autotuning trials.
+    cfg = FlareConfig(ray_grid=192, spectral_samples=16, frame_per_source=True, batch_mode=1, schedule=1)
     ctx = Context()
     pairs, boosts = filter_ghost_pairs(lens, hw, hh, cfg, ctx=ctx)
     t0 = time.perf_counter()

@@ -36,7 +36,7 @@ static constexpr long long kProbeBlocks = 16384;   // workgroups a probe launch 
 // distinct per-lambda table rows it gathers: the 24-surface 31-lambda render
 // is fastest at 4 wavelengths a pass, the 30-surface 16-lambda sweep at 16.
 struct Sched { int lam_div, blocks_per_cu, longest_first; };
-static constexpr Sched kSched[] = {{1, 8, 1}, {2, 8, 1}, {4, 8, 1}, {8, 8, 1}, {16, 8, 1}, {8, 8, 0}};
+static constexpr Sched kSched[] = {{1, 4, 1}, {1, 8, 1}, {2, 8, 1}, {4, 8, 1}, {8, 8, 1}, {16, 8, 1}};
 static constexpr int kNumSched = (int)(sizeof(kSched) / sizeof(kSched[0]));
 
 // hseed: the source-angle part of the per-ray HURB seed, float bits of
@@ -1209,7 +1209,17 @@ static int do_launch(Launch& Lc, float* out_r, float* out_g, float* out_b, float
         if (ts->ms.empty()) ts->ms.assign(kNumSched, -1.0f);
         if (p->schedule >= 0 && p->schedule < kNumSched) cand = p->schedule;
         else {
-            for (int k = 0; k < kNumSched && cand < 0; ++k) if (ts->ms[k] < 0.0f) { cand = k; trial = true; }
+            // Trial order: a lens whose coatings read coating rows (the
+            // interleaved-record form) starts from few wavelengths a pass, one
+            // without them from all of them on half the workers (which leaves
+            // room for a concurrent render on another stream) -- the schedule
+            // a one-off shape (a render never repeated) keeps.
+            static constexpr int kOrderRec[kNumSched] = {4, 3, 5, 2, 1, 0};
+            const bool rec_order = D.coat_classes != (CC_AIRY | CC_FRESNEL);
+            for (int q = 0; q < kNumSched && cand < 0; ++q) {
+                const int k = rec_order ? kOrderRec[q] : q;
+                if (ts->ms[k] < 0.0f) { cand = k; trial = true; }
+            }
             if (cand < 0) cand = (int)(std::min_element(ts->ms.begin(), ts->ms.end()) - ts->ms.begin());
         }
         const Sched& sc = kSched[cand];

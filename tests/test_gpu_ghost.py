@@ -324,7 +324,9 @@ def test_autotuner_settles_on_the_fastest_candidate():
         seen.append((s["candidate"], s["tuning"]))
     n = ctx.last_schedule()["candidates"]
     print(f"autotune sequence {seen}", flush=True)
-    assert [c for c, t in seen[:n]] == list(range(n)) and all(t for c, t in seen[:n])
+    # The first n renders try every candidate once (in the lens's trial
+    # order), then the fastest stays.
+    assert sorted(c for c, t in seen[:n]) == list(range(n)) and all(t for c, t in seen[:n])
     assert not seen[-1][1] and seen[-1][0] == seen[-2][0]
     ctx.close()
 
